@@ -1,0 +1,237 @@
+"""ctypes loader for the C oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, always as the checker.  Arrays follow the reference's Fortran
+layout: a 3-D nest field f(nk, nlond, nlatd) is held in numpy as shape
+(nlatd, nlond, nk) C-order (identical bytes), 2-D fields as (nlatd, nlond).
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+GRID_FIELDS = [
+    ("nk", C.c_int), ("nlon", C.c_int), ("nlat", C.c_int),
+    ("dz", C.c_double), ("dlamda", C.c_double), ("dphi", C.c_double),
+    ("dtx2inv", C.c_double), ("shapiro", C.c_double),
+    ("expzmid", C.c_double), ("expzmid_inv", C.c_double),
+    ("expz", C.c_void_p), ("difk", C.c_void_p), ("dift", C.c_void_p), ("xmue", C.c_void_p),
+    ("zpint", C.c_void_p), ("zpmid", C.c_void_p),
+    ("racs", C.c_void_p), ("cs", C.c_void_p), ("tanphi", C.c_void_p), ("cor", C.c_void_p),
+    ("glat", C.c_void_p),
+    ("nstep_sub", C.c_int), ("calc_helium", C.c_int),
+    ("colfac", C.c_double), ("opdiffcap", C.c_double), ("difhor", C.c_double),
+    ("b", C.c_double * 9), ("fb", C.c_double * 3), ("pshelb", C.c_double),
+]
+
+# Order must match tgx_ofields in tgx_oracle.h.
+FIELD_NAMES_3D = (
+    "tn tn_nm un vn un_nm vn_nm o2 o2_nm o1 o1_nm he he_nm "
+    "op op_nm n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm ti te ne o2p w z "
+    "mbar barm xnmbar xnmbari scht schti n2 w_upd "
+    "cp kt km hdt hdu hdv hdo2 hdo1 hdhe qji_tn cool_imp cool_exp qtotal rkm12 "
+    "ui vi wi lxx lyy lxy lyx qop2p qop2d qop qo2p qtef rj rk1 rk2 rk3 rk19 rk20 rk25 "
+    "beta1 beta3 beta5 beta8 beta9 beta9n beta17 ra1 ra2 ra3 n2p nplus nop").split()
+FIELD_NAMES_2D = ("t_lbc u_lbc v_lbc ulbc vlbc bx by bz bmod2 dipmag sndec csdec chi rlatm").split()
+LATERAL = "tn un vn o2 o1 he n2d n4s no ar op".split()
+
+_FIELDS = ([(n, C.c_void_p) for n in FIELD_NAMES_3D] + [(n, C.c_void_p) for n in FIELD_NAMES_2D]
+           + sum([[("lonb_" + n, C.c_void_p), ("latb_" + n, C.c_void_p)] for n in LATERAL], [])
+           + [("ar_glbm1", C.c_double), ("ar_glbm2", C.c_double)])
+
+
+class OGrid(C.Structure):
+    _fields_ = GRID_FIELDS
+
+
+class OFields(C.Structure):
+    _fields_ = _FIELDS
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        _LIB = C.CDLL(path)
+    return _LIB
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleState:
+    """Holds ctypes views of a synthetic state (dict of numpy arrays)."""
+
+    def __init__(self, grid, fields):
+        self.grid_np = grid
+        self.fields_np = fields
+        g = OGrid()
+        g.nk, g.nlon, g.nlat = grid["nk"], grid["nlon"], grid["nlat"]
+        for n in ("dz", "dlamda", "dphi", "dtx2inv", "shapiro", "expzmid", "expzmid_inv",
+                  "colfac", "opdiffcap", "difhor", "pshelb"):
+            setattr(g, n, float(grid[n]))
+        g.nstep_sub, g.calc_helium = int(grid["nstep_sub"]), int(grid["calc_helium"])
+        self._keep = []
+        for n in ("expz", "difk", "dift", "xmue", "zpint", "zpmid", "racs", "cs", "tanphi", "cor", "glat"):
+            a = np.ascontiguousarray(grid[n], dtype=np.float64)
+            self._keep.append(a)
+            setattr(g, n, a.ctypes.data)
+        bb = np.asarray(grid["b"], dtype=np.float64).reshape(3, 3)  # b[l][m] = b(l+1,m+1)
+        for i in range(9):
+            g.b[i] = float(bb.reshape(-1)[i])
+        for i in range(3):
+            g.fb[i] = float(grid["fb"][i])
+        self.g = g
+        f = OFields()
+        for n in FIELD_NAMES_3D + FIELD_NAMES_2D:
+            a = fields[n]
+            self._keep.append(a)
+            setattr(f, n, _p(a).value)
+        for n in LATERAL:
+            setattr(f, "lonb_" + n, _p(fields["lonb_" + n]).value)
+            setattr(f, "latb_" + n, _p(fields["latb_" + n]).value)
+        f.ar_glbm1, f.ar_glbm2 = float(grid["ar_glbm"][0]), float(grid["ar_glbm"][1])
+        self.f = f
+        self.shape3 = (grid["nlat"] + 4, grid["nlon"] + 4, grid["nk"])
+        self.shape2 = self.shape3[:2]
+
+    def z3(self):
+        return np.zeros(self.shape3)
+
+    def z2(self):
+        return np.zeros(self.shape2)
+
+    # ---- column-step routines --------------------------------------------
+    def dt_ng(self, tlbc):
+        tlbc = tlbc.copy()
+        tlbc_nm, tn, tn_nm = self.z2(), self.z3(), self.z3()
+        lib().tgx_o_dt_ng(C.byref(self.g), C.byref(self.f), _p(tlbc), _p(tlbc_nm), _p(tn), _p(tn_nm))
+        return dict(tn=tn, tn_nm=tn_nm, tlbc=tlbc, tlbc_nm=tlbc_nm)
+
+    def duv_ng(self, tn_upd, Fe, Fn, tlbc_nm, ulbc, vlbc):
+        ulbc, vlbc = ulbc.copy(), vlbc.copy()
+        ulbc_nm, vlbc_nm = self.z2(), self.z2()
+        un, unm, vn, vnm = self.z3(), self.z3(), self.z3(), self.z3()
+        lib().tgx_o_duv_ng(C.byref(self.g), C.byref(self.f), _p(tn_upd), _p(Fe), _p(Fn), _p(tlbc_nm),
+                           _p(ulbc), _p(vlbc), _p(ulbc_nm), _p(vlbc_nm), _p(un), _p(unm), _p(vn), _p(vnm))
+        return dict(un=un, un_nm=unm, vn=vn, vn_nm=vnm, ulbc=ulbc, vlbc=vlbc, ulbc_nm=ulbc_nm, vlbc_nm=vlbc_nm)
+
+    def minor(self, species, n2d_upd, tlbc):
+        out, out_nm = self.z3(), self.z3()
+        lib().tgx_o_minor_species(C.byref(self.g), C.byref(self.f), C.c_int(species), _p(n2d_upd),
+                                  _p(tlbc), _p(out), _p(out_nm))
+        return out, out_nm
+
+    def minors(self, tlbc):
+        """N2D first, then N4S and NO with the new N2D, then AR (dynamics_ng.F90:147-165)."""
+        n2d, n2d_nm = self.minor(2, np.zeros(self.shape3), tlbc)
+        n4s, n4s_nm = self.minor(1, n2d, tlbc)
+        no, no_nm = self.minor(0, n2d, tlbc)
+        ar, ar_nm = self.minor(3, n2d, tlbc)
+        return dict(n2d=n2d, n2d_nm=n2d_nm, n4s=n4s, n4s_nm=n4s_nm, no=no, no_nm=no_nm, ar=ar, ar_nm=ar_nm)
+
+    def oplus_ng(self, op, optm1, lonb, latb):
+        outs = [self.z3() for _ in range(6)]
+        lib().tgx_o_oplus_ng(C.byref(self.g), C.byref(self.f), _p(op), _p(optm1), _p(lonb), _p(latb),
+                             *[_p(o) for o in outs])
+        return dict(zip(("op", "op_nm", "xiop2p", "xiop2d", "Fe", "Fn"), outs))
+
+    def oplus_subcycled(self, lonb_sub, latb_sub):
+        """dynamics_ng.F90:99-121: nstep_sub sub-cycles, itp/itc swapped between them."""
+        op, optm1 = self.fields_np["op"], self.fields_np["op_nm"]
+        out = None
+        for s in range(int(self.grid_np["nstep_sub"])):
+            out = self.oplus_ng(op, optm1, lonb_sub[s], latb_sub[s])
+            op, optm1 = out["op"], out["op_nm"]
+        return out
+
+    def comp_o2o_ng(self):
+        fs = np.zeros((4, 3) + self.shape3)
+        lib().tgx_o_comp_o2o_ng(C.byref(self.g), C.byref(self.f), _p(fs))
+        return fs
+
+    def comp_ng(self, fs, tlbc, flx_he):
+        flx_he = flx_he.copy()
+        outs = [self.z3() for _ in range(6)]
+        lib().tgx_o_comp_ng(C.byref(self.g), C.byref(self.f), _p(fs), _p(tlbc), _p(flx_he),
+                            *[_p(o) for o in outs])
+        d = dict(zip(("o2", "o2_nm", "o1", "o1_nm", "he", "he_nm"), outs))
+        d["flx_he"] = flx_he
+        return d
+
+
+# ---- plain solvers ----------------------------------------------------------
+def trsolv(a, b, c, f, x, lev0, lev1, k1, k2, lon0, lon1, lonmax):
+    lib().tgx_o_trsolv(_p(a), _p(b), _p(c), _p(f), _p(x), lev0, lev1, k1, k2, lon0, lon1, lonmax)
+    return x
+
+
+def trsolv_ng(a, b, c, f, x, nk, ncol, k2):
+    lib().tgx_o_trsolv_ng(_p(a), _p(b), _p(c), _p(f), _p(x), nk, ncol, k2)
+    return x
+
+
+def blktri(a, b, c, f, x, lev0, lev1, lon0, lon1, lonmax):
+    lib().tgx_o_blktri(_p(a), _p(b), _p(c), _p(f), _p(x), lev0, lev1, lon0, lon1, lonmax)
+    return x
+
+
+def blktri3(p, q, r, f, x, nk, ncol, k2):
+    lib().tgx_o_blktri3(_p(p), _p(q), _p(r), _p(f), _p(x), nk, ncol, k2)
+    return x
+
+
+def matinv3(a):
+    """a: (3,3) numpy matrix; returns its inverse via the oracle (adjugate/det)."""
+    af = np.asfortranarray(a, dtype=np.float64).reshape(-1, order="F").copy()
+    out = np.zeros(9)
+    lib().tgx_o_matinv3(_p(af), _p(out))
+    return out.reshape(3, 3, order="F")
+
+
+def bndcmp(dz):
+    b = np.zeros(9)
+    fb = np.zeros(3)
+    lib().tgx_o_bndcmp(C.c_double(dz), _p(b), _p(fb))
+    return b.reshape(3, 3), fb
+
+
+def df3d(f, order):
+    """dffm_ng df_3d on a (nlatd, nlond, nk) array."""
+    out = np.zeros_like(f)
+    lib().tgx_o_df3d(_p(f), _p(out), int(order), f.shape[2], f.shape[1], f.shape[0])
+    return out
+
+
+def fm3d(f, order):
+    out = np.zeros_like(f)
+    lib().tgx_o_fm3d(_p(f), _p(out), int(order), f.shape[2], f.shape[1], f.shape[0])
+    return out
+
+
+def _methods():
+    def advec(self, f):
+        out = self.z3()
+        lib().tgx_o_advec(C.byref(self.g), _p(f), _p(self.fields_np["un"]), _p(self.fields_np["vn"]), _p(out))
+        return out
+
+    def smooth(self, f, shapiro=None):
+        out = self.z3()
+        s = self.grid_np["shapiro"] if shapiro is None else shapiro
+        lib().tgx_o_smooth(C.byref(self.g), _p(f), _p(out), C.c_double(s))
+        return out
+
+    def div(self, f, direction):
+        out = self.z3()
+        lib().tgx_o_div(C.byref(self.g), _p(f), _p(out), int(direction))
+        return out
+    OracleState.advec, OracleState.smooth, OracleState.div = advec, smooth, div
+
+
+_methods()

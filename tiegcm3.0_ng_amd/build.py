@@ -1,0 +1,66 @@
+"""Builds libtgx.so (HIP, gfx950) in-tree with hipcc.
+
+Used by __graft_entry__.build() and runnable directly:
+    python tiegcm3.0_ng_amd/build.py
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libtgx.so")
+SOURCES = ["solvers.hip", "step.hip", "api.cpp", "step_api.cpp"]
+
+# FP64 parity kernels follow the reference's arithmetic order: no contraction.
+# Packed FP32 instructions are disabled for the whole library.
+HIPFLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def hipcc():
+    for p in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if p and os.path.exists(p):
+            return p
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=True, jobs=4):
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for s in srcs:
+        obj = os.path.join(objdir, s + ".o")
+        objs.append(obj)
+        cmd = [hipcc()] + HIPFLAGS + ["-c", os.path.join(CSRC, s), "-o", obj]
+        if s.endswith(".cpp"):
+            cmd = [hipcc(), "-x", "hip"] + HIPFLAGS + ["-c", os.path.join(CSRC, s), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        if len([p for p in procs if p[1].poll() is None]) >= jobs:
+            procs[0][1].wait()
+    for s, p in procs:
+        out, _ = p.communicate()
+        txt = out.decode(errors="replace")
+        if p.returncode != 0:
+            raise RuntimeError("hipcc failed on %s:\n%s" % (s, txt))
+        if verbose and txt.strip():
+            print(txt)
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

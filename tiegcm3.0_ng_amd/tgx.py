@@ -1,0 +1,173 @@
+"""Host-side mirror of the reference's column-solver interface over libtgx.
+
+Names, argument order and meaning follow the Fortran subroutines
+(trsolv.F:2-3, trsolv_ng.F90:1-9, duv.F:801-820).  Host arrays are numpy
+float64 arrays in the reference's memory layout (level fastest): a Fortran
+array real(lev0:lev1, lon0:lon1) is a C-order numpy array of shape
+(nlon, nlev).  Device entry points take torch tensors already on the GPU.
+
+Everything here calls libtgx.so; there is no CPU fallback.  If the library is
+missing or no GPU is present, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtgx.so")
+_lib = None
+
+SCHEME_AUTO, SCHEME_THOMAS, SCHEME_PCR = 0, 1, 2
+
+_D = C.c_void_p
+_I = C.c_int
+_SIGS = {
+    "tgx_version": ([], _I),
+    "tgx_last_error": ([], C.c_char_p),
+    "tgx_last_status": ([], _I),
+    "tgx_clear_error": ([], None),
+    "tgx_trsolv_f64": ([_D] * 5 + [_I] * 9, None),
+    "tgx_trsolv_ng_f64": ([_D] * 5 + [_I] * 4, None),
+    "tgx_blktri_f64": ([_D] * 8 + [_I] * 5, None),
+    "tgx_blktri3_f64": ([_D] * 5 + [_I] * 3, None),
+    "tgx_bndcmp": ([C.c_double, _D, _D], None),
+    "tgx_trsolv_dev_f64": ([_D] * 5 + [_I] * 6 + [_D], _I),
+    "tgx_autotune_choice": ([_I] * 4, _I),
+    "tgx_trsolv_dev_mixed": ([_D] * 5 + [_I] * 6 + [_D], _I),
+    "tgx_blktri2_dev_f64": ([_D] * 5 + [_I] * 4 + [_D], _I),
+    "tgx_blktri3_dev_f64": ([_D] * 5 + [_I] * 4 + [_D], _I),
+}
+
+
+class TgxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libtgx.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TgxError("libtgx.so is not built; run tiegcm3.0_ng_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+        try:
+            from . import step as _step  # noqa: F401  (registers the step signatures)
+        except ImportError:
+            pass
+    return _lib
+
+
+def last_error():
+    return lib().tgx_last_error().decode()
+
+
+def _check_void():
+    L = lib()
+    if L.tgx_last_status() != 0:
+        raise TgxError(last_error())
+
+
+def _check(rc):
+    if rc != 0:
+        raise TgxError(last_error() or ("libtgx error %d" % rc))
+
+
+def _hp(a):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return a.ctypes.data
+
+
+def _dp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(stream)
+
+
+# ---- drop-in host entry points ------------------------------------------------
+def trsolv(a, b, c, f, x, lev0, lev1, k1, k2, lon0, lon1, lonmax, lat=0, idebug=0):
+    """trsolv.F:2-3.  Arrays of shape (lon1-lon0+1, lev1-lev0+1); x updated in place."""
+    lib().tgx_trsolv_f64(_hp(a), _hp(b), _hp(c), _hp(f), _hp(x), lev0, lev1, k1, k2, lon0, lon1,
+                         lonmax, lat, idebug)
+    _check_void()
+    return x
+
+
+def trsolv_ng(a, b, c, f, x, k2):
+    """trsolv_ng.F90:1-9.  Arrays of shape (nlatd, nlond, nk); x updated in place."""
+    nlatd, nlond, nk = a.shape
+    lib().tgx_trsolv_ng_f64(_hp(a), _hp(b), _hp(c), _hp(f), _hp(x), nk, nlond, nlatd, k2)
+    _check_void()
+    return x
+
+
+def blktri(a, b, c, f, x, lev0, lev1, lon0, lon1, lonmax, beta=None, gamma=None, y=None):
+    """duv.F:801-820.  a,b,c of shape (ncol, nlev, 2(col), 2(row)); f,x (ncol, nlev, 2)."""
+    z = C.c_void_p(0)
+    lib().tgx_blktri_f64(_hp(a), _hp(b), _hp(c), _hp(f), z if beta is None else _hp(beta),
+                         z if gamma is None else _hp(gamma), z if y is None else _hp(y), _hp(x),
+                         lev0, lev1, lon0, lon1, lonmax)
+    _check_void()
+    return x
+
+
+def blktri3(p, q, r, f, x, k2):
+    """comp's 3x3 elimination.  p,q,r (ncol, nk, 3, 3) column-major blocks; f,x (ncol, nk, 3)."""
+    ncol, nk = p.shape[0], p.shape[1]
+    lib().tgx_blktri3_f64(_hp(p), _hp(q), _hp(r), _hp(f), _hp(x), nk, ncol, k2)
+    _check_void()
+    return x
+
+
+def bndcmp(dz):
+    b = np.zeros(9)
+    fb = np.zeros(3)
+    lib().tgx_bndcmp(float(dz), _hp(b), _hp(fb))
+    return b.reshape(3, 3), fb
+
+
+# ---- device entry points -------------------------------------------------------
+def trsolv_dev(a, b, c, f, x, k1, k2, scheme=SCHEME_AUTO, col0=0, ncol=None, stream=None):
+    """Batched scalar solve on device tensors of shape (ncol, ldk) (level fastest)."""
+    ldk = a.shape[-1]
+    ncol = a.numel() // ldk - col0 if ncol is None else ncol
+    _check(lib().tgx_trsolv_dev_f64(_dp(a), _dp(b), _dp(c), _dp(f), _dp(x), ldk, k1, k2, col0,
+                                    ncol, scheme, _stream(stream)))
+    return x
+
+
+def trsolv_dev_mixed(a, b, c, f, x, k1, k2, nsweep=2, col0=0, ncol=None, stream=None):
+    ldk = a.shape[-1]
+    ncol = a.numel() // ldk - col0 if ncol is None else ncol
+    _check(lib().tgx_trsolv_dev_mixed(_dp(a), _dp(b), _dp(c), _dp(f), _dp(x), ldk, k1, k2, col0,
+                                      ncol, nsweep, _stream(stream)))
+    return x
+
+
+def autotune_choice(ldk, k1, k2, ncol):
+    return lib().tgx_autotune_choice(ldk, k1, k2, ncol)
+
+
+def blktri2_dev(a, b, c, f, x, nrow, stream=None):
+    ncol, ldk = a.shape[0], a.shape[1]
+    _check(lib().tgx_blktri2_dev_f64(_dp(a), _dp(b), _dp(c), _dp(f), _dp(x), ldk, nrow, 0, ncol,
+                                     _stream(stream)))
+    return x
+
+
+def blktri3_dev(p, q, r, f, x, nrow, stream=None):
+    ncol, ldk = p.shape[0], p.shape[1]
+    _check(lib().tgx_blktri3_dev_f64(_dp(p), _dp(q), _dp(r), _dp(f), _dp(x), ldk, nrow, 0, ncol,
+                                     _stream(stream)))
+    return x
