@@ -1,0 +1,107 @@
+/*
+ * libtgx — fused implicit column step of one nest domain, device resident.
+ *
+ * Replaces the assembly+solve of dt_ng (dt_ng.F90:1), duv_ng (duv_ng.F90:1),
+ * minor_ng with comp_{n2d,n4s,no,ar}_ng (minor_ng.F90:1, comp_*_ng.F90),
+ * oplus_ng with its nstep_sub sub-cycles (oplus_ng.F90:1,
+ * dynamics_ng.F90:99-121), and comp_o2o_ng + comp_ng (comp_o2o_ng.F90:1,
+ * comp_ng.F90:1).  Semantics are those of the reference running the nest on
+ * one task: all four sides are domain edges (lateral Dirichlet data applied,
+ * one-sided stencils at the ghost points).  addfld diagnostics are not
+ * produced.
+ *
+ * Device layout ("level planes", longitude fastest): element (k, i, j) of a
+ * 3-D field is at k*plane + j*ld + i with plane = ld*(nlat+4), i in
+ * [0, nlon+4) (i = Fortran lon + 1), j in [0, nlat+4), k in [0, nk);
+ * 2-D fields are at j*ld + i.  ld >= nlon+4 is the row pitch in elements.
+ * Lateral boundary data: lonb[(k*2 + side)*(nlat+4) + j] (side 0 = lon -1,
+ * side 1 = lon nlon+2), latb[(k*2 + side)*ld + i] (side 0 = lat -1, side 1 =
+ * lat nlat+2); the O+ data carries one such block per sub-step.
+ *
+ * All pointers are device pointers.  Every entry point is stream-ordered on
+ * the hipStream_t passed as void* and returns 0 or a TGX_ERR_* code.
+ */
+#ifndef TGX_STEP_H
+#define TGX_STEP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nk, nlon, nlat, ld;
+  int nstep_sub;      /* O+ sub-cycles per step (input namelist NSTEP_SUB) */
+  int calc_helium;
+  double dz;          /* dz(i_ng) = dlev_ng */
+  double dlamda, dphi, dtx2inv, shapiro, expzmid, expzmid_inv;
+  double colfac, opdiffcap, difhor, pshelb, ar_glbm1, ar_glbm2;
+  double b[9];        /* lbc b(l,m), row-major b[3*(l-1)+(m-1)] */
+  double fb[3];
+  const double *expz, *difk, *dift, *xmue, *zpint, *zpmid;   /* [nk] */
+  const double *racs, *cs, *tanphi, *cor, *glat;             /* [nlat+4] */
+} tgx_ng_grid;
+
+/* Model state read by the step (itp level unless noted). */
+typedef struct {
+  const double *tn, *tn_nm, *un, *vn, *un_nm, *vn_nm;
+  const double *o2, *o2_nm, *o1, *o1_nm, *he, *he_nm;
+  const double *op, *op_nm, *n2d, *n2d_nm, *n4s, *n4s_nm, *no, *no_nm, *ar, *ar_nm;
+  const double *ti, *te, *ne, *o2p, *w, *z;
+  const double *mbar, *barm, *xnmbar, *xnmbari, *scht, *schti, *n2;
+  const double *w_upd;                         /* flds%w(itc) from swdot_ng */
+  const double *cp, *kt, *km, *hdt, *hdu, *hdv, *hdo2, *hdo1, *hdhe;
+  const double *qji_tn, *cool_imp, *cool_exp, *qtotal, *rkm12;
+  const double *ui, *vi, *wi, *lxx, *lyy, *lxy, *lyx;
+  const double *qop2p, *qop2d, *qop, *qo2p, *qtef, *rj;
+  const double *rk1, *rk2, *rk3, *rk19, *rk20, *rk25;
+  const double *beta1, *beta3, *beta5, *beta8, *beta9, *beta9n, *beta17;
+  const double *ra1, *ra2, *ra3, *n2p, *nplus, *nop;
+  /* 2-D */
+  const double *t_lbc, *u_lbc, *v_lbc;   /* this step's lower-boundary values */
+  const double *tlbc, *ulbc, *vlbc;      /* flds%tlbc/ulbc/vlbc at the start of the step */
+  const double *flx_he;
+  const double *bx, *by, *bz, *bmod2, *dipmag, *sndec, *csdec, *chi, *rlatm;
+  /* lateral Dirichlet data */
+  const double *lonb_tn, *latb_tn, *lonb_un, *latb_un, *lonb_vn, *latb_vn;
+  const double *lonb_o2, *latb_o2, *lonb_o1, *latb_o1, *lonb_he, *latb_he;
+  const double *lonb_n2d, *latb_n2d, *lonb_n4s, *latb_n4s, *lonb_no, *latb_no;
+  const double *lonb_ar, *latb_ar, *lonb_op, *latb_op;
+} tgx_ng_fields;
+
+/* Outputs (itc level).  2-D: tlbc/tlbc_nm as dt_ng leaves them
+ * (dt_ng.F90:210-211), ulbc..vlbc_nm as duv_ng leaves them
+ * (duv_ng.F90:315-318), flx_he as comp_ng leaves it (comp_ng.F90:72). */
+typedef struct {
+  double *tn, *tn_nm;
+  double *un, *un_nm, *vn, *vn_nm;
+  double *n2d, *n2d_nm, *n4s, *n4s_nm, *no, *no_nm, *ar, *ar_nm;
+  double *op, *op_nm, *xiop2p, *xiop2d, *Fe, *Fn;
+  double *o2, *o2_nm, *o1, *o1_nm, *he, *he_nm;
+  double *tlbc, *tlbc_nm, *ulbc, *vlbc, *ulbc_nm, *vlbc_nm, *flx_he;
+} tgx_ng_out;
+
+/* Bytes of device workspace the step needs for this grid (Thomas ratios,
+ * block gammas, O+ and wind intermediates). */
+unsigned long long tgx_ng_workspace_bytes(const tgx_ng_grid *g);
+
+/* The routines, each on its own; work is a device buffer of
+ * tgx_ng_workspace_bytes bytes. */
+int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                 void *stream);
+int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                  void *stream);
+int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+              void *stream);
+/* needs out->tn (dt_ng) and out->Fe/Fn (oplus_ng) of the same step */
+int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+               void *stream);
+int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                void *stream);
+/* The whole column step in the reference's order (dynamics_ng.F90:99-233). */
+int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -16,16 +16,14 @@ int tgx_set_error(int code, const char *fmt, ...);
                            hipGetErrorString(e_));                                 \
   } while (0)
 
-// Grid-stride helpers.  Launches are 1-D over columns; blocks are remapped so
-// that each of the 8 XCDs (which receive blocks round-robin) works on one
-// contiguous range of columns, keeping neighbouring latitude rows in one L2.
+// Launches are 1-D over columns.  The hardware hands consecutive block ids
+// to the 8 XCDs round-robin; this remap gives every XCD one contiguous range
+// of block ids (neighbouring latitude rows then share one L2).  Bijective on
+// [0, nblocks) for any nblocks.
 __device__ __forceinline__ int tgx_xcd_block(int nblocks) {
   const int nxcd = 8;
   int b = blockIdx.x;
-  int per = (nblocks + nxcd - 1) / nxcd;
-  int xcd = b % nxcd, slot = b / nxcd;
-  int nb = xcd * per + slot;
-  // the last XCD ranges may be short: fold overflow back (keeps a bijection
-  // because per*nxcd >= nblocks and we only drop ids >= nblocks)
-  return nb;
+  int x = b % nxcd, slot = b / nxcd;
+  int q = nblocks / nxcd, r = nblocks % nxcd;
+  return x * q + (x < r ? x : r) + slot;
 }

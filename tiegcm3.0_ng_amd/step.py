@@ -1,0 +1,222 @@
+"""Host side of the fused column step over libtgx (include/tgx_step.h).
+
+`NestState` holds one nest domain's model state on the GPU in the library's
+level-plane layout (longitude fastest, row pitch `ld`), the outputs and the
+workspace, and calls the C-ABI entry points that mirror the reference's
+routines: oplus_ng (with its nstep_sub sub-cycles), the minor species,
+dt_ng, duv_ng and comp_o2o_ng + comp_ng, or the whole step in the
+reference's order (dynamics_ng.F90:99-233).
+"""
+import ctypes as C
+
+import torch
+
+from . import tgx as _tgx
+
+_P = C.c_void_p
+
+GRID_FIELDS = [
+    ("nk", C.c_int), ("nlon", C.c_int), ("nlat", C.c_int), ("ld", C.c_int),
+    ("nstep_sub", C.c_int), ("calc_helium", C.c_int),
+    ("dz", C.c_double), ("dlamda", C.c_double), ("dphi", C.c_double), ("dtx2inv", C.c_double),
+    ("shapiro", C.c_double), ("expzmid", C.c_double), ("expzmid_inv", C.c_double),
+    ("colfac", C.c_double), ("opdiffcap", C.c_double), ("difhor", C.c_double),
+    ("pshelb", C.c_double), ("ar_glbm1", C.c_double), ("ar_glbm2", C.c_double),
+    ("b", C.c_double * 9), ("fb", C.c_double * 3),
+    ("expz", _P), ("difk", _P), ("dift", _P), ("xmue", _P), ("zpint", _P), ("zpmid", _P),
+    ("racs", _P), ("cs", _P), ("tanphi", _P), ("cor", _P), ("glat", _P),
+]
+IN_3D = ("tn tn_nm un vn un_nm vn_nm o2 o2_nm o1 o1_nm he he_nm "
+         "op op_nm n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm ti te ne o2p w z "
+         "mbar barm xnmbar xnmbari scht schti n2 w_upd "
+         "cp kt km hdt hdu hdv hdo2 hdo1 hdhe qji_tn cool_imp cool_exp qtotal rkm12 "
+         "ui vi wi lxx lyy lxy lyx qop2p qop2d qop qo2p qtef rj rk1 rk2 rk3 rk19 rk20 rk25 "
+         "beta1 beta3 beta5 beta8 beta9 beta9n beta17 ra1 ra2 ra3 n2p nplus nop").split()
+IN_2D = "t_lbc u_lbc v_lbc tlbc ulbc vlbc flx_he bx by bz bmod2 dipmag sndec csdec chi rlatm".split()
+LATERAL = "tn un vn o2 o1 he n2d n4s no ar op".split()
+OUT_3D = ("tn tn_nm un un_nm vn vn_nm n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm "
+          "op op_nm xiop2p xiop2d Fe Fn o2 o2_nm o1 o1_nm he he_nm").split()
+OUT_2D = "tlbc tlbc_nm ulbc vlbc ulbc_nm vlbc_nm flx_he".split()
+
+
+class NgGrid(C.Structure):
+    _fields_ = GRID_FIELDS
+
+
+class NgFields(C.Structure):
+    _fields_ = ([(n, _P) for n in IN_3D] + [(n, _P) for n in IN_2D]
+                + sum([[("lonb_" + n, _P), ("latb_" + n, _P)] for n in LATERAL], []))
+
+
+class NgOut(C.Structure):
+    _fields_ = [(n, _P) for n in OUT_3D + OUT_2D]
+
+
+_SIGS = {
+    "tgx_ng_workspace_bytes": ([_P], C.c_ulonglong),
+    "tgx_ng_oplus": ([_P] * 5, C.c_int),
+    "tgx_ng_minors": ([_P] * 5, C.c_int),
+    "tgx_ng_dt": ([_P] * 5, C.c_int),
+    "tgx_ng_duv": ([_P] * 5, C.c_int),
+    "tgx_ng_comp": ([_P] * 5, C.c_int),
+    "tgx_ng_step": ([_P] * 5, C.c_int),
+}
+
+
+def _lib():
+    L = _tgx.lib()
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+def pitch(nlond, align=16):
+    return (nlond + align - 1) // align * align
+
+
+class NestState:
+    """One nest domain on the GPU.
+
+    grid:   dict from synth.make_grid
+    fields: dict of float64 tensors in logical layout (synth.make_state),
+            on the CPU or already on the GPU.
+    """
+
+    def __init__(self, grid, fields, device="cuda"):
+        self.grid = grid
+        nk, nlat, nlon = grid["nk"], grid["nlat"], grid["nlon"]
+        self.nk, self.nlatd, self.nlond = nk, nlat + 4, nlon + 4
+        self.ld = pitch(self.nlond)
+        self.dev = device
+        kw = dict(dtype=torch.float64, device=device)
+        self._keep = []
+
+        def lvl(name):
+            t = torch.tensor(grid[name], **kw)
+            self._keep.append(t)
+            return t.data_ptr()
+
+        g = NgGrid()
+        g.nk, g.nlon, g.nlat, g.ld = nk, nlon, nlat, self.ld
+        g.nstep_sub, g.calc_helium = int(grid["nstep_sub"]), int(grid["calc_helium"])
+        for n in ("dz", "dlamda", "dphi", "dtx2inv", "shapiro", "expzmid", "expzmid_inv",
+                  "colfac", "opdiffcap", "difhor", "pshelb"):
+            setattr(g, n, float(grid[n]))
+        g.ar_glbm1, g.ar_glbm2 = float(grid["ar_glbm"][0]), float(grid["ar_glbm"][1])
+        bb = [float(x) for row in grid["b"] for x in row]
+        for i in range(9):
+            g.b[i] = bb[i]
+        for i in range(3):
+            g.fb[i] = float(grid["fb"][i])
+        for n in ("expz", "difk", "dift", "xmue", "zpint", "zpmid", "racs", "cs", "tanphi", "cor", "glat"):
+            setattr(g, n, lvl(n))
+        self.g = g
+
+        self.inp = {}
+        f = NgFields()
+        for n in IN_3D:
+            self.inp[n] = self._put3(fields[n])
+        for n in IN_2D:
+            self.inp[n] = self._put2(fields[n])
+        for n in LATERAL:
+            lonb, latb = fields["lonb_" + n], fields["latb_" + n]
+            self.inp["lonb_" + n] = lonb.to(**kw).contiguous()
+            self.inp["latb_" + n] = self._padlast(latb.to(**kw))
+        for n, t in self.inp.items():
+            setattr(f, n, t.data_ptr())
+        self.f = f
+
+        self.out = {n: torch.zeros((nk, self.nlatd, self.ld), **kw) for n in OUT_3D}
+        self.out.update({n: torch.zeros((self.nlatd, self.ld), **kw) for n in OUT_2D})
+        o = NgOut()
+        for n, t in self.out.items():
+            setattr(o, n, t.data_ptr())
+        self.o = o
+        nbytes = _lib().tgx_ng_workspace_bytes(C.byref(self.g))
+        self.work = torch.empty((nbytes + 7) // 8, **kw)
+
+    # ---- layout helpers ------------------------------------------------------
+    def _padlast(self, t):
+        shp = t.shape[:-1] + (self.ld,)
+        out = torch.zeros(shp, dtype=torch.float64, device=self.dev)
+        out[..., :t.shape[-1]] = t
+        return out
+
+    def _put3(self, t):
+        return self._padlast(t.to(dtype=torch.float64, device=self.dev))
+
+    def _put2(self, t):
+        return self._padlast(t.to(dtype=torch.float64, device=self.dev))
+
+    def fortran(self, name):
+        """An output in the reference's Fortran layout, numpy (nlatd, nlond, nk) or (nlatd, nlond)."""
+        t = self.out[name][..., :self.nlond]
+        if t.dim() == 3:
+            t = t.permute(1, 2, 0)
+        return t.contiguous().cpu().numpy()
+
+    # ---- entry points -----------------------------------------------------------
+    def _call(self, name, stream=None):
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = getattr(_lib(), name)(C.byref(self.g), C.byref(self.f), C.byref(self.o),
+                                   C.c_void_p(self.work.data_ptr()), C.c_void_p(s))
+        if rc != 0:
+            raise _tgx.TgxError(_tgx.last_error())
+
+    def oplus(self, stream=None):
+        self._call("tgx_ng_oplus", stream)
+
+    def minors(self, stream=None):
+        self._call("tgx_ng_minors", stream)
+
+    def dt(self, stream=None):
+        self._call("tgx_ng_dt", stream)
+
+    def duv(self, stream=None):
+        self._call("tgx_ng_duv", stream)
+
+    def comp(self, stream=None):
+        self._call("tgx_ng_comp", stream)
+
+    def step(self, stream=None):
+        self._call("tgx_ng_step", stream)
+
+
+def run_oracle_step(oracle, grid, Fo):
+    """The oracle's column step in the reference's order (dynamics_ng.F90)."""
+    st = oracle.OracleState(grid, Fo)
+    res = {}
+    opl = st.oplus_subcycled(Fo["lonb_op"], Fo["latb_op"])
+    res.update(opl)
+    res.update(st.minors(Fo["tlbc"]))
+    d = st.dt_ng(Fo["tlbc"])
+    res.update(dict(tn=d["tn"], tn_nm=d["tn_nm"], tlbc=d["tlbc"], tlbc_nm=d["tlbc_nm"]))
+    uv = st.duv_ng(d["tn"], opl["Fe"], opl["Fn"], d["tlbc_nm"], Fo["ulbc"], Fo["vlbc"])
+    res.update(uv)
+    fs = st.comp_o2o_ng()
+    c = st.comp_ng(fs, d["tlbc"], Fo["flx_he"])
+    res.update(c)
+    return res
+
+
+def smoke_check(oracle, name="tiny", tol=1e-12):
+    """One small fused column step on cuda:0 against the oracle."""
+    import numpy as np
+    from . import synth
+    g = synth.make_grid(name, nstep_sub=2)
+    F = synth.make_state(g)
+    Fo = synth.to_fortran_layout(F)
+    ref = run_oracle_step(oracle, g, Fo)
+    ns = NestState(g, F)
+    ns.step()
+    torch.cuda.synchronize()
+    worst = 0.
+    for n in OUT_3D:
+        got, exp = ns.fortran(n), ref[n]
+        err = float(np.abs(got - exp).max() / max(np.abs(exp).max(), 1e-300))
+        worst = max(worst, err)
+        assert err <= tol, (n, err)
+    print("smoke ok: fused column step on %s, worst field-relative error %.3e" % (name, worst))
+    return worst
