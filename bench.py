@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused implicit column step (TIE-GCM NG) on one MI355X.
+
+Workload (default): the 0.625 deg global grid, 576 x 288 columns x 113 levels,
+all 11 unknowns per column per step (TN; UN+VN as the 2x2 block; O+; NO,
+N4S, N2D, Ar; O2+O+He as the 3x3 block), one O+ sub-cycle, FP64, seeded
+synthetic fields resident in HBM.  A step is one pass of the whole column
+step (dynamics_ng.F90 order) over that state.
+
+Prints one JSON line:
+  value   = Mcolumn-level solves/s over the interior columns (whole job),
+  roofline: the dominant kernel's algorithmic bytes / its hipEvent-timed
+            duration against the 8 TB/s HBM peak,
+  cpu_baseline: the C oracle (a restatement of the reference) on one host
+            core over a bounded sub-grid of the same workload.
+"""
+import argparse
+import importlib
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "tiegcm3.0_ng_amd")
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    name = "tiegcm_ng_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    for sub in ("synth", "tgx", "step"):
+        setattr(mod, sub, importlib.import_module(name + "." + sub))
+    return mod
+
+
+ROUTINES = ("oplus", "minors", "dt", "duv", "comp")
+
+
+def cpu_baseline(pkg, config, nstep_sub, budget_s=20.0):
+    """The oracle's column step, one host core, on a sub-grid of the workload
+    (same levels, timestep and physics; fewer columns)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nlon, nlat, nk, dlev, step, dstep, region = pkg.synth.CONFIGS[config]
+    sub = "cpu_sample"
+    pkg.synth.CONFIGS[sub] = (32, 16, nk, dlev, step, dstep, region)
+    g = pkg.synth.make_grid(sub, nstep_sub=nstep_sub)
+    Fo = pkg.synth.to_fortran_layout(pkg.synth.make_state(g))
+    cols = g["nlon"] * g["nlat"]
+    reps, t_total = 0, 0.0
+    while t_total < budget_s and reps < 50:
+        t0 = time.perf_counter()
+        pkg.step.run_oracle_step(oracle, g, Fo)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    per_step = t_total / reps
+    value = cols * nk * (10 + nstep_sub) / per_step / 1e6
+    return {"value": round(value, 4), "unit": "Mcolumn-level solves/s", "cores": 1, "kind": "port",
+            "sample": "oracle (C restatement, gcc -O2, no FMA) full column step on a 32x16-column "
+                      "x %d-level sub-grid of the same workload, %d reps, %.2f s/step" % (nk, reps, per_step)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="g0p625")
+    ap.add_argument("--nstep-sub", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of known size (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the column step needs no "
+              "communication; see DESIGN.md for how it shards). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    pkg = load_pkg()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    g = pkg.synth.make_grid(args.config, nstep_sub=args.nstep_sub)
+    print("# generating %s state on the GPU" % args.config, file=sys.stderr, flush=True)
+    F = pkg.synth.make_state(g, device="cuda")
+    ns = pkg.step.NestState(g, F)
+    del F
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    calls = {r: getattr(ns, r) for r in ROUTINES}
+    routines = (args.only,) if args.only else ROUTINES
+
+    if args.copy_calibration:
+        n = 1 << 28  # 2 GiB of float64 read + 2 GiB written
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("# copy calibration: %d bytes read, %d bytes written per launch" % (8 * n, 8 * n),
+              file=sys.stderr, flush=True)
+        del a, b
+
+    for _ in range(args.warmup):
+        for r in routines:
+            calls[r]()
+    torch.cuda.synchronize()
+    print("# timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    ev = {r: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)] for r in routines}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        for r in routines:
+            e0, e1 = ev[r][s]
+            e0.record()
+            calls[r]()
+            e1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    rt = {r: sum(e0.elapsed_time(e1) for e0, e1 in ev[r]) / args.steps for r in routines}
+
+    nk, cols = g["nk"], g["nlon"] * g["nlat"]
+    cols_all = (g["nlon"] + 4) * (g["nlat"] + 4)
+    nsub = args.nstep_sub
+    unknowns = sum(pkg.step.UNKNOWNS[r] * (nsub if r == "oplus" else 1) for r in routines)
+    value = cols * nk * unknowns / (ms_per_step * 1e-3) / 1e6
+    # bytes per launch of each routine (all columns the kernels process, ghosts included)
+    alg = {r: pkg.step.alg_bytes_per_column_level(r) * cols_all * nk * (nsub if r == "oplus" else 1)
+           for r in routines}
+    dom = max(routines, key=lambda r: rt[r])
+    achieved = alg[dom] / (rt[dom] * 1e-3) / 1e9
+    step_bytes = sum(alg.values())
+    out = {
+        "metric": "Mcolumn-level solves/s + %HBM roofline, FP32+refine vs FP64 at <=1e-10 rel",
+        "value": round(value, 2),
+        "unit": "Mcolumn-level solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded profiles, device-generated)",
+        "config": {"workload": "%s global %dx%d columns x %d levels, %d unknowns/column/level, "
+                               "nstep_sub=%d, fused assembly+solve FP64" % (
+                                   args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
+                   "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": dom, "alg_bytes_per_launch": alg[dom],
+                     "launch_ms": round(rt[dom], 4)},
+        "routine_ms": {r: round(rt[r], 4) for r in routines},
+        "step_alg_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+    }
+    if not args.no_cpu_baseline and not args.only:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(pkg, args.config, nsub, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -60,58 +60,65 @@ static Geo make_geo(const tgx_ng_grid &g) {
 }
 
 // ---- stencils on one level plane (dffm_ng.F90, single task) ----------------
+// Branch-free: every case of the reference's edge rules is a pair of loads at
+// clamped indices and a scale, so a wavefront issues all its loads at once.
+// Scaling by 0.5/0.25 is exactly the reference's division by 2/4.
 #define PV(f, i, j) (f)[(size_t)(j) * G.ld + (i)]
+// df_3d order 1 in longitude: edges one-sided, interior centred / 2
 __device__ __forceinline__ double dflon1(const double *f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
-  if (i == 0) return PV(f, 1, j) - PV(f, 0, j);
-  if (i == L1) return PV(f, L1, j) - PV(f, L1 - 1, j);
-  return (PV(f, i + 1, j) - PV(f, i - 1, j)) / 2.;
+  int ip = i < L1 ? i + 1 : L1, im = i > 0 ? i - 1 : 0;
+  double d = PV(f, ip, j) - PV(f, im, j);
+  return (i == 0 || i == L1) ? d : d * 0.5;
 }
+// df_3d order 2: edges one-sided, next-to-edge centred over 2 cells / 2,
+// interior (f(i+2)-f(i-2)) / 4
 __device__ __forceinline__ double dflon2(const double *f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
-  if (i == 0) return PV(f, 1, j) - PV(f, 0, j);
-  if (i == L1) return PV(f, L1, j) - PV(f, L1 - 1, j);
-  if (i == 1) return (PV(f, 2, j) - PV(f, 0, j)) / 2.;
-  if (i == L1 - 1) return (PV(f, L1, j) - PV(f, L1 - 2, j)) / 2.;
-  return (PV(f, i + 2, j) - PV(f, i - 2, j)) / 4.;
+  bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+  int ip = edge ? (i == 0 ? 1 : L1) : (near ? i + 1 : i + 2);
+  int im = edge ? (i == 0 ? 0 : L1 - 1) : (near ? i - 1 : i - 2);
+  double d = PV(f, ip, j) - PV(f, im, j);
+  return edge ? d : (near ? d * 0.5 : d * 0.25);
 }
+// fm_3d order 1/2: edges return f itself ((f+f)*0.5 == f exactly)
 __device__ __forceinline__ double fmlon1(const double *f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
-  if (i == 0 || i == L1) return PV(f, i, j);
-  return (PV(f, i + 1, j) + PV(f, i - 1, j)) / 2.;
+  bool edge = (i == 0 || i == L1);
+  int ip = edge ? i : i + 1, im = edge ? i : i - 1;
+  return (PV(f, ip, j) + PV(f, im, j)) * 0.5;
 }
 __device__ __forceinline__ double fmlon2(const double *f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
-  if (i == 0 || i == L1) return PV(f, i, j);
-  if (i == 1) return (PV(f, 2, j) + PV(f, 0, j)) / 2.;
-  if (i == L1 - 1) return (PV(f, L1, j) + PV(f, L1 - 2, j)) / 2.;
-  return (PV(f, i + 2, j) + PV(f, i - 2, j)) / 2.;
+  bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+  int ip = edge ? i : (near ? i + 1 : i + 2), im = edge ? i : (near ? i - 1 : i - 2);
+  return (PV(f, ip, j) + PV(f, im, j)) * 0.5;
 }
 __device__ __forceinline__ double dflat1(const double *f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
-  if (j == 0) return PV(f, i, 1) - PV(f, i, 0);
-  if (j == J1) return PV(f, i, J1) - PV(f, i, J1 - 1);
-  return (PV(f, i, j + 1) - PV(f, i, j - 1)) / 2.;
+  int jp = j < J1 ? j + 1 : J1, jm = j > 0 ? j - 1 : 0;
+  double d = PV(f, i, jp) - PV(f, i, jm);
+  return (j == 0 || j == J1) ? d : d * 0.5;
 }
 __device__ __forceinline__ double dflat2(const double *f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
-  if (j == 0) return PV(f, i, 1) - PV(f, i, 0);
-  if (j == J1) return PV(f, i, J1) - PV(f, i, J1 - 1);
-  if (j == 1) return (PV(f, i, 2) - PV(f, i, 0)) / 2.;
-  if (j == J1 - 1) return (PV(f, i, J1) - PV(f, i, J1 - 2)) / 2.;
-  return (PV(f, i, j + 2) - PV(f, i, j - 2)) / 4.;
+  bool edge = (j == 0 || j == J1), near = (j == 1 || j == J1 - 1);
+  int jp = edge ? (j == 0 ? 1 : J1) : (near ? j + 1 : j + 2);
+  int jm = edge ? (j == 0 ? 0 : J1 - 1) : (near ? j - 1 : j - 2);
+  double d = PV(f, i, jp) - PV(f, i, jm);
+  return edge ? d : (near ? d * 0.5 : d * 0.25);
 }
 __device__ __forceinline__ double fmlat1(const double *f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
-  if (j == 0 || j == J1) return PV(f, i, j);
-  return (PV(f, i, j - 1) + PV(f, i, j + 1)) / 2.;
+  bool edge = (j == 0 || j == J1);
+  int jp = edge ? j : j + 1, jm = edge ? j : j - 1;
+  return (PV(f, i, jm) + PV(f, i, jp)) * 0.5;
 }
 __device__ __forceinline__ double fmlat2(const double *f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
-  if (j == 0 || j == J1) return PV(f, i, j);
-  if (j == 1) return (PV(f, i, 2) + PV(f, i, 0)) / 2.;
-  if (j == J1 - 1) return (PV(f, i, J1) + PV(f, i, J1 - 2)) / 2.;
-  return (PV(f, i, j - 2) + PV(f, i, j + 2)) / 2.;
+  bool edge = (j == 0 || j == J1), near = (j == 1 || j == J1 - 1);
+  int jp = edge ? j : (near ? j + 1 : j + 2), jm = edge ? j : (near ? j - 1 : j - 2);
+  return (PV(f, i, jm) + PV(f, i, jp)) * 0.5;
 }
 
 // advec_ng.F90:244-265 at (i, j) of the plane
@@ -130,38 +137,46 @@ __device__ __forceinline__ double advec(const double *f, const double *u, const 
   return wk1 / dlamda + wk2 * RE_INV / dphi;
 }
 
-// smooth_ng.F90:283-294: latitude pass at (i, j)
+// smooth_ng.F90:283-294: latitude pass at (i, j); rows 0/J1 copy, rows 1/J1-1
+// use the 3-point form
 __device__ __forceinline__ double smooth_lat(const double *f, const Geo &G, int i, int j, double s) {
+  int J1 = G.nlatd - 1;
+  bool edge = (j == 0 || j == J1), near = (j == 1 || j == J1 - 1);
+  int j1p = j < J1 ? j + 1 : J1, j1m = j > 0 ? j - 1 : 0;
+  int j2p = j + 2 <= J1 ? j + 2 : J1, j2m = j >= 2 ? j - 2 : 0;
   double fin = PV(f, i, j);
-  if (j == 0 || j == G.nlatd - 1) return fin;
-  double fm1 = fmlat1(f, G, i, j);
-  if (j == 1 || j == G.nlatd - 2) return fin - s * (-8. * fm1 + 8. * fin);
-  double fm2 = fmlat2(f, G, i, j);
-  return fin - s * (2. * fm2 - 8. * fm1 + 6. * fin);
+  double fm1 = (PV(f, i, j1m) + PV(f, i, j1p)) * 0.5;
+  double fm2 = (PV(f, i, j2m) + PV(f, i, j2p)) * 0.5;
+  double gen = fin - s * (2. * fm2 - 8. * fm1 + 6. * fin);
+  double nr = fin - s * (-8. * fm1 + 8. * fin);
+  return edge ? fin : (near ? nr : gen);
 }
 // smooth_ng.F90:296-307: longitude pass of the latitude-smoothed field
 __device__ __forceinline__ double smooth(const double *f, const Geo &G, int i, int j, double s) {
   int L1 = G.nlond - 1;
+  bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+  int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
+  int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
   double t0 = smooth_lat(f, G, i, j, s);
-  if (i == 0 || i == L1) return t0;
-  double fm1 = (smooth_lat(f, G, i + 1, j, s) + smooth_lat(f, G, i - 1, j, s)) / 2.;
-  if (i == 1 || i == L1 - 1) return t0 - s * (-8. * fm1 + 8. * t0);
-  double fm2 = (smooth_lat(f, G, i + 2, j, s) + smooth_lat(f, G, i - 2, j, s)) / 2.;
-  return t0 - s * (2. * fm2 - 8. * fm1 + 6. * t0);
+  double fm1 = (smooth_lat(f, G, i1p, j, s) + smooth_lat(f, G, i1m, j, s)) * 0.5;
+  double fm2 = (smooth_lat(f, G, i2p, j, s) + smooth_lat(f, G, i2m, j, s)) * 0.5;
+  double gen = t0 - s * (2. * fm2 - 8. * fm1 + 6. * t0);
+  double nr = t0 - s * (-8. * fm1 + 8. * t0);
+  return edge ? t0 : (near ? nr : gen);
 }
 
 // div_ng.F90:331-346
 __device__ __forceinline__ double divx(const double *f, const Geo &G, int i, int j) {
   double d1 = dflon1(f, G, i, j);
-  if (i <= 1 || i >= G.nlon + 2) return d1;
   double d2 = dflon2(f, G, i, j);
-  return d1 * 2. * 2. / 3. - d2 * 4. / 12.;
+  double gen = d1 * 2. * 2. / 3. - d2 * 4. / 12.;
+  return (i <= 1 || i >= G.nlon + 2) ? d1 : gen;
 }
 __device__ __forceinline__ double divy(const double *f, const Geo &G, int i, int j) {
   double d1 = dflat1(f, G, i, j);
-  if (j <= 1 || j >= G.nlat + 2) return d1;
   double d2 = dflat2(f, G, i, j);
-  return d1 * 2. * 2. / 3. - d2 * 4. / 12.;
+  double gen = d1 * 2. * 2. / 3. - d2 * 4. / 12.;
+  return (j <= 1 || j >= G.nlat + 2) ? d1 : gen;
 }
 
 // oplus_ng.F90:344-365 (bdotdh)
@@ -202,9 +217,71 @@ __device__ __forceinline__ bool lateral_value(const Geo &G, const double *lonb, 
 }
 
 // =============================================================================
+// Horizontal-stencil prelude: one thread per (column, level), every term a
+// column kernel needs from neighbouring columns, written as a level plane.
+enum { ST_ADVEC = 0, ST_SMOOTH, ST_DIVX, ST_DIVY, ST_BDOTDH, ST_VDOTN };
+struct StencilJob {
+  int kind;
+  const double *f;
+  double *out;
+  double s;   // Shapiro coefficient for ST_SMOOTH
+};
+struct StencilJobs {
+  int n;
+  StencilJob job[8];
+};
+
+__global__ void __launch_bounds__(256) stencil_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                      StencilJobs J) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  long npt = (long)G.nlond * G.nlatd * G.nk;
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int k = (int)(r / G.nlatd);
+  const size_t off = (size_t)k * G.P, p = off + (size_t)j * G.ld + i, c2 = (size_t)j * G.ld + i;
+  for (int n = 0; n < J.n; ++n) {
+    const StencilJob &jb = J.job[n];
+    const double *f = jb.f + off;
+    double v;
+    switch (jb.kind) {
+      case ST_ADVEC:
+        v = advec(f, in.un + off, in.vn + off, G, i, j, g.racs[j], g.dlamda, g.dphi);
+        break;
+      case ST_SMOOTH:
+        v = smooth(f, G, i, j, jb.s);
+        break;
+      case ST_DIVX:
+        v = divx(f, G, i, j);
+        break;
+      case ST_DIVY:
+        v = divy(f, G, i, j);
+        break;
+      case ST_BDOTDH:
+        v = bdotdh(f, G, i, j, in.bx[c2], in.by[c2], g.cs[j], g.dlamda, g.dphi);
+        break;
+      default: {  // ST_VDOTN: oplus_ng.F90:196-209, f = op/bmod2**2
+        double uii = k < G.nk - 1 ? .5 * (in.ui[p] + in.ui[p + G.P]) : 1.5 * in.ui[p] - .5 * in.ui[p - G.P];
+        double vii = k < G.nk - 1 ? .5 * (in.vi[p] + in.vi[p + G.P]) : 1.5 * in.vi[p] - .5 * in.vi[p - G.P];
+        double vx = uii * dflon1(f, G, i, j), vy = vii * dflat1(f, G, i, j);
+        vx = vx / g.cs[j];
+        double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
+        double bm = in.bmod2[c2];
+        v = vh * (bm * bm);
+      }
+    }
+    jb.out[p] = v;
+  }
+}
+
+// =============================================================================
 // dt_ng.F90:54-213
 __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                 tgx_ng_out out, double *__restrict__ w2s) {
+                                                 tgx_ng_out out, double *__restrict__ w2s,
+                                                 const double *__restrict__ hadv_tn,
+                                                 const double *__restrict__ smooth_tn_nm) {
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
@@ -252,10 +329,8 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
     double kmi = .5 * (F3(in.km, k) + F3(in.km, k + 1));
     double wi = .5 * (w_k + w_k1);
     // cptn (:67, :80-83)
-    const double *tnk = in.tn + (size_t)k * G.P;
-    double horadv = advec(tnk, in.un + (size_t)k * G.P, in.vn + (size_t)k * G.P, G, i, j, racs,
-                          g.dlamda, g.dphi);
-    double tnsmooth = smooth(in.tn_nm + (size_t)k * G.P, G, i, j, g.shapiro);
+    double horadv = F3(hadv_tn, k);
+    double tnsmooth = F3(smooth_tn_nm, k);
     double advec_tn = horadv + veradv;
     double cptn = cpi * (advec_tn - dtx2inv * tnsmooth);
     cptn = cptn * g.expz[k];
@@ -388,10 +463,14 @@ struct MinorSpecies {  // per species and level
   double ax, thdiff, sloss, s0prod, ftm1s, hadvec;
 };
 
+struct MinorTerms {  // prelude planes: smoothed fcomp_tm1 and advection of fcomp, by species id
+  const double *ftm1s[4], *hadvec[4];
+};
+
 template <int SA, int SB>
 __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     tgx_ng_out out, double *__restrict__ w2a,
-                                                    double *__restrict__ w2b) {
+                                                    double *__restrict__ w2b, MinorTerms MT) {
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
@@ -400,6 +479,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   const int SP[2] = {SA, SB};
   double *outs[2], *outs_nm[2];
   const double *fc[2], *fctm1[2], *lonb[2], *latb[2];
+#pragma unroll
   for (int q = 0; q < 2; ++q) {
     int s = SP[q];
     outs[q] = s == 0 ? out.no : s == 1 ? out.n4s : s == 2 ? out.n2d : out.ar;
@@ -412,7 +492,8 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   const double small = 1.e-12;
   double v;
   if (lateral_value(G, in.lonb_no, in.latb_no, 0, i, j, v)) {
-    for (int q = 0; q < 2; ++q)
+  #pragma unroll
+  for (int q = 0; q < 2; ++q)
       for (int k = 0; k < nk; ++k) {
         lateral_value(G, lonb[q], latb[q], k, i, j, v);
         size_t p = G.a3(k, i, j);
@@ -559,9 +640,8 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     double sprod;
     sources(SP[q], L, T.sloss, sprod);
     T.s0prod = sprod * c.rmx / F3(in.xnmbar, L);
-    const size_t off = (size_t)L * G.P;
-    T.ftm1s = smooth(fctm1[q] + off, G, i, j, g.shapiro);
-    T.hadvec = advec(fc[q] + off, in.un + off, in.vn + off, G, i, j, racs, g.dlamda, g.dphi);
+    T.ftm1s = F3(MT.ftm1s[SP[q]], L);
+    T.hadvec = F3(MT.hadvec[SP[q]], L);
   };
   auto wi_at = [&](int L) -> double {
     if (L < nk - 1) return .5 * (F3(in.w_upd, L) + F3(in.w_upd, L + 1));
@@ -569,6 +649,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   };
   // lower-boundary data flbc(:,:,1:3) of each species (comp_*_ng.F90)
   double l1[2], l2[2], l3[2];
+#pragma unroll
   for (int q = 0; q < 2; ++q) {
     int s = SP[q];
     size_t p = G.a3(0, i, j);
@@ -597,6 +678,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   double tni_m = 0.;
   shared_at(0, Sc);
   double tni_1 = tni_at(1);
+#pragma unroll
   for (int q = 0; q < 2; ++q) species_at(q, 0, Sc, tni_1 - Sc.tni, Tc[q]);
   double w1[2] = {0., 0.}, w3[2] = {0., 0.}, rprev[2] = {0., 0.};
   double *w2s[2] = {w2a, w2b};
@@ -607,13 +689,15 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
       shared_at(k + 1, Sn);
       double tni_p = (k + 2 <= nk - 1) ? tni_at(k + 2) : 0.;
       double dtnidz = (k + 1 == nk - 1) ? Sn.tni - Sc.tni : (tni_p - Sc.tni) / 2.;
-      for (int q = 0; q < 2; ++q) species_at(q, k + 1, Sn, dtnidz, Tn[q]);
+    #pragma unroll
+  for (int q = 0; q < 2; ++q) species_at(q, k + 1, Sn, dtnidz, Tn[q]);
       wi_n = wi_at(k + 1);
     }
     (void)tni_m;
     const double dmdzc = Sc.dmdz / Sc.barm;
     const double dmdzn = k + 1 <= nk - 1 ? Sn.dmdz / Sn.barm : 0.;
-    for (int q = 0; q < 2; ++q) {
+  #pragma unroll
+  for (int q = 0; q < 2; ++q) {
       double pc, qc, rc, fr;
       if (k <= nk - 2) {
         double molp = Tc[q].ax * (1. / dzp + .5 * Tc[q].thdiff);
@@ -658,6 +742,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     wic = wi_n;
   }
   // back substitution and output stage (:234-237)
+#pragma unroll
   for (int q = 0; q < 2; ++q) {
     double xn = w3[q];
     for (int k = nk - 1; k >= 0; --k) {
@@ -675,8 +760,13 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 
 // =============================================================================
 // comp_o2o_ng.F90:44-81 (fused) + comp_ng.F90:71-457
+struct CompTerms {  // prelude planes: advection of O2, O, He and smoothed *_nm
+  const double *o2adv, *o1adv, *headv, *o2s, *o1s, *hes;
+};
+
 __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                   tgx_ng_out out, double *__restrict__ gs) {
+                                                   tgx_ng_out out, double *__restrict__ gs,
+                                                   CompTerms CT) {
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
@@ -785,36 +875,35 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
 
     Lev Qc, Qn;
     lev(0, Qc);
-    double ep[2][3], ak[2][9], alpha[9];
+    double epm[3], epp[3], akm[9], akp[9], alpha[9];  // levels k (km) and k+1 (kp)
     double wks3, wks4 = Qc.dembardz / (Qc.embari * 2.);
-    int km = 0, kp = 1;
-    for (int l = 0; l < 3; ++l) ep[kp][l] = 1. - 1. / Qc.embari * (rmass[l] + Qc.dembardz);
-    ep[kp][2] = ep[kp][2] - thdiffalpha * Qc.dtndz / Qc.tni;
+    for (int l = 0; l < 3; ++l) epp[l] = 1. - 1. / Qc.embari * (rmass[l] + Qc.dembardz);
+    epp[2] = epp[2] - thdiffalpha * Qc.dtndz / Qc.tni;
     alpha_of(Qc, alpha);
-    matinv3_ng(alpha, ak[kp]);
+    matinv3_ng(alpha, akp);
     double wks1 = Qc.embari * RIN2 * pow(T00 / tl, 0.25) / TAU;
-    for (int n = 0; n < 9; ++n) ak[kp][n] = ak[kp][n] * wks1;
+    for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
     double gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
     double alpha22 = 0, alpha33 = 0, alpha23 = 0, alpha32 = 0, flx00 = 0;
-    const double *o2nm = in.o2_nm, *o1nm = in.o1_nm, *henm = in.he_nm;
     for (int k = 0; k < nk - 1; ++k) {
-      int t = km; km = kp; kp = t;
+      for (int n = 0; n < 9; ++n) akm[n] = akp[n];
+      for (int n = 0; n < 3; ++n) epm[n] = epp[n];
       lev(k + 1, Qn);
-      for (int l = 0; l < 3; ++l) ep[kp][l] = 1. - 1. / Qn.embari * (rmass[l] + Qn.dembardz);
-      if (k != nk - 2) ep[kp][2] = ep[kp][2] - thdiffalpha * Qn.dtndz / Qn.tni;
+      for (int l = 0; l < 3; ++l) epp[l] = 1. - 1. / Qn.embari * (rmass[l] + Qn.dembardz);
+      if (k != nk - 2) epp[2] = epp[2] - thdiffalpha * Qn.dtndz / Qn.tni;
       alpha_of(Qn, alpha);
-      matinv3_ng(alpha, ak[kp]);
+      matinv3_ng(alpha, akp);
       wks1 = Qn.embari * RIN2 * pow(T00 / Qn.tni, 0.25) / TAU;
       wks3 = wks4;
       wks4 = Qn.dembardz / (Qn.embari * 2.);
-      for (int n = 0; n < 9; ++n) ak[kp][n] = ak[kp][n] * wks1;
+      for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
       double molp[9], molr[9], molq[9], pk[9], rk[9], qk[9], fk[3], fs[2][3];
       for (int isp = 0; isp < 3; ++isp)
         for (int m = 0; m < 3; ++m) {
           int e = isp + 3 * m;
-          molp[e] = ak[km][e] * (1. / dz + ep[km][m] / 2.);
-          molr[e] = ak[kp][e] * (1. / dz - ep[kp][m] / 2.);
-          molq[e] = ak[km][e] * (1. / dz - ep[km][m] / 2.) + ak[kp][e] * (1. / dz + ep[kp][m] / 2.);
+          molp[e] = akm[e] * (1. / dz + epm[m] / 2.);
+          molr[e] = akp[e] * (1. / dz - epp[m] / 2.);
+          molq[e] = akm[e] * (1. / dz - epm[m] / 2.) + akp[e] * (1. / dz + epp[m] / 2.);
         }
       double eddyp = g.expzmid_inv * g.difk[k] * (1. / dz - wks3);
       double eddyr = g.expzmid * g.difk[k + 1] * (1. / dz + wks4);
@@ -830,14 +919,8 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
           rk[e] = molr[e] / dz - g.expz[k] * delta / dz * (eddyr * dfac - .5 * wik);
           qk[e] = -molq[e] / dz + g.expz[k] * delta * (eddyq * dfac / dz + dtx2inv) - g.expz[k] * fsv;
         }
-      const size_t off = (size_t)k * G.P;
-      const double *un = in.un + off, *vn = in.vn + off;
-      double o2adv = advec(in.o2 + off, un, vn, G, i, j, racs, g.dlamda, g.dphi);
-      double o1adv = advec(in.o1 + off, un, vn, G, i, j, racs, g.dlamda, g.dphi);
-      double headv = advec(in.he + off, un, vn, G, i, j, racs, g.dlamda, g.dphi);
-      double o2s = smooth(o2nm + off, G, i, j, g.shapiro);
-      double o1s = smooth(o1nm + off, G, i, j, g.shapiro);
-      double hes = smooth(henm + off, G, i, j, g.shapiro);
+      double o2adv = F3(CT.o2adv, k), o1adv = F3(CT.o1adv, k), headv = F3(CT.headv, k);
+      double o2s = F3(CT.o2s, k), o1s = F3(CT.o1s, k), hes = F3(CT.hes, k);
       fk[0] = g.expz[k] * (o2s * dtx2inv - (o2adv - fs[0][0]) + F3(in.hdo2, k));
       fk[1] = g.expz[k] * (o1s * dtx2inv - (o1adv - fs[1][0]) + F3(in.hdo1, k));
       fk[2] = g.expz[k] * (hes * dtx2inv - (headv - 0.) + F3(in.hdhe, k));
@@ -851,7 +934,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       } else if (k == nk - 2) {
         for (int m = 0; m < 3; ++m)
           for (int isp = 0; isp < 3; ++isp)
-            qk[isp + 3 * m] = qk[isp + 3 * m] + (1. + .5 * ep[kp][m] * dz) / (1. - .5 * ep[kp][m] * dz) * rk[isp + 3 * m];
+            qk[isp + 3 * m] = qk[isp + 3 * m] + (1. + .5 * epp[m] * dz) / (1. - .5 * epp[m] * dz) * rk[isp + 3 * m];
         const double *d_ = Qn.df;
         alpha22 = -(PHI(2, 4) + (PHI(2, 1) - PHI(2, 4)) * Qn.o2i + (d_[1] * PHI(2, 3) - PHI(2, 4)) * Qn.hei);
         alpha33 = -(d_[2] * PHI(3, 4) + (d_[0] * PHI(3, 1) - d_[2] * PHI(3, 4)) * Qn.o2i +
@@ -860,8 +943,8 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
         alpha32 = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Qn.hei;
         flx00 = Qc.embari * RIN2 * P0 * pow(T00 / F3(in.tn, k), 0.25) / (TAU * GRAV);
         for (int isp = 0; isp < 3; ++isp)
-          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][1])) -
-                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][2]));
+          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * epp[1])) -
+                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * epp[2]));
         for (int n = 0; n < 9; ++n) rk[n] = 0.;
       }
       for (int m = 0; m < 3; ++m)
@@ -901,9 +984,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     }
     double top[3];
     for (int isp = 0; isp < 3; ++isp)
-      top[isp] = (1. + .5 * ep[kp][isp] * dz) / (1. - .5 * ep[kp][isp] * dz) * (isp == 0 ? F3(out.o2, nk - 2) : isp == 1 ? F3(out.o1, nk - 2) : F3(out.he, nk - 2));
-    top[1] = top[1] + (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][1]));
-    top[2] = top[2] + (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][2]));
+      top[isp] = (1. + .5 * epp[isp] * dz) / (1. - .5 * epp[isp] * dz) * (isp == 0 ? F3(out.o2, nk - 2) : isp == 1 ? F3(out.o1, nk - 2) : F3(out.he, nk - 2));
+    top[1] = top[1] + (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * epp[1]));
+    top[2] = top[2] + (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * epp[2]));
     size_t pt = G.a3(nk - 1, i, j);
     out.o2[pt] = top[0]; out.o1[pt] = top[1]; out.he[pt] = top[2];
     if (g.calc_helium == 0)
@@ -959,6 +1042,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
 // workspace and Fe, Fn to the outputs.
 struct OpWork {
   double *dj, *bdotu, *bvel, *tpop, *diffj, *djbz, *opbm2, *bdopj, *w2;
+};
+struct OpTerms {  // prelude planes of one sub-step
+  const double *bd_diff, *bd_diffj, *bd_bvel, *vdotn, *bd_djbz, *opts2;
 };
 
 __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng_fields in,
@@ -1030,6 +1116,7 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
                                                     const double *__restrict__ optm1,
                                                     const double *__restrict__ lonb,
                                                     const double *__restrict__ latb, OpWork W,
+                                                    OpTerms OT,
                                                     double *__restrict__ opout,
                                                     double *__restrict__ optm1out,
                                                     double *__restrict__ xiop2p,
@@ -1118,8 +1205,7 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
     return 0.5 * (F3(W.dj, L - 1) + F3(W.dj, L));
   };
   auto hdzmp_at = [&](int L, double &hm, double &hp) {  // :239-248
-    size_t off = (size_t)L * G.P;
-    double bd_djbz = bdotdh(W.djbz + off, G, i, j, bx, by, cs, g.dlamda, g.dphi);
+    double bd_djbz = F3(OT.bd_djbz, L);
     double divbz = dvbz + bd_djbz / (F3(W.dj, L) * (bz * bz));
     double h = hdz_at(L);
     hm = h - 0.5 * divbz;
@@ -1138,19 +1224,14 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
     else if (k <= nk - 3) dbd = (F3(W.bdopj, k + 1) - F3(W.bdopj, k - 1)) / 2.;
     else dbd = F3(W.bdopj, nk - 2) - F3(W.bdopj, nk - 3);
     double bdzdvb = bz / (in.scht[p] * dz) * dbd + dvb * W.bdopj[p];
-    double bd_diff = bdotdh(W.bdopj + off, G, i, j, bx, by, cs, g.dlamda, g.dphi);
-    double bd_op = W.djbz[p] * bdotdh(W.diffj + off, G, i, j, bx, by, cs, g.dlamda, g.dphi);
+    double bd_diff = OT.bd_diff[p];
+    double bd_op = W.djbz[p] * OT.bd_diffj[p];
     double diffexp = bdzdvb + bd_diff + bd_op;
     double expl = -1. * diffexp;
-    double uii = .5 * (in.ui[p] + in.ui[p + G.P]);
-    double vii = .5 * (in.vi[p] + in.vi[p + G.P]);
-    double vx = uii * dflon1(W.opbm2 + off, G, i, j), vy = vii * dflat1(W.opbm2 + off, G, i, j);
-    vx = vx / cs;
-    double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
-    vh = vh * (bm * bm);
-    double bd_bvel = bdotdh(W.bvel + off, G, i, j, bx, by, cs, g.dlamda, g.dphi);
+    double vh = OT.vdotn[p];
+    double bd_bvel = OT.bd_bvel[p];
     expl = expl + vh + bd_bvel;
-    double opts2 = smooth(optm1 + off, G, i, j, ss2);
+    double opts2 = OT.opts2[p];
     expl = expl - opts2 * dtx2inv * nsub;
     // diffusion, drift and wind coefficients (:252-278)
     double hm, hp;
@@ -1257,9 +1338,13 @@ __global__ void __launch_bounds__(256) duv_stage1(tgx_ng_grid g, Geo G, tgx_ng_f
 }
 
 // stage 2: assembly of the 2x2 (UN,VN) system, block Thomas, outputs (:83-321)
+struct UvTerms {  // prelude planes
+  const double *divx_ztmp, *divy_z, *unms, *vnms, *advu, *advv;
+};
+
 __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                  tgx_ng_out out, const double *__restrict__ ztmp,
-                                                  double *__restrict__ gs) {
+                                                  tgx_ng_out out, double *__restrict__ gs,
+                                                  UvTerms UT) {
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
@@ -1286,14 +1371,14 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
 #define F3(f, k) (f)[G.a3((k), i, j)]
   const double racs = g.racs[j], cor = g.cor[j], tanphi = g.tanphi[j];
   auto zl_at = [&](int L) {
-    double zl = divx(ztmp + (size_t)L * G.P, G, i, j);
-    if (L < nk - 1) zl = (zl + divx(ztmp + (size_t)(L + 1) * G.P, G, i, j)) * .5;
+    double zl = F3(UT.divx_ztmp, L);
+    if (L < nk - 1) zl = (zl + F3(UT.divx_ztmp, L + 1)) * .5;
     zl = zl * racs;
     return zl * GRAV / g.dlamda;
   };
   auto zp_at = [&](int L) {
-    double zp = divy(in.z + (size_t)L * G.P, G, i, j);
-    if (L < nk - 1) zp = (zp + divy(in.z + (size_t)(L + 1) * G.P, G, i, j)) * .5;
+    double zp = F3(UT.divy_z, L);
+    if (L < nk - 1) zp = (zp + F3(UT.divy_z, L + 1)) * .5;
     return zp * GRAV * RE_INV / g.dphi;
   };
   auto iavg = [&](const double *f, int L) {
@@ -1332,10 +1417,7 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
     double uii = iavg(in.ui, k), vii = iavg(in.vi, k);
     double udc = out.Fe[p] + lxxi * uii + lxyi * vii;
     double vdc = out.Fn[p] + lyyi * vii - lyxi * uii;
-    double unms = smooth(in.un_nm + off, G, i, j, g.shapiro);
-    double vnms = smooth(in.vn_nm + off, G, i, j, g.shapiro);
-    double advu = advec(in.un + off, in.un + off, in.vn + off, G, i, j, racs, g.dlamda, g.dphi);
-    double advv = advec(in.vn + off, in.un + off, in.vn + off, G, i, j, racs, g.dlamda, g.dphi);
+    double unms = UT.unms[p], vnms = UT.vnms[p], advu = UT.advu[p], advv = UT.advv[p];
     double s1 = dtx2inv * unms + in.hdu[p] + udc - advu - zl_at(k);
     double s2 = dtx2inv * vnms + in.hdv[p] + vdc - advv - zp_at(k);
     s1 = s1 * g.expz[k];
@@ -1437,50 +1519,116 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   Geo G = make_geo(g);
   long ncol = (long)G.nlond * G.nlatd;
   int nb = (int)((ncol + 255) / 256);
-  dim3 grid(nb), block(256);
+  int nbp = (int)((ncol * G.nk + 255) / 256);
+  dim3 grid(nb), gridp(nbp), block(256);
+  auto D = [&](int n) { return (double *)bufs[n]; };
+  auto job = [](int kind, const double *f, double *o, double sh) {
+    StencilJob jb;
+    jb.kind = kind; jb.f = f; jb.out = o; jb.s = sh;
+    return jb;
+  };
   switch (which) {
-    case 0:
-      hipLaunchKernelGGL(dt_kernel, grid, block, 0, s, g, G, in, out, (double *)bufs[0]);
+    case 0:  // dt: bufs = {w2, hadv_tn, smooth_tn_nm}
+      hipLaunchKernelGGL(dt_kernel, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
       break;
     case 1:
-      hipLaunchKernelGGL((minor_kernel<2, 3>), grid, block, 0, s, g, G, in, out, (double *)bufs[0],
-                         (double *)bufs[1]);
+    case 2: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4]} (species 0..3)
+      MinorTerms MT;
+      for (int q = 0; q < 4; ++q) { MT.ftm1s[q] = D(2 + q); MT.hadvec[q] = D(6 + q); }
+      if (which == 1)
+        hipLaunchKernelGGL((minor_kernel<2, 3>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      else
+        hipLaunchKernelGGL((minor_kernel<1, 0>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
       break;
-    case 2:
-      hipLaunchKernelGGL((minor_kernel<1, 0>), grid, block, 0, s, g, G, in, out, (double *)bufs[0],
-                         (double *)bufs[1]);
-      break;
+    }
     case 3:
     case 4:
-    case 5: {
-      OpWork W{(double *)bufs[0], (double *)bufs[1], (double *)bufs[2], (double *)bufs[3],
-               (double *)bufs[4], (double *)bufs[5], (double *)bufs[6], (double *)bufs[7],
-               (double *)bufs[8]};
-      const double *op = (const double *)bufs[9], *optm1 = (const double *)bufs[10];
-      double *opout = (double *)bufs[11], *optm1out = (double *)bufs[12];
+    case 5:
+    case 9: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
+               //           optm1out, bd_diff bd_diffj bd_bvel vdotn bd_djbz opts2}
+      OpWork W{D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8)};
+      const double *op = D(9), *optm1 = D(10);
+      double *opout = D(11), *optm1out = D(12);
+      OpTerms OT{D(13), D(14), D(15), D(16), D(17), D(18)};
       if (which == 3) {
         hipLaunchKernelGGL(oplus_stage1, grid, block, 0, s, g, G, in, op, W, out.Fe, out.Fn);
       } else if (which == 4) {
         hipLaunchKernelGGL(oplus_stage2, grid, block, 0, s, g, G, in, W);
+      } else if (which == 9) {
+        StencilJobs J;
+        J.n = 6;
+        J.job[0] = job(ST_BDOTDH, W.bdopj, D(13), 0.);
+        J.job[1] = job(ST_BDOTDH, W.diffj, D(14), 0.);
+        J.job[2] = job(ST_BDOTDH, W.bvel, D(15), 0.);
+        J.job[3] = job(ST_VDOTN, W.opbm2, D(16), 0.);
+        J.job[4] = job(ST_BDOTDH, W.djbz, D(17), 0.);
+        J.job[5] = job(ST_SMOOTH, optm1, D(18), g.shapiro / g.nstep_sub);
+        hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
       } else {
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
-        hipLaunchKernelGGL(oplus_stage3, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+        hipLaunchKernelGGL(oplus_stage3, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
                            opout, optm1out, out.xiop2p, out.xiop2d);
       }
       break;
     }
-    case 6:
-      hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn,
-                         (double *)bufs[0]);
+    case 6:  // duv stage 1: bufs = {ztmp}
+      hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
       break;
-    case 7:
-      hipLaunchKernelGGL(duv_stage2, grid, block, 0, s, g, G, in, out, (const double *)bufs[0],
-                         (double *)bufs[1]);
+    case 7: {  // duv stage 2: bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
+      UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
+      hipLaunchKernelGGL(duv_stage2, grid, block, 0, s, g, G, in, out, D(1), UT);
       break;
-    case 8:
-      hipLaunchKernelGGL(comp_kernel, grid, block, 0, s, g, G, in, out, (double *)bufs[0]);
+    }
+    case 8: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes}
+      CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6)};
+      hipLaunchKernelGGL(comp_kernel, grid, block, 0, s, g, G, in, out, D(0), CT);
       break;
+    }
+    case 10: {  // dt prelude: bufs = {w2, hadv_tn, smooth_tn_nm}
+      StencilJobs J;
+      J.n = 2;
+      J.job[0] = job(ST_ADVEC, in.tn, D(1), 0.);
+      J.job[1] = job(ST_SMOOTH, in.tn_nm, D(2), g.shapiro);
+      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      break;
+    }
+    case 11: {  // minors prelude (species 0..3 = NO, N4S, N2D, AR)
+      StencilJobs J;
+      J.n = 8;
+      const double *fc[4] = {in.no, in.n4s, in.n2d, in.ar};
+      const double *fm[4] = {in.no_nm, in.n4s_nm, in.n2d_nm, in.ar_nm};
+      for (int q = 0; q < 4; ++q) {
+        J.job[q] = job(ST_SMOOTH, fm[q], D(2 + q), g.shapiro);
+        J.job[4 + q] = job(ST_ADVEC, fc[q], D(6 + q), 0.);
+      }
+      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      break;
+    }
+    case 12: {  // comp prelude
+      StencilJobs J;
+      J.n = 6;
+      J.job[0] = job(ST_ADVEC, in.o2, D(1), 0.);
+      J.job[1] = job(ST_ADVEC, in.o1, D(2), 0.);
+      J.job[2] = job(ST_ADVEC, in.he, D(3), 0.);
+      J.job[3] = job(ST_SMOOTH, in.o2_nm, D(4), g.shapiro);
+      J.job[4] = job(ST_SMOOTH, in.o1_nm, D(5), g.shapiro);
+      J.job[5] = job(ST_SMOOTH, in.he_nm, D(6), g.shapiro);
+      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      break;
+    }
+    case 13: {  // duv prelude (after stage 1 wrote ztmp)
+      StencilJobs J;
+      J.n = 6;
+      J.job[0] = job(ST_DIVX, D(0), D(2), 0.);
+      J.job[1] = job(ST_DIVY, in.z, D(3), 0.);
+      J.job[2] = job(ST_SMOOTH, in.un_nm, D(4), g.shapiro);
+      J.job[3] = job(ST_SMOOTH, in.vn_nm, D(5), g.shapiro);
+      J.job[4] = job(ST_ADVEC, in.un, D(6), 0.);
+      J.job[5] = job(ST_ADVEC, in.vn, D(7), 0.);
+      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      break;
+    }
     default:
       return tgx_set_error(1, "tgx_ng_launch: unknown kernel %d", which);
   }

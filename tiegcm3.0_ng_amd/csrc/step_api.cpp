@@ -25,13 +25,13 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_field
 namespace {
 // Workspace planes, each nk * ld * (nlat+4) doubles.
 enum {
-  W_DT = 0,                 // dt Thomas ratios
-  W_MIN_A, W_MIN_B,         // minor Thomas ratios (two species per pass)
-  W_OP_DJ, W_OP_BDOTU, W_OP_BVEL, W_OP_TPOP, W_OP_DIFFJ, W_OP_DJBZ, W_OP_OPBM2, W_OP_BDOPJ,
-  W_OP_W2, W_OP_X, W_OP_XNM,  // O+ intermediates, ratios, ping-pong state
-  W_UV_ZTMP, W_UV_G,        // wind: vertical integral, 4 gamma planes
-  W_CO_G = W_UV_G + 4,      // comp: 9 gamma planes
-  W_NPLANES = W_CO_G + 9
+  W_DT = 0, W_DT_T = 1,                  // dt ratios; 2 prelude planes
+  W_MIN_A = 3, W_MIN_B = 4, W_MIN_T = 5, // minor ratios (two species per pass); 8 prelude planes
+  W_OP = 13,                             // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
+  W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
+  W_UV = W_OP_T + 6,                     // wind: ztmp, 4 gamma, 6 prelude planes
+  W_CO = W_UV + 11,                      // comp: 9 gamma, 6 prelude planes
+  W_NPLANES = W_CO + 15
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -57,22 +57,30 @@ extern "C" unsigned long long tgx_ng_workspace_bytes(const tgx_ng_grid *g) {
   return (unsigned long long)W_NPLANES * plane_elems(g) * sizeof(double);
 }
 
-enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP };
+enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
+       K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE };
 
 extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                          void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
-  void *b[1] = {plane(work, g, W_DT)};
-  return tgx_ng_launch(K_DT, g, in, out, b, 0, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
+  if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
+  return tgx_ng_launch(K_DT, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                              void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
-  void *b[2] = {plane(work, g, W_MIN_A), plane(work, g, W_MIN_B)};
+  hipStream_t s = (hipStream_t)stream;
+  void *b[10];
+  b[0] = plane(work, g, W_MIN_A);
+  b[1] = plane(work, g, W_MIN_B);
+  for (int q = 0; q < 8; ++q) b[2 + q] = plane(work, g, W_MIN_T + q);
+  if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
-  if (int rc = tgx_ng_launch(K_MIN_A, g, in, out, b, 0, (hipStream_t)stream)) return rc;
-  return tgx_ng_launch(K_MIN_B, g, in, out, b, 0, (hipStream_t)stream);
+  if (int rc = tgx_ng_launch(K_MIN_A, g, in, out, b, 0, s)) return rc;
+  return tgx_ng_launch(K_MIN_B, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -87,13 +95,16 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
     // a sub-step never reads the buffer it writes
     bool to_out = ((nsub - 1 - sub) % 2) == 0;
     double *op_o = to_out ? out->op : tmp_op, *nm_o = to_out ? out->op_nm : tmp_nm;
-    void *b[13] = {plane(work, g, W_OP_DJ),    plane(work, g, W_OP_BDOTU), plane(work, g, W_OP_BVEL),
-                   plane(work, g, W_OP_TPOP),  plane(work, g, W_OP_DIFFJ), plane(work, g, W_OP_DJBZ),
-                   plane(work, g, W_OP_OPBM2), plane(work, g, W_OP_BDOPJ), plane(work, g, W_OP_W2),
-                   (void *)op_in,              (void *)nm_in,              op_o,
-                   nm_o};
+    void *b[19];
+    for (int q = 0; q < 9; ++q) b[q] = plane(work, g, W_OP + q);
+    b[9] = (void *)op_in;
+    b[10] = (void *)nm_in;
+    b[11] = op_o;
+    b[12] = nm_o;
+    for (int q = 0; q < 6; ++q) b[13 + q] = plane(work, g, W_OP_T + q);
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
+    if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP3, g, in, out, b, sub, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
@@ -104,16 +115,25 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
 extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                           void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
-  void *b[2] = {plane(work, g, W_UV_ZTMP), plane(work, g, W_UV_G)};
-  if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, (hipStream_t)stream)) return rc;
-  return tgx_ng_launch(K_UV2, g, in, out, b, 0, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  void *b[8];
+  b[0] = plane(work, g, W_UV);
+  b[1] = plane(work, g, W_UV + 1);
+  for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 5 + q);
+  if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
+  if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
+  return tgx_ng_launch(K_UV2, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                            void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
-  void *b[1] = {plane(work, g, W_CO_G)};
-  return tgx_ng_launch(K_COMP, g, in, out, b, 0, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  void *b[7];
+  b[0] = plane(work, g, W_CO);
+  for (int q = 0; q < 6; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
+  if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
+  return tgx_ng_launch(K_COMP, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,

@@ -184,6 +184,40 @@ class NestState:
         self._call("tgx_ng_step", stream)
 
 
+# Algorithmic HBM traffic of each routine: the distinct 3-D fields its fused
+# kernels must read and write once (8 bytes per column-level each).  2-D and
+# 1-D inputs are negligible and not counted.
+ALG_FIELDS = {
+    "oplus": (  # oplus_ng.F90:35-75 inputs, :14-16 outputs (per sub-step)
+        "tn te ti o2 o1 n2 he n2d ne un vn w ui vi wi xnmbar scht op op_nm qop2p qop2d qop "
+        "rk1 rk2 rk19 rk20 rk25".split(),
+        "op op_nm xiop2p xiop2d Fe Fn".split()),
+    "minors": (  # minor_ng.F90:42-50 + comp_{n2d,n4s,no,ar}_ng.F90 inputs
+        "tn o2 o1 he mbar barm xnmbar w_upd un vn n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm "
+        "qtef ne rk3 n2p ra1 nop ra3 op beta8 beta5 rk2 n2 nplus beta1 beta3 beta17 o2p beta9 beta9n".split(),
+        "n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm".split()),
+    "dt": (  # dt_ng.F90:27-43
+        "tn tn_nm un vn o1 mbar xnmbar scht schti cp kt km hdt qji_tn cool_imp cool_exp w_upd "
+        "qtotal rkm12".split(),
+        "tn tn_nm".split()),
+    "duv": (  # duv_ng.F90:33-55 (tn_upd = this step's TN)
+        "tn tn_upd tn_nm un vn un_nm vn_nm w_upd mbar scht schti z hdu hdv ui vi lxx lyy lxy lyx "
+        "km Fe Fn".split(),
+        "un un_nm vn vn_nm".split()),
+    "comp": (  # comp_ng.F90:47-59 + comp_o2o_ng.F90:17-40
+        "tn o2 o2_nm o1 o1_nm he he_nm w mbar hdo2 hdo1 hdhe xnmbar un vn op no n4s n2d o2p ne "
+        "n2p nplus nop rj qo2p qop ra1 ra2 beta3 beta8 rk1 beta1 rkm12 rk3".split(),
+        "o2 o2_nm o1 o1_nm he he_nm".split()),
+}
+# unknowns (column-level solves) per column and level of each routine
+UNKNOWNS = {"oplus": 1, "minors": 4, "dt": 1, "duv": 2, "comp": 3}
+
+
+def alg_bytes_per_column_level(routine):
+    ins, outs = ALG_FIELDS[routine]
+    return 8 * (len(set(ins)) + len(set(outs)))
+
+
 def run_oracle_step(oracle, grid, Fo):
     """The oracle's column step in the reference's order (dynamics_ng.F90)."""
     st = oracle.OracleState(grid, Fo)
