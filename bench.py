@@ -41,7 +41,7 @@ def load_pkg():
     return mod
 
 
-ROUTINES = ("oplus", "minors", "dt", "duv", "comp")
+ROUTINES = ("oplus", "minors", "dt", "duv", "comp")  # the library's routine order
 
 
 def cpu_baseline(pkg, config, nstep_sub, budget_s=20.0):
@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the routines one after another on one stream (default: the "
+                         "independent routines run concurrently)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
@@ -111,25 +114,25 @@ def main():
               file=sys.stderr, flush=True)
         del a, b
 
+    step_fn = ns.step_serial if args.serial else ns.step
+    if args.only:
+        step_fn = calls[args.only]
     for _ in range(args.warmup):
-        for r in routines:
-            calls[r]()
+        step_fn()
     torch.cuda.synchronize()
     print("# timing %d steps" % args.steps, file=sys.stderr, flush=True)
-    ev = {r: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)] for r in routines}
+    # per-routine hipEvent pairs, recorded by the library on the stream each
+    # routine runs on, inside the timed region
+    pkg.step.profile_begin(args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        for r in routines:
-            e0, e1 = ev[r][s]
-            e0.record()
-            calls[r]()
-            e1.record()
+        step_fn()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    prof = pkg.step.profile_end()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    rt = {r: sum(e0.elapsed_time(e1) for e0, e1 in ev[r]) / args.steps for r in routines}
+    rt = {r: prof[r][0] / max(prof[r][1], 1) for r in routines}
 
     nk, cols = g["nk"], g["nlon"] * g["nlat"]
     cols_all = (g["nlon"] + 4) * (g["nlat"] + 4)
@@ -158,7 +161,8 @@ def main():
         "config": {"workload": "%s global %dx%d columns x %d levels, %d unknowns/column/level, "
                                "nstep_sub=%d, fused assembly+solve FP64" % (
                                    args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
-                   "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64"},
+                   "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64",
+                   "schedule": "serial" if args.serial else "concurrent"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": dom, "alg_bytes_per_launch": alg[dom],

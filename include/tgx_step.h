@@ -97,9 +97,25 @@ int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
                void *stream);
 int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                 void *stream);
-/* The whole column step in the reference's order (dynamics_ng.F90:99-233). */
+/* The whole column step (dynamics_ng.F90:99-233).  The independent routines
+ * (oplus, minors, dt, comp) run concurrently on library-owned streams forked
+ * from and joined back into `stream`; duv follows dt and oplus.  Results are
+ * identical to the serial order; the call is stream-ordered on `stream` and
+ * may be captured into a hipGraph. */
 int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                 void *stream);
+/* The same step strictly in the reference's order on `stream` alone. */
+int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                       void *work, void *stream);
+
+/* Timing hooks (no counterpart in the reference; used by benchmarks and
+ * profiling).  Between begin and end, each routine entry point records a
+ * hipEvent pair on the stream it runs on, for at most max_calls calls per
+ * routine.  end waits for the recorded events and returns per routine, in the
+ * order oplus, minors, dt, duv, comp, the summed milliseconds and the number
+ * of calls timed. */
+int tgx_ng_profile_begin(int max_calls);
+int tgx_ng_profile_end(double *ms_total /* [5] */, int *calls /* [5] */);
 
 #ifdef __cplusplus
 }

@@ -55,3 +55,33 @@ def test_step_2d_bookkeeping(case):
     name, ns, ref = case
     for n in ("tlbc", "tlbc_nm", "ulbc", "vlbc", "ulbc_nm", "vlbc_nm", "flx_he"):
         assert np.array_equal(ns.fortran(n), ref[n]), n
+
+
+def test_concurrent_step_equals_serial(pkg):
+    """The forked multi-stream step and the single-stream reference order give
+    bitwise the same outputs, also when called twice in a row."""
+    import torch
+    g = pkg.synth.make_grid("small", nstep_sub=2)
+    F = pkg.synth.make_state(g)
+    a = pkg.step.NestState(g, F)
+    b = pkg.step.NestState(g, F)
+    a.step_serial()
+    b.step()
+    b.step()
+    torch.cuda.synchronize()
+    for n in a.out:
+        assert torch.equal(a.out[n], b.out[n]), n
+
+
+def test_profile_hooks_count_calls(pkg):
+    import torch
+    g = pkg.synth.make_grid("tiny", nstep_sub=1)
+    ns = pkg.step.NestState(g, pkg.synth.make_state(g))
+    pkg.step.profile_begin(2)
+    for _ in range(3):
+        ns.step()
+    torch.cuda.synchronize()
+    prof = pkg.step.profile_end()
+    for r in pkg.step.ROUTINES:
+        ms, n = prof[r]
+        assert n == 2 and ms > 0.0, (r, prof[r])

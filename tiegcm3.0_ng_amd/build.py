@@ -62,5 +62,34 @@ def build(verbose=True, jobs=4):
     return OUT
 
 
+def build_fortran(verbose=True):
+    """Fortran drop-in shims (trsolv, blktri) and their test driver, built
+    with amdflang against libtgx.so.  Skipped if no Fortran compiler."""
+    fc = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
+    if not os.path.exists(fc):
+        if verbose:
+            print("amdflang not found: Fortran shims not built")
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(bdir, exist_ok=True)
+    flags = ["-fdefault-real-8", "-O2", "-cpp", "-DTGX_NLONP4=76", "-J", bdir]
+    objs = []
+    for src in ("tgx_c.F90", "trsolv_shim.F90", "shim_driver.F90"):
+        obj = os.path.join(bdir, src + ".o")
+        cmd = [fc] + flags + ["-c", os.path.join(fdir, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        objs.append(obj)
+    exe = os.path.join(fdir, "shim_driver")
+    cmd = [fc, "-o", exe] + objs + ["-L" + HERE, "-ltgx", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose="-q" not in sys.argv)
+    build_fortran(verbose="-q" not in sys.argv)

@@ -2,6 +2,10 @@
 // O+ sub-cycle ping-pong (dynamics_ng.F90:99-121).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "../../include/tgx.h"
 #include "../../include/tgx_step.h"
 #include "tgx_common.h"
@@ -57,6 +61,68 @@ extern "C" unsigned long long tgx_ng_workspace_bytes(const tgx_ng_grid *g) {
   return (unsigned long long)W_NPLANES * plane_elems(g) * sizeof(double);
 }
 
+// Optional per-routine timing: hipEvent pairs recorded on the stream each
+// routine runs on, accumulated until tgx_ng_profile_end.  Off by default.
+namespace {
+enum { R_OPLUS = 0, R_MINORS, R_DT, R_DUV, R_COMP, R_N };
+struct Prof {
+  bool on = false;
+  int cap = 0;
+  int used[R_N] = {};
+  std::vector<hipEvent_t> ev;  // [R_N][cap][2]
+};
+std::mutex g_prof_mu;
+Prof g_prof;
+
+struct ProfScope {
+  hipEvent_t e1 = nullptr;
+  hipStream_t s;
+  ProfScope(int r, hipStream_t st) : s(st) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof.on || g_prof.used[r] >= g_prof.cap) return;
+    size_t at = ((size_t)r * g_prof.cap + g_prof.used[r]++) * 2;
+    if (hipEventRecord(g_prof.ev[at], s) == hipSuccess) e1 = g_prof.ev[at + 1];
+  }
+  ~ProfScope() {
+    if (e1) (void)hipEventRecord(e1, s);
+  }
+};
+}  // namespace
+
+extern "C" int tgx_ng_profile_begin(int max_calls) {
+  if (max_calls < 1 || max_calls > 100000) return tgx_set_error(1, "tgx_ng_profile_begin: bad max_calls %d", max_calls);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  size_t need = (size_t)R_N * max_calls * 2;
+  while (g_prof.ev.size() < need) {
+    hipEvent_t e;
+    TGX_HIP(hipEventCreate(&e));
+    g_prof.ev.push_back(e);
+  }
+  g_prof.cap = max_calls;
+  for (int r = 0; r < R_N; ++r) g_prof.used[r] = 0;
+  g_prof.on = true;
+  return 0;
+}
+
+extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof.on) return tgx_set_error(1, "tgx_ng_profile_end: profiling not started");
+  g_prof.on = false;
+  for (int r = 0; r < R_N; ++r) {
+    double tot = 0.;
+    for (int c = 0; c < g_prof.used[r]; ++c) {
+      size_t at = ((size_t)r * g_prof.cap + c) * 2;
+      TGX_HIP(hipEventSynchronize(g_prof.ev[at + 1]));
+      float ms = 0.f;
+      TGX_HIP(hipEventElapsedTime(&ms, g_prof.ev[at], g_prof.ev[at + 1]));
+      tot += ms;
+    }
+    if (ms_total) ms_total[r] = tot;
+    if (calls) calls[r] = g_prof.used[r];
+  }
+  return 0;
+}
+
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE };
 
@@ -64,6 +130,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
                          void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
+  ProfScope prof(R_DT, s);
   void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
   return tgx_ng_launch(K_DT, g, in, out, b, 0, s);
@@ -73,6 +140,7 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
                              void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
+  ProfScope prof(R_MINORS, s);
   void *b[10];
   b[0] = plane(work, g, W_MIN_A);
   b[1] = plane(work, g, W_MIN_B);
@@ -87,6 +155,7 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
                             void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
+  ProfScope prof(R_OPLUS, s);
   const int nsub = g->nstep_sub;
   double *tmp_op = plane(work, g, W_OP_X), *tmp_nm = plane(work, g, W_OP_XNM);
   const double *op_in = in->op, *nm_in = in->op_nm;
@@ -116,6 +185,7 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
                           void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
+  ProfScope prof(R_DUV, s);
   void *b[8];
   b[0] = plane(work, g, W_UV);
   b[1] = plane(work, g, W_UV + 1);
@@ -129,6 +199,7 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
                            void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
+  ProfScope prof(R_COMP, s);
   void *b[7];
   b[0] = plane(work, g, W_CO);
   for (int q = 0; q < 6; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
@@ -136,10 +207,62 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   return tgx_ng_launch(K_COMP, g, in, out, b, 0, s);
 }
 
+// Concurrency of the step: oplus, the minor species, dt and comp read only
+// the step's inputs and write disjoint outputs and workspace planes, so they
+// run on four streams forked from the caller's stream; duv needs dt's TN and
+// oplus' Fe/Fn and joins both.  All streams have the default priority.
+namespace {
+struct Fork {
+  hipStream_t s[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {};
+};
+std::mutex g_fork_mu;
+std::map<int, Fork> g_fork;
+
+int get_fork(Fork **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return tgx_set_error(3, "no HIP device: %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(g_fork_mu);
+  Fork &f = g_fork[dev];
+  if (!f.s[0]) {
+    for (int i = 0; i < 3; ++i) TGX_HIP(hipStreamCreateWithFlags(&f.s[i], hipStreamNonBlocking));
+    for (int i = 0; i < 8; ++i) TGX_HIP(hipEventCreateWithFlags(&f.ev[i], hipEventDisableTiming));
+  }
+  *out = &f;
+  return 0;
+}
+}  // namespace
+
 extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                            void *work, void *stream) {
-  // dynamics_ng.F90: oplus (:99-121), minors (:147-165), dt (:203), duv (:210),
-  // comp_o2o + comp (:221-233)
+  // dynamics_ng.F90 order: oplus (:99-121), minors (:147-165), dt (:203),
+  // duv (:210), comp_o2o + comp (:221-233)
+  if (int rc = check(g, work)) return rc;
+  Fork *F = nullptr;
+  if (int rc = get_fork(&F)) return rc;
+  hipStream_t s0 = (hipStream_t)stream, sa = F->s[0], sb = F->s[1], sc = F->s[2];
+  TGX_HIP(hipEventRecord(F->ev[0], s0));
+  TGX_HIP(hipStreamWaitEvent(sa, F->ev[0], 0));
+  TGX_HIP(hipStreamWaitEvent(sb, F->ev[0], 0));
+  TGX_HIP(hipStreamWaitEvent(sc, F->ev[0], 0));
+  if (int rc = tgx_ng_oplus(g, in, out, work, sa)) return rc;    // stream a
+  if (int rc = tgx_ng_minors(g, in, out, work, sb)) return rc;   // stream b
+  if (int rc = tgx_ng_comp(g, in, out, work, sc)) return rc;     // stream c
+  if (int rc = tgx_ng_dt(g, in, out, work, s0)) return rc;       // caller's stream
+  TGX_HIP(hipEventRecord(F->ev[1], sa));
+  TGX_HIP(hipStreamWaitEvent(s0, F->ev[1], 0));
+  if (int rc = tgx_ng_duv(g, in, out, work, s0)) return rc;
+  TGX_HIP(hipEventRecord(F->ev[2], sb));
+  TGX_HIP(hipEventRecord(F->ev[3], sc));
+  TGX_HIP(hipStreamWaitEvent(s0, F->ev[2], 0));
+  TGX_HIP(hipStreamWaitEvent(s0, F->ev[3], 0));
+  return 0;
+}
+
+// The same step strictly in the reference's order on one stream.
+extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
+                                  const tgx_ng_out *out, void *work, void *stream) {
   if (int rc = tgx_ng_oplus(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_minors(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_dt(g, in, out, work, stream)) return rc;

@@ -60,7 +60,28 @@ _SIGS = {
     "tgx_ng_duv": ([_P] * 5, C.c_int),
     "tgx_ng_comp": ([_P] * 5, C.c_int),
     "tgx_ng_step": ([_P] * 5, C.c_int),
+    "tgx_ng_step_serial": ([_P] * 5, C.c_int),
+    "tgx_ng_profile_begin": ([C.c_int], C.c_int),
+    "tgx_ng_profile_end": ([_P, _P], C.c_int),
 }
+
+ROUTINES = ("oplus", "minors", "dt", "duv", "comp")
+
+
+def profile_begin(max_calls):
+    """Start recording a hipEvent pair around every routine call (on the
+    stream the routine runs on), for at most max_calls calls per routine."""
+    if _lib().tgx_ng_profile_begin(int(max_calls)) != 0:
+        raise _tgx.TgxError(_tgx.last_error())
+
+
+def profile_end():
+    """Stop recording; returns {routine: (summed ms, calls)}."""
+    ms = (C.c_double * 5)()
+    n = (C.c_int * 5)()
+    if _lib().tgx_ng_profile_end(C.cast(ms, _P), C.cast(n, _P)) != 0:
+        raise _tgx.TgxError(_tgx.last_error())
+    return {r: (ms[i], n[i]) for i, r in enumerate(ROUTINES)}
 
 
 def _lib():
@@ -182,6 +203,9 @@ class NestState:
 
     def step(self, stream=None):
         self._call("tgx_ng_step", stream)
+
+    def step_serial(self, stream=None):
+        self._call("tgx_ng_step_serial", stream)
 
 
 # Algorithmic HBM traffic of each routine: the distinct 3-D fields its fused
