@@ -1,6 +1,6 @@
-"""Oracle vs the reference's own compiled matutil.F90 (oracle/_ref, built from
-the reference source where it lies by oracle/Makefile 'ref').  Skipped when
-oracle/_ref has not been built."""
+"""Oracle vs the reference's own compiled matutil.F90: live against oracle/_ref
+(built from the reference source where it lies by oracle/Makefile 'ref';
+skipped when not built) and against the committed vectors written from it."""
 import ctypes as C
 import os
 
@@ -10,9 +10,9 @@ import pytest
 from helpers import ROOT
 
 REF = os.path.join(ROOT, "oracle", "_ref", "libref_matutil.so")
-pytestmark = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built")
 
 
+@pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built")
 def test_matinv3_matinv2_bitwise_equal_to_reference(oracle):
     ref = C.CDLL(REF)
     rng = np.random.default_rng(0)
@@ -28,3 +28,18 @@ def test_matinv3_matinv2_bitwise_equal_to_reference(oracle):
         oracle.lib().tgx_o_matinv2(B.reshape(-1, order="F").copy().ctypes.data_as(C.c_void_p),
                                    mine.ctypes.data_as(C.c_void_p))
         assert np.array_equal(out2.reshape(-1, order="F"), mine)
+
+
+def test_oracle_matches_committed_reference_vectors(oracle):
+    """The committed fixture (written from the compiled reference matutil by
+    tests/golden/gen_matutil_vectors.py) pins the oracle's matinv3/matinv2
+    bit for bit without the reference tree."""
+    golden = os.path.join(ROOT, "tests", "golden", "matutil_vectors.npz")
+    d = np.load(golden)
+    for A, R in zip(d["a3"], d["inv3"]):
+        assert np.array_equal(oracle.matinv3(np.ascontiguousarray(A)), R)
+    for B, R in zip(d["a2"], d["inv2"]):
+        mine = np.zeros(4)
+        oracle.lib().tgx_o_matinv2(B.reshape(-1, order="F").copy().ctypes.data_as(C.c_void_p),
+                                   mine.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(mine.reshape(2, 2, order="F"), R)

@@ -300,48 +300,70 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
     }
     return;
   }
-  const double racs = g.racs[j];
 #define F3(f, k) (f)[G.a3((k), i, j)]
-  // g(L), f(L) of dt_ng.F90:121-139
-  auto gf = [&](int L, double tni, double &gg, double &ff) {
-    double h = F3(in.schti, L);
-    double rho = P0 * g.expzmid_inv / (h * GRAV);
-    rho = rho * g.expz[L];
-    double gpart = h * h * rho * F3(in.cp, L);
-    double gpart_a = gpart * g.dift[L];
-    gg = GRAV * (F3(in.kt, L) + gpart_a) / (P0 * h * (dz * dz));
-    ff = GRAV * GRAV * (h * h) * rho / (tni * P0 * 2. * dz);
-    ff = ff * g.dift[L];
+  // Fields used at two neighbouring levels, carried one level in registers;
+  // every load of a level is issued before its arithmetic.
+  struct DL { double w, cp, km, qtotal, schti, kt, un, vn; };
+  struct DS { double hadv, tsm, xn, o1, rkm12, mbar, scht, hdt, qji, cool_imp, cool_exp; };
+  auto load_dl = [&](int L, DL &D) {
+    size_t p = G.a3(L, i, j);
+    D.w = in.w_upd[p]; D.cp = in.cp[p]; D.km = in.km[p]; D.qtotal = in.qtotal[p];
+    D.schti = in.schti[p]; D.kt = in.kt[p]; D.un = in.un[p]; D.vn = in.vn[p];
   };
+  auto load_ds = [&](int L, DS &S) {
+    size_t p = G.a3(L, i, j);
+    S.hadv = hadv_tn[p]; S.tsm = smooth_tn_nm[p]; S.xn = in.xnmbar[p]; S.o1 = in.o1[p];
+    S.rkm12 = in.rkm12[p]; S.mbar = in.mbar[p]; S.scht = in.scht[p]; S.hdt = in.hdt[p];
+    S.qji = in.qji_tn[p]; S.cool_imp = in.cool_imp[p]; S.cool_exp = in.cool_exp[p];
+  };
+  // g(L), f(L) of dt_ng.F90:121-139 from the level-L fields
+  auto gf = [&](const DL &D, double expzL, double diftL, double tni, double &gg, double &ff) {
+    double h = D.schti;
+    double rho = P0 * g.expzmid_inv / (h * GRAV);
+    rho = rho * expzL;
+    double gpart = h * h * rho * D.cp;
+    double gpart_a = gpart * diftL;
+    gg = GRAV * (D.kt + gpart_a) / (P0 * h * (dz * dz));
+    ff = GRAV * GRAV * (h * h) * rho / (tni * P0 * 2. * dz);
+    ff = ff * diftL;
+  };
+  const double ulbc = in.ulbc[c2], vlbc = in.vlbc[c2];
+  DL dc, dn;
+  load_dl(0, dc);
   double tn_m = 0., tn_c = F3(in.tn, 0), tn_p = F3(in.tn, 1);
+  double un_m = 0., vn_m = 0.;
   double gk, fk;
-  gf(0, tl, gk, fk);
+  gf(dc, g.expz[0], g.dift[0], tl, gk, fk);
   double w1 = 0., w3 = 0., rprev = 0., dudz_prev = 0., dvdz_prev = 0.;
   for (int k = 0; k <= nk - 2; ++k) {
-    const double w_k = F3(in.w_upd, k), w_k1 = F3(in.w_upd, k + 1);
+    DS ds;
+    load_dl(k + 1, dn);
+    load_ds(k, ds);
+    double tn_pp = k + 2 <= nk - 1 ? F3(in.tn, k + 2) : 0.;
+    const double expzk = g.expz[k], expzk1 = g.expz[k + 1], dift1 = g.dift[k + 1];
+    asm volatile("" ::: "memory");
+    const double w_k = dc.w, w_k1 = dn.w;
     // veradv (:57-65)
     double raw_k = k == 0 ? (tn_c - t_lbc) * w_k * 2. : (tn_c - tn_m) * w_k;
     double raw_k1 = (k + 1 <= nk - 2) ? (tn_p - tn_c) * w_k1 : 0.;
     double veradv = (raw_k + raw_k1) * .5;
     veradv = veradv / dz;
     // interface means (:71-78)
-    double cpi = .5 * (F3(in.cp, k) + F3(in.cp, k + 1));
-    double kmi = .5 * (F3(in.km, k) + F3(in.km, k + 1));
+    double cpi = .5 * (dc.cp + dn.cp);
+    double kmi = .5 * (dc.km + dn.km);
     double wi = .5 * (w_k + w_k1);
     // cptn (:67, :80-83)
-    double horadv = F3(hadv_tn, k);
-    double tnsmooth = F3(smooth_tn_nm, k);
-    double advec_tn = horadv + veradv;
-    double cptn = cpi * (advec_tn - dtx2inv * tnsmooth);
-    cptn = cptn * g.expz[k];
+    double advec_tn = ds.hadv + veradv;
+    double cptn = cpi * (advec_tn - dtx2inv * ds.tsm);
+    cptn = cptn * expzk;
     // dudz (:85-94)
     double dudz, dvdz;
     if (k == 0) {
-      dudz = (F3(in.un, 0) + 1. / 3. * F3(in.un, 1) - 4. / 3. * in.ulbc[c2]) / dz;
-      dvdz = (F3(in.vn, 0) + 1. / 3. * F3(in.vn, 1) - 4. / 3. * in.vlbc[c2]) / dz;
+      dudz = (dc.un + 1. / 3. * dn.un - 4. / 3. * ulbc) / dz;
+      dvdz = (dc.vn + 1. / 3. * dn.vn - 4. / 3. * vlbc) / dz;
     } else if (k <= nk - 3) {
-      dudz = (F3(in.un, k + 1) - F3(in.un, k - 1)) / (2. * dz);
-      dvdz = (F3(in.vn, k + 1) - F3(in.vn, k - 1)) / (2. * dz);
+      dudz = (dn.un - un_m) / (2. * dz);
+      dvdz = (dn.vn - vn_m) / (2. * dz);
     } else {
       dudz = dudz_prev / 3.;
       dvdz = dvdz_prev / 3.;
@@ -349,20 +371,20 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
     dudz_prev = dudz;
     dvdz_prev = dvdz;
     // heating (:96-113)
-    double qsolar = .5 * (F3(in.qtotal, k) + F3(in.qtotal, k + 1));
-    double t = F3(in.xnmbar, k) * F3(in.o1, k) * RIO1;
-    double qrecomb = TSURPLUS * F3(in.rkm12, k) * (t * t) * AVO / F3(in.mbar, k);
-    double qm = GRAV * kmi / (P0 * F3(in.scht, k)) * (dudz * dudz + dvdz * dvdz);
-    qm = qm / g.expz[k];
-    double total_heat = qsolar + F3(in.hdt, k) + qrecomb + F3(in.qji_tn, k) + qm;
-    double total_heat_a = total_heat * g.expz[k];
+    double qsolar = .5 * (dc.qtotal + dn.qtotal);
+    double t = ds.xn * ds.o1 * RIO1;
+    double qrecomb = TSURPLUS * ds.rkm12 * (t * t) * AVO / ds.mbar;
+    double qm = GRAV * kmi / (P0 * ds.scht) * (dudz * dudz + dvdz * dvdz);
+    qm = qm / expzk;
+    double total_heat = qsolar + ds.hdt + qrecomb + ds.qji + qm;
+    double total_heat_a = total_heat * expzk;
     cptn = cptn - total_heat_a;
     // coefficients (:141-163)
     double p = gk - fk, q, r;
     double g1 = 0., f1 = 0.;
     if (k <= nk - 3) {
       double tni1 = .5 * (tn_c + tn_p);
-      gf(k + 1, tni1, g1, f1);
+      gf(dn, expzk1, dift1, tni1, g1, f1);
       q = -(g1 - f1 + gk + fk);
       r = g1 + f1;
     } else {
@@ -370,9 +392,9 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
       r = 0.;
     }
     double rhs = cptn;
-    double qpart = cpi * (dtx2inv + F3(in.cool_imp, k)) + wi * GASK / F3(in.mbar, k);
-    rhs = rhs + F3(in.cool_exp, k);
-    double qpart_a = qpart * g.expz[k];
+    double qpart = cpi * (dtx2inv + ds.cool_imp) + wi * GASK / ds.mbar;
+    rhs = rhs + ds.cool_exp;
+    double qpart_a = qpart * expzk;
     q = q - qpart_a;
     if (k == 0) {
       q = q - p;
@@ -393,23 +415,36 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
     out.tn[pk] = w3;
     rprev = r;
     gk = g1; fk = f1;
-    tn_m = tn_c; tn_c = tn_p;
-    if (k + 2 <= nk - 1) tn_p = F3(in.tn, k + 2);
+    tn_m = tn_c; tn_c = tn_p; tn_p = tn_pp;
+    un_m = dc.un; vn_m = dc.vn;
+    dc = dn;
   }
-  // back substitution (:165-167) fused with the output stage (:208-213)
-  double xn = w3, xnn = 0.;
-  auto emit = [&](int k, double x) {
+  // back substitution (:165-167) fused with the output stage (:208-213); the
+  // next level's loads are in flight while this level is stored
+  struct DB { double x, w2, tnm, tn; };
+  auto load_db = [&](int k, DB &B) {
     size_t p = G.a3(k, i, j);
-    out.tn_nm[p] = DTSMOOTH_DIV2 * (in.tn_nm[p] + x) + DTSMOOTH * in.tn[p];
-    out.tn[p] = x > 100. ? x : 100.;
+    B.x = out.tn[p]; B.w2 = k < nk - 2 ? w2s[p] : 0.; B.tnm = in.tn_nm[p]; B.tn = in.tn[p];
   };
+  const double tnm_top = F3(in.tn_nm, nk - 1), tn_top = F3(in.tn, nk - 1);
+  double xn = w3;
+  DB bc, bn;
+  load_db(nk - 2, bc);
   for (int k = nk - 2; k >= 0; --k) {
-    double x = k == nk - 2 ? xn : out.tn[G.a3(k, i, j)] - w2s[G.a3(k, i, j)] * xn;
-    if (k == nk - 3) emit(nk - 1, xn * xn / x);
-    if (k < nk - 2) xnn = xn;
-    (void)xnn;
-    emit(k, x);
+    if (k > 0) load_db(k - 1, bn);
+    asm volatile("" ::: "memory");
+    double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
+    if (k == nk - 3) {
+      double xt = xn * xn / x;
+      size_t pt = G.a3(nk - 1, i, j);
+      out.tn_nm[pt] = DTSMOOTH_DIV2 * (tnm_top + xt) + DTSMOOTH * tn_top;
+      out.tn[pt] = xt > 100. ? xt : 100.;
+    }
+    size_t p = G.a3(k, i, j);
+    out.tn_nm[p] = DTSMOOTH_DIV2 * (bc.tnm + x) + DTSMOOTH * bc.tn;
+    out.tn[p] = x > 100. ? x : 100.;
     xn = x;
+    bc = bn;
   }
 #undef F3
 }
@@ -463,14 +498,107 @@ struct MinorSpecies {  // per species and level
   double ax, thdiff, sloss, s0prod, ftm1s, hadvec;
 };
 
-struct MinorTerms {  // prelude planes: smoothed fcomp_tm1 and advection of fcomp, by species id
-  const double *ftm1s[4], *hadvec[4];
+struct MinorTerms {  // prelude planes by species id: smoothed fcomp_tm1, advection of
+                     // fcomp, and the loss / scaled production of minor_src_kernel
+  const double *ftm1s[4], *hadvec[4], *sloss[4], *s0prod[4];
 };
+
+// Production and loss of species s at level L of column (i, j): the pointwise
+// chemistry of comp_n2d_ng.F90:195-203, comp_n4s_ng.F90:122-138 and
+// comp_no_ng.F90:48-57 (Ar has none).  n2d_upd is this step's N2D.
+__device__ __forceinline__ void minor_sources(const Geo &G, const tgx_ng_fields &in,
+                                              const double *__restrict__ n2d_upd, int s, int L,
+                                              int i, int j, double &sloss, double &sprod) {
+  const int nk = G.nk;
+  size_t p = G.a3(L, i, j);
+  double xn = in.xnmbar[p];
+  if (s == 3) { sloss = 0.; sprod = 0.; return; }
+  if (s == 2) {  // comp_n2d_ng.F90:195-203
+    double qtefi, nei;
+    if (L < nk - 1) {
+      qtefi = .5 * (in.qtef[p] + in.qtef[p + G.P]);
+      nei = sqrt(in.ne[p] * in.ne[p + G.P]);
+    } else {
+      qtefi = 1.5 * in.qtef[p] - .5 * in.qtef[p - G.P];
+      nei = sqrt(in.ne[p] * in.ne[p] * in.ne[p] / in.ne[p - G.P]);
+    }
+    sprod = qtefi * BRN2D + in.rk3[p] * in.n2p[p] * xn * in.o1[p] * RIO1 +
+            (in.ra1[p] * in.nop[p] * 0.85 + in.ra3[p] * in.n2p[p] * 0.9) * nei;
+    sloss = -(xn * (BETA2 * in.o2[p] * RIO2 + BETA4 * in.o1[p] * RIO1 + BETA6 * in.no[p] * RINO) + BETA7 +
+              in.beta5[p] * nei + RK10 * in.op[p]);
+    return;
+  }
+  double n2d = n2d_upd[p];
+  if (s == 1) {  // comp_n4s_ng.F90:122-138
+    double qtefi, beta8i, nei1, nei2;
+    if (L < nk - 1) {
+      qtefi = .5 * (in.qtef[p] + in.qtef[p + G.P]);
+      beta8i = .5 * (in.beta8[p] + in.beta8[p + G.P]);
+      nei1 = .5 * (in.ne[p] + in.ne[p + G.P]);
+      nei2 = sqrt(in.ne[p] * in.ne[p + G.P]);
+    } else {
+      qtefi = 0.;  // undefined in the reference (:128); reaches only f_rhs(nk), overwritten
+      beta8i = 1.5 * in.beta8[p] - .5 * in.beta8[p - G.P];
+      nei1 = 1.5 * in.ne[p] - .5 * in.ne[p - G.P];
+      nei2 = sqrt(in.ne[p] * in.ne[p] * in.ne[p] / in.ne[p - G.P]);
+    }
+    sprod = qtefi * (1. - BRN2D) +
+            xn * (n2d * RIN2D * (xn * BETA4 * in.o1[p] * RIO1 + in.beta5[p] * nei1 + BETA7) +
+                  beta8i * in.no[p] * RINO) +
+            xn * (in.rk2[p] * in.op[p] * in.n2[p] * RIN2 + RK6 * in.nplus[p] * in.o2[p] * RIO2 +
+                  RK8 * in.nplus[p] * in.o1[p] * RIO1) +
+            nei2 * (in.ra1[p] * in.nop[p] * 0.15 + in.ra3[p] * in.n2p[p] * 1.1);
+    sloss = -xn * (in.beta1[p] * in.o2[p] * RIO2 + in.beta3[p] * in.no[p] * RINO +
+                   xn * in.beta17[p] * in.o1[p] * RIO1 * in.n2[p] * RIN2) -
+            RK4 * in.o2p[p];
+    return;
+  }
+  // NO: comp_no_ng.F90:48-57
+  sprod = xn * xn * in.o2[p] * RIO2 * (in.beta1[p] * in.n4s[p] * RIN4S + BETA2 * n2d * RIN2D) +
+          xn * xn * xn * in.beta17[p] * in.o1[p] * RIO1 * in.n2[p] * RIN2 * in.n4s[p] * RIN4S;
+  double betai;
+  if (L < nk - 1)
+    betai = .5 * (in.beta8[p] + in.beta9[p] + in.beta9n[p] + in.beta8[p + G.P] + in.beta9[p + G.P] +
+                  in.beta9n[p + G.P]);
+  else
+    betai = 1.5 * (in.beta8[p] + in.beta9[p] + in.beta9n[p]) -
+            .5 * (in.beta8[p - G.P] + in.beta9[p - G.P] + in.beta9n[p - G.P]);
+  sloss = -xn * (in.beta3[p] * in.n4s[p] * RIN4S + BETA6 * n2d * RIN2D) - RK5 * in.o2p[p] - betai;
+}
+
+// sloss and s0prod = sprod*rmx/xnmbar of species SA and SB at every level
+// (level-parallel; Ar has no planes).
+template <int SA, int SB>
+__global__ void __launch_bounds__(256) minor_src_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                        tgx_ng_out out, MinorTerms MT) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  long npt = (long)G.nlond * G.nlatd * G.nk;
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int k = (int)(r / G.nlatd);
+  const size_t p = G.a3(k, i, j);
+  const int SP[2] = {SA, SB};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    if (SP[q] == 3) continue;
+    double sl, sp;
+    minor_sources(G, in, out.n2d, SP[q], k, i, j, sl, sp);
+    double *dl = (double *)MT.sloss[SP[q]], *dp = (double *)MT.s0prod[SP[q]];
+    dl[p] = sl;
+    dp[p] = sp * species_const(SP[q]).rmx / in.xnmbar[p];
+  }
+}
 
 template <int SA, int SB>
 __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     tgx_ng_out out, double *__restrict__ w2a,
                                                     double *__restrict__ w2b, MinorTerms MT) {
+  // Column solve of two species (minor_ng.F90:60-237); their sources come
+  // from minor_src_kernel.  Each level's loads are issued together ahead of
+  // its arithmetic, with a window of three levels of the shared fields.
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
@@ -526,31 +654,46 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   const double bhe = g.b[6] * o2_0 + g.b[7] * o1_0 + g.b[8] * he_0 + g.fb[2];
   const double xmbari = 1. / (bo2 * RIO2 + bo1 * RIO1 + (1. - bo2 - bo1 - bhe) * RIN2 + bhe * RIHE);
 
-  auto tni_at = [&](int L) -> double {  // :89-93
-    if (L == 0) return in.tlbc[c2];
-    if (L == nk - 1) return F3(in.tn, nk - 2);
-    return .5 * (F3(in.tn, L) + F3(in.tn, L - 1));
+  struct MR { double o1, o2, he, mbar, tn, barm, w; };  // shared fields at one level
+  struct SR { double ftm1s, hadvec, sloss, s0prod; };     // one species at one level
+  auto load_mr = [&](int L, MR &R) {
+    size_t p = G.a3(L, i, j);
+    R.o1 = in.o1[p]; R.o2 = in.o2[p]; R.he = in.he[p]; R.mbar = in.mbar[p]; R.tn = in.tn[p];
+    R.barm = in.barm[p]; R.w = in.w_upd[p];
   };
-  auto shared_at = [&](int L, MinorShared &S) {
+  auto load_sr = [&](int L, SR *Q) {
+    size_t p = G.a3(L, i, j);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      Q[q].ftm1s = MT.ftm1s[SP[q]][p];
+      Q[q].hadvec = MT.hadvec[SP[q]][p];
+      if (SP[q] == 3) { Q[q].sloss = 0.; Q[q].s0prod = 0.; }
+      else { Q[q].sloss = MT.sloss[SP[q]][p]; Q[q].s0prod = MT.s0prod[SP[q]][p]; }
+    }
+  };
+  // :66-126 interface quantities at L from levels A = L-1 and B = L (L >= 1),
+  // or from the lower boundary and B = level 0 (L = 0)
+  auto shared_at = [&](int L, const MR &A, const MR &B, MinorShared &S) {
     if (L == 0) {
-      S.dmdz = (F3(in.mbar, 0) - xmbari) / dzp;
+      S.dmdz = (B.mbar - xmbari) / dzp;
       S.pso1 = .5 * (bo1 + o1_0);
       S.pso2 = .5 * (bo2 + o2_0);
       S.pshe = .5 * (bhe + he_0);
       S.do1dz = (o1_0 - bo1) / dzp;
       S.do2dz = (o2_0 - bo2) / dzp;
       S.dhedz = (he_0 - bhe) / dzp;
+      S.tni = in.tlbc[c2];
     } else {
-      S.dmdz = (F3(in.mbar, L) - F3(in.mbar, L - 1)) / dzp;
-      S.pso1 = .5 * (F3(in.o1, L) + F3(in.o1, L - 1));
-      S.pso2 = .5 * (F3(in.o2, L) + F3(in.o2, L - 1));
-      S.pshe = .5 * (F3(in.he, L) + F3(in.he, L - 1));
-      S.do1dz = (F3(in.o1, L) - F3(in.o1, L - 1)) / dzp;
-      S.do2dz = (F3(in.o2, L) - F3(in.o2, L - 1)) / dzp;
-      S.dhedz = (F3(in.he, L) - F3(in.he, L - 1)) / dzp;
+      S.dmdz = (B.mbar - A.mbar) / dzp;
+      S.pso1 = .5 * (B.o1 + A.o1);
+      S.pso2 = .5 * (B.o2 + A.o2);
+      S.pshe = .5 * (B.he + A.he);
+      S.do1dz = (B.o1 - A.o1) / dzp;
+      S.do2dz = (B.o2 - A.o2) / dzp;
+      S.dhedz = (B.he - A.he) / dzp;
+      S.tni = L == nk - 1 ? A.tn : .5 * (B.tn + A.tn);  // :89-93
     }
-    S.tni = tni_at(L);
-    S.barm = F3(in.barm, L);
+    S.barm = B.barm;
     double df[3], al[9];
     for (int n = 0; n < 3; ++n) df[n] = pow(S.tni / T00, 1.75 - ssn(n));
     al[0] = -(PHI(1, 4) + (PHI(1, 2) - PHI(1, 4)) * S.pso1 + (df[0] * PHI(1, 3) - PHI(1, 4)) * S.pshe);
@@ -565,64 +708,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     al[5] = (df[1] * PHI(3, 2) - df[2] * PHI(3, 4)) * S.pshe;
     matinv3_ng(al, S.inv);
   };
-  // sources of species s at level L (comp_*_ng.F90)
-  auto sources = [&](int s, int L, double &sloss, double &sprod) {
-    size_t p = G.a3(L, i, j);
-    double xn = in.xnmbar[p];
-    if (s == 3) { sloss = 0.; sprod = 0.; return; }
-    if (s == 2) {  // comp_n2d_ng.F90:195-203
-      double qtefi, nei;
-      if (L < nk - 1) {
-        qtefi = .5 * (in.qtef[p] + in.qtef[p + G.P]);
-        nei = sqrt(in.ne[p] * in.ne[p + G.P]);
-      } else {
-        qtefi = 1.5 * in.qtef[p] - .5 * in.qtef[p - G.P];
-        nei = sqrt(in.ne[p] * in.ne[p] * in.ne[p] / in.ne[p - G.P]);
-      }
-      sprod = qtefi * BRN2D + in.rk3[p] * in.n2p[p] * xn * in.o1[p] * RIO1 +
-              (in.ra1[p] * in.nop[p] * 0.85 + in.ra3[p] * in.n2p[p] * 0.9) * nei;
-      sloss = -(xn * (BETA2 * in.o2[p] * RIO2 + BETA4 * in.o1[p] * RIO1 + BETA6 * in.no[p] * RINO) + BETA7 +
-                in.beta5[p] * nei + RK10 * in.op[p]);
-      return;
-    }
-    double n2d = n2d_upd[p];
-    if (s == 1) {  // comp_n4s_ng.F90:122-138
-      double qtefi, beta8i, nei1, nei2;
-      if (L < nk - 1) {
-        qtefi = .5 * (in.qtef[p] + in.qtef[p + G.P]);
-        beta8i = .5 * (in.beta8[p] + in.beta8[p + G.P]);
-        nei1 = .5 * (in.ne[p] + in.ne[p + G.P]);
-        nei2 = sqrt(in.ne[p] * in.ne[p + G.P]);
-      } else {
-        qtefi = 0.;  // undefined in the reference (:128); reaches only f_rhs(nk), overwritten
-        beta8i = 1.5 * in.beta8[p] - .5 * in.beta8[p - G.P];
-        nei1 = 1.5 * in.ne[p] - .5 * in.ne[p - G.P];
-        nei2 = sqrt(in.ne[p] * in.ne[p] * in.ne[p] / in.ne[p - G.P]);
-      }
-      sprod = qtefi * (1. - BRN2D) +
-              xn * (n2d * RIN2D * (xn * BETA4 * in.o1[p] * RIO1 + in.beta5[p] * nei1 + BETA7) +
-                    beta8i * in.no[p] * RINO) +
-              xn * (in.rk2[p] * in.op[p] * in.n2[p] * RIN2 + RK6 * in.nplus[p] * in.o2[p] * RIO2 +
-                    RK8 * in.nplus[p] * in.o1[p] * RIO1) +
-              nei2 * (in.ra1[p] * in.nop[p] * 0.15 + in.ra3[p] * in.n2p[p] * 1.1);
-      sloss = -xn * (in.beta1[p] * in.o2[p] * RIO2 + in.beta3[p] * in.no[p] * RINO +
-                     xn * in.beta17[p] * in.o1[p] * RIO1 * in.n2[p] * RIN2) -
-              RK4 * in.o2p[p];
-      return;
-    }
-    // NO: comp_no_ng.F90:48-57
-    sprod = xn * xn * in.o2[p] * RIO2 * (in.beta1[p] * in.n4s[p] * RIN4S + BETA2 * n2d * RIN2D) +
-            xn * xn * xn * in.beta17[p] * in.o1[p] * RIO1 * in.n2[p] * RIN2 * in.n4s[p] * RIN4S;
-    double betai;
-    if (L < nk - 1)
-      betai = .5 * (in.beta8[p] + in.beta9[p] + in.beta9n[p] + in.beta8[p + G.P] + in.beta9[p + G.P] +
-                    in.beta9n[p + G.P]);
-    else
-      betai = 1.5 * (in.beta8[p] + in.beta9[p] + in.beta9n[p]) -
-              .5 * (in.beta8[p - G.P] + in.beta9[p - G.P] + in.beta9n[p - G.P]);
-    sloss = -xn * (in.beta3[p] * in.n4s[p] * RIN4S + BETA6 * n2d * RIN2D) - RK5 * in.o2p[p] - betai;
-  };
-  auto species_at = [&](int q, int L, const MinorShared &S, double dtnidz, MinorSpecies &T) {
+  auto species_at = [&](int q, const MinorShared &S, double dtnidz, const SR &R, MinorSpecies &T) {
     const SpeciesConst &c = sc[q];
     double sai[3];
     for (int n = 0; n < 3; ++n) {
@@ -637,15 +723,10 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     T.ax = -(S.barm / (TAU * RMN2) * pow(T00 / S.tni, 0.25) /
              (c.phix[2] + c.salfax[0] * S.pso2 + c.salfax[1] * S.pso1 + c.salfax[2] * S.pshe));
     T.thdiff = ex - c.alfax * dtnidz / (dzp * S.tni);
-    double sprod;
-    sources(SP[q], L, T.sloss, sprod);
-    T.s0prod = sprod * c.rmx / F3(in.xnmbar, L);
-    T.ftm1s = F3(MT.ftm1s[SP[q]], L);
-    T.hadvec = F3(MT.hadvec[SP[q]], L);
-  };
-  auto wi_at = [&](int L) -> double {
-    if (L < nk - 1) return .5 * (F3(in.w_upd, L) + F3(in.w_upd, L + 1));
-    return 1.5 * F3(in.w_upd, nk - 1) - .5 * F3(in.w_upd, nk - 2);
+    T.sloss = R.sloss;
+    T.s0prod = R.s0prod;
+    T.ftm1s = R.ftm1s;
+    T.hadvec = R.hadvec;
   };
   // lower-boundary data flbc(:,:,1:3) of each species (comp_*_ng.F90)
   double l1[2], l2[2], l3[2];
@@ -672,46 +753,55 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   }
   const double fubc = 0.;  // every species sets fubc = 0 (comp_*_ng.F90)
 
-  // sliding window: levels k (cur) and k+1 (nxt)
+  // window of the shared fields at levels k (R0), k+1 (R1), k+2 (R2)
+  MR R0, R1, R2;
+  SR Y[2];
+  load_mr(0, R0);
+  load_mr(1, R1);
+  load_sr(0, Y);
   MinorShared Sc, Sn;
   MinorSpecies Tc[2], Tn[2];
-  double tni_m = 0.;
-  shared_at(0, Sc);
-  double tni_1 = tni_at(1);
+  shared_at(0, R0, R0, Sc);
+  // tni at level 1 (:89-93)
+  double tni_1 = 1 == nk - 1 ? R0.tn : .5 * (R1.tn + R0.tn);
 #pragma unroll
-  for (int q = 0; q < 2; ++q) species_at(q, 0, Sc, tni_1 - Sc.tni, Tc[q]);
+  for (int q = 0; q < 2; ++q) species_at(q, Sc, tni_1 - Sc.tni, Y[q], Tc[q]);
   double w1[2] = {0., 0.}, w3[2] = {0., 0.}, rprev[2] = {0., 0.};
   double *w2s[2] = {w2a, w2b};
-  double wic = wi_at(0);
+  double wic = .5 * (R0.w + R1.w);
   for (int k = 0; k < nk; ++k) {
+    if (k + 2 <= nk - 1) load_mr(k + 2, R2);
+    if (k + 1 <= nk - 1) load_sr(k + 1, Y);
+    const double difk0 = g.difk[k], difk1 = k + 1 <= nk - 1 ? g.difk[k + 1] : 0., expzk = g.expz[k];
+    asm volatile("" ::: "memory");
     double wi_n = 0.;
     if (k + 1 <= nk - 1) {
-      shared_at(k + 1, Sn);
-      double tni_p = (k + 2 <= nk - 1) ? tni_at(k + 2) : 0.;
+      shared_at(k + 1, R0, R1, Sn);
+      double tni_p = 0.;
+      if (k + 2 <= nk - 1) tni_p = k + 2 == nk - 1 ? R1.tn : .5 * (R2.tn + R1.tn);
       double dtnidz = (k + 1 == nk - 1) ? Sn.tni - Sc.tni : (tni_p - Sc.tni) / 2.;
-    #pragma unroll
-  for (int q = 0; q < 2; ++q) species_at(q, k + 1, Sn, dtnidz, Tn[q]);
-      wi_n = wi_at(k + 1);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) species_at(q, Sn, dtnidz, Y[q], Tn[q]);
+      wi_n = k + 1 < nk - 1 ? .5 * (R1.w + R2.w) : 1.5 * R1.w - .5 * R0.w;
     }
-    (void)tni_m;
     const double dmdzc = Sc.dmdz / Sc.barm;
     const double dmdzn = k + 1 <= nk - 1 ? Sn.dmdz / Sn.barm : 0.;
-  #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
       double pc, qc, rc, fr;
       if (k <= nk - 2) {
         double molp = Tc[q].ax * (1. / dzp + .5 * Tc[q].thdiff);
         double molr = Tn[q].ax * (1. / dzp - .5 * Tn[q].thdiff);
         double molq = Tc[q].ax * (1. / dzp - .5 * Tc[q].thdiff) + Tn[q].ax * (1. / dzp + .5 * Tn[q].thdiff);
-        double eddyp = g.expzmid_inv * g.difk[k] * (1. / dzp - .5 * dmdzc);
-        double eddyr = g.expzmid * g.difk[k + 1] * (1. / dzp + .5 * dmdzn);
-        double eddyq = g.expzmid_inv * g.difk[k] * (1. / dzp + .5 * dmdzc) +
-                       g.expzmid * g.difk[k + 1] * (1. / dzp - .5 * dmdzn);
-        pc = molp / dzp - g.expz[k] * (eddyp * dfac + .5 * wic) / dzp;
-        rc = molr / dzp - g.expz[k] * (eddyr * dfac - .5 * wic) / dzp;
-        qc = -molq / dzp + g.expz[k] * (eddyq * dfac / dzp + dtx2inv - Tc[q].sloss);
+        double eddyp = g.expzmid_inv * difk0 * (1. / dzp - .5 * dmdzc);
+        double eddyr = g.expzmid * difk1 * (1. / dzp + .5 * dmdzn);
+        double eddyq = g.expzmid_inv * difk0 * (1. / dzp + .5 * dmdzc) +
+                       g.expzmid * difk1 * (1. / dzp - .5 * dmdzn);
+        pc = molp / dzp - expzk * (eddyp * dfac + .5 * wic) / dzp;
+        rc = molr / dzp - expzk * (eddyr * dfac - .5 * wic) / dzp;
+        qc = -molq / dzp + expzk * (eddyq * dfac / dzp + dtx2inv - Tc[q].sloss);
         fr = Tc[q].ftm1s * dtx2inv - Tc[q].hadvec + Tc[q].s0prod;
-        fr = fr * g.expz[k];
+        fr = fr * expzk;
         if (k == 0) {
           qc = qc + pc * (l1[q] + .5 * l2[q] * dzp) / (l1[q] - .5 * l2[q] * dzp);
           fr = fr - pc * l3[q] * dzp / (l1[q] - .5 * l2[q] * dzp);
@@ -740,18 +830,34 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     Tc[0] = Tn[0];
     Tc[1] = Tn[1];
     wic = wi_n;
+    R0 = R1;
+    R1 = R2;
   }
-  // back substitution and output stage (:234-237)
+  // back substitution and output stage (:234-237); the next level's loads
+  // are in flight while this level is stored
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
+    struct BL { double x, w2, f, fm; };
+    auto load_bl = [&](int k, BL &B) {
+      size_t p = G.a3(k, i, j);
+      B.x = outs[q][p];
+      B.w2 = k < nk - 1 ? w2s[q][p] : 0.;
+      B.f = fc[q][p];
+      B.fm = fctm1[q][p];
+    };
+    BL bc, bn;
+    load_bl(nk - 1, bc);
     double xn = w3[q];
     for (int k = nk - 1; k >= 0; --k) {
+      if (k > 0) load_bl(k - 1, bn);
+      asm volatile("" ::: "memory");
       size_t p = G.a3(k, i, j);
-      double x = k == nk - 1 ? xn : outs[q][p] - w2s[q][p] * xn;
+      double x = k == nk - 1 ? xn : bc.x - bc.w2 * xn;
       xn = x;
-      double nm = DTSMOOTH * fc[q][p] + DTSMOOTH_DIV2 * (fctm1[q][p] + x);
+      double nm = DTSMOOTH * bc.f + DTSMOOTH_DIV2 * (bc.fm + x);
       outs[q][p] = x > small ? x : small;
       outs_nm[q][p] = nm > small ? nm : small;
+      bc = bn;
     }
   }
 #undef F3
@@ -760,9 +866,52 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 
 // =============================================================================
 // comp_o2o_ng.F90:44-81 (fused) + comp_ng.F90:71-457
-struct CompTerms {  // prelude planes: advection of O2, O, He and smoothed *_nm
+struct CompTerms {  // prelude planes: advection of O2, O, He, smoothed *_nm, and
+                    // fs(:, isp, m) of comp_o2o_ng as fs01 fs02 fs11 fs12 fs00 fs10
   const double *o2adv, *o1adv, *headv, *o2s, *o1s, *hes;
+  const double *fs[6];
 };
+
+// fs(k, isp, m) of comp_o2o_ng.F90:44-81 at the levels comp_ng uses (k <
+// nk-1), nonzero planes only (the He row and column are zero), level-parallel.
+__global__ void __launch_bounds__(256) o2o_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                  CompTerms CT) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  long npt = (long)G.nlond * G.nlatd * (G.nk - 1);
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int k = (int)(r / G.nlatd);
+  const size_t p = G.a3(k, i, j);
+  double rji = 0.5 * (in.rj[p] + in.rj[p + G.P]);
+  double qo2pi = 0.5 * (in.qo2p[p] + in.qo2p[p + G.P]);
+  double qopi = 0.5 * (in.qop[p] + in.qop[p + G.P]);
+  double beta8i = 0.5 * (in.beta8[p] + in.beta8[p + G.P]);
+  double nei = sqrt(in.ne[p] * in.ne[p + G.P]);
+  double xn = in.xnmbar[p], no = in.no[p], n4s = in.n4s[p], n2d = in.n2d[p], o2p = in.o2p[p];
+  double op = in.op[p], nplus = in.nplus[p];
+  double pox1 = xn * xn * (in.beta3[p] * n4s / RMN4S * no / RMNO + BETA6 * n2d / RMN2D * no / RMNO) +
+                beta8i * no / RMNO * xn + xn * (RK4 * o2p * n4s / RMN4S + RK10 * op * n2d / RMN2D) +
+                (in.ra1[p] * in.nop[p] + 2. * in.ra2[p] * o2p) * nei;
+  double pox2 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + RK7 * nplus + 2. * rji;
+  double lox1 = 2. * in.rkm12[p] * xn / in.mbar[p];
+  double lox2 = in.rk3[p] * in.n2p[p] + RK8 * nplus;
+  double lox3 = qopi;
+  double po21 = in.rkm12[p] * xn / in.mbar[p];
+  double po23 = RK5 * no / RMNO * o2p * xn;
+  double lo21 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + (RK6 + RK7) * nplus +
+                RK9 * in.n2p[p] + rji;
+  double lo22 = qo2pi;
+  double o1 = in.o1[p];
+  ((double *)CT.fs[0])[p] = -lo21;                                   // fs(k,1,1)
+  ((double *)CT.fs[1])[p] = xn * po21 * o1 * (RIO1 * RIO1) * RMO2;   // fs(k,1,2)
+  ((double *)CT.fs[2])[p] = pox2 * RMO1 * RIO2;                      // fs(k,2,1)
+  ((double *)CT.fs[3])[p] = -lox2 - lox1 * o1 * RIO1 * xn;           // fs(k,2,2)
+  ((double *)CT.fs[4])[p] = (po23 - lo22) * RMO2 / xn;               // fs(k,1,0)
+  ((double *)CT.fs[5])[p] = (pox1 - lox3) * RMO1 / xn;               // fs(k,2,0)
+}
 
 __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                    tgx_ng_out out, double *__restrict__ gs,
@@ -795,27 +944,52 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     const double embar0 =
         1. / (ps0[0] * RIO2 + ps0[1] * RIO1 + ps0[2] * RIHE + (1. - ps0[0] - ps0[1] - ps0[2]) * RIN2);
     struct Lev { double o2i, o1i, hei, embari, tni, dembardz, dtndz, df[3]; };
-    auto lev = [&](int L, Lev &Q) {  // comp_ng.F90:80-97, :124-126
-      if (L == 0) {
-        Q.o2i = .5 * (ps0[0] + o2_0);
-        Q.o1i = .5 * (ps0[1] + o1_0);
-        Q.hei = .5 * (ps0[2] + he_0);
-        Q.embari = .5 * (embar0 + F3(in.mbar, 0));
-        Q.tni = tl;
-        Q.dembardz = F3(in.mbar, 0) - embar0;
-        Q.dtndz = (F3(in.tn, 0) - tl) * 2;
-      } else {
-        Q.o2i = .5 * (F3(in.o2, L - 1) + F3(in.o2, L));
-        Q.o1i = .5 * (F3(in.o1, L - 1) + F3(in.o1, L));
-        Q.hei = .5 * (F3(in.he, L - 1) + F3(in.he, L));
-        Q.embari = .5 * (F3(in.mbar, L - 1) + F3(in.mbar, L));
-        Q.tni = .5 * (F3(in.tn, L - 1) + F3(in.tn, L));
-        Q.dembardz = F3(in.mbar, L) - F3(in.mbar, L - 1);
-        Q.dtndz = F3(in.tn, L) - F3(in.tn, L - 1);
-      }
+    // Fields read at two neighbouring levels (interface means); one level of
+    // them is carried in registers.
+    struct LR { double o2, o1, he, mbar, tn, w; };
+    // Fields and level constants read at the level being eliminated only.
+    struct LS {
+      double fs[2][3];
+      double o2adv, o1adv, headv, o2s, o1s, hes, hdo2, hdo1, hdhe, expz, difk0, difk1;
+    };
+    auto load_lr = [&](int L, LR &R) {
+      size_t p = G.a3(L, i, j);
+      R.o2 = in.o2[p]; R.o1 = in.o1[p]; R.he = in.he[p]; R.mbar = in.mbar[p]; R.tn = in.tn[p];
+      R.w = in.w[p];
+    };
+    auto load_ls = [&](int L, LS &S) {
+      size_t p = G.a3(L, i, j);
+      S.fs[0][1] = CT.fs[0][p]; S.fs[0][2] = CT.fs[1][p]; S.fs[1][1] = CT.fs[2][p];
+      S.fs[1][2] = CT.fs[3][p]; S.fs[0][0] = CT.fs[4][p]; S.fs[1][0] = CT.fs[5][p];
+      S.o2adv = CT.o2adv[p]; S.o1adv = CT.o1adv[p]; S.headv = CT.headv[p];
+      S.o2s = CT.o2s[p]; S.o1s = CT.o1s[p]; S.hes = CT.hes[p];
+      S.hdo2 = in.hdo2[p]; S.hdo1 = in.hdo1[p]; S.hdhe = in.hdhe[p];
+      S.expz = g.expz[L]; S.difk0 = g.difk[L]; S.difk1 = g.difk[L + 1];
+    };
+    auto lev_df = [&](Lev &Q) {  // comp_ng.F90:124-126
       Q.dembardz = Q.dembardz / dz;
       Q.dtndz = Q.dtndz / dz;
       for (int n = 0; n < 3; ++n) Q.df[n] = pow(Q.tni / T00, 1.75 - ssn(n));
+    };
+    auto lev0 = [&](const LR &R, Lev &Q) {  // comp_ng.F90:80-97 at the bottom interface
+      Q.o2i = .5 * (ps0[0] + o2_0);
+      Q.o1i = .5 * (ps0[1] + o1_0);
+      Q.hei = .5 * (ps0[2] + he_0);
+      Q.embari = .5 * (embar0 + R.mbar);
+      Q.tni = tl;
+      Q.dembardz = R.mbar - embar0;
+      Q.dtndz = (R.tn - tl) * 2;
+      lev_df(Q);
+    };
+    auto levk = [&](const LR &A, const LR &B, Lev &Q) {  // interface between A (L-1) and B (L)
+      Q.o2i = .5 * (A.o2 + B.o2);
+      Q.o1i = .5 * (A.o1 + B.o1);
+      Q.hei = .5 * (A.he + B.he);
+      Q.embari = .5 * (A.mbar + B.mbar);
+      Q.tni = .5 * (A.tn + B.tn);
+      Q.dembardz = B.mbar - A.mbar;
+      Q.dtndz = B.tn - A.tn;
+      lev_df(Q);
     };
     auto alpha_of = [&](const Lev &Q, double *al) {  // :139-151
       const double *d_ = Q.df;
@@ -830,51 +1004,10 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       al[2] = (d_[0] * PHI(3, 1) - d_[2] * PHI(3, 4)) * Q.hei;
       al[5] = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Q.hei;
     };
-    auto wi_at = [&](int L) -> double {
-      if (L < nk - 1) return .5 * (F3(in.w, L) + F3(in.w, L + 1));
-      return 1.5 * F3(in.w, nk - 1) - .5 * F3(in.w, nk - 2);
-    };
-    // fs(L, isp, m) of comp_o2o_ng.F90:44-81 (nonzero planes only)
-    auto fs_at = [&](int L, double fs[2][3]) {
-      size_t p = G.a3(L, i, j);
-      double rji, qo2pi, qopi, beta8i, nei;
-      if (L < nk - 1) {
-        rji = 0.5 * (in.rj[p] + in.rj[p + G.P]);
-        qo2pi = 0.5 * (in.qo2p[p] + in.qo2p[p + G.P]);
-        qopi = 0.5 * (in.qop[p] + in.qop[p + G.P]);
-        beta8i = 0.5 * (in.beta8[p] + in.beta8[p + G.P]);
-        nei = sqrt(in.ne[p] * in.ne[p + G.P]);
-      } else {
-        rji = 1.5 * in.rj[p] - .5 * in.rj[p - G.P];
-        qo2pi = 1.5 * in.qo2p[p] - .5 * in.qo2p[p - G.P];
-        qopi = 1.5 * in.qop[p] - .5 * in.qop[p - G.P];
-        beta8i = 1.5 * in.beta8[p] - .5 * in.beta8[p - G.P];
-        nei = sqrt(in.ne[p] * in.ne[p] * in.ne[p] / in.ne[p - G.P]);
-      }
-      double xn = in.xnmbar[p], no = in.no[p], n4s = in.n4s[p], n2d = in.n2d[p], o2p = in.o2p[p];
-      double op = in.op[p], nplus = in.nplus[p];
-      double pox1 = xn * xn * (in.beta3[p] * n4s / RMN4S * no / RMNO + BETA6 * n2d / RMN2D * no / RMNO) +
-                    beta8i * no / RMNO * xn + xn * (RK4 * o2p * n4s / RMN4S + RK10 * op * n2d / RMN2D) +
-                    (in.ra1[p] * in.nop[p] + 2. * in.ra2[p] * o2p) * nei;
-      double pox2 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + RK7 * nplus + 2. * rji;
-      double lox1 = 2. * in.rkm12[p] * xn / in.mbar[p];
-      double lox2 = in.rk3[p] * in.n2p[p] + RK8 * nplus;
-      double lox3 = qopi;
-      double po21 = in.rkm12[p] * xn / in.mbar[p];
-      double po23 = RK5 * no / RMNO * o2p * xn;
-      double lo21 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + (RK6 + RK7) * nplus +
-                    RK9 * in.n2p[p] + rji;
-      double lo22 = qo2pi;
-      fs[0][1] = -lo21;
-      fs[0][2] = xn * po21 * in.o1[p] * (RIO1 * RIO1) * RMO2;
-      fs[1][1] = pox2 * RMO1 * RIO2;
-      fs[1][2] = -lox2 - lox1 * in.o1[p] * RIO1 * xn;
-      fs[0][0] = (po23 - lo22) * RMO2 / xn;
-      fs[1][0] = (pox1 - lox3) * RMO1 / xn;
-    };
-
+    LR r1, r2;
+    load_lr(0, r1);
     Lev Qc, Qn;
-    lev(0, Qc);
+    lev0(r1, Qc);
     double epm[3], epp[3], akm[9], akp[9], alpha[9];  // levels k (km) and k+1 (kp)
     double wks3, wks4 = Qc.dembardz / (Qc.embari * 2.);
     for (int l = 0; l < 3; ++l) epp[l] = 1. - 1. / Qc.embari * (rmass[l] + Qc.dembardz);
@@ -885,10 +1018,17 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
     double gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
     double alpha22 = 0, alpha33 = 0, alpha23 = 0, alpha32 = 0, flx00 = 0;
+    const size_t GP = (size_t)nk * G.P;
     for (int k = 0; k < nk - 1; ++k) {
+      // every load of this level is issued before any of its arithmetic, so
+      // the level costs one memory round trip
+      LS ls;
+      load_lr(k + 1, r2);
+      load_ls(k, ls);
+      asm volatile("" ::: "memory");
       for (int n = 0; n < 9; ++n) akm[n] = akp[n];
       for (int n = 0; n < 3; ++n) epm[n] = epp[n];
-      lev(k + 1, Qn);
+      levk(r1, r2, Qn);
       for (int l = 0; l < 3; ++l) epp[l] = 1. - 1. / Qn.embari * (rmass[l] + Qn.dembardz);
       if (k != nk - 2) epp[2] = epp[2] - thdiffalpha * Qn.dtndz / Qn.tni;
       alpha_of(Qn, alpha);
@@ -897,7 +1037,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       wks3 = wks4;
       wks4 = Qn.dembardz / (Qn.embari * 2.);
       for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
-      double molp[9], molr[9], molq[9], pk[9], rk[9], qk[9], fk[3], fs[2][3];
+      double molp[9], molr[9], molq[9], pk[9], rk[9], qk[9], fk[3];
       for (int isp = 0; isp < 3; ++isp)
         for (int m = 0; m < 3; ++m) {
           int e = isp + 3 * m;
@@ -905,25 +1045,24 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
           molr[e] = akp[e] * (1. / dz - epp[m] / 2.);
           molq[e] = akm[e] * (1. / dz - epm[m] / 2.) + akp[e] * (1. / dz + epp[m] / 2.);
         }
-      double eddyp = g.expzmid_inv * g.difk[k] * (1. / dz - wks3);
-      double eddyr = g.expzmid * g.difk[k + 1] * (1. / dz + wks4);
-      double eddyq = g.expzmid_inv * g.difk[k] * (1. / dz + wks3) + g.expzmid * g.difk[k + 1] * (1. / dz - wks4);
-      fs_at(k, fs);
-      const double wik = wi_at(k);
+      double eddyp = g.expzmid_inv * ls.difk0 * (1. / dz - wks3);
+      double eddyr = g.expzmid * ls.difk1 * (1. / dz + wks4);
+      double eddyq = g.expzmid_inv * ls.difk0 * (1. / dz + wks3) + g.expzmid * ls.difk1 * (1. / dz - wks4);
+      const double(&fs)[2][3] = ls.fs;
+      const double expzk = ls.expz;
+      const double wik = .5 * (r1.w + r2.w);
       for (int m = 0; m < 3; ++m)
         for (int isp = 0; isp < 3; ++isp) {
           int e = isp + 3 * m;
           double delta = isp == m ? 1. : 0.;
           double fsv = (isp < 2 && m < 2) ? fs[isp][m + 1] : 0.;
-          pk[e] = molp[e] / dz - g.expz[k] * delta / dz * (eddyp * dfac + .5 * wik);
-          rk[e] = molr[e] / dz - g.expz[k] * delta / dz * (eddyr * dfac - .5 * wik);
-          qk[e] = -molq[e] / dz + g.expz[k] * delta * (eddyq * dfac / dz + dtx2inv) - g.expz[k] * fsv;
+          pk[e] = molp[e] / dz - expzk * delta / dz * (eddyp * dfac + .5 * wik);
+          rk[e] = molr[e] / dz - expzk * delta / dz * (eddyr * dfac - .5 * wik);
+          qk[e] = -molq[e] / dz + expzk * delta * (eddyq * dfac / dz + dtx2inv) - expzk * fsv;
         }
-      double o2adv = F3(CT.o2adv, k), o1adv = F3(CT.o1adv, k), headv = F3(CT.headv, k);
-      double o2s = F3(CT.o2s, k), o1s = F3(CT.o1s, k), hes = F3(CT.hes, k);
-      fk[0] = g.expz[k] * (o2s * dtx2inv - (o2adv - fs[0][0]) + F3(in.hdo2, k));
-      fk[1] = g.expz[k] * (o1s * dtx2inv - (o1adv - fs[1][0]) + F3(in.hdo1, k));
-      fk[2] = g.expz[k] * (hes * dtx2inv - (headv - 0.) + F3(in.hdhe, k));
+      fk[0] = expzk * (ls.o2s * dtx2inv - (ls.o2adv - fs[0][0]) + ls.hdo2);
+      fk[1] = expzk * (ls.o1s * dtx2inv - (ls.o1adv - fs[1][0]) + ls.hdo1);
+      fk[2] = expzk * (ls.hes * dtx2inv - (ls.headv - 0.) + ls.hdhe);
       if (k == 0) {
         for (int m = 0; m < 3; ++m)
           for (int n = 0; n < 3; ++n)
@@ -941,7 +1080,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
                     (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Qn.o1i);
         alpha23 = (d_[1] * PHI(2, 3) - PHI(2, 4)) * Qn.o1i;
         alpha32 = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Qn.hei;
-        flx00 = Qc.embari * RIN2 * P0 * pow(T00 / F3(in.tn, k), 0.25) / (TAU * GRAV);
+        flx00 = Qc.embari * RIN2 * P0 * pow(T00 / r1.tn, 0.25) / (TAU * GRAV);
         for (int isp = 0; isp < 3; ++isp)
           fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * epp[1])) -
                     rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * epp[2]));
@@ -961,26 +1100,38 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       for (int m = 0; m < 3; ++m)
         for (int isp = 0; isp < 3; ++isp) nz[isp] = nz[isp] + wkm1[isp + 3 * m] * wkv1[m];
       for (int n = 0; n < 9; ++n) gama[n] = ng[n];
-      for (int n = 0; n < 9; ++n) gs[(size_t)n * ((size_t)nk * G.P) + G.a3(k, i, j)] = ng[n];
+      for (int n = 0; n < 9; ++n) gs[(size_t)n * GP + G.a3(k, i, j)] = ng[n];
       for (int isp = 0; isp < 3; ++isp) zz[isp] = nz[isp];
       out.o2[G.a3(k, i, j)] = nz[0];
       out.o1[G.a3(k, i, j)] = nz[1];
       out.he[G.a3(k, i, j)] = nz[2];
       Qc = Qn;
+      r1 = r2;
     }
-    // :317-332 back substitution and the closed-form top level
+    // :317-332 back substitution (the next level's loads are in flight while
+    // this level is computed and stored) and the closed-form top level
     double u1[3] = {0., 0., 0.};
-    for (int k = nk - 2; k >= 0; --k) {
+    struct BS { double z[3], gm[9]; };
+    auto load_bs = [&](int k, BS &B) {
       size_t p = G.a3(k, i, j);
-      double z0 = out.o2[p], z1 = out.o1[p], z2 = out.he[p], u[3];
-      double zr[3] = {z0, z1, z2};
+      B.z[0] = out.o2[p]; B.z[1] = out.o1[p]; B.z[2] = out.he[p];
+      for (int n = 0; n < 9; ++n) B.gm[n] = gs[(size_t)n * GP + p];
+    };
+    BS bc, bn;
+    load_bs(nk - 2, bc);
+    for (int k = nk - 2; k >= 0; --k) {
+      if (k > 0) load_bs(k - 1, bn);
+      asm volatile("" ::: "memory");
+      size_t p = G.a3(k, i, j);
+      double u[3];
       for (int isp = 0; isp < 3; ++isp) {
-        double uu = zr[isp];
-        for (int m = 0; m < 3; ++m) uu = uu - gs[(size_t)(isp + 3 * m) * ((size_t)nk * G.P) + p] * u1[m];
+        double uu = bc.z[isp];
+        for (int m = 0; m < 3; ++m) uu = uu - bc.gm[isp + 3 * m] * u1[m];
         u[isp] = uu;
       }
       out.o2[p] = u[0]; out.o1[p] = u[1]; out.he[p] = u[2];
       for (int isp = 0; isp < 3; ++isp) u1[isp] = u[isp];
+      bc = bn;
     }
     double top[3];
     for (int isp = 0; isp < 3; ++isp)
@@ -992,15 +1143,28 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     if (g.calc_helium == 0)
       for (int k = 0; k < nk; ++k) out.he[G.a3(k, i, j)] = g.pshelb;
   }
-  // :415-457 time smoothing, N2 above the O-dominated top, clamps, normalisation
+  // :415-457 time smoothing, N2 above the O-dominated top, clamps,
+  // normalisation; the next level's loads are issued before this level's stores
   bool found = false, extrap = false;
-  double n2top = 0., dn2dz = 0., zmtop = 0., n2prev = 0.;
-  for (int k = 0; k < nk; ++k) {
+  double n2top = 0., dn2dz = 0., zmtop = 0., n2prev = 0., xnprev = 0.;
+  struct PS { double o2u, o1u, heu, o2nm, o1nm, henm, o2, o1, he, xn; };
+  auto load_ps = [&](int k, PS &Q) {
     size_t p = G.a3(k, i, j);
-    double o2u = out.o2[p], o1u = out.o1[p], heu = out.he[p];
-    double o2nm = DTSMOOTH_DIV2 * (in.o2_nm[p] + o2u) + DTSMOOTH * in.o2[p];
-    double o1nm = DTSMOOTH_DIV2 * (in.o1_nm[p] + o1u) + DTSMOOTH * in.o1[p];
-    double henm = DTSMOOTH_DIV2 * (in.he_nm[p] + heu) + DTSMOOTH * in.he[p];
+    Q.o2u = out.o2[p]; Q.o1u = out.o1[p]; Q.heu = out.he[p];
+    Q.o2nm = in.o2_nm[p]; Q.o1nm = in.o1_nm[p]; Q.henm = in.he_nm[p];
+    Q.o2 = in.o2[p]; Q.o1 = in.o1[p]; Q.he = in.he[p];
+    Q.xn = in.xnmbar[p];
+  };
+  PS pc, pn;
+  load_ps(0, pc);
+  for (int k = 0; k < nk; ++k) {
+    if (k + 1 < nk) load_ps(k + 1, pn);
+    asm volatile("" ::: "memory");
+    size_t p = G.a3(k, i, j);
+    double o2u = pc.o2u, o1u = pc.o1u, heu = pc.heu;
+    double o2nm = DTSMOOTH_DIV2 * (pc.o2nm + o2u) + DTSMOOTH * pc.o2;
+    double o1nm = DTSMOOTH_DIV2 * (pc.o1nm + o1u) + DTSMOOTH * pc.o1;
+    double henm = DTSMOOTH_DIV2 * (pc.henm + heu) + DTSMOOTH * pc.he;
     double n2 = 1. - o2u - o1u - heu;
     double n2nm = 1. - o2nm - o1nm - henm;
     if (!found) {
@@ -1009,15 +1173,16 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
         if (k + 1 < nk && k >= 1) {
           extrap = true;
           zmtop = g.zpmid[k];
-          n2top = log(n2 * in.xnmbar[p]);
-          double n2top1 = log(n2prev * in.xnmbar[p - G.P]);
+          n2top = log(n2 * pc.xn);
+          double n2top1 = log(n2prev * xnprev);
           dn2dz = (n2top - n2top1) / dz;
         }
       }
     } else if (extrap) {
-      n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / in.xnmbar[p];
+      n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / pc.xn;
     }
     n2prev = n2;
+    xnprev = pc.xn;
     double a = o2u > small ? o2u : small, b = o1u > small ? o1u : small;
     double c = heu > small ? heu : small, d = n2 > small ? n2 : small;
     double norm = a + b + c + d;
@@ -1030,6 +1195,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     out.o2_nm[p] = an / normn;
     out.o1_nm[p] = bn / normn;
     out.he_nm[p] = cn / normn;
+    pc = pn;
   }
 #undef F3
 }
@@ -1111,42 +1277,62 @@ __global__ void __launch_bounds__(256) oplus_stage2(tgx_ng_grid g, Geo G, tgx_ng
 }
 
 // stage 3: assembly, solve and outputs of one sub-step (:172-340)
+// O+ chemistry of oplus_ng.F90:288-299 at every column and level: the
+// xiop2p/xiop2d outputs, the loss and the production.  None of it depends on
+// O+, so it is computed once per step for all sub-cycles (level-parallel).
+__global__ void __launch_bounds__(256) oplus_chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                         double *__restrict__ oploss_o,
+                                                         double *__restrict__ opprod_o,
+                                                         double *__restrict__ xiop2p,
+                                                         double *__restrict__ xiop2d) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  long npt = (long)G.nlond * G.nlatd * G.nk;
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int L = (int)(r / G.nlatd);
+  const int nk = G.nk;
+  size_t p = G.a3(L, i, j);
+  double xn = in.xnmbar[p], ne = in.ne[p];
+  double qop2pi = L < nk - 1 ? .5 * (in.qop2p[p] + in.qop2p[p + G.P]) : 1.5 * in.qop2p[p] - .5 * in.qop2p[p - G.P];
+  double qop2di = L < nk - 1 ? .5 * (in.qop2d[p] + in.qop2d[p + G.P]) : 1.5 * in.qop2d[p] - .5 * in.qop2d[p - G.P];
+  double x2p = qop2pi / (xn * ((RK16 + RK17) * in.n2[p] * RIN2 + RK18 * in.o1[p] * RIO1) +
+                         (in.rk19[p] + in.rk20[p]) * ne + RK21 + RK22);
+  double x2d = (qop2di + (in.rk20[p] * ne + RK22) * x2p) /
+               (xn * (RK23 * in.n2[p] * RIN2 + RK24 * in.o1[p] * RIO1 + RK26 * in.o2[p] * RIO2) + in.rk25[p] * ne + RK27);
+  double oploss = xn * (in.rk1[p] * in.o2[p] * RIO2 + in.rk2[p] * in.n2[p] * RIN2 + RK10 * in.n2d[p] * RIN2D);
+  xiop2p[p] = x2p;
+  xiop2d[p] = x2d;
+  oploss_o[p] = oploss;
+  if (L < nk - 1) {  // the production enters rows 1..nk-1 only
+    double qopi = .5 * (in.qop[p] + in.qop[p + G.P]);
+    opprod_o[p] = qopi + (in.rk19[p] * ne + RK21) * x2p + (in.rk25[p] * ne + RK27) * x2d +
+                  (RK18 * x2p + RK24 * x2d) * in.o1[p] * RIO1 * xn;
+  }
+}
+
 __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     const double *__restrict__ op,
                                                     const double *__restrict__ optm1,
                                                     const double *__restrict__ lonb,
                                                     const double *__restrict__ latb, OpWork W,
                                                     OpTerms OT,
+                                                    const double *__restrict__ oploss_p,
+                                                    const double *__restrict__ opprod_p,
                                                     double *__restrict__ opout,
-                                                    double *__restrict__ optm1out,
-                                                    double *__restrict__ xiop2p,
-                                                    double *__restrict__ xiop2d) {
+                                                    double *__restrict__ optm1out) {
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk, nsub = g.nstep_sub;
   const double dz = g.dz, dtx2inv = g.dtx2inv;
   const size_t c2 = G.a2(i, j);
 #define F3(f, k) (f)[G.a3((k), i, j)]
-  // xiop2p, xiop2d and the O+ loss (:288-293), every column and level
-  auto chem = [&](int L, double &x2p, double &x2d, double &oploss) {
-    size_t p = G.a3(L, i, j);
-    double xn = in.xnmbar[p], ne = in.ne[p];
-    double qop2pi = L < nk - 1 ? .5 * (in.qop2p[p] + in.qop2p[p + G.P]) : 1.5 * in.qop2p[p] - .5 * in.qop2p[p - G.P];
-    double qop2di = L < nk - 1 ? .5 * (in.qop2d[p] + in.qop2d[p + G.P]) : 1.5 * in.qop2d[p] - .5 * in.qop2d[p - G.P];
-    x2p = qop2pi / (xn * ((RK16 + RK17) * in.n2[p] * RIN2 + RK18 * in.o1[p] * RIO1) +
-                    (in.rk19[p] + in.rk20[p]) * ne + RK21 + RK22);
-    x2d = (qop2di + (in.rk20[p] * ne + RK22) * x2p) /
-          (xn * (RK23 * in.n2[p] * RIN2 + RK24 * in.o1[p] * RIO1 + RK26 * in.o2[p] * RIO2) + in.rk25[p] * ne + RK27);
-    oploss = xn * (in.rk1[p] * in.o2[p] * RIO2 + in.rk2[p] * in.n2[p] * RIN2 + RK10 * in.n2d[p] * RIN2D);
-  };
   double v;
   if (lateral_value(G, lonb, latb, 0, i, j, v)) {
     for (int k = 0; k < nk; ++k) {
       size_t p = G.a3(k, i, j);
-      double x2p, x2d, ol;
-      chem(k, x2p, x2d, ol);
-      xiop2p[p] = x2p;
-      xiop2d[p] = x2d;
       lateral_value(G, lonb, latb, k, i, j, v);
       double nm = DTSMOOTH * op[p] + DTSMOOTH_DIV2 * (optm1[p] + v);
       opout[p] = v > 0. ? v : 0.;
@@ -1155,7 +1341,7 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
     return;
   }
   const double pi = 4.0 * atan(1.0), rtd = 180. / pi;
-  const double bx = in.bx[c2], by = in.by[c2], bz = in.bz[c2], bm = in.bmod2[c2], cs = g.cs[j];
+  const double bz = in.bz[c2], cs = g.cs[j];
   double opflux;
   {  // :77-91
     const double phid = 2.0e8, phin = -2.0e8, ppolar = 0.;
@@ -1184,103 +1370,89 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
   }
   const double dvbz = dvb / bz;
   const double gmr = GRAV * RMOP / (2. * GASK);
-  auto hdz_at = [&](int L) { return 1. / (F3(in.scht, L) * dz); };
-  auto tp1_at = [&](int L) { return 0.5 * (F3(in.ti, L) + F3(in.te, L)); };
-  auto hdzi_at = [&](int L) {
-    if (L == 0) return 1.5 * hdz_at(0) - 0.5 * hdz_at(1);
-    if (L == nk - 1) return 1.5 * hdz_at(nk - 2) - 0.5 * hdz_at(nk - 3);
-    return 0.5 * (hdz_at(L - 1) + hdz_at(L));
+  // Fields used at up to three neighbouring levels, in a window A = k-1,
+  // B = k, C = k+1 carried in registers; each level's loads are issued
+  // before its arithmetic.  hdz = 1/(scht dz), tp1 = (ti+te)/2 (:163-178).
+  struct OL { double scht, hdz, tp1, dj, bdotu, bdopj, wi; };
+  struct OS { double bd_diff, bd_diffj, djbz, vdotn, bd_bvel, opts2, bd_djbz, oploss, opprod; };
+  auto load_ol = [&](int L, OL &Q) {
+    size_t p = G.a3(L, i, j);
+    Q.scht = in.scht[p];
+    Q.hdz = 1. / (Q.scht * dz);
+    Q.tp1 = 0.5 * (in.ti[p] + in.te[p]);
+    Q.dj = W.dj[p]; Q.bdotu = W.bdotu[p]; Q.bdopj = W.bdopj[p]; Q.wi = in.wi[p];
   };
-  auto tphdz1_at = [&](int L) {
-    if (L == nk - 1) return 2. * (2. * tp1_at(nk - 2) - tp1_at(nk - 3)) * hdzi_at(nk - 1) + gmr;
-    return 2. * tp1_at(L) * hdzi_at(L) + gmr;
+  auto load_os = [&](int L, OS &S) {
+    size_t p = G.a3(L, i, j);
+    S.bd_diff = OT.bd_diff[p]; S.bd_diffj = OT.bd_diffj[p]; S.djbz = W.djbz[p]; S.vdotn = OT.vdotn[p];
+    S.bd_bvel = OT.bd_bvel[p]; S.opts2 = OT.opts2[p]; S.bd_djbz = OT.bd_djbz[p];
+    S.oploss = oploss_p[p]; S.opprod = opprod_p[p];
   };
-  auto tphdz0_at = [&](int L) {
-    if (L == 0) return 2. * (2. * tp1_at(0) - tp1_at(1)) * hdzi_at(0) - gmr;
-    return 2. * tp1_at(L - 1) * hdzi_at(L) - gmr;
-  };
-  auto djint_at = [&](int L) {
-    if (L == 0) return 1.5 * F3(W.dj, 0) - 0.5 * F3(W.dj, 1);
-    if (L == nk - 1) return 1.5 * F3(W.dj, nk - 2) - 0.5 * F3(W.dj, nk - 3);
-    return 0.5 * (F3(W.dj, L - 1) + F3(W.dj, L));
-  };
-  auto hdzmp_at = [&](int L, double &hm, double &hp) {  // :239-248
-    double bd_djbz = F3(OT.bd_djbz, L);
-    double divbz = dvbz + bd_djbz / (F3(W.dj, L) * (bz * bz));
-    double h = hdz_at(L);
-    hm = h - 0.5 * divbz;
-    hp = h + 0.5 * divbz;
-    hm = hm * (bz * bz);
-    hp = hp * (bz * bz);
-  };
-  const double ss2 = g.shapiro / nsub;
+  OL qa = {0, 0, 0, 0, 0, 0, 0}, qb, qc;
+  load_ol(0, qb);
+  load_ol(1, qc);
+  const double ol1 = F3(oploss_p, 1), qop0 = F3(in.qop, 0);  // lower boundary (:301-303)
   double w1 = 0., w3 = 0., rprev = 0.;
-  double ol_next_cache = 0.;
   for (int k = 0; k <= nk - 2; ++k) {
-    size_t p = G.a3(k, i, j), off = (size_t)k * G.P;
+    size_t p = G.a3(k, i, j);
+    OS os;
+    if (k > 0) load_ol(k + 1, qc);
+    load_os(k, os);
+    asm volatile("" ::: "memory");
+    const bool top = k == nk - 2;
     // explicit terms (:181-211, :250)
     double dbd;
-    if (k == 0) dbd = F3(W.bdopj, 1) - F3(W.bdopj, 0);
-    else if (k <= nk - 3) dbd = (F3(W.bdopj, k + 1) - F3(W.bdopj, k - 1)) / 2.;
-    else dbd = F3(W.bdopj, nk - 2) - F3(W.bdopj, nk - 3);
-    double bdzdvb = bz / (in.scht[p] * dz) * dbd + dvb * W.bdopj[p];
-    double bd_diff = OT.bd_diff[p];
-    double bd_op = W.djbz[p] * OT.bd_diffj[p];
-    double diffexp = bdzdvb + bd_diff + bd_op;
+    if (k == 0) dbd = qc.bdopj - qb.bdopj;
+    else if (k <= nk - 3) dbd = (qc.bdopj - qa.bdopj) / 2.;
+    else dbd = qb.bdopj - qa.bdopj;
+    double bdzdvb = bz / (qb.scht * dz) * dbd + dvb * qb.bdopj;
+    double bd_op = os.djbz * os.bd_diffj;
+    double diffexp = bdzdvb + os.bd_diff + bd_op;
     double expl = -1. * diffexp;
-    double vh = OT.vdotn[p];
-    double bd_bvel = OT.bd_bvel[p];
-    expl = expl + vh + bd_bvel;
-    double opts2 = OT.opts2[p];
-    expl = expl - opts2 * dtx2inv * nsub;
-    // diffusion, drift and wind coefficients (:252-278)
-    double hm, hp;
-    hdzmp_at(k, hm, hp);
-    double dji = djint_at(k), dji1 = djint_at(k + 1);
-    double t0 = tphdz0_at(k), t01 = tphdz0_at(k + 1), t1 = tphdz1_at(k), t11 = tphdz1_at(k + 1);
+    expl = expl + os.vdotn + os.bd_bvel;
+    expl = expl - os.opts2 * dtx2inv * nsub;
+    // diffusion, drift and wind coefficients (:239-278)
+    double divbz = dvbz + os.bd_djbz / (qb.dj * (bz * bz));
+    double hm = qb.hdz - 0.5 * divbz, hp = qb.hdz + 0.5 * divbz;
+    hm = hm * (bz * bz);
+    hp = hp * (bz * bz);
+    double hdzi_k = k == 0 ? 1.5 * qb.hdz - 0.5 * qc.hdz : 0.5 * (qa.hdz + qb.hdz);
+    double hdzi_k1 = top ? 1.5 * qb.hdz - 0.5 * qa.hdz : 0.5 * (qb.hdz + qc.hdz);
+    double dji = k == 0 ? 1.5 * qb.dj - 0.5 * qc.dj : 0.5 * (qa.dj + qb.dj);
+    double dji1 = top ? 1.5 * qb.dj - 0.5 * qa.dj : 0.5 * (qb.dj + qc.dj);
+    double t0 = k == 0 ? 2. * (2. * qb.tp1 - qc.tp1) * hdzi_k - gmr : 2. * qa.tp1 * hdzi_k - gmr;
+    double t01 = 2. * qb.tp1 * hdzi_k1 - gmr;
+    double t1 = 2. * qb.tp1 * hdzi_k + gmr;
+    double t11 = top ? 2. * (2. * qb.tp1 - qa.tp1) * hdzi_k1 + gmr : 2. * qc.tp1 * hdzi_k1 + gmr;
     double diffp = hm * dji * t0;
     double diffq = -(hp * dji1 * t01 + hm * dji * t1);
     double diffr = hp * dji1 * t11;
-    double hdzk = hdz_at(k);
-    double wii = .5 * (in.wi[p] + in.wi[p + G.P]);
+    double hdzk = qb.hdz;
+    double wii = .5 * (qb.wi + qc.wi);
     double driftp = wii * 0.5 * hdzk;
     double driftq = -wii * 6. / RE;
     double driftr = -wii * 0.5 * hdzk;
-    double bu = W.bdotu[p];
-    double windp = k == 0 ? bz * (2. * F3(W.bdotu, 0) - F3(W.bdotu, 1)) * 0.5 * hdzk
-                          : bz * F3(W.bdotu, k - 1) * 0.5 * hdzk;
-    double windq = -bu * dvbz * bz;
-    double windr = k < nk - 2 ? -bz * F3(W.bdotu, k + 1) * 0.5 * hdzk
-                              : bz * (2. * F3(W.bdotu, nk - 2) - F3(W.bdotu, nk - 3)) * 0.5 * hdzk;
+    double windp = k == 0 ? bz * (2. * qb.bdotu - qc.bdotu) * 0.5 * hdzk : bz * qa.bdotu * 0.5 * hdzk;
+    double windq = -qb.bdotu * dvbz * bz;
+    double windr = !top ? -bz * qc.bdotu * 0.5 * hdzk : bz * (2. * qb.bdotu - qa.bdotu) * 0.5 * hdzk;
     double pcf = diffp + driftp + windp;
     double qcf = diffq + driftq + windq - dtx2inv * nsub;
     double rcf = diffr + driftr + windr;
-    if (k == nk - 2) {  // upper boundary (:280-286)
-      double djn = djint_at(nk - 1);
-      double ubcb = -(bz * bz) * djn * tphdz0_at(nk - 1);
-      double ubca = -(bz * bz) * djn * tphdz1_at(nk - 1);
+    if (top) {  // upper boundary (:280-286)
+      double djn = dji1;
+      double ubcb = -(bz * bz) * djn * t01;
+      double ubca = -(bz * bz) * djn * t11;
       qcf = qcf + ubcb / ubca * rcf;
       expl = expl - opflux * rcf / ubca;
       rcf = 0.;
     }
-    double x2p, x2d, oploss;
-    chem(k, x2p, x2d, oploss);
-    xiop2p[p] = x2p;
-    xiop2d[p] = x2d;
-    qcf = qcf - oploss;
-    double qopi = .5 * (in.qop[p] + in.qop[p + G.P]);
-    double ne = in.ne[p], xn = in.xnmbar[p];
-    double op_prod = qopi + (in.rk19[p] * ne + RK21) * x2p + (in.rk25[p] * ne + RK27) * x2d +
-                     (RK18 * x2p + RK24 * x2d) * in.o1[p] * RIO1 * xn;
-    expl = expl - op_prod;
+    qcf = qcf - os.oploss;
+    expl = expl - os.opprod;
     if (k == 0) {  // lower boundary (:301-303)
-      double a1, a2, ol1;
-      chem(1, a1, a2, ol1);
       qcf = qcf - pcf;
-      expl = expl - 2. * pcf * in.qop[p] / (1.5 * oploss - 0.5 * ol1);
+      expl = expl - 2. * pcf * qop0 / (1.5 * os.oploss - 0.5 * ol1);
       pcf = 0.;
     }
-    (void)ol_next_cache;
     // Thomas forward (trsolv_ng, rows 1..nk-1)
     if (k == 0) {
       w1 = qcf;
@@ -1293,26 +1465,37 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
     }
     opout[p] = w3;
     rprev = rcf;
+    qa = qb;
+    qb = qc;
   }
-  {  // xiop2p/xiop2d at the top level
-    double x2p, x2d, ol;
-    chem(nk - 1, x2p, x2d, ol);
-    xiop2p[G.a3(nk - 1, i, j)] = x2p;
-    xiop2d[G.a3(nk - 1, i, j)] = x2d;
-  }
-  // back substitution, top extrapolation (:307) and outputs (:337-340)
-  auto emit = [&](int k, double x) {
+  // back substitution, top extrapolation (:307) and outputs (:337-340); the
+  // next level's loads are in flight while this level is stored
+  struct OB { double x, w2, op, optm1; };
+  auto load_ob = [&](int k, OB &B) {
     size_t p = G.a3(k, i, j);
-    double nm = DTSMOOTH * op[p] + DTSMOOTH_DIV2 * (optm1[p] + x);
+    B.x = opout[p]; B.w2 = k < nk - 2 ? W.w2[p] : 0.; B.op = op[p]; B.optm1 = optm1[p];
+  };
+  const double op_top = F3(op, nk - 1), optm1_top = F3(optm1, nk - 1);
+  double xn = w3;
+  OB bc, bn;
+  load_ob(nk - 2, bc);
+  for (int k = nk - 2; k >= 0; --k) {
+    if (k > 0) load_ob(k - 1, bn);
+    asm volatile("" ::: "memory");
+    double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
+    if (k == nk - 3) {
+      double xt = xn * xn / x;
+      size_t pt = G.a3(nk - 1, i, j);
+      double nm = DTSMOOTH * op_top + DTSMOOTH_DIV2 * (optm1_top + xt);
+      opout[pt] = xt > 0. ? xt : 0.;
+      optm1out[pt] = nm > 0. ? nm : 0.;
+    }
+    size_t p = G.a3(k, i, j);
+    double nm = DTSMOOTH * bc.op + DTSMOOTH_DIV2 * (bc.optm1 + x);
     opout[p] = x > 0. ? x : 0.;
     optm1out[p] = nm > 0. ? nm : 0.;
-  };
-  double xn = w3;
-  for (int k = nk - 2; k >= 0; --k) {
-    double x = k == nk - 2 ? xn : opout[G.a3(k, i, j)] - W.w2[G.a3(k, i, j)] * xn;
-    if (k == nk - 3) emit(nk - 1, xn * xn / x);
-    emit(k, x);
     xn = x;
+    bc = bn;
   }
 #undef F3
 }
@@ -1370,74 +1553,83 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
   }
 #define F3(f, k) (f)[G.a3((k), i, j)]
   const double racs = g.racs[j], cor = g.cor[j], tanphi = g.tanphi[j];
-  auto zl_at = [&](int L) {
-    double zl = F3(UT.divx_ztmp, L);
-    if (L < nk - 1) zl = (zl + F3(UT.divx_ztmp, L + 1)) * .5;
-    zl = zl * racs;
-    return zl * GRAV / g.dlamda;
-  };
-  auto zp_at = [&](int L) {
-    double zp = F3(UT.divy_z, L);
-    if (L < nk - 1) zp = (zp + F3(UT.divy_z, L + 1)) * .5;
-    return zp * GRAV * RE_INV / g.dphi;
-  };
-  auto iavg = [&](const double *f, int L) {
+  // Fields averaged to interfaces (iavg) or used at two levels, carried one
+  // level in registers; each level's loads are issued before its arithmetic.
+  struct UL { double w, lxx, lyy, lxy, lyx, ui, vi, divx, divy, km, schti, scht; };
+  struct US { double Fe, Fn, unms, vnms, advu, advv, hdu, hdv, un, expz; };
+  auto load_ul = [&](int L, UL &U) {
     size_t p = G.a3(L, i, j);
-    return L < nk - 1 ? .5 * (f[p] + f[p + G.P]) : 1.5 * f[p] - .5 * f[p - G.P];
+    U.w = in.w_upd[p]; U.lxx = in.lxx[p]; U.lyy = in.lyy[p]; U.lxy = in.lxy[p]; U.lyx = in.lyx[p];
+    U.ui = in.ui[p]; U.vi = in.vi[p]; U.divx = UT.divx_ztmp[p]; U.divy = UT.divy_z[p];
+    U.km = in.km[p]; U.schti = in.schti[p]; U.scht = in.scht[p];
   };
-  // eddy viscosity (:143-152)
-  auto e1b = [&](int L) -> double {
-    double e = P0 * F3(in.scht, L) / GRAV;
+  auto load_us = [&](int L, US &S) {
+    size_t p = G.a3(L, i, j);
+    S.Fe = out.Fe[p]; S.Fn = out.Fn[p]; S.unms = UT.unms[p]; S.vnms = UT.vnms[p];
+    S.advu = UT.advu[p]; S.advv = UT.advv[p]; S.hdu = in.hdu[p]; S.hdv = in.hdv[p]; S.un = in.un[p];
+    S.expz = g.expz[L];
+  };
+  // eddy viscosity (:143-152): e1b(L) from the level-L scale height
+  auto e1b = [&](double scht, int L) -> double {
+    double e = P0 * scht / GRAV;
     return e * g.expz[L] * g.xmue[L];
   };
-  auto e1 = [&](int L) -> double {
-    if (L <= nk - 2) return e1b(L);
-    double a = e1b(nk - 2), b = e1b(nk - 3);
-    return sqrt(a * a * a / b);
-  };
-  auto evf = [&](int L) -> double {
-    if (L >= 1) return sqrt(e1(L - 1) * e1(L));
-    double e0 = e1(0);
-    return e0 * e0 / sqrt(e1(0) * e1(1));
-  };
-  auto gg_at = [&](int L) { return GRAV * (F3(in.km, L) + evf(L)) / (P0 * F3(in.schti, L) * (dz * dz)); };
+  auto gg = [&](const UL &U, double evf) { return GRAV * (U.km + evf) / (P0 * U.schti * (dz * dz)); };
   double ulbc = in.u_lbc[c2], vlbc = in.v_lbc[c2];
+  UL ua, ub;
+  load_ul(0, ua);
+  load_ul(1, ub);
+  double e1_m = 0., e1_c = e1b(ua.scht, 0);            // e1 at levels k-1 and k
+  const double e1_1 = e1b(ub.scht, 1);                  // nk >= 4: level 1 is interior
+  double gk = gg(ua, e1_c * e1_c / sqrt(e1_c * e1_1));  // evf(0)
   // block Thomas state (duv_ng.F90:218-243): beta(k-1), yy(k-1), rr(k-1)
   double be11 = 0, be12 = 0, be21 = 0, be22 = 0, y1 = 0, y2 = 0;
   double r11p = 0, r12p = 0, r21p = 0, r22p = 0;
-  double gk = gg_at(0);
+  const size_t NP = (size_t)nk * G.P;
   for (int k = 0; k <= nk - 2; ++k) {
-    size_t p = G.a3(k, i, j), off = (size_t)k * G.P;
-    double gk1 = gg_at(k + 1);
+    size_t p = G.a3(k, i, j);
+    US us;
+    if (k > 0) load_ul(k + 1, ub);
+    load_us(k, us);
+    asm volatile("" ::: "memory");
+    // e1(k+1): interior levels directly, the top one extrapolated (:148-150)
+    double e1_n = k + 1 <= nk - 2 ? e1b(ub.scht, k + 1) : sqrt(e1_c * e1_c * e1_c / e1_m);
+    double gk1 = gg(ub, sqrt(e1_c * e1_n));
     double g_1 = gk1;  // k <= nk-2
-    double wi = iavg(in.w_upd, k);
+    const double expzk = us.expz;
+    double wi = .5 * (ua.w + ub.w);
     double dwdz = wi / (2. * dz);
-    dwdz = dwdz * g.expz[k];
-    double lxxi = iavg(in.lxx, k), lyyi = iavg(in.lyy, k), lxyi = iavg(in.lxy, k), lyxi = iavg(in.lyx, k);
-    double uii = iavg(in.ui, k), vii = iavg(in.vi, k);
-    double udc = out.Fe[p] + lxxi * uii + lxyi * vii;
-    double vdc = out.Fn[p] + lyyi * vii - lyxi * uii;
-    double unms = UT.unms[p], vnms = UT.vnms[p], advu = UT.advu[p], advv = UT.advv[p];
-    double s1 = dtx2inv * unms + in.hdu[p] + udc - advu - zl_at(k);
-    double s2 = dtx2inv * vnms + in.hdv[p] + vdc - advv - zp_at(k);
-    s1 = s1 * g.expz[k];
-    s2 = s2 * g.expz[k];
+    dwdz = dwdz * expzk;
+    double lxxi = .5 * (ua.lxx + ub.lxx), lyyi = .5 * (ua.lyy + ub.lyy);
+    double lxyi = .5 * (ua.lxy + ub.lxy), lyxi = .5 * (ua.lyx + ub.lyx);
+    double uii = .5 * (ua.ui + ub.ui), vii = .5 * (ua.vi + ub.vi);
+    double udc = us.Fe + lxxi * uii + lxyi * vii;
+    double vdc = us.Fn + lyyi * vii - lyxi * uii;
+    double zl = (ua.divx + ub.divx) * .5;
+    zl = zl * racs;
+    zl = zl * GRAV / g.dlamda;
+    double zp = (ua.divy + ub.divy) * .5;
+    zp = zp * GRAV * RE_INV / g.dphi;
+    double s1 = dtx2inv * us.unms + us.hdu + udc - us.advu - zl;
+    double s2 = dtx2inv * us.vnms + us.hdv + vdc - us.advv - zp;
+    s1 = s1 * expzk;
+    s2 = s2 * expzk;
     double p11 = -(gk + dwdz), p12 = 0., p21 = 0., p22 = p11;
     double r11 = -(g_1 - dwdz), r12 = 0., r21 = 0., r22 = r11;
     double q11 = gk + g_1;
     double qa = dtx2inv + lxxi;
-    qa = qa * g.expz[k];
+    qa = qa * expzk;
     double qq11 = q11 + qa;
     qa = dtx2inv + lyyi;
-    qa = qa * g.expz[k];
+    qa = qa * expzk;
     double qq22 = q11 + qa;
-    double q12 = cor + F3(in.un, k) / RE * tanphi;
+    double q12 = cor + us.un / RE * tanphi;
     double q21 = q12;
     qa = q12 - lxyi;
-    qa = qa * g.expz[k];
+    qa = qa * expzk;
     double qq12 = -qa;
     qa = q21 - lyxi;
-    qa = qa * g.expz[k];
+    qa = qa * expzk;
     double qq21 = qa;
     if (k == 0) {  // :207-212
       qq11 = qq11 - p11; qq12 = qq12 - p12; qq21 = qq21 - p21; qq22 = qq22 - p22;
@@ -1459,7 +1651,7 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
       double g12 = be11 * r12p + be12 * r22p;
       double g21 = be21 * r11p + be22 * r21p;
       double g22 = be21 * r12p + be22 * r22p;
-      size_t pm = G.a3(k - 1, i, j), NP = (size_t)nk * G.P;
+      size_t pm = G.a3(k - 1, i, j);
       gs[pm] = g11; gs[pm + NP] = g21; gs[pm + 2 * NP] = g12; gs[pm + 3 * NP] = g22;
       double h11 = qq11 - p11 * g11 - p12 * g21;
       double h12 = qq12 - p11 * g12 - p12 * g22;
@@ -1476,31 +1668,50 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
     out.vn[p] = y2;
     r11p = r11; r12p = r12; r21p = r21; r22p = r22;
     gk = gk1;
+    e1_m = e1_c;
+    e1_c = e1_n;
+    ua = ub;
   }
-  // back substitution (:244-250) and outputs (:320-321)
-  auto emit = [&](int k, double u, double vv) {
+  // back substitution (:244-250) and outputs (:320-321); the next level's
+  // loads are in flight while this level is stored
+  struct UB { double x1, x2, g11, g21, g12, g22, unm, un, vnm, vn; };
+  auto load_ub = [&](int k, UB &B) {
     size_t p = G.a3(k, i, j);
-    out.un[p] = u;
-    out.vn[p] = vv;
-    out.un_nm[p] = DTSMOOTH_DIV2 * (in.un_nm[p] + u) + DTSMOOTH * in.un[p];
-    out.vn_nm[p] = DTSMOOTH_DIV2 * (in.vn_nm[p] + vv) + DTSMOOTH * in.vn[p];
+    B.x1 = out.un[p]; B.x2 = out.vn[p];
+    if (k < nk - 2) { B.g11 = gs[p]; B.g21 = gs[p + NP]; B.g12 = gs[p + 2 * NP]; B.g22 = gs[p + 3 * NP]; }
+    else { B.g11 = B.g21 = B.g12 = B.g22 = 0.; }
+    B.unm = in.un_nm[p]; B.un = in.un[p]; B.vnm = in.vn_nm[p]; B.vn = in.vn[p];
   };
-  double xn1 = y1, xn2 = y2, xp1 = 0., xp2 = 0.;
-  const size_t NP = (size_t)nk * G.P;
+  const double unm_top = F3(in.un_nm, nk - 1), un_top = F3(in.un, nk - 1);
+  const double vnm_top = F3(in.vn_nm, nk - 1), vn_top = F3(in.vn, nk - 1);
+  double xn1 = y1, xn2 = y2;
+  UB bc, bn;
+  load_ub(nk - 2, bc);
   for (int k = nk - 2; k >= 0; --k) {
-    size_t p = G.a3(k, i, j);
+    if (k > 0) load_ub(k - 1, bn);
+    asm volatile("" ::: "memory");
     double x1, x2;
     if (k == nk - 2) { x1 = xn1; x2 = xn2; }
     else {
-      x1 = out.un[p] - gs[p] * xn1 - gs[p + 2 * NP] * xn2;
-      x2 = out.vn[p] - gs[p + NP] * xn1 - gs[p + 3 * NP] * xn2;
+      x1 = bc.x1 - bc.g11 * xn1 - bc.g12 * xn2;
+      x2 = bc.x2 - bc.g21 * xn1 - bc.g22 * xn2;
     }
-    if (k == nk - 3) emit(nk - 1, 2 * xn1 - x1, 2 * xn2 - x2);
-    emit(k, x1, x2);
-    xp1 = xn1; xp2 = xn2;
+    if (k == nk - 3) {
+      size_t pt = G.a3(nk - 1, i, j);
+      double u = 2 * xn1 - x1, vv = 2 * xn2 - x2;
+      out.un[pt] = u;
+      out.vn[pt] = vv;
+      out.un_nm[pt] = DTSMOOTH_DIV2 * (unm_top + u) + DTSMOOTH * un_top;
+      out.vn_nm[pt] = DTSMOOTH_DIV2 * (vnm_top + vv) + DTSMOOTH * vn_top;
+    }
+    size_t p = G.a3(k, i, j);
+    out.un[p] = x1;
+    out.vn[p] = x2;
+    out.un_nm[p] = DTSMOOTH_DIV2 * (bc.unm + x1) + DTSMOOTH * bc.un;
+    out.vn_nm[p] = DTSMOOTH_DIV2 * (bc.vnm + x2) + DTSMOOTH * bc.vn;
     xn1 = x1; xn2 = x2;
+    bc = bn;
   }
-  (void)xp1; (void)xp2;
 #undef F3
 }
 
@@ -1532,25 +1743,41 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       hipLaunchKernelGGL(dt_kernel, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
       break;
     case 1:
-    case 2: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4]} (species 0..3)
+    case 2:
+    case 15:
+    case 16: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
+                // NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO after N2D's solve
       MinorTerms MT;
       for (int q = 0; q < 4; ++q) { MT.ftm1s[q] = D(2 + q); MT.hadvec[q] = D(6 + q); }
+      MT.sloss[0] = MT.sloss[2] = D(10);
+      MT.s0prod[0] = MT.s0prod[2] = D(11);
+      MT.sloss[1] = D(12);
+      MT.s0prod[1] = D(13);
+      MT.sloss[3] = MT.s0prod[3] = nullptr;
       if (which == 1)
         hipLaunchKernelGGL((minor_kernel<2, 3>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
-      else
+      else if (which == 2)
         hipLaunchKernelGGL((minor_kernel<1, 0>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      else if (which == 15)
+        hipLaunchKernelGGL((minor_src_kernel<2, 3>), gridp, block, 0, s, g, G, in, out, MT);
+      else
+        hipLaunchKernelGGL((minor_src_kernel<1, 0>), gridp, block, 0, s, g, G, in, out, MT);
       break;
     }
     case 3:
     case 4:
     case 5:
-    case 9: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
-               //           optm1out, bd_diff bd_diffj bd_bvel vdotn bd_djbz opts2}
+    case 9:
+    case 17: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
+                //           optm1out, bd_diff bd_diffj bd_bvel vdotn bd_djbz opts2, oploss, opprod}
       OpWork W{D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8)};
       const double *op = D(9), *optm1 = D(10);
       double *opout = D(11), *optm1out = D(12);
       OpTerms OT{D(13), D(14), D(15), D(16), D(17), D(18)};
-      if (which == 3) {
+      if (which == 17) {
+        hipLaunchKernelGGL(oplus_chem_kernel, gridp, block, 0, s, g, G, in, D(19), D(20), out.xiop2p,
+                           out.xiop2d);
+      } else if (which == 3) {
         hipLaunchKernelGGL(oplus_stage1, grid, block, 0, s, g, G, in, op, W, out.Fe, out.Fn);
       } else if (which == 4) {
         hipLaunchKernelGGL(oplus_stage2, grid, block, 0, s, g, G, in, W);
@@ -1568,7 +1795,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
         hipLaunchKernelGGL(oplus_stage3, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
-                           opout, optm1out, out.xiop2p, out.xiop2d);
+                           (const double *)D(19), (const double *)D(20), opout, optm1out);
       }
       break;
     }
@@ -1580,9 +1807,15 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       hipLaunchKernelGGL(duv_stage2, grid, block, 0, s, g, G, in, out, D(1), UT);
       break;
     }
-    case 8: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes}
-      CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6)};
-      hipLaunchKernelGGL(comp_kernel, grid, block, 0, s, g, G, in, out, D(0), CT);
+    case 8:
+    case 14: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6]}
+      CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6), {D(7), D(8), D(9), D(10), D(11), D(12)}};
+      if (which == 8) {
+        hipLaunchKernelGGL(comp_kernel, grid, block, 0, s, g, G, in, out, D(0), CT);
+      } else {
+        dim3 grido((unsigned)((ncol * (G.nk - 1) + 255) / 256));
+        hipLaunchKernelGGL(o2o_kernel, grido, block, 0, s, g, G, in, CT);
+      }
       break;
     }
     case 10: {  // dt prelude: bufs = {w2, hadv_tn, smooth_tn_nm}

@@ -31,11 +31,13 @@ namespace {
 enum {
   W_DT = 0, W_DT_T = 1,                  // dt ratios; 2 prelude planes
   W_MIN_A = 3, W_MIN_B = 4, W_MIN_T = 5, // minor ratios (two species per pass); 8 prelude planes
-  W_OP = 13,                             // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
+  W_MIN_S = 13,                          // minor sources: 2 species x (sloss, s0prod)
+  W_OP = 17,                             // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
-  W_UV = W_OP_T + 6,                     // wind: ztmp, 4 gamma, 6 prelude planes
-  W_CO = W_UV + 11,                      // comp: 9 gamma, 6 prelude planes
-  W_NPLANES = W_CO + 15
+  W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
+  W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
+  W_CO = W_UV + 11,                      // comp: 9 gamma, 6 prelude planes, 6 fs planes
+  W_NPLANES = W_CO + 21
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -124,7 +126,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 }
 
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
-       K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE };
+       K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM };
 
 extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                          void *work, void *stream) {
@@ -141,13 +143,16 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_MINORS, s);
-  void *b[10];
+  void *b[14];
   b[0] = plane(work, g, W_MIN_A);
   b[1] = plane(work, g, W_MIN_B);
   for (int q = 0; q < 8; ++q) b[2 + q] = plane(work, g, W_MIN_T + q);
+  for (int q = 0; q < 4; ++q) b[10 + q] = plane(work, g, W_MIN_S + q);
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
+  if (int rc = tgx_ng_launch(K_MIN_SRC_A, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_MIN_A, g, in, out, b, 0, s)) return rc;
+  if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
   return tgx_ng_launch(K_MIN_B, g, in, out, b, 0, s);
 }
 
@@ -159,18 +164,24 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
   const int nsub = g->nstep_sub;
   double *tmp_op = plane(work, g, W_OP_X), *tmp_nm = plane(work, g, W_OP_XNM);
   const double *op_in = in->op, *nm_in = in->op_nm;
+  double *oploss = plane(work, g, W_OP_C), *opprod = plane(work, g, W_OP_C + 1);
   for (int sub = 0; sub < nsub; ++sub) {
     // the last sub-step writes the outputs; earlier ones alternate so that
     // a sub-step never reads the buffer it writes
     bool to_out = ((nsub - 1 - sub) % 2) == 0;
     double *op_o = to_out ? out->op : tmp_op, *nm_o = to_out ? out->op_nm : tmp_nm;
-    void *b[19];
+    void *b[21];
     for (int q = 0; q < 9; ++q) b[q] = plane(work, g, W_OP + q);
     b[9] = (void *)op_in;
     b[10] = (void *)nm_in;
     b[11] = op_o;
     b[12] = nm_o;
     for (int q = 0; q < 6; ++q) b[13 + q] = plane(work, g, W_OP_T + q);
+    b[19] = oploss;
+    b[20] = opprod;
+    // the O+ chemistry does not depend on O+: once for all sub-cycles
+    if (sub == 0)
+      if (int rc = tgx_ng_launch(K_OP_CHEM, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
@@ -200,10 +211,11 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_COMP, s);
-  void *b[7];
+  void *b[13];
   b[0] = plane(work, g, W_CO);
-  for (int q = 0; q < 6; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
+  for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
+  if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
   return tgx_ng_launch(K_COMP, g, in, out, b, 0, s);
 }
 

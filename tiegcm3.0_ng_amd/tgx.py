@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtgx.so")
+# TGX_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("TGX_LIBRARY") or os.path.join(HERE, "libtgx.so")
 _lib = None
 
 SCHEME_AUTO, SCHEME_THOMAS, SCHEME_PCR = 0, 1, 2
