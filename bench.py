@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
+    ap.add_argument("--scheme", default="thomas", choices=("thomas", "pcr", "auto"),
+                    help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
     ap.add_argument("--serial", action="store_true",
                     help="run the routines one after another on one stream (default: the "
                          "independent routines run concurrently)")
@@ -96,7 +98,7 @@ def main():
     g = pkg.synth.make_grid(args.config, nstep_sub=args.nstep_sub)
     print("# generating %s state on the GPU" % args.config, file=sys.stderr, flush=True)
     F = pkg.synth.make_state(g, device="cuda")
-    ns = pkg.step.NestState(g, F)
+    ns = pkg.step.NestState(g, F, scheme=args.scheme)
     del F
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
@@ -162,7 +164,7 @@ def main():
                                "nstep_sub=%d, fused assembly+solve FP64" % (
                                    args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
                    "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64",
-                   "schedule": "serial" if args.serial else "concurrent"},
+                   "schedule": "serial" if args.serial else "concurrent", "scheme": args.scheme},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": dom, "alg_bytes_per_launch": alg[dom],

@@ -32,6 +32,10 @@ typedef struct {
   int nk, nlon, nlat, ld;
   int nstep_sub;      /* O+ sub-cycles per step (input namelist NSTEP_SUB) */
   int calc_helium;
+  int scheme;         /* column solve: TGX_SCHEME_THOMAS (thread per column, the
+                         reference's operation order), TGX_SCHEME_PCR (one
+                         wavefront per column, cyclic reduction, nk <= 128) or
+                         TGX_SCHEME_AUTO (the faster one, timed once per shape) */
   double dz;          /* dz(i_ng) = dlev_ng */
   double dlamda, dphi, dtx2inv, shapiro, expzmid, expzmid_inv;
   double colfac, opdiffcap, difhor, pshelb, ar_glbm1, ar_glbm2;

@@ -21,15 +21,21 @@ def field_err(got, ref):
     return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
 
 
-@pytest.fixture(scope="module", params=[("tiny", 2), ("small", 1), ("small", 3)])
+# (grid, O+ sub-cycles, column-solve scheme): "thomas" follows the reference's
+# operation order; "pcr" (one wavefront per column, cyclic reduction) reorders
+# the elimination and is held to the same 1e-12 bound.
+@pytest.fixture(scope="module", params=[("tiny", 2, "thomas"), ("small", 1, "thomas"), ("small", 3, "thomas"),
+                                        ("tall", 1, "thomas"), ("tiny", 2, "pcr"), ("small", 1, "pcr"),
+                                        ("tall", 1, "pcr")],
+                ids=lambda p: "%s-nsub%d-%s" % p)
 def case(request, pkg, oracle):
     import torch
-    name, nsub = request.param
+    name, nsub, scheme = request.param
     g = pkg.synth.make_grid(name, nstep_sub=nsub)
     F = pkg.synth.make_state(g)
     Fo = pkg.synth.to_fortran_layout(F)
     ref = pkg.step.run_oracle_step(oracle, g, Fo)
-    ns = pkg.step.NestState(g, F)
+    ns = pkg.step.NestState(g, F, scheme=scheme)
     ns.step()
     torch.cuda.synchronize()
     return name, ns, ref

@@ -865,6 +865,710 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 
 
 // =============================================================================
+// One wavefront per column.  Lane l holds level l ("lo") and level l+64
+// ("hi"), so a wave covers up to 128 levels; neighbouring levels come from
+// other lanes by ds_bpermute, and the column's tridiagonal systems are solved
+// in registers by parallel cyclic reduction (no ratio planes, no second pass
+// over memory).  A workgroup holds consecutive columns of one latitude row, so
+// the waves of a workgroup share the cache lines of every level plane.
+
+// Value at level L-1 (dn) and L+1 (up) for the lo and hi rows of this lane.
+// Lane 0 lo (level -1) and lane 63 hi (level 128) receive unused values.
+__device__ __forceinline__ void wave_dn(double lo, double hi, int lane, double &dlo, double &dhi) {
+  const int src = (lane - 1) & 63;
+  dlo = __shfl(lo, src, 64);
+  dhi = __shfl(lane == 63 ? lo : hi, src, 64);  // lane 0 hi (level 64) takes lane 63 lo
+}
+__device__ __forceinline__ void wave_up(double lo, double hi, int lane, double &ulo, double &uhi) {
+  const int src = (lane + 1) & 63;
+  ulo = __shfl(lane == 0 ? hi : lo, src, 64);  // lane 63 lo (level 63) takes lane 0 hi
+  uhi = __shfl(hi, src, 64);
+}
+
+struct TriRow { double a, b, c, f; };  // a x(k-1) + b x(k) + c x(k+1) = f
+
+__device__ __forceinline__ TriRow tri_ident() { return TriRow{0., 1., 0., 0.}; }
+
+__device__ __forceinline__ TriRow tri_shfl(const TriRow &r, int src) {
+  return TriRow{__shfl(r.a, src, 64), __shfl(r.b, src, 64), __shfl(r.c, src, 64), __shfl(r.f, src, 64)};
+}
+
+// Eliminates the couplings of row r to rows m (r-d) and p (r+d).
+__device__ __forceinline__ TriRow tri_reduce(const TriRow &r, const TriRow &m, const TriRow &p) {
+  double alpha = -r.a / m.b, gamma = -r.c / p.b;
+  TriRow o;
+  o.a = alpha * m.a;
+  o.c = gamma * p.c;
+  o.b = r.b + alpha * m.c + gamma * p.a;
+  o.f = r.f + alpha * m.f + gamma * p.f;
+  return o;
+}
+
+// Parallel cyclic reduction of NS independent 128-row systems held as
+// (lo, hi) rows per lane.  Rows beyond the system are identity rows; the
+// first row has a = 0 and the last c = 0.  On return x = f / b.
+template <int NS>
+__device__ __forceinline__ void wave_pcr(TriRow (&lo)[NS], TriRow (&hi)[NS], int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
+    const bool has_m = lane >= d, low_p = lane + d < 64;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      TriRow Lm = tri_shfl(lo[q], sm), Hm = tri_shfl(hi[q], sm);
+      TriRow Lp = tri_shfl(lo[q], sp), Hp = tri_shfl(hi[q], sp);
+      TriRow lm = has_m ? Lm : tri_ident();
+      TriRow lp = low_p ? Lp : Hp;
+      TriRow hm = has_m ? Hm : Lm;
+      TriRow hp = low_p ? Hp : tri_ident();
+      TriRow nlo = tri_reduce(lo[q], lm, lp);
+      TriRow nhi = tri_reduce(hi[q], hm, hp);
+      lo[q] = nlo;
+      hi[q] = nhi;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {  // d = 64: lo and hi of the same lane
+    TriRow nlo = tri_reduce(lo[q], tri_ident(), hi[q]);
+    TriRow nhi = tri_reduce(hi[q], lo[q], tri_ident());
+    lo[q] = nlo;
+    hi[q] = nhi;
+  }
+}
+
+// Block rows A x(k-1) + B x(k) + C x(k+1) = F with N x N blocks, row-major
+// (e = n*N + m is row n, column m).
+template <int N>
+struct BlkRow { double A[N * N], B[N * N], C[N * N], F[N]; };
+
+template <int N>
+__device__ __forceinline__ void blk_inv(const double *M, double *R);
+template <>
+__device__ __forceinline__ void blk_inv<2>(const double *M, double *R) {
+  double det = M[0] * M[3] - M[1] * M[2];
+  R[0] = M[3] / det; R[1] = -M[1] / det; R[2] = -M[2] / det; R[3] = M[0] / det;
+}
+template <>
+__device__ __forceinline__ void blk_inv<3>(const double *M, double *R) {
+  double c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8], c02 = M[3] * M[7] - M[4] * M[6];
+  double det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+  double r = 1. / det;
+  R[0] = c00 * r; R[3] = c01 * r; R[6] = c02 * r;
+  R[1] = (M[2] * M[7] - M[1] * M[8]) * r; R[4] = (M[0] * M[8] - M[2] * M[6]) * r;
+  R[7] = (M[1] * M[6] - M[0] * M[7]) * r;
+  R[2] = (M[1] * M[5] - M[2] * M[4]) * r; R[5] = (M[2] * M[3] - M[0] * M[5]) * r;
+  R[8] = (M[0] * M[4] - M[1] * M[3]) * r;
+}
+template <int N>
+__device__ __forceinline__ void blk_mm(const double *X, const double *Y, double *Z) {  // Z = X Y
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      double t = 0.;
+#pragma unroll
+      for (int l = 0; l < N; ++l) t += X[n * N + l] * Y[l * N + m];
+      Z[n * N + m] = t;
+    }
+}
+template <int N>
+__device__ __forceinline__ void blk_mv(const double *X, const double *y, double *z) {  // z = X y
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    double t = 0.;
+#pragma unroll
+    for (int l = 0; l < N; ++l) t += X[n * N + l] * y[l];
+    z[n] = t;
+  }
+}
+
+// A row divided by its diagonal block: Ah = B^-1 A, Ch = B^-1 C, Fh = B^-1 F.
+template <int N>
+struct BlkNorm { double A[N * N], C[N * N], F[N]; };
+
+template <int N>
+__device__ __forceinline__ BlkNorm<N> blk_normalize(const BlkRow<N> &r) {
+  double Bi[N * N];
+  blk_inv<N>(r.B, Bi);
+  BlkNorm<N> o;
+  blk_mm<N>(Bi, r.A, o.A);
+  blk_mm<N>(Bi, r.C, o.C);
+  blk_mv<N>(Bi, r.F, o.F);
+  return o;
+}
+template <int N>
+__device__ __forceinline__ BlkNorm<N> blk_shfl(const BlkNorm<N> &r, int src) {
+  BlkNorm<N> o;
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) { o.A[e] = __shfl(r.A[e], src, 64); o.C[e] = __shfl(r.C[e], src, 64); }
+#pragma unroll
+  for (int e = 0; e < N; ++e) o.F[e] = __shfl(r.F[e], src, 64);
+  return o;
+}
+template <int N>
+__device__ __forceinline__ BlkNorm<N> blk_zero() {
+  BlkNorm<N> o;
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) { o.A[e] = 0.; o.C[e] = 0.; }
+#pragma unroll
+  for (int e = 0; e < N; ++e) o.F[e] = 0.;
+  return o;
+}
+template <int N>
+__device__ __forceinline__ BlkNorm<N> blk_sel(bool c, const BlkNorm<N> &x, const BlkNorm<N> &y) {
+  BlkNorm<N> o;
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) { o.A[e] = c ? x.A[e] : y.A[e]; o.C[e] = c ? x.C[e] : y.C[e]; }
+#pragma unroll
+  for (int e = 0; e < N; ++e) o.F[e] = c ? x.F[e] : y.F[e];
+  return o;
+}
+// Eliminates the couplings of r to the normalised rows m (r-d) and p (r+d).
+template <int N>
+__device__ __forceinline__ void blk_reduce(BlkRow<N> &r, const BlkNorm<N> &m, const BlkNorm<N> &p) {
+  double T[N * N], nA[N * N], nC[N * N], t[N];
+  blk_mm<N>(r.A, m.C, T);
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) r.B[e] -= T[e];
+  blk_mm<N>(r.C, p.A, T);
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) r.B[e] -= T[e];
+  blk_mv<N>(r.A, m.F, t);
+#pragma unroll
+  for (int e = 0; e < N; ++e) r.F[e] -= t[e];
+  blk_mv<N>(r.C, p.F, t);
+#pragma unroll
+  for (int e = 0; e < N; ++e) r.F[e] -= t[e];
+  blk_mm<N>(r.A, m.A, nA);
+  blk_mm<N>(r.C, p.C, nC);
+#pragma unroll
+  for (int e = 0; e < N * N; ++e) { r.A[e] = -nA[e]; r.C[e] = -nC[e]; }
+}
+
+// Block parallel cyclic reduction over 128 rows held as (lo, hi) per lane;
+// rows beyond the system are identity rows.  On return x = B^-1 F per row.
+template <int N>
+__device__ __forceinline__ void wave_block_pcr(BlkRow<N> &lo, BlkRow<N> &hi, int lane, double *xlo,
+                                               double *xhi) {
+#pragma unroll 1
+  for (int d = 1; d < 64; d <<= 1) {
+    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
+    const bool has_m = lane >= d, low_p = lane + d < 64;
+    BlkNorm<N> nlo = blk_normalize<N>(lo), nhi = blk_normalize<N>(hi);
+    BlkNorm<N> Lm = blk_shfl<N>(nlo, sm), Hm = blk_shfl<N>(nhi, sm);
+    BlkNorm<N> Lp = blk_shfl<N>(nlo, sp), Hp = blk_shfl<N>(nhi, sp);
+    const BlkNorm<N> Z = blk_zero<N>();
+    blk_reduce<N>(lo, blk_sel<N>(has_m, Lm, Z), blk_sel<N>(low_p, Lp, Hp));
+    blk_reduce<N>(hi, blk_sel<N>(has_m, Hm, Lm), blk_sel<N>(low_p, Hp, Z));
+  }
+  {  // d = 64
+    BlkNorm<N> nlo = blk_normalize<N>(lo), nhi = blk_normalize<N>(hi);
+    blk_reduce<N>(lo, blk_zero<N>(), nhi);
+    blk_reduce<N>(hi, nlo, blk_zero<N>());
+  }
+  double Bi[N * N];
+  blk_inv<N>(lo.B, Bi);
+  blk_mv<N>(Bi, lo.F, xlo);
+  blk_inv<N>(hi.B, Bi);
+  blk_mv<N>(Bi, hi.F, xhi);
+}
+
+struct Nrm3 { double A[9], C[9], F[3]; };
+
+// dst = (ok ? the source lane's (pick_a ? a : b) : 0), element by element
+template <int K>
+__device__ __forceinline__ void fetchk(const double *a, const double *b, bool pick_a, int lane_src,
+                                       bool ok, double *dst) {
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    double t = __shfl(pick_a ? a[e] : b[e], lane_src, 64);
+    dst[e] = ok ? t : 0.;
+  }
+}
+// The reduced row of r from its neighbours m (lane sm, row ma or mb) and p
+// (lane sp, row pa or pb), normalised again by its new diagonal block.
+__device__ __forceinline__ void nrm3_update(const Nrm3 &r, int sm, const Nrm3 &ma, const Nrm3 &mb,
+                                            bool m_a, bool okm, int sp, const Nrm3 &pa,
+                                            const Nrm3 &pb, bool p_a, bool okp, Nrm3 &o) {
+  double Bp[9] = {1., 0., 0., 0., 1., 0., 0., 0., 1.}, Ap[9], Cp[9], Fp[3], T[9], t[3], nb[9], nf[3];
+  // m side: B -= A Cm, A' = -A Am, F -= A Fm
+  fetchk<9>(ma.C, mb.C, m_a, sm, okm, nb);
+  blk_mm<3>(r.A, nb, T);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Bp[e] -= T[e];
+  fetchk<9>(ma.A, mb.A, m_a, sm, okm, nb);
+  blk_mm<3>(r.A, nb, T);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Ap[e] = -T[e];
+  fetchk<3>(ma.F, mb.F, m_a, sm, okm, nf);
+  blk_mv<3>(r.A, nf, t);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) Fp[e] = r.F[e] - t[e];
+  // p side: B -= C Ap, C' = -C Cp, F -= C Fp
+  fetchk<9>(pa.A, pb.A, p_a, sp, okp, nb);
+  blk_mm<3>(r.C, nb, T);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Bp[e] -= T[e];
+  fetchk<9>(pa.C, pb.C, p_a, sp, okp, nb);
+  blk_mm<3>(r.C, nb, T);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Cp[e] = -T[e];
+  fetchk<3>(pa.F, pb.F, p_a, sp, okp, nf);
+  blk_mv<3>(r.C, nf, t);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) Fp[e] -= t[e];
+  double Bi[9];
+  blk_inv<3>(Bp, Bi);
+  blk_mm<3>(Bi, Ap, o.A);
+  blk_mm<3>(Bi, Cp, o.C);
+  blk_mv<3>(Bi, Fp, o.F);
+}
+// 3x3 block cyclic reduction on rows kept normalised by their diagonal
+// block (B = I): each lane holds A, C, F of its lo and hi rows.  A source
+// lane offers the row its requester needs, so one neighbour block at a time
+// is live.  On return lo.F / hi.F hold the solution.
+__device__ __forceinline__ void wave_pcr3(Nrm3 &lo, Nrm3 &hi, int lane) {
+#pragma unroll 1
+  for (int d = 1; d <= 64; d <<= 1) {
+    int sm, sp;
+    bool lo_m_ok, hi_p_ok, offer_p_lo, offer_m_hi;
+    if (d < 64) {
+      sm = (lane - d) & 63;
+      sp = (lane + d) & 63;
+      lo_m_ok = lane >= d;       // lo's m-neighbour: the source's lo
+      hi_p_ok = lane + d < 64;   // hi's p-neighbour: the source's hi
+      offer_p_lo = lane >= d;    // lo's p-neighbour: source lo if source >= d, else its hi
+      offer_m_hi = lane + d < 64;  // hi's m-neighbour: source hi if source + d < 64, else its lo
+    } else {                     // the partner row sits in the same lane
+      sm = sp = lane;
+      lo_m_ok = false;
+      hi_p_ok = false;
+      offer_p_lo = false;        // lo's p-neighbour: own hi
+      offer_m_hi = false;        // hi's m-neighbour: own lo
+    }
+    Nrm3 nlo, nhi;
+    nrm3_update(lo, sm, lo, lo, true, lo_m_ok, sp, lo, hi, offer_p_lo, true, nlo);
+    nrm3_update(hi, sm, hi, lo, offer_m_hi, true, sp, hi, hi, true, hi_p_ok, nhi);
+    lo = nlo;
+    hi = nhi;
+  }
+}
+
+// Column index of this wave (consecutive waves take consecutive columns).
+__device__ __forceinline__ bool wave_column_of(const Geo &G, int &i, int &j) {
+  int nb = gridDim.x;
+  int b = tgx_xcd_block(nb);
+  long c = (long)b * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= (long)G.nlond * G.nlatd) return false;
+  j = (int)(c / G.nlond);
+  i = (int)(c - (long)j * G.nlond);
+  return true;
+}
+
+// minor_ng.F90:60-237 for species SA and SB, one wavefront per column, the
+// solve by parallel cyclic reduction over the nk rows.
+template <int SA, int SB>
+__global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                         tgx_ng_out out, MinorTerms MT) {
+  int i, j;
+  if (!wave_column_of(G, i, j)) return;
+  const int lane = threadIdx.x & 63, nk = G.nk;
+  const int Lr[2] = {lane, lane + 64};
+  const double dzp = g.dz, dtx2inv = g.dtx2inv, small = 1.e-12;
+  const size_t c2 = G.a2(i, j);
+  const int SP[2] = {SA, SB};
+  double *outs[2], *outs_nm[2];
+  const double *fc[2], *fctm1[2], *lonb[2], *latb[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int sp = SP[q];
+    outs[q] = sp == 0 ? out.no : sp == 1 ? out.n4s : sp == 2 ? out.n2d : out.ar;
+    outs_nm[q] = sp == 0 ? out.no_nm : sp == 1 ? out.n4s_nm : sp == 2 ? out.n2d_nm : out.ar_nm;
+    fc[q] = sp == 0 ? in.no : sp == 1 ? in.n4s : sp == 2 ? in.n2d : in.ar;
+    fctm1[q] = sp == 0 ? in.no_nm : sp == 1 ? in.n4s_nm : sp == 2 ? in.n2d_nm : in.ar_nm;
+    lonb[q] = sp == 0 ? in.lonb_no : sp == 1 ? in.lonb_n4s : sp == 2 ? in.lonb_n2d : in.lonb_ar;
+    latb[q] = sp == 0 ? in.latb_no : sp == 1 ? in.latb_n4s : sp == 2 ? in.latb_n2d : in.latb_ar;
+  }
+  double v;
+  if (lateral_value(G, in.lonb_no, in.latb_no, 0, i, j, v)) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int L = Lr[h];
+      if (L >= nk) continue;
+      size_t p = G.a3(L, i, j);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        lateral_value(G, lonb[q], latb[q], L, i, j, v);
+        double nm = DTSMOOTH * fc[q][p] + DTSMOOTH_DIV2 * (fctm1[q][p] + v);
+        outs[q][p] = v > small ? v : small;
+        outs_nm[q][p] = nm > small ? nm : small;
+      }
+    }
+    return;
+  }
+  const double pi = 4.0 * atan(1.0);
+  double dfac;
+  {  // minor_ng.F90:145-154
+    const double hor = .25, dtr = pi / 180.;
+    if (!(g.difhor > 0)) dfac = 1.;
+    else {
+      double rlat = g.glat[j] * dtr;
+      if (fabs(rlat) >= pi / 4.5) dfac = hor + 1.;
+      else dfac = hor + .5 * (1. + sin(pi * (fabs(rlat) - pi / 9.) / (pi / 4.5)));
+    }
+  }
+  SpeciesConst sc[2] = {species_const(SA), species_const(SB)};
+  const double *n2d_upd = out.n2d;
+  // lower boundary of the major species (:66-69)
+  const double o2_0 = in.o2[c2], o1_0 = in.o1[c2], he_0 = in.he[c2];
+  const double bo2 = g.b[0] * o2_0 + g.b[1] * o1_0 + g.b[2] * he_0 + g.fb[0];
+  const double bo1 = g.b[3] * o2_0 + g.b[4] * o1_0 + g.b[5] * he_0 + g.fb[1];
+  const double bhe = g.b[6] * o2_0 + g.b[7] * o1_0 + g.b[8] * he_0 + g.fb[2];
+  const double xmbari = 1. / (bo2 * RIO2 + bo1 * RIO1 + (1. - bo2 - bo1 - bhe) * RIN2 + bhe * RIHE);
+  // lower-boundary data flbc(:,:,1:3) of each species (comp_*_ng.F90)
+  double l1[2], l2[2], l3[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int s = SP[q];
+    size_t p = c2;
+    double xb = in.xnmbari[p];
+    l1[q] = 0.;
+    l2[q] = 1.;
+    if (s == 0) l3[q] = -4.e6 * RMNO / xb;
+    else if (s == 1)
+      l3[q] = -(RMN4S / xb *
+                (in.qtef[p] * (1. - BRN2D) / xb +
+                 n2d_upd[p] * RIN2D * (BETA4 * xb * in.o1[p] * RIO1 + in.beta5[p] * in.ne[p] + BETA7) +
+                 in.beta8[p] * in.no[p] * RINO) /
+                (in.beta1[p] * in.o2[p] * RIO2 + in.beta3[p] * in.no[p] * RINO +
+                 xb * in.beta17[p] * in.o1[p] * RIO1 * in.n2[p] * RIN2));
+    else if (s == 2)
+      l3[q] = -(RMN2D / xb * in.qtef[p] * BRN2D /
+                (xb * (BETA2 * in.o2[p] * RIO2 + BETA4 * in.o1[p] * RIO1 + BETA6 * in.no[p] * RINO) + BETA7 +
+                 in.beta5[p] * in.ne[p]));
+    else l3[q] = -sqrt(g.ar_glbm1 * g.ar_glbm2);
+  }
+  const double fubc = 0.;  // every species sets fubc = 0 (comp_*_ng.F90)
+
+  // own-level fields (levels beyond the column read the top level; their
+  // rows become identity rows)
+  struct MR { double o1, o2, he, mbar, tn, barm, w; };
+  MR R[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
+    R[h].o1 = in.o1[p]; R[h].o2 = in.o2[p]; R[h].he = in.he[p]; R[h].mbar = in.mbar[p];
+    R[h].tn = in.tn[p]; R[h].barm = in.barm[p]; R[h].w = in.w_upd[p];
+  }
+  MR A[2];  // level L-1
+  wave_dn(R[0].o1, R[1].o1, lane, A[0].o1, A[1].o1);
+  wave_dn(R[0].o2, R[1].o2, lane, A[0].o2, A[1].o2);
+  wave_dn(R[0].he, R[1].he, lane, A[0].he, A[1].he);
+  wave_dn(R[0].mbar, R[1].mbar, lane, A[0].mbar, A[1].mbar);
+  wave_dn(R[0].tn, R[1].tn, lane, A[0].tn, A[1].tn);
+  wave_dn(R[0].w, R[1].w, lane, A[0].w, A[1].w);
+  // :66-126 interface quantities at level L
+  MinorShared S[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    MinorShared &Q = S[h];
+    if (L == 0) {
+      Q.dmdz = (R[h].mbar - xmbari) / dzp;
+      Q.pso1 = .5 * (bo1 + o1_0);
+      Q.pso2 = .5 * (bo2 + o2_0);
+      Q.pshe = .5 * (bhe + he_0);
+      Q.do1dz = (o1_0 - bo1) / dzp;
+      Q.do2dz = (o2_0 - bo2) / dzp;
+      Q.dhedz = (he_0 - bhe) / dzp;
+      Q.tni = in.tlbc[c2];
+    } else {
+      Q.dmdz = (R[h].mbar - A[h].mbar) / dzp;
+      Q.pso1 = .5 * (R[h].o1 + A[h].o1);
+      Q.pso2 = .5 * (R[h].o2 + A[h].o2);
+      Q.pshe = .5 * (R[h].he + A[h].he);
+      Q.do1dz = (R[h].o1 - A[h].o1) / dzp;
+      Q.do2dz = (R[h].o2 - A[h].o2) / dzp;
+      Q.dhedz = (R[h].he - A[h].he) / dzp;
+      Q.tni = L == nk - 1 ? A[h].tn : .5 * (R[h].tn + A[h].tn);  // :89-93
+    }
+    Q.barm = R[h].barm;
+    double df[3], al[9];
+    for (int n = 0; n < 3; ++n) df[n] = pow(Q.tni / T00, 1.75 - ssn(n));
+    al[0] = -(PHI(1, 4) + (PHI(1, 2) - PHI(1, 4)) * Q.pso1 + (df[0] * PHI(1, 3) - PHI(1, 4)) * Q.pshe);
+    al[4] = -(PHI(2, 4) + (PHI(2, 1) - PHI(2, 4)) * Q.pso2 + (df[1] * PHI(2, 3) - PHI(2, 4)) * Q.pshe);
+    al[8] = -(df[2] * PHI(3, 4) + (df[0] * PHI(3, 1) - df[2] * PHI(3, 4)) * Q.pso2 +
+              (df[1] * PHI(3, 2) - df[2] * PHI(3, 4)) * Q.pso1);
+    al[3] = (PHI(1, 2) - PHI(1, 4)) * Q.pso2;
+    al[6] = (df[0] * PHI(1, 3) - PHI(1, 4)) * Q.pso2;
+    al[1] = (PHI(2, 1) - PHI(2, 4)) * Q.pso1;
+    al[7] = (df[1] * PHI(2, 3) - PHI(2, 4)) * Q.pso1;
+    al[2] = (df[0] * PHI(3, 1) - df[2] * PHI(3, 4)) * Q.pshe;
+    al[5] = (df[1] * PHI(3, 2) - df[2] * PHI(3, 4)) * Q.pshe;
+    matinv3_ng(al, Q.inv);
+  }
+  double tnim[2], tnip[2];
+  wave_dn(S[0].tni, S[1].tni, lane, tnim[0], tnim[1]);
+  wave_up(S[0].tni, S[1].tni, lane, tnip[0], tnip[1]);
+  // species terms at level L
+  double ax[2][2], thd[2][2], slo[2][2], s0p[2][2], fts[2][2], hadv[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    const int Lc = L < nk ? L : nk - 1;
+    size_t p = G.a3(Lc, i, j);
+    double dtnidz;
+    if (L == 0) dtnidz = tnip[h] - S[h].tni;
+    else if (L == nk - 1) dtnidz = S[h].tni - tnim[h];
+    else dtnidz = (tnip[h] - tnim[h]) / 2.;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const SpeciesConst &cq = sc[q];
+      const MinorShared &Q = S[h];
+      double sai[3];
+      for (int n = 0; n < 3; ++n) {
+        double t = 0.;
+        for (int m = 0; m < 3; ++m) t = t + cq.salfax[m] * Q.inv[m + 3 * n];
+        sai[n] = t;
+      }
+      double ex = 1. - (cq.rmx + Q.dmdz) / Q.barm +
+                  sai[0] * (Q.do2dz - (1. - (RMO2 + Q.dmdz) / Q.barm) * Q.pso2) +
+                  sai[1] * (Q.do1dz - (1. - (RMO1 + Q.dmdz) / Q.barm) * Q.pso1) +
+                  sai[2] * (Q.dhedz - (1. - (RMHE + Q.dmdz) / Q.barm) * Q.pshe);
+      ax[h][q] = -(Q.barm / (TAU * RMN2) * pow(T00 / Q.tni, 0.25) /
+                   (cq.phix[2] + cq.salfax[0] * Q.pso2 + cq.salfax[1] * Q.pso1 + cq.salfax[2] * Q.pshe));
+      thd[h][q] = ex - cq.alfax * dtnidz / (dzp * Q.tni);
+      fts[h][q] = MT.ftm1s[SP[q]][p];
+      hadv[h][q] = MT.hadvec[SP[q]][p];
+      if (SP[q] == 3) { slo[h][q] = 0.; s0p[h][q] = 0.; }
+      else { slo[h][q] = MT.sloss[SP[q]][p]; s0p[h][q] = MT.s0prod[SP[q]][p]; }
+    }
+  }
+  // level L+1 terms from the next lane
+  double axn[2][2], thdn[2][2], dmn[2], wn[2];
+  {
+    double dm0 = S[0].dmdz / S[0].barm, dm1 = S[1].dmdz / S[1].barm;
+    wave_up(dm0, dm1, lane, dmn[0], dmn[1]);
+    wave_up(R[0].w, R[1].w, lane, wn[0], wn[1]);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      double u0, u1;
+      wave_up(ax[0][q], ax[1][q], lane, u0, u1);
+      axn[0][q] = u0; axn[1][q] = u1;
+      wave_up(thd[0][q], thd[1][q], lane, u0, u1);
+      thdn[0][q] = u0; thdn[1][q] = u1;
+    }
+  }
+  // rows of the two systems (:160-198), identity beyond the column
+  TriRow rlo[2], rhi[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    const double dmdzc = S[h].dmdz / S[h].barm;
+    const int Lc = L < nk ? L : nk - 1;
+    const double wic = L < nk - 1 ? .5 * (R[h].w + wn[h]) : 1.5 * R[h].w - .5 * A[h].w;
+    const double difk0 = g.difk[Lc], difk1 = g.difk[L + 1 < nk ? L + 1 : nk - 1], expzk = g.expz[Lc];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      double pc, qc, rc, fr;
+      if (L <= nk - 2) {
+        double molp = ax[h][q] * (1. / dzp + .5 * thd[h][q]);
+        double molr = axn[h][q] * (1. / dzp - .5 * thdn[h][q]);
+        double molq = ax[h][q] * (1. / dzp - .5 * thd[h][q]) + axn[h][q] * (1. / dzp + .5 * thdn[h][q]);
+        double eddyp = g.expzmid_inv * difk0 * (1. / dzp - .5 * dmdzc);
+        double eddyr = g.expzmid * difk1 * (1. / dzp + .5 * dmn[h]);
+        double eddyq = g.expzmid_inv * difk0 * (1. / dzp + .5 * dmdzc) +
+                       g.expzmid * difk1 * (1. / dzp - .5 * dmn[h]);
+        pc = molp / dzp - expzk * (eddyp * dfac + .5 * wic) / dzp;
+        rc = molr / dzp - expzk * (eddyr * dfac - .5 * wic) / dzp;
+        qc = -molq / dzp + expzk * (eddyq * dfac / dzp + dtx2inv - slo[h][q]);
+        fr = fts[h][q] * dtx2inv - hadv[h][q] + s0p[h][q];
+        fr = fr * expzk;
+        if (L == 0) {
+          qc = qc + pc * (l1[q] + .5 * l2[q] * dzp) / (l1[q] - .5 * l2[q] * dzp);
+          fr = fr - pc * l3[q] * dzp / (l1[q] - .5 * l2[q] * dzp);
+          pc = 0.;
+        }
+      } else {
+        pc = 1. + .5 * dzp * thd[h][q];
+        qc = pc - 2.;
+        rc = 0.;
+        fr = -GRAV * sc[q].rmx * fubc * dzp / (P0 * ax[h][q] * AVO);
+      }
+      TriRow t = L < nk ? TriRow{pc, qc, rc, fr} : tri_ident();
+      if (h == 0) rlo[q] = t; else rhi[q] = t;
+    }
+  }
+  wave_pcr<2>(rlo, rhi, lane);
+  // outputs (:234-237)
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    if (L >= nk) continue;
+    size_t p = G.a3(L, i, j);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const TriRow &t = h == 0 ? rlo[q] : rhi[q];
+      double x = t.f / t.b;
+      double nm = DTSMOOTH * fc[q][p] + DTSMOOTH_DIV2 * (fctm1[q][p] + x);
+      outs[q][p] = x > small ? x : small;
+      outs_nm[q][p] = nm > small ? nm : small;
+    }
+  }
+}
+
+
+// dt_ng.F90:54-213, one wavefront per column: rows 0..nk-2 solved by
+// parallel cyclic reduction, the top level extrapolated (:167).
+__global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                      tgx_ng_out out,
+                                                      const double *__restrict__ hadv_tn,
+                                                      const double *__restrict__ smooth_tn_nm) {
+  int i, j;
+  if (!wave_column_of(G, i, j)) return;
+  const int lane = threadIdx.x & 63, nk = G.nk;
+  const int Lr[2] = {lane, lane + 64};
+  const double dz = g.dz, dtx2inv = g.dtx2inv;
+  const size_t c2 = G.a2(i, j);
+  const double t_lbc = in.t_lbc[c2], tl = in.tlbc[c2];
+  if (lane == 0) {
+    out.tlbc_nm[c2] = tl;
+    out.tlbc[c2] = t_lbc;
+  }
+  double v;
+  if (lateral_value(G, in.lonb_tn, in.latb_tn, 0, i, j, v)) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int L = Lr[h];
+      if (L >= nk) continue;
+      lateral_value(G, in.lonb_tn, in.latb_tn, L, i, j, v);
+      size_t p = G.a3(L, i, j);
+      out.tn_nm[p] = DTSMOOTH_DIV2 * (in.tn_nm[p] + v) + DTSMOOTH * in.tn[p];
+      out.tn[p] = v > 100. ? v : 100.;
+    }
+    return;
+  }
+  struct DL { double tn, w, cp, km, un, vn, qtotal, schti, kt; };
+  DL R[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
+    R[h].tn = in.tn[p]; R[h].w = in.w_upd[p]; R[h].cp = in.cp[p]; R[h].km = in.km[p];
+    R[h].un = in.un[p]; R[h].vn = in.vn[p]; R[h].qtotal = in.qtotal[p]; R[h].schti = in.schti[p];
+    R[h].kt = in.kt[p];
+  }
+  double tnm[2], tnp[2], unm[2], unp[2], vnm[2], vnp[2], wp[2], cpp[2], kmp[2], qtp[2];
+  wave_dn(R[0].tn, R[1].tn, lane, tnm[0], tnm[1]);
+  wave_up(R[0].tn, R[1].tn, lane, tnp[0], tnp[1]);
+  wave_dn(R[0].un, R[1].un, lane, unm[0], unm[1]);
+  wave_up(R[0].un, R[1].un, lane, unp[0], unp[1]);
+  wave_dn(R[0].vn, R[1].vn, lane, vnm[0], vnm[1]);
+  wave_up(R[0].vn, R[1].vn, lane, vnp[0], vnp[1]);
+  wave_up(R[0].w, R[1].w, lane, wp[0], wp[1]);
+  wave_up(R[0].cp, R[1].cp, lane, cpp[0], cpp[1]);
+  wave_up(R[0].km, R[1].km, lane, kmp[0], kmp[1]);
+  wave_up(R[0].qtotal, R[1].qtotal, lane, qtp[0], qtp[1]);
+  // g(L), f(L) of dt_ng.F90:121-139 at this lane's levels
+  double gg[2], ff[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h], Lc = L < nk ? L : nk - 1;
+    double tni = L == 0 ? tl : .5 * (tnm[h] + R[h].tn);
+    double hh = R[h].schti;
+    double rho = P0 * g.expzmid_inv / (hh * GRAV);
+    rho = rho * g.expz[Lc];
+    double gpart = hh * hh * rho * R[h].cp;
+    double gpart_a = gpart * g.dift[Lc];
+    gg[h] = GRAV * (R[h].kt + gpart_a) / (P0 * hh * (dz * dz));
+    double f = GRAV * GRAV * (hh * hh) * rho / (tni * P0 * 2. * dz);
+    ff[h] = f * g.dift[Lc];
+  }
+  double ggp[2], ffp[2];
+  wave_up(gg[0], gg[1], lane, ggp[0], ggp[1]);
+  wave_up(ff[0], ff[1], lane, ffp[0], ffp[1]);
+  // dudz, dvdz (:85-94); the top row takes a third of the row below's
+  double dudz[2], dvdz[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    if (L == 0) {
+      dudz[h] = (R[h].un + 1. / 3. * unp[h] - 4. / 3. * in.ulbc[c2]) / dz;
+      dvdz[h] = (R[h].vn + 1. / 3. * vnp[h] - 4. / 3. * in.vlbc[c2]) / dz;
+    } else {
+      dudz[h] = (unp[h] - unm[h]) / (2. * dz);
+      dvdz[h] = (vnp[h] - vnm[h]) / (2. * dz);
+    }
+  }
+  double dudzm[2], dvdzm[2];
+  wave_dn(dudz[0], dudz[1], lane, dudzm[0], dudzm[1]);
+  wave_dn(dvdz[0], dvdz[1], lane, dvdzm[0], dvdzm[1]);
+  TriRow rows[2][1];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    if (L > nk - 2) { rows[h][0] = tri_ident(); continue; }
+    size_t p = G.a3(L, i, j);
+    const double expzk = g.expz[L];
+    double du = dudz[h], dv = dvdz[h];
+    if (L == nk - 2) { du = dudzm[h] / 3.; dv = dvdzm[h] / 3.; }
+    // veradv (:57-65)
+    double raw_k = L == 0 ? (R[h].tn - t_lbc) * R[h].w * 2. : (R[h].tn - tnm[h]) * R[h].w;
+    double raw_k1 = (L + 1 <= nk - 2) ? (tnp[h] - R[h].tn) * wp[h] : 0.;
+    double veradv = (raw_k + raw_k1) * .5;
+    veradv = veradv / dz;
+    double cpi = .5 * (R[h].cp + cpp[h]);
+    double kmi = .5 * (R[h].km + kmp[h]);
+    double wi = .5 * (R[h].w + wp[h]);
+    double advec_tn = hadv_tn[p] + veradv;
+    double cptn = cpi * (advec_tn - dtx2inv * smooth_tn_nm[p]);
+    cptn = cptn * expzk;
+    double qsolar = .5 * (R[h].qtotal + qtp[h]);
+    double t = in.xnmbar[p] * in.o1[p] * RIO1;
+    const double mbar = in.mbar[p];
+    double qrecomb = TSURPLUS * in.rkm12[p] * (t * t) * AVO / mbar;
+    double qm = GRAV * kmi / (P0 * in.scht[p]) * (du * du + dv * dv);
+    qm = qm / expzk;
+    double total_heat = qsolar + in.hdt[p] + qrecomb + in.qji_tn[p] + qm;
+    double total_heat_a = total_heat * expzk;
+    cptn = cptn - total_heat_a;
+    double pc = gg[h] - ff[h], qc, rc;
+    if (L <= nk - 3) {
+      qc = -(ggp[h] - ffp[h] + gg[h] + ff[h]);
+      rc = ggp[h] + ffp[h];
+    } else {
+      qc = -gg[h] - ff[h];
+      rc = 0.;
+    }
+    double rhs = cptn;
+    double qpart = cpi * (dtx2inv + in.cool_imp[p]) + wi * GASK / mbar;
+    rhs = rhs + in.cool_exp[p];
+    double qpart_a = qpart * expzk;
+    qc = qc - qpart_a;
+    if (L == 0) {
+      qc = qc - pc;
+      rhs = rhs - 2. * pc * t_lbc;
+      pc = 0.;
+    }
+    rows[h][0] = TriRow{pc, qc, rc, rhs};
+  }
+  TriRow lo[1] = {rows[0][0]}, hi[1] = {rows[1][0]};
+  wave_pcr<1>(lo, hi, lane);
+  double x[2] = {lo[0].f / lo[0].b, hi[0].f / hi[0].b}, xm1[2], xm2[2];
+  wave_dn(x[0], x[1], lane, xm1[0], xm1[1]);
+  wave_dn(xm1[0], xm1[1], lane, xm2[0], xm2[1]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    if (L >= nk) continue;
+    double xv = L == nk - 1 ? xm1[h] * xm1[h] / xm2[h] : x[h];  // :167
+    size_t p = G.a3(L, i, j);
+    out.tn_nm[p] = DTSMOOTH_DIV2 * (in.tn_nm[p] + xv) + DTSMOOTH * in.tn[p];
+    out.tn[p] = xv > 100. ? xv : 100.;
+  }
+}
+
+
+// =============================================================================
 // comp_o2o_ng.F90:44-81 (fused) + comp_ng.F90:71-457
 struct CompTerms {  // prelude planes: advection of O2, O, He, smoothed *_nm, and
                     // fs(:, isp, m) of comp_o2o_ng as fs01 fs02 fs11 fs12 fs00 fs10
@@ -1200,6 +1904,265 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
 #undef F3
 }
 
+// comp_ng.F90:71-457 with one wavefront per column: the 3x3 block rows
+// 0..nk-2 by block cyclic reduction, the closed-form top level (:317-332),
+// then the post pass (:415-457) with the N2 extrapolation level found by a
+// wave ballot.
+__global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                        tgx_ng_out out, CompTerms CT) {
+  int i, j;
+  if (!wave_column_of(G, i, j)) return;
+  const int lane = threadIdx.x & 63, nk = G.nk;
+  const int Lr[2] = {lane, lane + 64};
+  const double dz = g.dz, dtx2inv = g.dtx2inv, thdiffalpha = -0.38, small = 1.e-9;
+  const size_t c2 = G.a2(i, j);
+  const double fhe = g.calc_helium == 0 ? 0. : in.flx_he[c2];
+  if (lane == 0) out.flx_he[c2] = fhe;
+  double x[2][3];  // O2, O, He at this lane's levels before the post pass
+  double v;
+  const bool bnd = lateral_value(G, in.lonb_o2, in.latb_o2, 0, i, j, v);
+  if (bnd) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int Lc = Lr[h] < nk ? Lr[h] : nk - 1;
+      lateral_value(G, in.lonb_o2, in.latb_o2, Lc, i, j, x[h][0]);
+      lateral_value(G, in.lonb_o1, in.latb_o1, Lc, i, j, x[h][1]);
+      lateral_value(G, in.lonb_he, in.latb_he, Lc, i, j, x[h][2]);
+    }
+  } else {
+    const double dfac = dfactor_at(g, j);
+    const double tl = in.t_lbc[c2];
+    const double rmass[3] = {RMO2, RMO1, RMHE};
+    const double o2_0 = in.o2[c2], o1_0 = in.o1[c2], he_0 = in.he[c2];
+    double ps0[3];
+    for (int l = 0; l < 3; ++l)
+      ps0[l] = g.b[3 * l + 0] * o2_0 + g.b[3 * l + 1] * o1_0 + g.b[3 * l + 2] * he_0 + g.fb[l];
+    const double embar0 =
+        1. / (ps0[0] * RIO2 + ps0[1] * RIO1 + ps0[2] * RIHE + (1. - ps0[0] - ps0[1] - ps0[2]) * RIN2);
+    struct LR { double o2, o1, he, mbar, tn, w; };
+    LR R[2], A[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
+      R[h].o2 = in.o2[p]; R[h].o1 = in.o1[p]; R[h].he = in.he[p]; R[h].mbar = in.mbar[p];
+      R[h].tn = in.tn[p]; R[h].w = in.w[p];
+    }
+    wave_dn(R[0].o2, R[1].o2, lane, A[0].o2, A[1].o2);
+    wave_dn(R[0].o1, R[1].o1, lane, A[0].o1, A[1].o1);
+    wave_dn(R[0].he, R[1].he, lane, A[0].he, A[1].he);
+    wave_dn(R[0].mbar, R[1].mbar, lane, A[0].mbar, A[1].mbar);
+    wave_dn(R[0].tn, R[1].tn, lane, A[0].tn, A[1].tn);
+    // interface L (between levels L-1 and L): ak = matinv3(alpha)*wks1, ep, wks
+    double ak[2][9], ep[2][3], wks[2], a22[2], a33[2], a23[2], a32[2], flx[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int L = Lr[h];
+      double o2i, o1i, hei, embari, tni, dembardz, dtndz, df[3];
+      if (L == 0) {  // :80-97
+        o2i = .5 * (ps0[0] + o2_0);
+        o1i = .5 * (ps0[1] + o1_0);
+        hei = .5 * (ps0[2] + he_0);
+        embari = .5 * (embar0 + R[h].mbar);
+        tni = tl;
+        dembardz = R[h].mbar - embar0;
+        dtndz = (R[h].tn - tl) * 2;
+      } else {
+        o2i = .5 * (A[h].o2 + R[h].o2);
+        o1i = .5 * (A[h].o1 + R[h].o1);
+        hei = .5 * (A[h].he + R[h].he);
+        embari = .5 * (A[h].mbar + R[h].mbar);
+        tni = .5 * (A[h].tn + R[h].tn);
+        dembardz = R[h].mbar - A[h].mbar;
+        dtndz = R[h].tn - A[h].tn;
+      }
+      dembardz = dembardz / dz;
+      dtndz = dtndz / dz;
+      for (int n = 0; n < 3; ++n) df[n] = pow(tni / T00, 1.75 - ssn(n));
+      for (int l = 0; l < 3; ++l) ep[h][l] = 1. - 1. / embari * (rmass[l] + dembardz);
+      if (L <= nk - 2) ep[h][2] = ep[h][2] - thdiffalpha * dtndz / tni;
+      double al[9];
+      const double *d_ = df;
+      al[0] = -(PHI(1, 4) + (PHI(1, 2) - PHI(1, 4)) * o1i + (d_[0] * PHI(1, 3) - PHI(1, 4)) * hei);
+      al[4] = -(PHI(2, 4) + (PHI(2, 1) - PHI(2, 4)) * o2i + (d_[1] * PHI(2, 3) - PHI(2, 4)) * hei);
+      al[8] = -(d_[2] * PHI(3, 4) + (d_[0] * PHI(3, 1) - d_[2] * PHI(3, 4)) * o2i +
+                (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * o1i);
+      al[3] = (PHI(1, 2) - PHI(1, 4)) * o2i;
+      al[6] = (d_[0] * PHI(1, 3) - PHI(1, 4)) * o2i;
+      al[1] = (PHI(2, 1) - PHI(2, 4)) * o1i;
+      al[7] = (d_[1] * PHI(2, 3) - PHI(2, 4)) * o1i;
+      al[2] = (d_[0] * PHI(3, 1) - d_[2] * PHI(3, 4)) * hei;
+      al[5] = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * hei;
+      matinv3_ng(al, ak[h]);
+      double wks1 = embari * RIN2 * pow(T00 / tni, 0.25) / TAU;
+      for (int n = 0; n < 9; ++n) ak[h][n] = ak[h][n] * wks1;
+      wks[h] = dembardz / (embari * 2.);
+      // upper boundary terms of the top interface (:601-652)
+      a22[h] = al[4];
+      a33[h] = al[8];
+      a23[h] = al[7];
+      a32[h] = al[5];
+      flx[h] = 0.;
+      if (L == nk - 2) flx[h] = embari * RIN2 * P0 * pow(T00 / R[h].tn, 0.25) / (TAU * GRAV);
+    }
+    // the interface above (L+1) from the next lane
+    double akp[2][9], epp[2][3], wksp[2], wp[2], a22p[2], a33p[2], a23p[2], a32p[2];
+    for (int n = 0; n < 9; ++n) wave_up(ak[0][n], ak[1][n], lane, akp[0][n], akp[1][n]);
+    for (int n = 0; n < 3; ++n) wave_up(ep[0][n], ep[1][n], lane, epp[0][n], epp[1][n]);
+    wave_up(wks[0], wks[1], lane, wksp[0], wksp[1]);
+    wave_up(R[0].w, R[1].w, lane, wp[0], wp[1]);
+    wave_up(a22[0], a22[1], lane, a22p[0], a22p[1]);
+    wave_up(a33[0], a33[1], lane, a33p[0], a33p[1]);
+    wave_up(a23[0], a23[1], lane, a23p[0], a23p[1]);
+    wave_up(a32[0], a32[1], lane, a32p[0], a32p[1]);
+    Nrm3 rows[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = Lr[h];
+      Nrm3 &N3 = rows[h];
+      if (k > nk - 2) {
+        for (int e = 0; e < 9; ++e) { N3.A[e] = 0.; N3.C[e] = 0.; }
+        for (int e = 0; e < 3; ++e) N3.F[e] = 0.;
+        continue;
+      }
+      size_t p = G.a3(k, i, j);
+      const double expzk = g.expz[k];
+      double fs[2][3];
+      fs[0][1] = CT.fs[0][p]; fs[0][2] = CT.fs[1][p]; fs[1][1] = CT.fs[2][p];
+      fs[1][2] = CT.fs[3][p]; fs[0][0] = CT.fs[4][p]; fs[1][0] = CT.fs[5][p];
+      const double *akm = ak[h], *epm = ep[h], *akpp = akp[h], *eppp = epp[h];
+      const double wks3 = wks[h], wks4 = wksp[h];
+      double pk[9], rk[9], qk[9], fk[3];
+      double eddyp = g.expzmid_inv * g.difk[k] * (1. / dz - wks3);
+      double eddyr = g.expzmid * g.difk[k + 1] * (1. / dz + wks4);
+      double eddyq = g.expzmid_inv * g.difk[k] * (1. / dz + wks3) + g.expzmid * g.difk[k + 1] * (1. / dz - wks4);
+      const double wik = .5 * (R[h].w + wp[h]);
+      for (int m = 0; m < 3; ++m)
+        for (int isp = 0; isp < 3; ++isp) {
+          int e = isp + 3 * m;
+          double molp = akm[e] * (1. / dz + epm[m] / 2.);
+          double molr = akpp[e] * (1. / dz - eppp[m] / 2.);
+          double molq = akm[e] * (1. / dz - epm[m] / 2.) + akpp[e] * (1. / dz + eppp[m] / 2.);
+          double delta = isp == m ? 1. : 0.;
+          double fsv = (isp < 2 && m < 2) ? fs[isp][m + 1] : 0.;
+          pk[e] = molp / dz - expzk * delta / dz * (eddyp * dfac + .5 * wik);
+          rk[e] = molr / dz - expzk * delta / dz * (eddyr * dfac - .5 * wik);
+          qk[e] = -molq / dz + expzk * delta * (eddyq * dfac / dz + dtx2inv) - expzk * fsv;
+        }
+      fk[0] = expzk * (CT.o2s[p] * dtx2inv - (CT.o2adv[p] - fs[0][0]) + in.hdo2[p]);
+      fk[1] = expzk * (CT.o1s[p] * dtx2inv - (CT.o1adv[p] - fs[1][0]) + in.hdo1[p]);
+      fk[2] = expzk * (CT.hes[p] * dtx2inv - (CT.headv[p] - 0.) + in.hdhe[p]);
+      if (k == 0) {
+        for (int m = 0; m < 3; ++m)
+          for (int n = 0; n < 3; ++n)
+            for (int isp = 0; isp < 3; ++isp) qk[isp + 3 * m] = qk[isp + 3 * m] + pk[isp + 3 * n] * g.b[3 * n + m];
+        for (int m = 0; m < 3; ++m)
+          for (int isp = 0; isp < 3; ++isp) fk[isp] = fk[isp] - pk[isp + 3 * m] * g.fb[m];
+        for (int n = 0; n < 9; ++n) pk[n] = 0.;
+      }
+      if (k == nk - 2) {
+        for (int m = 0; m < 3; ++m)
+          for (int isp = 0; isp < 3; ++isp)
+            qk[isp + 3 * m] = qk[isp + 3 * m] + (1. + .5 * eppp[m] * dz) / (1. - .5 * eppp[m] * dz) * rk[isp + 3 * m];
+        const double flx00 = flx[h];
+        const double alpha22 = a22p[h], alpha33 = a33p[h], alpha23 = a23p[h], alpha32 = a32p[h];
+        for (int isp = 0; isp < 3; ++isp)
+          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * eppp[1])) -
+                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * eppp[2]));
+        for (int n = 0; n < 9; ++n) rk[n] = 0.;
+      }
+      // to row-major blocks, normalised by the diagonal block
+      double Bq[9], Ap[9], Cr[9];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          Bq[3 * r + c] = qk[r + 3 * c];
+          Ap[3 * r + c] = pk[r + 3 * c];
+          Cr[3 * r + c] = rk[r + 3 * c];
+        }
+      double Bi[9];
+      blk_inv<3>(Bq, Bi);
+      blk_mm<3>(Bi, Ap, N3.A);
+      blk_mm<3>(Bi, Cr, N3.C);
+      blk_mv<3>(Bi, fk, N3.F);
+    }
+    wave_pcr3(rows[0], rows[1], lane);
+    // the closed-form top level (:317-332) from level nk-2
+    double xm[2][3], flxm[2];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) wave_dn(rows[0].F[n], rows[1].F[n], lane, xm[0][n], xm[1][n]);
+    wave_dn(flx[0], flx[1], lane, flxm[0], flxm[1]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      for (int n = 0; n < 3; ++n) x[h][n] = rows[h].F[n];
+      if (Lr[h] == nk - 1) {
+        const double *e_ = ep[h];  // ep of the top interface
+        double top[3];
+        for (int isp = 0; isp < 3; ++isp) top[isp] = (1. + .5 * e_[isp] * dz) / (1. - .5 * e_[isp] * dz) * xm[h][isp];
+        const double flx00 = flxm[h];
+        top[1] = top[1] + (a23[h] - a22[h]) * fhe / (flx00 * (1. / dz - 0.5 * e_[1]));
+        top[2] = top[2] + (a33[h] - a32[h]) * fhe / (flx00 * (1. / dz - 0.5 * e_[2]));
+        for (int n = 0; n < 3; ++n) x[h][n] = top[n];
+      }
+      if (g.calc_helium == 0) x[h][2] = g.pshelb;
+    }
+  }
+  // :415-457 time smoothing, N2 above the O-dominated top, clamps, normalisation
+  double n2[2], xn[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
+    xn[h] = in.xnmbar[p];
+    n2[h] = 1. - x[h][0] - x[h][1] - x[h][2];
+  }
+  // first level with n2 < 0.01 over the column
+  const unsigned long long blo = __ballot(Lr[0] < nk && n2[0] < 0.01);
+  const unsigned long long bhi = __ballot(Lr[1] < nk && n2[1] < 0.01);
+  int kf = -1;
+  if (blo) kf = __ffsll((long long)blo) - 1;
+  else if (bhi) kf = 64 + __ffsll((long long)bhi) - 1;
+  bool extrap = false;
+  double n2top = 0., dn2dz = 0., zmtop = 0.;
+  {
+    // n2 and xnmbar at kf and kf-1 from the lanes holding them (uniform)
+    int kk = kf >= 0 ? kf : 0, km1 = kf >= 1 ? kf - 1 : 0;
+    double n2a = __shfl(kk < 64 ? n2[0] : n2[1], kk & 63, 64);
+    double xna = __shfl(kk < 64 ? xn[0] : xn[1], kk & 63, 64);
+    double n2b = __shfl(km1 < 64 ? n2[0] : n2[1], km1 & 63, 64);
+    double xnb = __shfl(km1 < 64 ? xn[0] : xn[1], km1 & 63, 64);
+    if (kf >= 1 && kf + 1 < nk) {
+      extrap = true;
+      zmtop = g.zpmid[kf];
+      n2top = log(n2a * xna);
+      double n2top1 = log(n2b * xnb);
+      dn2dz = (n2top - n2top1) / dz;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = Lr[h];
+    if (k >= nk) continue;
+    size_t p = G.a3(k, i, j);
+    double o2u = x[h][0], o1u = x[h][1], heu = x[h][2];
+    double o2nm = DTSMOOTH_DIV2 * (in.o2_nm[p] + o2u) + DTSMOOTH * in.o2[p];
+    double o1nm = DTSMOOTH_DIV2 * (in.o1_nm[p] + o1u) + DTSMOOTH * in.o1[p];
+    double henm = DTSMOOTH_DIV2 * (in.he_nm[p] + heu) + DTSMOOTH * in.he[p];
+    double n2v = n2[h];
+    double n2nm = 1. - o2nm - o1nm - henm;
+    if (extrap && k > kf) n2v = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / xn[h];
+    double a = o2u > small ? o2u : small, b = o1u > small ? o1u : small;
+    double c = heu > small ? heu : small, d = n2v > small ? n2v : small;
+    double norm = a + b + c + d;
+    out.o2[p] = a / norm;
+    out.o1[p] = b / norm;
+    out.he[p] = c / norm;
+    double an = o2nm > small ? o2nm : small, bn = o1nm > small ? o1nm : small;
+    double cn = henm > small ? henm : small, dn = n2nm > small ? n2nm : small;
+    double normn = an + bn + cn + dn;
+    out.o2_nm[p] = an / normn;
+    out.o1_nm[p] = bn / normn;
+    out.he_nm[p] = cn / normn;
+  }
+}
+
 
 // =============================================================================
 // oplus_ng.F90, sub-step stage 1 (every column, ghosts included): the
@@ -1500,6 +2463,165 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
 #undef F3
 }
 
+// oplus_stage3 with one wavefront per column: rows 0..nk-2 by parallel
+// cyclic reduction, the top level extrapolated (:307).
+__global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                         const double *__restrict__ op,
+                                                         const double *__restrict__ optm1,
+                                                         const double *__restrict__ lonb,
+                                                         const double *__restrict__ latb, OpWork W,
+                                                         OpTerms OT,
+                                                         const double *__restrict__ oploss_p,
+                                                         const double *__restrict__ opprod_p,
+                                                         double *__restrict__ opout,
+                                                         double *__restrict__ optm1out) {
+  int i, j;
+  if (!wave_column_of(G, i, j)) return;
+  const int lane = threadIdx.x & 63, nk = G.nk, nsub = g.nstep_sub;
+  const int Lr[2] = {lane, lane + 64};
+  const double dz = g.dz, dtx2inv = g.dtx2inv;
+  const size_t c2 = G.a2(i, j);
+  double v;
+  if (lateral_value(G, lonb, latb, 0, i, j, v)) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int L = Lr[h];
+      if (L >= nk) continue;
+      size_t p = G.a3(L, i, j);
+      lateral_value(G, lonb, latb, L, i, j, v);
+      double nm = DTSMOOTH * op[p] + DTSMOOTH_DIV2 * (optm1[p] + v);
+      opout[p] = v > 0. ? v : 0.;
+      optm1out[p] = nm > 0. ? nm : 0.;
+    }
+    return;
+  }
+  const double pi = 4.0 * atan(1.0), rtd = 180. / pi;
+  const double bz = in.bz[c2], cs = g.cs[j];
+  double opflux;
+  {  // :77-91
+    const double phid = 2.0e8, phin = -2.0e8, ppolar = 0.;
+    double rl = in.rlatm[c2], a = 0.;
+    if (fabs(rl) >= pi / 4.5) a = 1.;
+    if (fabs(rl) < pi / 4.5 && fabs(rl) > pi / 18) a = .5 * (1. + cos(fabs(rl) * 6. - pi * 4. / 3.));
+    if (fabs(rl) <= pi / 18) a = 0.;
+    double fed = phid * a, fen = phin * a, chi = in.chi[c2];
+    if (chi >= 0.5 * pi) opflux = fen; else opflux = fed;
+    if ((chi * rtd - 80.) * (chi * rtd - 100.) < 0.)
+      opflux = .5 * (fed + fen) + .5 * (fed - fen) * cos(pi * (chi * rtd - 80.) / 20.);
+    if (fabs(rl) >= pi / 3.) opflux = opflux + ppolar;
+  }
+  double dvb;
+  {  // :94-104 (df_2d of bx and of cs*by)
+    double dbxdx = dflon1(in.bx, G, i, j);
+    auto csby = [&](int jj) { return g.cs[jj] * in.by[G.a2(i, jj)]; };
+    int J1 = G.nlatd - 1;
+    double dcsbydy;
+    if (j == 0) dcsbydy = csby(1) - csby(0);
+    else if (j == J1) dcsbydy = csby(J1) - csby(J1 - 1);
+    else dcsbydy = (csby(j + 1) - csby(j - 1)) / 2.;
+    double vv = dbxdx / g.dlamda + dcsbydy / g.dphi;
+    vv = vv / cs;
+    dvb = (vv + 2. * bz) / RE;
+  }
+  const double dvbz = dvb / bz;
+  const double gmr = GRAV * RMOP / (2. * GASK);
+  struct OL { double scht, hdz, tp1, dj, bdotu, bdopj, wi; };
+  OL B[2], A[2], Cn[2];  // this lane's level, the one below and the one above
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
+    B[h].scht = in.scht[p];
+    B[h].hdz = 1. / (B[h].scht * dz);
+    B[h].tp1 = 0.5 * (in.ti[p] + in.te[p]);
+    B[h].dj = W.dj[p]; B[h].bdotu = W.bdotu[p]; B[h].bdopj = W.bdopj[p]; B[h].wi = in.wi[p];
+  }
+#define TGX_NBR(fld)                                                          \
+  wave_dn(B[0].fld, B[1].fld, lane, A[0].fld, A[1].fld);                      \
+  wave_up(B[0].fld, B[1].fld, lane, Cn[0].fld, Cn[1].fld);
+  TGX_NBR(hdz) TGX_NBR(tp1) TGX_NBR(dj) TGX_NBR(bdotu) TGX_NBR(bdopj) TGX_NBR(wi)
+#undef TGX_NBR
+  const double ol1 = oploss_p[G.a3(1, i, j)], qop0 = in.qop[c2];  // lower boundary (:301-303)
+  TriRow lo[1], hi[1];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = Lr[h];
+    TriRow t = tri_ident();
+    if (k <= nk - 2) {
+      size_t p = G.a3(k, i, j);
+      const OL &qa = A[h], &qb = B[h], &qc = Cn[h];
+      const bool top = k == nk - 2;
+      double dbd;
+      if (k == 0) dbd = qc.bdopj - qb.bdopj;
+      else if (k <= nk - 3) dbd = (qc.bdopj - qa.bdopj) / 2.;
+      else dbd = qb.bdopj - qa.bdopj;
+      double bdzdvb = bz / (qb.scht * dz) * dbd + dvb * qb.bdopj;
+      double bd_op = W.djbz[p] * OT.bd_diffj[p];
+      double diffexp = bdzdvb + OT.bd_diff[p] + bd_op;
+      double expl = -1. * diffexp;
+      expl = expl + OT.vdotn[p] + OT.bd_bvel[p];
+      expl = expl - OT.opts2[p] * dtx2inv * nsub;
+      double divbz = dvbz + OT.bd_djbz[p] / (qb.dj * (bz * bz));
+      double hm = qb.hdz - 0.5 * divbz, hp = qb.hdz + 0.5 * divbz;
+      hm = hm * (bz * bz);
+      hp = hp * (bz * bz);
+      double hdzi_k = k == 0 ? 1.5 * qb.hdz - 0.5 * qc.hdz : 0.5 * (qa.hdz + qb.hdz);
+      double hdzi_k1 = top ? 1.5 * qb.hdz - 0.5 * qa.hdz : 0.5 * (qb.hdz + qc.hdz);
+      double dji = k == 0 ? 1.5 * qb.dj - 0.5 * qc.dj : 0.5 * (qa.dj + qb.dj);
+      double dji1 = top ? 1.5 * qb.dj - 0.5 * qa.dj : 0.5 * (qb.dj + qc.dj);
+      double t0 = k == 0 ? 2. * (2. * qb.tp1 - qc.tp1) * hdzi_k - gmr : 2. * qa.tp1 * hdzi_k - gmr;
+      double t01 = 2. * qb.tp1 * hdzi_k1 - gmr;
+      double t1 = 2. * qb.tp1 * hdzi_k + gmr;
+      double t11 = top ? 2. * (2. * qb.tp1 - qa.tp1) * hdzi_k1 + gmr : 2. * qc.tp1 * hdzi_k1 + gmr;
+      double diffp = hm * dji * t0;
+      double diffq = -(hp * dji1 * t01 + hm * dji * t1);
+      double diffr = hp * dji1 * t11;
+      double hdzk = qb.hdz;
+      double wii = .5 * (qb.wi + qc.wi);
+      double driftp = wii * 0.5 * hdzk;
+      double driftq = -wii * 6. / RE;
+      double driftr = -wii * 0.5 * hdzk;
+      double windp = k == 0 ? bz * (2. * qb.bdotu - qc.bdotu) * 0.5 * hdzk : bz * qa.bdotu * 0.5 * hdzk;
+      double windq = -qb.bdotu * dvbz * bz;
+      double windr = !top ? -bz * qc.bdotu * 0.5 * hdzk : bz * (2. * qb.bdotu - qa.bdotu) * 0.5 * hdzk;
+      double pcf = diffp + driftp + windp;
+      double qcf = diffq + driftq + windq - dtx2inv * nsub;
+      double rcf = diffr + driftr + windr;
+      if (top) {  // upper boundary (:280-286)
+        double djn = dji1;
+        double ubcb = -(bz * bz) * djn * t01;
+        double ubca = -(bz * bz) * djn * t11;
+        qcf = qcf + ubcb / ubca * rcf;
+        expl = expl - opflux * rcf / ubca;
+        rcf = 0.;
+      }
+      const double oploss = oploss_p[p];
+      qcf = qcf - oploss;
+      expl = expl - opprod_p[p];
+      if (k == 0) {  // lower boundary (:301-303)
+        qcf = qcf - pcf;
+        expl = expl - 2. * pcf * qop0 / (1.5 * oploss - 0.5 * ol1);
+        pcf = 0.;
+      }
+      t = TriRow{pcf, qcf, rcf, expl};
+    }
+    if (h == 0) lo[0] = t; else hi[0] = t;
+  }
+  wave_pcr<1>(lo, hi, lane);
+  double x[2] = {lo[0].f / lo[0].b, hi[0].f / hi[0].b}, xm1[2], xm2[2];
+  wave_dn(x[0], x[1], lane, xm1[0], xm1[1]);
+  wave_dn(xm1[0], xm1[1], lane, xm2[0], xm2[1]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    if (L >= nk) continue;
+    double xv = L == nk - 1 ? xm1[h] * xm1[h] / xm2[h] : x[h];
+    size_t p = G.a3(L, i, j);
+    double nm = DTSMOOTH * op[p] + DTSMOOTH_DIV2 * (optm1[p] + xv);
+    opout[p] = xv > 0. ? xv : 0.;
+    optm1out[p] = nm > 0. ? nm : 0.;
+  }
+}
+
 // =============================================================================
 // duv_ng.F90 stage 1 (every column): the vertical integral ztmp (:73-80)
 __global__ void __launch_bounds__(256) duv_stage1(tgx_ng_grid g, Geo G, tgx_ng_fields in,
@@ -1715,6 +2837,159 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
 #undef F3
 }
 
+// duv_stage2 with one wavefront per column: the 2x2 block rows 0..nk-2 by
+// block cyclic reduction, the top level extrapolated (:250).
+__global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                       tgx_ng_out out, UvTerms UT) {
+  int i, j;
+  if (!wave_column_of(G, i, j)) return;
+  const int lane = threadIdx.x & 63, nk = G.nk;
+  const int Lr[2] = {lane, lane + 64};
+  const double dz = g.dz, dtx2inv = g.dtx2inv;
+  const size_t c2 = G.a2(i, j);
+  if (lane == 0) {
+    out.ulbc_nm[c2] = in.ulbc[c2];
+    out.vlbc_nm[c2] = in.vlbc[c2];
+    out.ulbc[c2] = in.u_lbc[c2];
+    out.vlbc[c2] = in.v_lbc[c2];
+  }
+  double v;
+  if (lateral_value(G, in.lonb_un, in.latb_un, 0, i, j, v)) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int L = Lr[h];
+      if (L >= nk) continue;
+      size_t p = G.a3(L, i, j);
+      double vu, vv;
+      lateral_value(G, in.lonb_un, in.latb_un, L, i, j, vu);
+      lateral_value(G, in.lonb_vn, in.latb_vn, L, i, j, vv);
+      out.un[p] = vu;
+      out.vn[p] = vv;
+      out.un_nm[p] = DTSMOOTH_DIV2 * (in.un_nm[p] + vu) + DTSMOOTH * in.un[p];
+      out.vn_nm[p] = DTSMOOTH_DIV2 * (in.vn_nm[p] + vv) + DTSMOOTH * in.vn[p];
+    }
+    return;
+  }
+  const double racs = g.racs[j], cor = g.cor[j], tanphi = g.tanphi[j];
+  struct UL { double w, lxx, lyy, lxy, lyx, ui, vi, divx, divy, km, schti; };
+  UL U[2], Un[2];
+  double e1r[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int Lc = Lr[h] < nk ? Lr[h] : nk - 1;
+    size_t p = G.a3(Lc, i, j);
+    U[h].w = in.w_upd[p]; U[h].lxx = in.lxx[p]; U[h].lyy = in.lyy[p]; U[h].lxy = in.lxy[p];
+    U[h].lyx = in.lyx[p]; U[h].ui = in.ui[p]; U[h].vi = in.vi[p]; U[h].divx = UT.divx_ztmp[p];
+    U[h].divy = UT.divy_z[p]; U[h].km = in.km[p]; U[h].schti = in.schti[p];
+    double e = P0 * in.scht[p] / GRAV;  // eddy viscosity (:143-152)
+    e1r[h] = e * g.expz[Lc] * g.xmue[Lc];
+  }
+#define TGX_UP(fld) wave_up(U[0].fld, U[1].fld, lane, Un[0].fld, Un[1].fld);
+  TGX_UP(w) TGX_UP(lxx) TGX_UP(lyy) TGX_UP(lxy) TGX_UP(lyx) TGX_UP(ui) TGX_UP(vi) TGX_UP(divx) TGX_UP(divy)
+#undef TGX_UP
+  // e1 at the top level from the two below (:148-150)
+  double e1m[2], e1mm[2], e1[2];
+  wave_dn(e1r[0], e1r[1], lane, e1m[0], e1m[1]);
+  wave_dn(e1m[0], e1m[1], lane, e1mm[0], e1mm[1]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) e1[h] = Lr[h] == nk - 1 ? sqrt(e1m[h] * e1m[h] * e1m[h] / e1mm[h]) : e1r[h];
+  double e1d[2], e1u[2];
+  wave_dn(e1[0], e1[1], lane, e1d[0], e1d[1]);
+  wave_up(e1[0], e1[1], lane, e1u[0], e1u[1]);
+  double gg[2], ggp[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    double evf = Lr[h] >= 1 ? sqrt(e1d[h] * e1[h]) : e1[h] * e1[h] / sqrt(e1[h] * e1u[h]);
+    gg[h] = GRAV * (U[h].km + evf) / (P0 * U[h].schti * (dz * dz));
+  }
+  wave_up(gg[0], gg[1], lane, ggp[0], ggp[1]);
+  const double ulbc = in.u_lbc[c2], vlbc = in.v_lbc[c2];
+  BlkRow<2> rw[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = Lr[h];
+    BlkRow<2> &R = rw[h];
+    if (k > nk - 2) {
+      R.A[0] = R.A[1] = R.A[2] = R.A[3] = 0.;
+      R.C[0] = R.C[1] = R.C[2] = R.C[3] = 0.;
+      R.B[0] = 1.; R.B[1] = 0.; R.B[2] = 0.; R.B[3] = 1.;
+      R.F[0] = R.F[1] = 0.;
+      continue;
+    }
+    size_t p = G.a3(k, i, j);
+    const double expzk = g.expz[k];
+    const double gk = gg[h], g_1 = ggp[h];
+    double wi = .5 * (U[h].w + Un[h].w);
+    double dwdz = wi / (2. * dz);
+    dwdz = dwdz * expzk;
+    double lxxi = .5 * (U[h].lxx + Un[h].lxx), lyyi = .5 * (U[h].lyy + Un[h].lyy);
+    double lxyi = .5 * (U[h].lxy + Un[h].lxy), lyxi = .5 * (U[h].lyx + Un[h].lyx);
+    double uii = .5 * (U[h].ui + Un[h].ui), vii = .5 * (U[h].vi + Un[h].vi);
+    double udc = out.Fe[p] + lxxi * uii + lxyi * vii;
+    double vdc = out.Fn[p] + lyyi * vii - lyxi * uii;
+    double zl = (U[h].divx + Un[h].divx) * .5;
+    zl = zl * racs;
+    zl = zl * GRAV / g.dlamda;
+    double zp = (U[h].divy + Un[h].divy) * .5;
+    zp = zp * GRAV * RE_INV / g.dphi;
+    double s1 = dtx2inv * UT.unms[p] + in.hdu[p] + udc - UT.advu[p] - zl;
+    double s2 = dtx2inv * UT.vnms[p] + in.hdv[p] + vdc - UT.advv[p] - zp;
+    s1 = s1 * expzk;
+    s2 = s2 * expzk;
+    double p11 = -(gk + dwdz), p12 = 0., p21 = 0., p22 = p11;
+    double r11 = -(g_1 - dwdz), r12 = 0., r21 = 0., r22 = r11;
+    double q11 = gk + g_1;
+    double qa = dtx2inv + lxxi;
+    qa = qa * expzk;
+    double qq11 = q11 + qa;
+    qa = dtx2inv + lyyi;
+    qa = qa * expzk;
+    double qq22 = q11 + qa;
+    double q12 = cor + in.un[p] / RE * tanphi;
+    double q21 = q12;
+    qa = q12 - lxyi;
+    qa = qa * expzk;
+    double qq12 = -qa;
+    qa = q21 - lyxi;
+    qa = qa * expzk;
+    double qq21 = qa;
+    if (k == 0) {  // :207-212
+      qq11 = qq11 - p11; qq12 = qq12 - p12; qq21 = qq21 - p21; qq22 = qq22 - p22;
+      s1 = s1 - 2. * (p11 * ulbc + p12 * vlbc);
+      s2 = s2 - 2. * (p21 * ulbc + p22 * vlbc);
+      p11 = 0.; p12 = 0.; p21 = 0.; p22 = 0.;
+    }
+    if (k == nk - 2) {  // :214-215
+      qq11 = qq11 + r11; qq12 = qq12 + r12; qq21 = qq21 + r21; qq22 = qq22 + r22;
+      r11 = 0.; r12 = 0.; r21 = 0.; r22 = 0.;
+    }
+    R.A[0] = p11; R.A[1] = p12; R.A[2] = p21; R.A[3] = p22;
+    R.B[0] = qq11; R.B[1] = qq12; R.B[2] = qq21; R.B[3] = qq22;
+    R.C[0] = r11; R.C[1] = r12; R.C[2] = r21; R.C[3] = r22;
+    R.F[0] = s1; R.F[1] = s2;
+  }
+  double xlo[2], xhi[2];
+  wave_block_pcr<2>(rw[0], rw[1], lane, xlo, xhi);
+  double x1[2] = {xlo[0], xhi[0]}, x2[2] = {xlo[1], xhi[1]};
+  double x1m[2], x1mm[2], x2m[2], x2mm[2];
+  wave_dn(x1[0], x1[1], lane, x1m[0], x1m[1]);
+  wave_dn(x1m[0], x1m[1], lane, x1mm[0], x1mm[1]);
+  wave_dn(x2[0], x2[1], lane, x2m[0], x2m[1]);
+  wave_dn(x2m[0], x2m[1], lane, x2mm[0], x2mm[1]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int L = Lr[h];
+    if (L >= nk) continue;
+    double u = x1[h], vv = x2[h];
+    if (L == nk - 1) { u = 2 * x1m[h] - x1mm[h]; vv = 2 * x2m[h] - x2mm[h]; }  // :250
+    size_t p = G.a3(L, i, j);
+    out.un[p] = u;
+    out.vn[p] = vv;
+    out.un_nm[p] = DTSMOOTH_DIV2 * (in.un_nm[p] + u) + DTSMOOTH * in.un[p];
+    out.vn_nm[p] = DTSMOOTH_DIV2 * (in.vn_nm[p] + vv) + DTSMOOTH * in.vn[p];
+  }
+}
+
 }  // namespace ng
 }  // namespace tgx
 
@@ -1731,7 +3006,9 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   long ncol = (long)G.nlond * G.nlatd;
   int nb = (int)((ncol + 255) / 256);
   int nbp = (int)((ncol * G.nk + 255) / 256);
-  dim3 grid(nb), gridp(nbp), block(256);
+  // thread per column (grid), per column-level (gridp), wavefront per column (gridw)
+  int nbw = (int)((ncol + 3) / 4);
+  dim3 grid(nb), gridp(nbp), gridw(nbw), block(256);
   auto D = [&](int n) { return (double *)bufs[n]; };
   auto job = [](int kind, const double *f, double *o, double sh) {
     StencilJob jb;
@@ -1742,10 +3019,16 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 0:  // dt: bufs = {w2, hadv_tn, smooth_tn_nm}
       hipLaunchKernelGGL(dt_kernel, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
       break;
+    case 20:
+      hipLaunchKernelGGL(dt_wave_kernel, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
+                         (const double *)D(2));
+      break;
     case 1:
     case 2:
     case 15:
-    case 16: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
+    case 16:
+    case 18:
+    case 19: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
                 // NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO after N2D's solve
       MinorTerms MT;
       for (int q = 0; q < 4; ++q) { MT.ftm1s[q] = D(2 + q); MT.hadvec[q] = D(6 + q); }
@@ -1758,6 +3041,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL((minor_kernel<2, 3>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
       else if (which == 2)
         hipLaunchKernelGGL((minor_kernel<1, 0>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      else if (which == 18)
+        hipLaunchKernelGGL((minor_wave_kernel<2, 3>), gridw, block, 0, s, g, G, in, out, MT);
+      else if (which == 19)
+        hipLaunchKernelGGL((minor_wave_kernel<1, 0>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 15)
         hipLaunchKernelGGL((minor_src_kernel<2, 3>), gridp, block, 0, s, g, G, in, out, MT);
       else
@@ -1768,7 +3055,8 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 4:
     case 5:
     case 9:
-    case 17: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
+    case 17:
+    case 21: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
                 //           optm1out, bd_diff bd_diffj bd_bvel vdotn bd_djbz opts2, oploss, opprod}
       OpWork W{D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8)};
       const double *op = D(9), *optm1 = D(10);
@@ -1794,24 +3082,35 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       } else {
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
-        hipLaunchKernelGGL(oplus_stage3, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
-                           (const double *)D(19), (const double *)D(20), opout, optm1out);
+        if (which == 5)
+          hipLaunchKernelGGL(oplus_stage3, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
+                             (const double *)D(19), (const double *)D(20), opout, optm1out);
+        else
+          hipLaunchKernelGGL(oplus_wave_stage3, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+                             OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
       }
       break;
     }
     case 6:  // duv stage 1: bufs = {ztmp}
       hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
       break;
-    case 7: {  // duv stage 2: bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
+    case 7:
+    case 22: {  // duv stage 2: bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
       UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
-      hipLaunchKernelGGL(duv_stage2, grid, block, 0, s, g, G, in, out, D(1), UT);
+      if (which == 7)
+        hipLaunchKernelGGL(duv_stage2, grid, block, 0, s, g, G, in, out, D(1), UT);
+      else
+        hipLaunchKernelGGL(duv_wave_stage2, gridw, block, 0, s, g, G, in, out, UT);
       break;
     }
     case 8:
-    case 14: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6]}
+    case 14:
+    case 23: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6]}
       CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6), {D(7), D(8), D(9), D(10), D(11), D(12)}};
       if (which == 8) {
         hipLaunchKernelGGL(comp_kernel, grid, block, 0, s, g, G, in, out, D(0), CT);
+      } else if (which == 23) {
+        hipLaunchKernelGGL(comp_wave_kernel, gridw, block, 0, s, g, G, in, out, CT);
       } else {
         dim3 grido((unsigned)((ncol * (G.nk - 1) + 255) / 256));
         hipLaunchKernelGGL(o2o_kernel, grido, block, 0, s, g, G, in, CT);

@@ -47,6 +47,10 @@ int check(const tgx_ng_grid *g, const void *work) {
   if (g->nk < 4 || g->nlon < 2 || g->nlat < 2 || g->ld < g->nlon + 4)
     return tgx_set_error(1, "tgx_ng: bad grid nk=%d nlon=%d nlat=%d ld=%d", g->nk, g->nlon, g->nlat, g->ld);
   if (g->nstep_sub < 1) return tgx_set_error(1, "tgx_ng: nstep_sub must be >= 1");
+  if (g->scheme < TGX_SCHEME_AUTO || g->scheme > TGX_SCHEME_PCR)
+    return tgx_set_error(1, "tgx_ng: unknown scheme %d", g->scheme);
+  if (g->scheme == TGX_SCHEME_PCR && g->nk > 128)
+    return tgx_set_error(1, "tgx_ng: the PCR scheme holds at most 128 levels (nk=%d)", g->nk);
   if ((size_t)g->ld * (size_t)(g->nlat + 4) * (size_t)g->nk > ((size_t)1 << 40))
     return tgx_set_error(1, "tgx_ng: grid too large");
   if (!work) return tgx_set_error(1, "tgx_ng: null workspace");
@@ -126,7 +130,17 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 }
 
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
-       K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM };
+       K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
+       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W };
+
+namespace {
+// Solve scheme of a routine: the wavefront-per-column cyclic reduction holds
+// at most 128 levels.
+bool use_pcr(const tgx_ng_grid *g) {
+  if (g->nk > 128) return false;
+  return g->scheme == TGX_SCHEME_PCR || g->scheme == TGX_SCHEME_AUTO;
+}
+}  // namespace
 
 extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                          void *work, void *stream) {
@@ -135,7 +149,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   ProfScope prof(R_DT, s);
   void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
-  return tgx_ng_launch(K_DT, g, in, out, b, 0, s);
+  return tgx_ng_launch(use_pcr(g) ? K_DT_W : K_DT, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -150,10 +164,11 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
   for (int q = 0; q < 4; ++q) b[10 + q] = plane(work, g, W_MIN_S + q);
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
+  const bool pcr = use_pcr(g);
   if (int rc = tgx_ng_launch(K_MIN_SRC_A, g, in, out, b, 0, s)) return rc;
-  if (int rc = tgx_ng_launch(K_MIN_A, g, in, out, b, 0, s)) return rc;
+  if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
-  return tgx_ng_launch(K_MIN_B, g, in, out, b, 0, s);
+  return tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -185,7 +200,7 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
-    if (int rc = tgx_ng_launch(K_OP3, g, in, out, b, sub, s)) return rc;
+    if (int rc = tgx_ng_launch(use_pcr(g) ? K_OP3_W : K_OP3, g, in, out, b, sub, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
   }
@@ -203,7 +218,7 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 5 + q);
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
-  return tgx_ng_launch(K_UV2, g, in, out, b, 0, s);
+  return tgx_ng_launch(use_pcr(g) ? K_UV2_W : K_UV2, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -216,7 +231,7 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
-  return tgx_ng_launch(K_COMP, g, in, out, b, 0, s);
+  return tgx_ng_launch(use_pcr(g) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
 }
 
 // Concurrency of the step: oplus, the minor species, dt and comp read only

@@ -17,7 +17,7 @@ _P = C.c_void_p
 
 GRID_FIELDS = [
     ("nk", C.c_int), ("nlon", C.c_int), ("nlat", C.c_int), ("ld", C.c_int),
-    ("nstep_sub", C.c_int), ("calc_helium", C.c_int),
+    ("nstep_sub", C.c_int), ("calc_helium", C.c_int), ("scheme", C.c_int),
     ("dz", C.c_double), ("dlamda", C.c_double), ("dphi", C.c_double), ("dtx2inv", C.c_double),
     ("shapiro", C.c_double), ("expzmid", C.c_double), ("expzmid_inv", C.c_double),
     ("colfac", C.c_double), ("opdiffcap", C.c_double), ("difhor", C.c_double),
@@ -66,6 +66,7 @@ _SIGS = {
 }
 
 ROUTINES = ("oplus", "minors", "dt", "duv", "comp")
+SCHEMES = {"auto": 0, "thomas": 1, "pcr": 2}
 
 
 def profile_begin(max_calls):
@@ -103,9 +104,11 @@ class NestState:
     grid:   dict from synth.make_grid
     fields: dict of float64 tensors in logical layout (synth.make_state),
             on the CPU or already on the GPU.
+    scheme: column solve, "thomas" (the reference's operation order), "pcr"
+            (one wavefront per column) or "auto" (timed once per shape).
     """
 
-    def __init__(self, grid, fields, device="cuda"):
+    def __init__(self, grid, fields, device="cuda", scheme="thomas"):
         self.grid = grid
         nk, nlat, nlon = grid["nk"], grid["nlat"], grid["nlon"]
         self.nk, self.nlatd, self.nlond = nk, nlat + 4, nlon + 4
@@ -122,6 +125,7 @@ class NestState:
         g = NgGrid()
         g.nk, g.nlon, g.nlat, g.ld = nk, nlon, nlat, self.ld
         g.nstep_sub, g.calc_helium = int(grid["nstep_sub"]), int(grid["calc_helium"])
+        g.scheme = SCHEMES[scheme]
         for n in ("dz", "dlamda", "dphi", "dtx2inv", "shapiro", "expzmid", "expzmid_inv",
                   "colfac", "opdiffcap", "difhor", "pshelb"):
             setattr(g, n, float(grid[n]))

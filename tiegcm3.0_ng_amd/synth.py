@@ -32,6 +32,7 @@ SMOOTH_FAC = 3.e-3
 CONFIGS = {
     "tiny": (8, 6, 12, 1.0, 60., 60., None),
     "small": (16, 12, 29, 0.5, 60., 60., None),
+    "tall": (12, 8, 113, 0.125, 10., 10., None),         # few columns, all 113 levels
     "g5": (72, 36, 29, 0.5, 60., 60., None),            # 5 deg, 29 levels
     "g2p5": (144, 72, 57, 0.25, 30., 30., None),        # 2.5 deg, 57 levels
     "g1p25": (288, 144, 113, 0.125, 10., 10., None),    # 1.25 deg, 113 levels
