@@ -78,7 +78,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
-    ap.add_argument("--scheme", default="thomas", choices=("thomas", "pcr", "auto"),
+    ap.add_argument("--scheme", default="auto", choices=("thomas", "pcr", "auto"),
                     help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
     ap.add_argument("--serial", action="store_true",
                     help="run the routines one after another on one stream (default: the "
@@ -164,7 +164,8 @@ def main():
                                "nstep_sub=%d, fused assembly+solve FP64" % (
                                    args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
                    "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64",
-                   "schedule": "serial" if args.serial else "concurrent", "scheme": args.scheme},
+                   "schedule": "serial" if args.serial else "concurrent", "scheme": args.scheme,
+                   "schemes": ns.schemes()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": dom, "alg_bytes_per_launch": alg[dom],

@@ -108,6 +108,14 @@ int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out 
  * may be captured into a hipGraph. */
 int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                 void *stream);
+/* Autotuner of TGX_SCHEME_AUTO: times each routine under both schemes on
+ * `stream` (host-synchronising; not for graph capture) and keeps the faster
+ * one per device, shape and routine.  tgx_ng_step calls it on the first AUTO
+ * step of a shape.  tgx_ng_scheme_choice reports the scheme a routine (0..4 =
+ * oplus, minors, dt, duv, comp) runs with for this grid. */
+int tgx_ng_autotune(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                    void *stream);
+int tgx_ng_scheme_choice(const tgx_ng_grid *g, int routine);
 /* The same step strictly in the reference's order on `stream` alone. */
 int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                        void *work, void *stream);

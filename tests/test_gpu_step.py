@@ -79,6 +79,22 @@ def test_concurrent_step_equals_serial(pkg):
         assert torch.equal(a.out[n], b.out[n]), n
 
 
+def test_auto_scheme_tunes_and_matches_oracle(pkg, oracle):
+    """TGX_SCHEME_AUTO times both schemes on its first step, records a choice
+    per routine, and its results stay within the FP64 bound."""
+    import torch
+    g = pkg.synth.make_grid("small", nstep_sub=1)
+    F = pkg.synth.make_state(g)
+    ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
+    ns = pkg.step.NestState(g, F, scheme="auto")
+    ns.step()
+    ns.step()
+    torch.cuda.synchronize()
+    assert set(ns.schemes().values()) <= {"thomas", "pcr"}
+    for n in ("tn", "un", "vn", "op", "no", "n4s", "n2d", "ar", "o2", "o1", "he"):
+        assert field_err(ns.fortran(n), ref[n]) <= TOL, n
+
+
 def test_profile_hooks_count_calls(pkg):
     import torch
     g = pkg.synth.make_grid("tiny", nstep_sub=1)

@@ -63,9 +63,24 @@ static Geo make_geo(const tgx_ng_grid &g) {
 // Branch-free: every case of the reference's edge rules is a pair of loads at
 // clamped indices and a scale, so a wavefront issues all its loads at once.
 // Scaling by 0.5/0.25 is exactly the reference's division by 2/4.
-#define PV(f, i, j) (f)[(size_t)(j) * G.ld + (i)]
+// A plane read either from global memory (a pointer to the level's plane) or
+// from an LDS tile that holds it around the block's points.
+__device__ __forceinline__ double pv(const double *f, const Geo &G, int i, int j) {
+  return f[(size_t)j * G.ld + i];
+}
+constexpr int TILE_X = 32, TILE_Y = 8, TILE_H = 2;
+constexpr int TILE_W = TILE_X + 2 * TILE_H, TILE_R = TILE_Y + 2 * TILE_H;
+struct LdsPlane {
+  const double *t;  // TILE_R x TILE_W, row-major
+  int i0, j0;       // plane indices of t[0]
+};
+__device__ __forceinline__ double pv(const LdsPlane &f, const Geo &G, int i, int j) {
+  return f.t[(j - f.j0) * TILE_W + (i - f.i0)];
+}
+#define PV(f, i, j) pv(f, G, i, j)
 // df_3d order 1 in longitude: edges one-sided, interior centred / 2
-__device__ __forceinline__ double dflon1(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double dflon1(const F &f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
   int ip = i < L1 ? i + 1 : L1, im = i > 0 ? i - 1 : 0;
   double d = PV(f, ip, j) - PV(f, im, j);
@@ -73,7 +88,8 @@ __device__ __forceinline__ double dflon1(const double *f, const Geo &G, int i, i
 }
 // df_3d order 2: edges one-sided, next-to-edge centred over 2 cells / 2,
 // interior (f(i+2)-f(i-2)) / 4
-__device__ __forceinline__ double dflon2(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double dflon2(const F &f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
   bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
   int ip = edge ? (i == 0 ? 1 : L1) : (near ? i + 1 : i + 2);
@@ -82,25 +98,29 @@ __device__ __forceinline__ double dflon2(const double *f, const Geo &G, int i, i
   return edge ? d : (near ? d * 0.5 : d * 0.25);
 }
 // fm_3d order 1/2: edges return f itself ((f+f)*0.5 == f exactly)
-__device__ __forceinline__ double fmlon1(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double fmlon1(const F &f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
   bool edge = (i == 0 || i == L1);
   int ip = edge ? i : i + 1, im = edge ? i : i - 1;
   return (PV(f, ip, j) + PV(f, im, j)) * 0.5;
 }
-__device__ __forceinline__ double fmlon2(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double fmlon2(const F &f, const Geo &G, int i, int j) {
   int L1 = G.nlond - 1;
   bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
   int ip = edge ? i : (near ? i + 1 : i + 2), im = edge ? i : (near ? i - 1 : i - 2);
   return (PV(f, ip, j) + PV(f, im, j)) * 0.5;
 }
-__device__ __forceinline__ double dflat1(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double dflat1(const F &f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
   int jp = j < J1 ? j + 1 : J1, jm = j > 0 ? j - 1 : 0;
   double d = PV(f, i, jp) - PV(f, i, jm);
   return (j == 0 || j == J1) ? d : d * 0.5;
 }
-__device__ __forceinline__ double dflat2(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double dflat2(const F &f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
   bool edge = (j == 0 || j == J1), near = (j == 1 || j == J1 - 1);
   int jp = edge ? (j == 0 ? 1 : J1) : (near ? j + 1 : j + 2);
@@ -108,13 +128,15 @@ __device__ __forceinline__ double dflat2(const double *f, const Geo &G, int i, i
   double d = PV(f, i, jp) - PV(f, i, jm);
   return edge ? d : (near ? d * 0.5 : d * 0.25);
 }
-__device__ __forceinline__ double fmlat1(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double fmlat1(const F &f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
   bool edge = (j == 0 || j == J1);
   int jp = edge ? j : j + 1, jm = edge ? j : j - 1;
   return (PV(f, i, jm) + PV(f, i, jp)) * 0.5;
 }
-__device__ __forceinline__ double fmlat2(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double fmlat2(const F &f, const Geo &G, int i, int j) {
   int J1 = G.nlatd - 1;
   bool edge = (j == 0 || j == J1), near = (j == 1 || j == J1 - 1);
   int jp = edge ? j : (near ? j + 1 : j + 2), jm = edge ? j : (near ? j - 1 : j - 2);
@@ -122,7 +144,8 @@ __device__ __forceinline__ double fmlat2(const double *f, const Geo &G, int i, i
 }
 
 // advec_ng.F90:244-265 at (i, j) of the plane
-__device__ __forceinline__ double advec(const double *f, const double *u, const double *v,
+template <class F, class U>
+__device__ __forceinline__ double advec(const F &f, const U &u, const U &v,
                                         const Geo &G, int i, int j, double racs, double dlamda,
                                         double dphi) {
   double ub1 = fmlon1(u, G, i, j), ub2 = fmlon2(u, G, i, j);
@@ -139,7 +162,8 @@ __device__ __forceinline__ double advec(const double *f, const double *u, const 
 
 // smooth_ng.F90:283-294: latitude pass at (i, j); rows 0/J1 copy, rows 1/J1-1
 // use the 3-point form
-__device__ __forceinline__ double smooth_lat(const double *f, const Geo &G, int i, int j, double s) {
+template <class F>
+__device__ __forceinline__ double smooth_lat(const F &f, const Geo &G, int i, int j, double s) {
   int J1 = G.nlatd - 1;
   bool edge = (j == 0 || j == J1), near = (j == 1 || j == J1 - 1);
   int j1p = j < J1 ? j + 1 : J1, j1m = j > 0 ? j - 1 : 0;
@@ -152,7 +176,8 @@ __device__ __forceinline__ double smooth_lat(const double *f, const Geo &G, int 
   return edge ? fin : (near ? nr : gen);
 }
 // smooth_ng.F90:296-307: longitude pass of the latitude-smoothed field
-__device__ __forceinline__ double smooth(const double *f, const Geo &G, int i, int j, double s) {
+template <class F>
+__device__ __forceinline__ double smooth(const F &f, const Geo &G, int i, int j, double s) {
   int L1 = G.nlond - 1;
   bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
   int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
@@ -166,13 +191,15 @@ __device__ __forceinline__ double smooth(const double *f, const Geo &G, int i, i
 }
 
 // div_ng.F90:331-346
-__device__ __forceinline__ double divx(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double divx(const F &f, const Geo &G, int i, int j) {
   double d1 = dflon1(f, G, i, j);
   double d2 = dflon2(f, G, i, j);
   double gen = d1 * 2. * 2. / 3. - d2 * 4. / 12.;
   return (i <= 1 || i >= G.nlon + 2) ? d1 : gen;
 }
-__device__ __forceinline__ double divy(const double *f, const Geo &G, int i, int j) {
+template <class F>
+__device__ __forceinline__ double divy(const F &f, const Geo &G, int i, int j) {
   double d1 = dflat1(f, G, i, j);
   double d2 = dflat2(f, G, i, j);
   double gen = d1 * 2. * 2. / 3. - d2 * 4. / 12.;
@@ -180,7 +207,8 @@ __device__ __forceinline__ double divy(const double *f, const Geo &G, int i, int
 }
 
 // oplus_ng.F90:344-365 (bdotdh)
-__device__ __forceinline__ double bdotdh(const double *f, const Geo &G, int i, int j, double bx,
+template <class F>
+__device__ __forceinline__ double bdotdh(const F &f, const Geo &G, int i, int j, double bx,
                                          double by, double cs, double dlamda, double dphi) {
   double a1 = bx * dflon1(f, G, i, j);
   double a2 = by * dflat1(f, G, i, j);
@@ -231,41 +259,85 @@ struct StencilJobs {
   StencilJob job[8];
 };
 
+// Horizontal stencils of one level: a block takes a TILE_X x TILE_Y tile of
+// the plane and loads every job's input once into LDS with a 2-point halo
+// (coalesced rows), all jobs' loads in flight together, then evaluates the
+// stencil functions from LDS.  Halo points outside the domain are never read:
+// the stencils clamp at the domain edges.  Dynamic LDS: (jobs + 2) tiles.
 __global__ void __launch_bounds__(256) stencil_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                       StencilJobs J) {
-  int nb = gridDim.x;
-  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
-  long npt = (long)G.nlond * G.nlatd * G.nk;
-  if (t >= npt) return;
-  int i = (int)(t % G.nlond);
-  long r = t / G.nlond;
-  int j = (int)(r % G.nlatd);
-  int k = (int)(r / G.nlatd);
-  const size_t off = (size_t)k * G.P, p = off + (size_t)j * G.ld + i, c2 = (size_t)j * G.ld + i;
+  extern __shared__ double smem[];
+  constexpr int TS = TILE_R * TILE_W;
+  const int ntx = (G.nlond + TILE_X - 1) / TILE_X, nty = (G.nlatd + TILE_Y - 1) / TILE_Y;
+  const int b = tgx_xcd_block(gridDim.x);
+  if (b >= ntx * nty * G.nk) return;  // uniform per block
+  const int bxi = b % ntx, rr = b / ntx, byj = rr % nty, k = rr / nty;
+  const int i0 = bxi * TILE_X, j0 = byj * TILE_Y;
+  const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
+  const bool inside = i < G.nlond && j < G.nlatd;
+  const size_t off = (size_t)k * G.P;
+  bool adv = false;
+  for (int n = 0; n < J.n; ++n) adv = adv || J.job[n].kind == ST_ADVEC;
+  double *tu = smem + (size_t)J.n * TS, *tv = tu + TS;
+  // every input tile of the block is requested before any is used: up to
+  // 2 x (8 + 2) loads per thread in flight
+  double ld[2][10];
+  bool okr[2];
+  int er[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int e = (int)threadIdx.x + r * 256;
+    const int ii = i0 - TILE_H + e % TILE_W, jj = j0 - TILE_H + e / TILE_W;
+    er[r] = e;
+    okr[r] = e < TS && ii >= 0 && ii < G.nlond && jj >= 0 && jj < G.nlatd;
+    const size_t q = okr[r] ? off + (size_t)jj * G.ld + ii : off;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+      if (n < J.n && okr[r]) ld[r][n] = J.job[n].f[q];
+    if (adv && okr[r]) {
+      ld[r][8] = in.un[q];
+      ld[r][9] = in.vn[q];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (!okr[r]) continue;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+      if (n < J.n) smem[n * TS + er[r]] = ld[r][n];
+    if (adv) {
+      tu[er[r]] = ld[r][8];
+      tv[er[r]] = ld[r][9];
+    }
+  }
+  __syncthreads();
+  if (!inside) return;
+  const LdsPlane U{tu, i0 - TILE_H, j0 - TILE_H}, V{tv, i0 - TILE_H, j0 - TILE_H};
+  const size_t p = off + (size_t)j * G.ld + i, c2 = (size_t)j * G.ld + i;
   for (int n = 0; n < J.n; ++n) {
     const StencilJob &jb = J.job[n];
-    const double *f = jb.f + off;
+    const LdsPlane F{smem + n * TS, i0 - TILE_H, j0 - TILE_H};
     double v;
     switch (jb.kind) {
       case ST_ADVEC:
-        v = advec(f, in.un + off, in.vn + off, G, i, j, g.racs[j], g.dlamda, g.dphi);
+        v = advec(F, U, V, G, i, j, g.racs[j], g.dlamda, g.dphi);
         break;
       case ST_SMOOTH:
-        v = smooth(f, G, i, j, jb.s);
+        v = smooth(F, G, i, j, jb.s);
         break;
       case ST_DIVX:
-        v = divx(f, G, i, j);
+        v = divx(F, G, i, j);
         break;
       case ST_DIVY:
-        v = divy(f, G, i, j);
+        v = divy(F, G, i, j);
         break;
       case ST_BDOTDH:
-        v = bdotdh(f, G, i, j, in.bx[c2], in.by[c2], g.cs[j], g.dlamda, g.dphi);
+        v = bdotdh(F, G, i, j, in.bx[c2], in.by[c2], g.cs[j], g.dlamda, g.dphi);
         break;
       default: {  // ST_VDOTN: oplus_ng.F90:196-209, f = op/bmod2**2
         double uii = k < G.nk - 1 ? .5 * (in.ui[p] + in.ui[p + G.P]) : 1.5 * in.ui[p] - .5 * in.ui[p - G.P];
         double vii = k < G.nk - 1 ? .5 * (in.vi[p] + in.vi[p + G.P]) : 1.5 * in.vi[p] - .5 * in.vi[p - G.P];
-        double vx = uii * dflon1(f, G, i, j), vy = vii * dflat1(f, G, i, j);
+        double vx = uii * dflon1(F, G, i, j), vy = vii * dflat1(F, G, i, j);
         vx = vx / g.cs[j];
         double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
         double bm = in.bmod2[c2];
@@ -2996,6 +3068,10 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
 // ---------------------------------------------------------------------------
 // Launcher used by step_api.cpp.  bufs holds the workspace planes (and for
 // O+ the input/output state of the sub-step) in the order documented there.
+static size_t stencil_lds(const tgx::ng::StencilJobs &J) {
+  return (size_t)(J.n + 2) * tgx::ng::TILE_R * tgx::ng::TILE_W * sizeof(double);
+}
+
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fields *inp,
                              const tgx_ng_out *outp, void *const *bufs, int sub, hipStream_t s) {
   using namespace tgx::ng;
@@ -3006,9 +3082,11 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   long ncol = (long)G.nlond * G.nlatd;
   int nb = (int)((ncol + 255) / 256);
   int nbp = (int)((ncol * G.nk + 255) / 256);
-  // thread per column (grid), per column-level (gridp), wavefront per column (gridw)
+  // thread per column (grid), per column-level (gridp), wavefront per column
+  // (gridw), one stencil tile of one level per block (grids)
   int nbw = (int)((ncol + 3) / 4);
-  dim3 grid(nb), gridp(nbp), gridw(nbw), block(256);
+  int nbs = ((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y) * G.nk;
+  dim3 grid(nb), gridp(nbp), gridw(nbw), grids(nbs), block(256);
   auto D = [&](int n) { return (double *)bufs[n]; };
   auto job = [](int kind, const double *f, double *o, double sh) {
     StencilJob jb;
@@ -3078,7 +3156,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         J.job[3] = job(ST_VDOTN, W.opbm2, D(16), 0.);
         J.job[4] = job(ST_BDOTDH, W.djbz, D(17), 0.);
         J.job[5] = job(ST_SMOOTH, optm1, D(18), g.shapiro / g.nstep_sub);
-        hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+        hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
       } else {
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
@@ -3122,7 +3200,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.n = 2;
       J.job[0] = job(ST_ADVEC, in.tn, D(1), 0.);
       J.job[1] = job(ST_SMOOTH, in.tn_nm, D(2), g.shapiro);
-      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
       break;
     }
     case 11: {  // minors prelude (species 0..3 = NO, N4S, N2D, AR)
@@ -3134,7 +3212,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         J.job[q] = job(ST_SMOOTH, fm[q], D(2 + q), g.shapiro);
         J.job[4 + q] = job(ST_ADVEC, fc[q], D(6 + q), 0.);
       }
-      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
       break;
     }
     case 12: {  // comp prelude
@@ -3146,7 +3224,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.job[3] = job(ST_SMOOTH, in.o2_nm, D(4), g.shapiro);
       J.job[4] = job(ST_SMOOTH, in.o1_nm, D(5), g.shapiro);
       J.job[5] = job(ST_SMOOTH, in.he_nm, D(6), g.shapiro);
-      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
       break;
     }
     case 13: {  // duv prelude (after stage 1 wrote ztmp)
@@ -3158,7 +3236,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.job[3] = job(ST_SMOOTH, in.vn_nm, D(5), g.shapiro);
       J.job[4] = job(ST_ADVEC, in.un, D(6), 0.);
       J.job[5] = job(ST_ADVEC, in.vn, D(7), 0.);
-      hipLaunchKernelGGL(stencil_kernel, gridp, block, 0, s, g, G, in, J);
+      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
       break;
     }
     default:

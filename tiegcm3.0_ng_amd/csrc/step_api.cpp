@@ -4,6 +4,7 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "../../include/tgx.h"
@@ -134,11 +135,29 @@ enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W };
 
 namespace {
-// Solve scheme of a routine: the wavefront-per-column cyclic reduction holds
-// at most 128 levels.
-bool use_pcr(const tgx_ng_grid *g) {
+// Solve scheme of each routine.  The wavefront-per-column cyclic reduction
+// holds at most 128 levels.  TGX_SCHEME_AUTO uses the choice timed by
+// tgx_ng_autotune for this device and shape (PCR until then).
+typedef std::tuple<int, int, int, int, int, int, int> TuneKey;  // dev nk nlon nlat ld nsub routine
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+thread_local int t_forced = -1;  // scheme forced while the autotuner times a routine
+
+TuneKey tune_key(const tgx_ng_grid *g, int routine) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return TuneKey(dev, g->nk, g->nlon, g->nlat, g->ld, g->nstep_sub, routine);
+}
+int tuned_choice(const tgx_ng_grid *g, int routine) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(g, routine));
+  return it == g_tune.end() ? TGX_SCHEME_AUTO : it->second;
+}
+bool use_pcr(const tgx_ng_grid *g, int routine) {
   if (g->nk > 128) return false;
-  return g->scheme == TGX_SCHEME_PCR || g->scheme == TGX_SCHEME_AUTO;
+  if (t_forced >= 0) return t_forced == TGX_SCHEME_PCR;
+  if (g->scheme != TGX_SCHEME_AUTO) return g->scheme == TGX_SCHEME_PCR;
+  return tuned_choice(g, routine) != TGX_SCHEME_THOMAS;
 }
 }  // namespace
 
@@ -149,7 +168,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   ProfScope prof(R_DT, s);
   void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
-  return tgx_ng_launch(use_pcr(g) ? K_DT_W : K_DT, g, in, out, b, 0, s);
+  return tgx_ng_launch(use_pcr(g, R_DT) ? K_DT_W : K_DT, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -164,7 +183,7 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
   for (int q = 0; q < 4; ++q) b[10 + q] = plane(work, g, W_MIN_S + q);
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
-  const bool pcr = use_pcr(g);
+  const bool pcr = use_pcr(g, R_MINORS);
   if (int rc = tgx_ng_launch(K_MIN_SRC_A, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
@@ -200,7 +219,7 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
-    if (int rc = tgx_ng_launch(use_pcr(g) ? K_OP3_W : K_OP3, g, in, out, b, sub, s)) return rc;
+    if (int rc = tgx_ng_launch(use_pcr(g, R_OPLUS) ? K_OP3_W : K_OP3, g, in, out, b, sub, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
   }
@@ -218,7 +237,7 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 5 + q);
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
-  return tgx_ng_launch(use_pcr(g) ? K_UV2_W : K_UV2, g, in, out, b, 0, s);
+  return tgx_ng_launch(use_pcr(g, R_DUV) ? K_UV2_W : K_UV2, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -231,7 +250,7 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
-  return tgx_ng_launch(use_pcr(g) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
+  return tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
 }
 
 // Concurrency of the step: oplus, the minor species, dt and comp read only
@@ -261,11 +280,71 @@ int get_fork(Fork **out) {
 }
 }  // namespace
 
+typedef int (*RoutineFn)(const tgx_ng_grid *, const tgx_ng_fields *, const tgx_ng_out *, void *, void *);
+
+// Times every routine under both schemes on `stream` (reference order, so
+// duv sees this step's dt and oplus outputs) and keeps the faster scheme per
+// routine for this device and shape.  The outputs are left as a full step
+// computed with the last scheme timed; the caller runs its step afterwards.
+extern "C" int tgx_ng_autotune(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                               void *work, void *stream) {
+  if (int rc = check(g, work)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const RoutineFn fns[R_N] = {tgx_ng_oplus, tgx_ng_minors, tgx_ng_dt, tgx_ng_duv, tgx_ng_comp};
+  hipEvent_t e0, e1;
+  TGX_HIP(hipEventCreate(&e0));
+  TGX_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (int r = 0; r < R_N && rc == 0; ++r) {
+    float best[3] = {0.f, 0.f, 0.f};
+    const int schemes[2] = {TGX_SCHEME_THOMAS, TGX_SCHEME_PCR};
+    const int nsch = g->nk > 128 ? 1 : 2;
+    for (int q = 0; q < nsch && rc == 0; ++q) {
+      t_forced = schemes[q];
+      float tmin = 1e30f;
+      for (int rep = 0; rep < 3 && rc == 0; ++rep) {  // the first run is a warm-up
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = tgx_set_error(2, "tgx_ng_autotune: event record"); break; }
+        rc = fns[r](g, in, out, work, stream);
+        if (rc) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          rc = tgx_set_error(2, "tgx_ng_autotune: event wait");
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < tmin) tmin = ms;
+      }
+      best[schemes[q]] = tmin;
+    }
+    t_forced = -1;
+    if (rc == 0) {
+      int choice = (nsch == 2 && best[TGX_SCHEME_PCR] < best[TGX_SCHEME_THOMAS]) ? TGX_SCHEME_PCR
+                                                                               : TGX_SCHEME_THOMAS;
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[tune_key(g, r)] = choice;
+    }
+  }
+  t_forced = -1;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+extern "C" int tgx_ng_scheme_choice(const tgx_ng_grid *g, int routine) {
+  if (!g || routine < 0 || routine >= R_N) return tgx_set_error(1, "tgx_ng_scheme_choice: bad arguments");
+  if (g->scheme != TGX_SCHEME_AUTO) return g->scheme;
+  if (g->nk > 128) return TGX_SCHEME_THOMAS;
+  int c = tuned_choice(g, routine);
+  return c == TGX_SCHEME_AUTO ? TGX_SCHEME_PCR : c;
+}
+
 extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                            void *work, void *stream) {
   // dynamics_ng.F90 order: oplus (:99-121), minors (:147-165), dt (:203),
   // duv (:210), comp_o2o + comp (:221-233)
   if (int rc = check(g, work)) return rc;
+  if (g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
+    if (int rc = tgx_ng_autotune(g, in, out, work, stream)) return rc;
   Fork *F = nullptr;
   if (int rc = get_fork(&F)) return rc;
   hipStream_t s0 = (hipStream_t)stream, sa = F->s[0], sb = F->s[1], sc = F->s[2];
@@ -290,6 +369,9 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
 // The same step strictly in the reference's order on one stream.
 extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
                                   const tgx_ng_out *out, void *work, void *stream) {
+  if (int rc = check(g, work)) return rc;
+  if (g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
+    if (int rc = tgx_ng_autotune(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_oplus(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_minors(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_dt(g, in, out, work, stream)) return rc;

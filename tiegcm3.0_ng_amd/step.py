@@ -62,6 +62,8 @@ _SIGS = {
     "tgx_ng_step": ([_P] * 5, C.c_int),
     "tgx_ng_step_serial": ([_P] * 5, C.c_int),
     "tgx_ng_profile_begin": ([C.c_int], C.c_int),
+    "tgx_ng_autotune": ([_P] * 5, C.c_int),
+    "tgx_ng_scheme_choice": ([_P, C.c_int], C.c_int),
     "tgx_ng_profile_end": ([_P, _P], C.c_int),
 }
 
@@ -210,6 +212,15 @@ class NestState:
 
     def step_serial(self, stream=None):
         self._call("tgx_ng_step_serial", stream)
+
+    def autotune(self, stream=None):
+        self._call("tgx_ng_autotune", stream)
+
+    def schemes(self):
+        """The column-solve scheme each routine runs with ("thomas" / "pcr")."""
+        names = {1: "thomas", 2: "pcr"}
+        return {r: names.get(_lib().tgx_ng_scheme_choice(C.byref(self.g), n), "?")
+                for n, r in enumerate(ROUTINES)}
 
 
 # Algorithmic HBM traffic of each routine: the distinct 3-D fields its fused
