@@ -539,6 +539,38 @@ __device__ __forceinline__ void matinv3_ng(const double *A, double *Bm) {
 #undef M3
 }
 
+// The same inverse with one division: the adjugate times 1/det.  Within
+// 2 ulp of matinv3_ng per element; used where the assembly is evaluated with
+// shared reciprocals (column kernels of the minor species and comp).
+__device__ __forceinline__ void matinv3_r(const double *A, double *Bm) {
+#define M3(r, c) A[((r) - 1) + 3 * ((c) - 1)]
+  double d = M3(1,1)*M3(2,2)*M3(3,3) - M3(1,1)*M3(2,3)*M3(3,2)
+           - M3(1,2)*M3(2,1)*M3(3,3) + M3(1,2)*M3(2,3)*M3(3,1)
+           + M3(1,3)*M3(2,1)*M3(3,2) - M3(1,3)*M3(2,2)*M3(3,1);
+  const double r = 1. / d;
+  Bm[0] =  (M3(2,2)*M3(3,3) - M3(2,3)*M3(3,2)) * r;
+  Bm[1] = -(M3(2,1)*M3(3,3) - M3(2,3)*M3(3,1)) * r;
+  Bm[2] =  (M3(2,1)*M3(3,2) - M3(2,2)*M3(3,1)) * r;
+  Bm[3] = -(M3(1,2)*M3(3,3) - M3(1,3)*M3(3,2)) * r;
+  Bm[4] =  (M3(1,1)*M3(3,3) - M3(1,3)*M3(3,1)) * r;
+  Bm[5] = -(M3(1,1)*M3(3,2) - M3(1,2)*M3(3,1)) * r;
+  Bm[6] =  (M3(1,2)*M3(2,3) - M3(1,3)*M3(2,2)) * r;
+  Bm[7] = -(M3(1,1)*M3(2,3) - M3(1,3)*M3(2,1)) * r;
+  Bm[8] =  (M3(1,1)*M3(2,2) - M3(1,2)*M3(2,1)) * r;
+#undef M3
+}
+// The temperature powers of the diffusion coefficients, (tni/T00)^(1.75-s_n)
+// (n = 0..2, comp_ng.F90:124-126, minor_ng.F90:95-97) and (T00/tni)^0.25
+// (comp_ng.F90:153, minor_ng.F90:136), from one logarithm: exp(y log x)
+// differs from pow(x, y) by a few ulp for these arguments.
+struct TPow { double df[3], q; };
+__device__ __forceinline__ TPow tpows(double tni) {
+  const double L = log(tni / T00);
+  TPow t;
+  for (int n = 0; n < 3; ++n) t.df[n] = exp((1.75 - ssn(n)) * L);
+  t.q = exp(-0.25 * L);
+  return t;
+}
 __device__ __forceinline__ double dfactor_at(const tgx_ng_grid &g, int j) {
   // minor_ng.F90:145-154 / comp_ng.F90:113-122
   const double pi = 4.0 * atan(1.0), hor = .25, dtr = pi / 180.;
@@ -564,7 +596,7 @@ __device__ __forceinline__ SpeciesConst species_const(int s) {
 }
 
 struct MinorShared {   // level quantities shared by the species (minor_ng.F90:66-126)
-  double pso1, pso2, pshe, dmdz, do1dz, do2dz, dhedz, tni, barm, inv[9];
+  double pso1, pso2, pshe, dmdz, do1dz, do2dz, dhedz, tni, barm, inv[9], q;
 };
 struct MinorSpecies {  // per species and level
   double ax, thdiff, sloss, s0prod, ftm1s, hadvec;
@@ -674,7 +706,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
-  const double dzp = g.dz, dtx2inv = g.dtx2inv;
+  const double dzp = g.dz, dtx2inv = g.dtx2inv, rdz = 1. / g.dz;
   const size_t c2 = G.a2(i, j);
   const int SP[2] = {SA, SB};
   double *outs[2], *outs_nm[2];
@@ -747,27 +779,29 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   // or from the lower boundary and B = level 0 (L = 0)
   auto shared_at = [&](int L, const MR &A, const MR &B, MinorShared &S) {
     if (L == 0) {
-      S.dmdz = (B.mbar - xmbari) / dzp;
+      S.dmdz = (B.mbar - xmbari) * rdz;
       S.pso1 = .5 * (bo1 + o1_0);
       S.pso2 = .5 * (bo2 + o2_0);
       S.pshe = .5 * (bhe + he_0);
-      S.do1dz = (o1_0 - bo1) / dzp;
-      S.do2dz = (o2_0 - bo2) / dzp;
-      S.dhedz = (he_0 - bhe) / dzp;
+      S.do1dz = (o1_0 - bo1) * rdz;
+      S.do2dz = (o2_0 - bo2) * rdz;
+      S.dhedz = (he_0 - bhe) * rdz;
       S.tni = in.tlbc[c2];
     } else {
-      S.dmdz = (B.mbar - A.mbar) / dzp;
+      S.dmdz = (B.mbar - A.mbar) * rdz;
       S.pso1 = .5 * (B.o1 + A.o1);
       S.pso2 = .5 * (B.o2 + A.o2);
       S.pshe = .5 * (B.he + A.he);
-      S.do1dz = (B.o1 - A.o1) / dzp;
-      S.do2dz = (B.o2 - A.o2) / dzp;
-      S.dhedz = (B.he - A.he) / dzp;
+      S.do1dz = (B.o1 - A.o1) * rdz;
+      S.do2dz = (B.o2 - A.o2) * rdz;
+      S.dhedz = (B.he - A.he) * rdz;
       S.tni = L == nk - 1 ? A.tn : .5 * (B.tn + A.tn);  // :89-93
     }
     S.barm = B.barm;
-    double df[3], al[9];
-    for (int n = 0; n < 3; ++n) df[n] = pow(S.tni / T00, 1.75 - ssn(n));
+    double al[9];
+    const TPow tp = tpows(S.tni);
+    const double *df = tp.df;
+    S.q = tp.q;
     al[0] = -(PHI(1, 4) + (PHI(1, 2) - PHI(1, 4)) * S.pso1 + (df[0] * PHI(1, 3) - PHI(1, 4)) * S.pshe);
     al[4] = -(PHI(2, 4) + (PHI(2, 1) - PHI(2, 4)) * S.pso2 + (df[1] * PHI(2, 3) - PHI(2, 4)) * S.pshe);
     al[8] = -(df[2] * PHI(3, 4) + (df[0] * PHI(3, 1) - df[2] * PHI(3, 4)) * S.pso2 +
@@ -778,7 +812,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     al[7] = (df[1] * PHI(2, 3) - PHI(2, 4)) * S.pso1;
     al[2] = (df[0] * PHI(3, 1) - df[2] * PHI(3, 4)) * S.pshe;
     al[5] = (df[1] * PHI(3, 2) - df[2] * PHI(3, 4)) * S.pshe;
-    matinv3_ng(al, S.inv);
+    matinv3_r(al, S.inv);
   };
   auto species_at = [&](int q, const MinorShared &S, double dtnidz, const SR &R, MinorSpecies &T) {
     const SpeciesConst &c = sc[q];
@@ -788,11 +822,12 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
       for (int m = 0; m < 3; ++m) s = s + c.salfax[m] * S.inv[m + 3 * n];
       sai[n] = s;
     }
-    double ex = 1. - (c.rmx + S.dmdz) / S.barm +
-                sai[0] * (S.do2dz - (1. - (RMO2 + S.dmdz) / S.barm) * S.pso2) +
-                sai[1] * (S.do1dz - (1. - (RMO1 + S.dmdz) / S.barm) * S.pso1) +
-                sai[2] * (S.dhedz - (1. - (RMHE + S.dmdz) / S.barm) * S.pshe);
-    T.ax = -(S.barm / (TAU * RMN2) * pow(T00 / S.tni, 0.25) /
+    const double rb = 1. / S.barm;
+    double ex = 1. - (c.rmx + S.dmdz) * rb +
+                sai[0] * (S.do2dz - (1. - (RMO2 + S.dmdz) * rb) * S.pso2) +
+                sai[1] * (S.do1dz - (1. - (RMO1 + S.dmdz) * rb) * S.pso1) +
+                sai[2] * (S.dhedz - (1. - (RMHE + S.dmdz) * rb) * S.pshe);
+    T.ax = -(S.barm * (1. / (TAU * RMN2)) * S.q /
              (c.phix[2] + c.salfax[0] * S.pso2 + c.salfax[1] * S.pso1 + c.salfax[2] * S.pshe));
     T.thdiff = ex - c.alfax * dtnidz / (dzp * S.tni);
     T.sloss = R.sloss;
@@ -862,16 +897,16 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     for (int q = 0; q < 2; ++q) {
       double pc, qc, rc, fr;
       if (k <= nk - 2) {
-        double molp = Tc[q].ax * (1. / dzp + .5 * Tc[q].thdiff);
-        double molr = Tn[q].ax * (1. / dzp - .5 * Tn[q].thdiff);
-        double molq = Tc[q].ax * (1. / dzp - .5 * Tc[q].thdiff) + Tn[q].ax * (1. / dzp + .5 * Tn[q].thdiff);
-        double eddyp = g.expzmid_inv * difk0 * (1. / dzp - .5 * dmdzc);
-        double eddyr = g.expzmid * difk1 * (1. / dzp + .5 * dmdzn);
-        double eddyq = g.expzmid_inv * difk0 * (1. / dzp + .5 * dmdzc) +
-                       g.expzmid * difk1 * (1. / dzp - .5 * dmdzn);
-        pc = molp / dzp - expzk * (eddyp * dfac + .5 * wic) / dzp;
-        rc = molr / dzp - expzk * (eddyr * dfac - .5 * wic) / dzp;
-        qc = -molq / dzp + expzk * (eddyq * dfac / dzp + dtx2inv - Tc[q].sloss);
+        double molp = Tc[q].ax * (rdz + .5 * Tc[q].thdiff);
+        double molr = Tn[q].ax * (rdz - .5 * Tn[q].thdiff);
+        double molq = Tc[q].ax * (rdz - .5 * Tc[q].thdiff) + Tn[q].ax * (rdz + .5 * Tn[q].thdiff);
+        double eddyp = g.expzmid_inv * difk0 * (rdz - .5 * dmdzc);
+        double eddyr = g.expzmid * difk1 * (rdz + .5 * dmdzn);
+        double eddyq = g.expzmid_inv * difk0 * (rdz + .5 * dmdzc) +
+                       g.expzmid * difk1 * (rdz - .5 * dmdzn);
+        pc = molp * rdz - expzk * (eddyp * dfac + .5 * wic) * rdz;
+        rc = molr * rdz - expzk * (eddyr * dfac - .5 * wic) * rdz;
+        qc = -molq * rdz + expzk * (eddyq * dfac * rdz + dtx2inv - Tc[q].sloss);
         fr = Tc[q].ftm1s * dtx2inv - Tc[q].hadvec + Tc[q].s0prod;
         fr = fr * expzk;
         if (k == 0) {
@@ -1695,7 +1730,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
   int i, j;
   if (!column_of(G, i, j)) return;
   const int nk = G.nk;
-  const double dz = g.dz, dtx2inv = g.dtx2inv, thdiffalpha = -0.38, small = 1.e-9;
+  const double dz = g.dz, dtx2inv = g.dtx2inv, thdiffalpha = -0.38, small = 1.e-9, rdz = 1. / g.dz;
   const size_t c2 = G.a2(i, j);
   const double fhe = g.calc_helium == 0 ? 0. : in.flx_he[c2];
   out.flx_he[c2] = fhe;
@@ -1719,7 +1754,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       ps0[l] = g.b[3 * l + 0] * o2_0 + g.b[3 * l + 1] * o1_0 + g.b[3 * l + 2] * he_0 + g.fb[l];
     const double embar0 =
         1. / (ps0[0] * RIO2 + ps0[1] * RIO1 + ps0[2] * RIHE + (1. - ps0[0] - ps0[1] - ps0[2]) * RIN2);
-    struct Lev { double o2i, o1i, hei, embari, tni, dembardz, dtndz, df[3]; };
+    struct Lev { double o2i, o1i, hei, embari, tni, dembardz, dtndz, df[3], q; };
     // Fields read at two neighbouring levels (interface means); one level of
     // them is carried in registers.
     struct LR { double o2, o1, he, mbar, tn, w; };
@@ -1743,9 +1778,11 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       S.expz = g.expz[L]; S.difk0 = g.difk[L]; S.difk1 = g.difk[L + 1];
     };
     auto lev_df = [&](Lev &Q) {  // comp_ng.F90:124-126
-      Q.dembardz = Q.dembardz / dz;
-      Q.dtndz = Q.dtndz / dz;
-      for (int n = 0; n < 3; ++n) Q.df[n] = pow(Q.tni / T00, 1.75 - ssn(n));
+      Q.dembardz = Q.dembardz * rdz;
+      Q.dtndz = Q.dtndz * rdz;
+      const TPow tp = tpows(Q.tni);
+      for (int n = 0; n < 3; ++n) Q.df[n] = tp.df[n];
+      Q.q = tp.q;
     };
     auto lev0 = [&](const LR &R, Lev &Q) {  // comp_ng.F90:80-97 at the bottom interface
       Q.o2i = .5 * (ps0[0] + o2_0);
@@ -1789,8 +1826,8 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     for (int l = 0; l < 3; ++l) epp[l] = 1. - 1. / Qc.embari * (rmass[l] + Qc.dembardz);
     epp[2] = epp[2] - thdiffalpha * Qc.dtndz / Qc.tni;
     alpha_of(Qc, alpha);
-    matinv3_ng(alpha, akp);
-    double wks1 = Qc.embari * RIN2 * pow(T00 / tl, 0.25) / TAU;
+    matinv3_r(alpha, akp);
+    double wks1 = Qc.embari * RIN2 * Qc.q * (1. / TAU);
     for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
     double gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
     double alpha22 = 0, alpha33 = 0, alpha23 = 0, alpha32 = 0, flx00 = 0;
@@ -1805,11 +1842,12 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       for (int n = 0; n < 9; ++n) akm[n] = akp[n];
       for (int n = 0; n < 3; ++n) epm[n] = epp[n];
       levk(r1, r2, Qn);
-      for (int l = 0; l < 3; ++l) epp[l] = 1. - 1. / Qn.embari * (rmass[l] + Qn.dembardz);
+      const double rem = 1. / Qn.embari;
+      for (int l = 0; l < 3; ++l) epp[l] = 1. - rem * (rmass[l] + Qn.dembardz);
       if (k != nk - 2) epp[2] = epp[2] - thdiffalpha * Qn.dtndz / Qn.tni;
       alpha_of(Qn, alpha);
-      matinv3_ng(alpha, akp);
-      wks1 = Qn.embari * RIN2 * pow(T00 / Qn.tni, 0.25) / TAU;
+      matinv3_r(alpha, akp);
+      wks1 = Qn.embari * RIN2 * Qn.q * (1. / TAU);
       wks3 = wks4;
       wks4 = Qn.dembardz / (Qn.embari * 2.);
       for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
@@ -1817,13 +1855,13 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       for (int isp = 0; isp < 3; ++isp)
         for (int m = 0; m < 3; ++m) {
           int e = isp + 3 * m;
-          molp[e] = akm[e] * (1. / dz + epm[m] / 2.);
-          molr[e] = akp[e] * (1. / dz - epp[m] / 2.);
-          molq[e] = akm[e] * (1. / dz - epm[m] / 2.) + akp[e] * (1. / dz + epp[m] / 2.);
+          molp[e] = akm[e] * (rdz + epm[m] / 2.);
+          molr[e] = akp[e] * (rdz - epp[m] / 2.);
+          molq[e] = akm[e] * (rdz - epm[m] / 2.) + akp[e] * (rdz + epp[m] / 2.);
         }
-      double eddyp = g.expzmid_inv * ls.difk0 * (1. / dz - wks3);
-      double eddyr = g.expzmid * ls.difk1 * (1. / dz + wks4);
-      double eddyq = g.expzmid_inv * ls.difk0 * (1. / dz + wks3) + g.expzmid * ls.difk1 * (1. / dz - wks4);
+      double eddyp = g.expzmid_inv * ls.difk0 * (rdz - wks3);
+      double eddyr = g.expzmid * ls.difk1 * (rdz + wks4);
+      double eddyq = g.expzmid_inv * ls.difk0 * (rdz + wks3) + g.expzmid * ls.difk1 * (rdz - wks4);
       const double(&fs)[2][3] = ls.fs;
       const double expzk = ls.expz;
       const double wik = .5 * (r1.w + r2.w);
@@ -1832,9 +1870,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
           int e = isp + 3 * m;
           double delta = isp == m ? 1. : 0.;
           double fsv = (isp < 2 && m < 2) ? fs[isp][m + 1] : 0.;
-          pk[e] = molp[e] / dz - expzk * delta / dz * (eddyp * dfac + .5 * wik);
-          rk[e] = molr[e] / dz - expzk * delta / dz * (eddyr * dfac - .5 * wik);
-          qk[e] = -molq[e] / dz + expzk * delta * (eddyq * dfac / dz + dtx2inv) - expzk * fsv;
+          pk[e] = molp[e] * rdz - expzk * delta * rdz * (eddyp * dfac + .5 * wik);
+          rk[e] = molr[e] * rdz - expzk * delta * rdz * (eddyr * dfac - .5 * wik);
+          qk[e] = -molq[e] * rdz + expzk * delta * (eddyq * dfac * rdz + dtx2inv) - expzk * fsv;
         }
       fk[0] = expzk * (ls.o2s * dtx2inv - (ls.o2adv - fs[0][0]) + ls.hdo2);
       fk[1] = expzk * (ls.o1s * dtx2inv - (ls.o1adv - fs[1][0]) + ls.hdo1);
@@ -1858,15 +1896,15 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
         alpha32 = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Qn.hei;
         flx00 = Qc.embari * RIN2 * P0 * pow(T00 / r1.tn, 0.25) / (TAU * GRAV);
         for (int isp = 0; isp < 3; ++isp)
-          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * epp[1])) -
-                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * epp[2]));
+          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * epp[1])) -
+                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * epp[2]));
         for (int n = 0; n < 9; ++n) rk[n] = 0.;
       }
       for (int m = 0; m < 3; ++m)
         for (int n = 0; n < 3; ++n)
           for (int isp = 0; isp < 3; ++isp) qk[isp + 3 * m] = qk[isp + 3 * m] - pk[isp + 3 * n] * gama[n + 3 * m];
       double wkm1[9], wkv1[3], ng[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, nz[3] = {0, 0, 0};
-      matinv3_ng(qk, wkm1);
+      matinv3_r(qk, wkm1);
       for (int isp = 0; isp < 3; ++isp) wkv1[isp] = fk[isp];
       for (int m = 0; m < 3; ++m) {
         for (int isp = 0; isp < 3; ++isp) wkv1[isp] = wkv1[isp] - pk[isp + 3 * m] * zz[m];
@@ -1912,8 +1950,8 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     double top[3];
     for (int isp = 0; isp < 3; ++isp)
       top[isp] = (1. + .5 * epp[isp] * dz) / (1. - .5 * epp[isp] * dz) * (isp == 0 ? F3(out.o2, nk - 2) : isp == 1 ? F3(out.o1, nk - 2) : F3(out.he, nk - 2));
-    top[1] = top[1] + (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * epp[1]));
-    top[2] = top[2] + (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * epp[2]));
+    top[1] = top[1] + (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * epp[1]));
+    top[2] = top[2] + (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * epp[2]));
     size_t pt = G.a3(nk - 1, i, j);
     out.o2[pt] = top[0]; out.o1[pt] = top[1]; out.he[pt] = top[2];
     if (g.calc_helium == 0)
@@ -1951,7 +1989,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
           zmtop = g.zpmid[k];
           n2top = log(n2 * pc.xn);
           double n2top1 = log(n2prev * xnprev);
-          dn2dz = (n2top - n2top1) / dz;
+          dn2dz = (n2top - n2top1) * rdz;
         }
       }
     } else if (extrap) {
