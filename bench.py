@@ -9,8 +9,10 @@ step (dynamics_ng.F90 order) over that state.
 
 Prints one JSON line:
   value   = Mcolumn-level solves/s over the interior columns (whole job),
-  roofline: the dominant kernel's algorithmic bytes / its hipEvent-timed
-            duration against the 8 TB/s HBM peak,
+  roofline: the column kernel with the longest launch: its algorithmic bytes
+            (the planes it must read and write once) / its hipEvent-timed
+            duration, against the 8 TB/s HBM peak; traffic from the
+            committed rocprofv3 --pmc summary of that kernel when present,
   cpu_baseline: the C oracle (a restatement of the reference) on one host
             core over a bounded sub-grid of the same workload.
 """
@@ -68,6 +70,24 @@ def cpu_baseline(pkg, config, nstep_sub, budget_s=20.0):
                       "x %d-level sub-grid of the same workload, %d reps, %.2f s/step" % (nk, reps, per_step)}
 
 
+def traffic_from_profiles(kernel, workload, scheme):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary
+    (profiles/*.txt lines 'TRAFFIC <kernel>|<workload>|<scheme> <bytes>'), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    key = "%s|%s|%s" % (kernel.replace(" ", ""), workload, scheme)
+    for fn in sorted(os.listdir(d)):
+        if not fn.endswith(".txt"):
+            continue
+        with open(os.path.join(d, fn)) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == key:
+                    return int(float(parts[2]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,9 +100,10 @@ def main():
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
     ap.add_argument("--scheme", default="auto", choices=("thomas", "pcr", "auto"),
                     help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
-    ap.add_argument("--serial", action="store_true",
-                    help="run the routines one after another on one stream (default: the "
-                         "independent routines run concurrently)")
+    ap.add_argument("--concurrent", action="store_true",
+                    help="run the independent routines concurrently on forked streams (default: the "
+                         "reference's order on one stream, so each kernel's duration is its own)")
+    ap.add_argument("--serial", action="store_true", help="(default) the reference's order on one stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
@@ -116,37 +137,42 @@ def main():
               file=sys.stderr, flush=True)
         del a, b
 
-    step_fn = ns.step_serial if args.serial else ns.step
+    step_fn = ns.step if args.concurrent else ns.step_serial
     if args.only:
         step_fn = calls[args.only]
-    for _ in range(args.warmup):
+    for _ in range(args.warmup):  # the first AUTO step times both schemes
         step_fn()
     torch.cuda.synchronize()
     print("# timing %d steps" % args.steps, file=sys.stderr, flush=True)
-    # per-routine hipEvent pairs, recorded by the library on the stream each
-    # routine runs on, inside the timed region
-    pkg.step.profile_begin(args.steps)
+    # per-routine and per-column-kernel hipEvent pairs, recorded by the
+    # library on the stream each runs on, inside the timed region
+    pkg.step.profile_begin(args.steps * max(1, args.nstep_sub))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         step_fn()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = pkg.step.profile_end()
+    prof, kern = pkg.step.profile_end()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     rt = {r: prof[r][0] / max(prof[r][1], 1) for r in routines}
+    kt = {r: kern[r][0] / max(kern[r][1], 1) for r in routines}
 
     nk, cols = g["nk"], g["nlon"] * g["nlat"]
     cols_all = (g["nlon"] + 4) * (g["nlat"] + 4)
     nsub = args.nstep_sub
     unknowns = sum(pkg.step.UNKNOWNS[r] * (nsub if r == "oplus" else 1) for r in routines)
     value = cols * nk * unknowns / (ms_per_step * 1e-3) / 1e6
-    # bytes per launch of each routine (all columns the kernels process, ghosts included)
-    alg = {r: pkg.step.alg_bytes_per_column_level(r) * cols_all * nk * (nsub if r == "oplus" else 1)
-           for r in routines}
-    dom = max(routines, key=lambda r: rt[r])
-    achieved = alg[dom] / (rt[dom] * 1e-3) / 1e9
-    step_bytes = sum(alg.values())
+    schemes = ns.schemes()
+    # the dominant kernel: the column kernel with the longest average launch
+    dom = max(routines, key=lambda r: kt[r])
+    names = pkg.step.COLUMN_KERNELS[dom][0]
+    kname = names[1] if schemes[dom] == "pcr" else names[0]
+    alg = pkg.step.column_kernel_bytes_per_column_level(dom) * cols_all * nk
+    achieved = alg / (kt[dom] * 1e-3) / 1e9
+    traffic = traffic_from_profiles(kname, args.config, schemes[dom])
+    step_bytes = sum(pkg.step.alg_bytes_per_column_level(r) * cols_all * nk * (nsub if r == "oplus" else 1)
+                     for r in routines)
     out = {
         "metric": "Mcolumn-level solves/s + %HBM roofline, FP32+refine vs FP64 at <=1e-10 rel",
         "value": round(value, 2),
@@ -164,13 +190,13 @@ def main():
                                "nstep_sub=%d, fused assembly+solve FP64" % (
                                    args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
                    "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64",
-                   "schedule": "serial" if args.serial else "concurrent", "scheme": args.scheme,
-                   "schemes": ns.schemes()},
+                   "schedule": "concurrent" if args.concurrent else "serial", "scheme": args.scheme,
+                   "schemes": schemes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": dom, "alg_bytes_per_launch": alg[dom],
-                     "launch_ms": round(rt[dom], 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kname, "alg_bytes_per_launch": alg, "launch_ms": round(kt[dom], 4)},
         "routine_ms": {r: round(rt[r], 4) for r in routines},
+        "column_kernel_ms": {r: round(kt[r], 4) for r in routines},
         "step_alg_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
     }
     if not args.no_cpu_baseline and not args.only:

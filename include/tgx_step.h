@@ -122,12 +122,14 @@ int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_
 
 /* Timing hooks (no counterpart in the reference; used by benchmarks and
  * profiling).  Between begin and end, each routine entry point records a
- * hipEvent pair on the stream it runs on, for at most max_calls calls per
- * routine.  end waits for the recorded events and returns per routine, in the
- * order oplus, minors, dt, duv, comp, the summed milliseconds and the number
- * of calls timed. */
+ * hipEvent pair on the stream it runs on around the whole routine and another
+ * around its column kernel (oplus: the O+ solve of each sub-step; minors: the
+ * N2D + Ar solve), for at most max_calls calls each.  end waits for the
+ * recorded events and returns the summed milliseconds and the number of calls
+ * timed: entries 0..4 the routines oplus, minors, dt, duv, comp, entries 5..9
+ * their column kernels in the same order. */
 int tgx_ng_profile_begin(int max_calls);
-int tgx_ng_profile_end(double *ms_total /* [5] */, int *calls /* [5] */);
+int tgx_ng_profile_end(double *ms_total /* [10] */, int *calls /* [10] */);
 
 #ifdef __cplusplus
 }

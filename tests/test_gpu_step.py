@@ -103,7 +103,9 @@ def test_profile_hooks_count_calls(pkg):
     for _ in range(3):
         ns.step()
     torch.cuda.synchronize()
-    prof = pkg.step.profile_end()
+    prof, kern = pkg.step.profile_end()
     for r in pkg.step.ROUTINES:
         ms, n = prof[r]
         assert n == 2 and ms > 0.0, (r, prof[r])
+        km, kn = kern[r]
+        assert kn == 2 and 0.0 < km <= ms, (r, kern[r], prof[r])

@@ -263,7 +263,8 @@ struct StencilJobs {
 // the plane and loads every job's input once into LDS with a 2-point halo
 // (coalesced rows), all jobs' loads in flight together, then evaluates the
 // stencil functions from LDS.  Halo points outside the domain are never read:
-// the stencils clamp at the domain edges.  Dynamic LDS: (jobs + 2) tiles.
+// the stencils clamp at the domain edges.  Dynamic LDS: (jobs + 2) tiles and
+// one latitude-pass buffer.
 __global__ void __launch_bounds__(256) stencil_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                       StencilJobs J) {
   extern __shared__ double smem[];
@@ -311,19 +312,41 @@ __global__ void __launch_bounds__(256) stencil_kernel(tgx_ng_grid g, Geo G, tgx_
     }
   }
   __syncthreads();
-  if (!inside) return;
   const LdsPlane U{tu, i0 - TILE_H, j0 - TILE_H}, V{tv, i0 - TILE_H, j0 - TILE_H};
   const size_t p = off + (size_t)j * G.ld + i, c2 = (size_t)j * G.ld + i;
+  double *tl = tv + TS;  // latitude pass of a smoothed field: TILE_Y rows x TILE_W
   for (int n = 0; n < J.n; ++n) {
     const StencilJob &jb = J.job[n];
     const LdsPlane F{smem + n * TS, i0 - TILE_H, j0 - TILE_H};
+    if (jb.kind == ST_SMOOTH) {
+      // smooth_ng.F90:283-294 once per point of the tile rows and their
+      // longitude halo, then the longitude pass (:296-307) from LDS
+      for (int e = threadIdx.x; e < TILE_W * TILE_Y; e += blockDim.x) {
+        int ii = i0 - TILE_H + e % TILE_W, jj = j0 + e / TILE_W;
+        if (ii >= 0 && ii < G.nlond && jj < G.nlatd) tl[e] = smooth_lat(F, G, ii, jj, jb.s);
+      }
+      __syncthreads();
+      if (inside) {
+        const double *row = tl + (j - j0) * TILE_W + TILE_H - i0;  // row[ii] = lat pass at (ii, j)
+        int L1 = G.nlond - 1;
+        bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+        int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
+        int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
+        double t0 = row[i];
+        double fm1 = (row[i1p] + row[i1m]) * 0.5;
+        double fm2 = (row[i2p] + row[i2m]) * 0.5;
+        double gen = t0 - jb.s * (2. * fm2 - 8. * fm1 + 6. * t0);
+        double nr = t0 - jb.s * (-8. * fm1 + 8. * t0);
+        jb.out[p] = edge ? t0 : (near ? nr : gen);
+      }
+      __syncthreads();
+      continue;
+    }
+    if (!inside) continue;
     double v;
     switch (jb.kind) {
       case ST_ADVEC:
         v = advec(F, U, V, G, i, j, g.racs[j], g.dlamda, g.dphi);
-        break;
-      case ST_SMOOTH:
-        v = smooth(F, G, i, j, jb.s);
         break;
       case ST_DIVX:
         v = divx(F, G, i, j);
@@ -3107,7 +3130,9 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
 // Launcher used by step_api.cpp.  bufs holds the workspace planes (and for
 // O+ the input/output state of the sub-step) in the order documented there.
 static size_t stencil_lds(const tgx::ng::StencilJobs &J) {
-  return (size_t)(J.n + 2) * tgx::ng::TILE_R * tgx::ng::TILE_W * sizeof(double);
+  // one tile per job, the u and v tiles, and one latitude-pass buffer
+  return ((size_t)(J.n + 2) * tgx::ng::TILE_R * tgx::ng::TILE_W +
+          (size_t)tgx::ng::TILE_Y * tgx::ng::TILE_W) * sizeof(double);
 }
 
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fields *inp,

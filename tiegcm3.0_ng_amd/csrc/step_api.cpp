@@ -71,12 +71,12 @@ extern "C" unsigned long long tgx_ng_workspace_bytes(const tgx_ng_grid *g) {
 // Optional per-routine timing: hipEvent pairs recorded on the stream each
 // routine runs on, accumulated until tgx_ng_profile_end.  Off by default.
 namespace {
-enum { R_OPLUS = 0, R_MINORS, R_DT, R_DUV, R_COMP, R_N };
+enum { R_OPLUS = 0, R_MINORS, R_DT, R_DUV, R_COMP, R_N, P_N = 2 * R_N };  // routines, then their column kernels
 struct Prof {
   bool on = false;
   int cap = 0;
-  int used[R_N] = {};
-  std::vector<hipEvent_t> ev;  // [R_N][cap][2]
+  int used[P_N] = {};
+  std::vector<hipEvent_t> ev;  // [P_N][cap][2]
 };
 std::mutex g_prof_mu;
 Prof g_prof;
@@ -99,14 +99,14 @@ struct ProfScope {
 extern "C" int tgx_ng_profile_begin(int max_calls) {
   if (max_calls < 1 || max_calls > 100000) return tgx_set_error(1, "tgx_ng_profile_begin: bad max_calls %d", max_calls);
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  size_t need = (size_t)R_N * max_calls * 2;
+  size_t need = (size_t)P_N * max_calls * 2;
   while (g_prof.ev.size() < need) {
     hipEvent_t e;
     TGX_HIP(hipEventCreate(&e));
     g_prof.ev.push_back(e);
   }
   g_prof.cap = max_calls;
-  for (int r = 0; r < R_N; ++r) g_prof.used[r] = 0;
+  for (int r = 0; r < P_N; ++r) g_prof.used[r] = 0;
   g_prof.on = true;
   return 0;
 }
@@ -115,7 +115,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   if (!g_prof.on) return tgx_set_error(1, "tgx_ng_profile_end: profiling not started");
   g_prof.on = false;
-  for (int r = 0; r < R_N; ++r) {
+  for (int r = 0; r < P_N; ++r) {
     double tot = 0.;
     for (int c = 0; c < g_prof.used[r]; ++c) {
       size_t at = ((size_t)r * g_prof.cap + c) * 2;
@@ -168,6 +168,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   ProfScope prof(R_DT, s);
   void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
+  ProfScope kern(R_N + R_DT, s);
   return tgx_ng_launch(use_pcr(g, R_DT) ? K_DT_W : K_DT, g, in, out, b, 0, s);
 }
 
@@ -185,7 +186,10 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
   const bool pcr = use_pcr(g, R_MINORS);
   if (int rc = tgx_ng_launch(K_MIN_SRC_A, g, in, out, b, 0, s)) return rc;
-  if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
+  {
+    ProfScope kern(R_N + R_MINORS, s);  // the N2D + Ar column kernel
+    if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
+  }
   if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
   return tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s);
 }
@@ -219,6 +223,7 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
+    ProfScope kern(R_N + R_OPLUS, s);
     if (int rc = tgx_ng_launch(use_pcr(g, R_OPLUS) ? K_OP3_W : K_OP3, g, in, out, b, sub, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
@@ -237,6 +242,7 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 5 + q);
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
+  ProfScope kern(R_N + R_DUV, s);
   return tgx_ng_launch(use_pcr(g, R_DUV) ? K_UV2_W : K_UV2, g, in, out, b, 0, s);
 }
 
@@ -250,6 +256,7 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
+  ProfScope kern(R_N + R_COMP, s);
   return tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
 }
 

@@ -79,12 +79,14 @@ def profile_begin(max_calls):
 
 
 def profile_end():
-    """Stop recording; returns {routine: (summed ms, calls)}."""
-    ms = (C.c_double * 5)()
-    n = (C.c_int * 5)()
+    """Stop recording; returns ({routine: (summed ms, calls)}, {routine: (summed
+    ms, calls) of its column kernel})."""
+    ms = (C.c_double * 10)()
+    n = (C.c_int * 10)()
     if _lib().tgx_ng_profile_end(C.cast(ms, _P), C.cast(n, _P)) != 0:
         raise _tgx.TgxError(_tgx.last_error())
-    return {r: (ms[i], n[i]) for i, r in enumerate(ROUTINES)}
+    return ({r: (ms[i], n[i]) for i, r in enumerate(ROUTINES)},
+            {r: (ms[5 + i], n[5 + i]) for i, r in enumerate(ROUTINES)})
 
 
 def _lib():
@@ -254,6 +256,40 @@ UNKNOWNS = {"oplus": 1, "minors": 4, "dt": 1, "duv": 2, "comp": 3}
 
 def alg_bytes_per_column_level(routine):
     ins, outs = ALG_FIELDS[routine]
+    return 8 * (len(set(ins)) + len(set(outs)))
+
+
+# The column kernel of each routine: its name per scheme and the distinct
+# planes it must read and write once per launch (8 bytes per column-level
+# each).  Horizontal-stencil and chemistry terms arrive as planes from the
+# routine's level-parallel kernels; Thomas ratio / gamma planes are scratch,
+# not algorithmic.
+COLUMN_KERNELS = {
+    "comp": (("comp_kernel", "comp_wave_kernel"),
+             "o2 o1 he mbar tn w fs01 fs02 fs11 fs12 fs00 fs10 o2adv o1adv headv o2s o1s hes "
+             "hdo2 hdo1 hdhe o2_nm o1_nm he_nm xnmbar".split(),
+             "o2 o1 he o2_nm o1_nm he_nm".split()),
+    "minors": (("minor_kernel<2, 3>", "minor_wave_kernel<2, 3>"),  # N2D + Ar
+               "o1 o2 he mbar tn barm w_upd ftm1s_n2d hadvec_n2d ftm1s_ar hadvec_ar sloss_n2d "
+               "s0prod_n2d n2d n2d_nm ar ar_nm".split(),
+               "n2d n2d_nm ar ar_nm".split()),
+    "dt": (("dt_kernel", "dt_wave_kernel"),
+           "tn w_upd cp km un vn qtotal schti kt hadv_tn smooth_tn_nm xnmbar o1 rkm12 mbar scht "
+           "hdt qji_tn cool_imp cool_exp tn_nm".split(),
+           "tn tn_nm".split()),
+    "duv": (("duv_stage2", "duv_wave_stage2"),
+            "w_upd lxx lyy lxy lyx ui vi divx_ztmp divy_z km schti scht Fe Fn unms vnms advu advv "
+            "hdu hdv un un_nm vn_nm vn".split(),
+            "un vn un_nm vn_nm".split()),
+    "oplus": (("oplus_stage3", "oplus_wave_stage3"),  # per O+ sub-step
+              "scht ti te dj bdotu bdopj wi bd_diff bd_diffj djbz vdotn bd_bvel opts2 bd_djbz "
+              "oploss opprod op optm1".split(),
+              "op op_nm".split()),
+}
+
+
+def column_kernel_bytes_per_column_level(routine):
+    _, ins, outs = COLUMN_KERNELS[routine]
     return 8 * (len(set(ins)) + len(set(outs)))
 
 
