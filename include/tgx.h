@@ -109,6 +109,19 @@ int tgx_blktri2_dev_f64(const double *a, const double *b, const double *c, const
 int tgx_blktri3_dev_f64(const double *p, const double *q, const double *r, const double *f,
                         double *x, int ldk, int nrow, int col0, int ncol, void *stream);
 
+/* Mixed precision for N x N block rows (N = 1, 2, 3; N = 1 is the scalar
+ * system): FP32 elimination with each block pivot inverted in FP64, FP64
+ * residual r = F - A x, nsweep (0..8) refinement sweeps reusing the FP32
+ * factorisation (the precision variant of trsolv_ng / blktri / comp's 3x3
+ * elimination).  Element e (row-major) of block A at (col, k) is at
+ * A[e*pstride + col*nk + k]; rows 0..nrow-1; columns with valid[col] == 0 are
+ * skipped.  work: tgx_blk_mixed_work_bytes(N, nrow, ncol) bytes of device
+ * memory. */
+int tgx_blk_mixed_dev(int nblk, const double *A, const double *B, const double *C, const double *F,
+                      double *X, long pstride, int nk, int nrow, const unsigned char *valid, long ncol,
+                      int nsweep, void *work, void *stream);
+unsigned long long tgx_blk_mixed_work_bytes(int nblk, int nrow, long ncol);
+
 #ifdef __cplusplus
 }
 #endif

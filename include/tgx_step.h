@@ -43,7 +43,18 @@ typedef struct {
   double fb[3];
   const double *expz, *difk, *dift, *xmue, *zpint, *zpmid;   /* [nk] */
   const double *racs, *cs, *tanphi, *cor, *glat;             /* [nlat+4] */
+  /* Optional (NULL in normal use): when set, the Thomas column kernels also
+   * write the rows they assemble, for analysis of the column systems.  Layout:
+   * capture[(plane*ncol + col)*nk + k], col = j*(nlon+4) + i, planes
+   *   0-3 dt a,b,c,f;  4-7 O+ a,b,c,f (last sub-step);  8+4s..11+4s minor
+   *   species s (0 NO, 1 N4S, 2 N2D, 3 Ar) a,b,c,f;  24-27 duv A, 28-31 B,
+   *   32-35 C (2x2 row-major), 36-37 F;  38-46 comp A, 47-55 B, 56-64 C (3x3
+   *   row-major), 65-67 F.  a x(k-1) + b x(k) + c x(k+1) = f, rows 0..nk-2
+   *   (minor species 0..nk-1); lateral-boundary columns are not written. */
+  double *capture;
 } tgx_ng_grid;
+
+#define TGX_NG_CAPTURE_PLANES 68
 
 /* Model state read by the step (itp level unless noted). */
 typedef struct {

@@ -182,6 +182,12 @@ def blktri(a, b, c, f, x, lev0, lev1, lon0, lon1, lonmax):
     return x
 
 
+def blktri_ng(pp, qq, rr, ss, xx, nk, ncol):
+    """duv_ng.F90:218-248 on assembled 2x2 blocks (2,2,nk,ncol), rows 1..nk-1."""
+    lib().tgx_o_blktri_ng(_p(pp), _p(qq), _p(rr), _p(ss), _p(xx), nk, ncol)
+    return xx
+
+
 def blktri3(p, q, r, f, x, nk, ncol, k2):
     lib().tgx_o_blktri3(_p(p), _p(q), _p(r), _p(f), _p(x), nk, ncol, k2)
     return x

@@ -26,7 +26,10 @@ def field_err(got, ref):
 # the elimination and is held to the same 1e-12 bound.
 @pytest.fixture(scope="module", params=[("tiny", 2, "thomas"), ("small", 1, "thomas"), ("small", 3, "thomas"),
                                         ("tall", 1, "thomas"), ("tiny", 2, "pcr"), ("small", 1, "pcr"),
-                                        ("tall", 1, "pcr")],
+                                        ("tall", 1, "pcr"),
+                                        # the 5 and 2.5 degree global grids and a nest at full size
+                                        ("g5", 1, "thomas"), ("g2p5", 1, "thomas"), ("g2p5", 1, "pcr"),
+                                        ("nest1", 2, "thomas")],
                 ids=lambda p: "%s-nsub%d-%s" % p)
 def case(request, pkg, oracle):
     import torch
@@ -109,3 +112,33 @@ def test_profile_hooks_count_calls(pkg):
         assert n == 2 and ms > 0.0, (r, prof[r])
         km, kn = kern[r]
         assert kn == 2 and 0.0 < km <= ms, (r, kern[r], prof[r])
+
+
+def test_nested_outer_step_graph_replay_matches_eager(pkg):
+    """The global grid and two nests stepped in the reference's recursion
+    (level i runs prod(nstep_ng[:i+1]) sub-steps) captured into one HIP graph:
+    every replay reproduces the eager outer step bit for bit, with eager work
+    and synchronisation between replays."""
+    import torch
+    doms = []
+    for name, nsub in (("small", 1), ("tiny", 2), ("tall", 1)):
+        g = pkg.synth.make_grid(name, nstep_sub=nsub)
+        doms.append((g, pkg.synth.make_state(g)))
+    ns = pkg.step.NestedStep(doms, [1, 2, 3])
+    assert ns.substeps() == {0: 1, 1: 2, 2: 6}
+    ns.run()
+    torch.cuda.synchronize()
+    eager = [{n: t.clone() for n, t in st.out.items()} for st in ns.states]
+    ns.capture()
+    for rep in range(3):
+        for st in ns.states:
+            for t in st.out.values():
+                t.zero_()
+        torch.cuda.synchronize()
+        ns.replay()
+        torch.cuda.synchronize()
+        for d, st in enumerate(ns.states):
+            for n, t in st.out.items():
+                assert torch.equal(t, eager[d][n]), (rep, d, n)
+        ns.run()  # eager work between replays
+        torch.cuda.synchronize()

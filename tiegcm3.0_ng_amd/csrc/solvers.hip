@@ -364,6 +364,131 @@ __global__ void __launch_bounds__(256) blktri3_kfast_f64(
 
 // ---------------------------------------------------------------------------
 // Launchers (called from api.cpp)
+namespace tgx {
+// ---------------------------------------------------------------------------
+// Mixed-precision block Thomas for N x N block rows (N = 1, 2, 3):
+// A x(k-1) + B x(k) + C x(k+1) = F.  The elimination runs in FP32 (the
+// coefficients rounded to float, Schur complements and multipliers in float);
+// each block pivot is inverted in FP64 and applied in FP64; the residual
+// r = F - A x is formed in FP64 from the FP64 coefficients and nsweep
+// refinement sweeps solve for the correction with the stored FP32
+// factorisation.  Layout: element e (row-major n*N+m) of A for (col, k) at
+// A[e*pstride + col*nk + k]; rows 0..nrow-1; columns with valid[col] == 0
+// are skipped.  Work per column and row: N*N floats (multipliers), N*N
+// doubles (pivot inverses), 2N doubles.
+template <int N>
+__device__ __forceinline__ void inv_nd(const double *M, double *R) {
+  if (N == 1) {
+    R[0] = 1. / M[0];
+  } else if (N == 2) {
+    double d = M[0] * M[3] - M[1] * M[2];
+    R[0] = M[3] / d; R[1] = -M[1] / d; R[2] = -M[2] / d; R[3] = M[0] / d;
+  } else {
+    double c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8], c02 = M[3] * M[7] - M[4] * M[6];
+    double r = 1. / (M[0] * c00 + M[1] * c01 + M[2] * c02);
+    R[0] = c00 * r; R[3] = c01 * r; R[6] = c02 * r;
+    R[1] = (M[2] * M[7] - M[1] * M[8]) * r; R[4] = (M[0] * M[8] - M[2] * M[6]) * r;
+    R[7] = (M[1] * M[6] - M[0] * M[7]) * r;
+    R[2] = (M[1] * M[5] - M[2] * M[4]) * r; R[5] = (M[2] * M[3] - M[0] * M[5]) * r;
+    R[8] = (M[0] * M[4] - M[1] * M[3]) * r;
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) blk_mixed_kernel(const double *__restrict__ A, const double *__restrict__ B,
+                                                        const double *__restrict__ C, const double *__restrict__ F,
+                                                        double *__restrict__ X, long pstride, int nk, int nrow,
+                                                        const unsigned char *__restrict__ valid, long ncol,
+                                                        int nsweep, float *__restrict__ gam,
+                                                        double *__restrict__ ginv, double *__restrict__ yv) {
+  long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol || !valid[col]) return;
+  constexpr int NN = N * N;
+  const size_t base = (size_t)col * nk;
+  auto el = [&](const double *M, int e, int k) { return M[(size_t)e * pstride + base + k]; };
+  float *Gm = gam + (size_t)col * nrow * NN;      // multipliers gamma_k = Gamma_k^-1 C_k (float)
+  double *Gi = ginv + (size_t)col * nrow * NN;    // pivot inverses Gamma_k^-1 (double)
+  double *Y = yv + (size_t)col * nrow * 2 * N;    // y and the right-hand side of a sweep
+  // factorisation: Gamma_k = B_k - A_k gamma_{k-1} in FP32
+  float gp[NN];
+  for (int e = 0; e < NN; ++e) gp[e] = 0.f;
+  for (int k = 0; k < nrow; ++k) {
+    float Af[NN], Gf[NN];
+    for (int e = 0; e < NN; ++e) { Af[e] = (float)el(A, e, k); Gf[e] = (float)el(B, e, k); }
+    for (int n = 0; n < N; ++n)
+      for (int m = 0; m < N; ++m) {
+        float t = Gf[n * N + m];
+        for (int l = 0; l < N; ++l) t -= Af[n * N + l] * gp[l * N + m];
+        Gf[n * N + m] = t;
+      }
+    double Gd[NN], Id[NN];
+    for (int e = 0; e < NN; ++e) Gd[e] = (double)Gf[e];
+    inv_nd<N>(Gd, Id);  // the block pivot inverted in FP64
+    for (int e = 0; e < NN; ++e) Gi[(size_t)k * NN + e] = Id[e];
+    float Cf[NN];
+    for (int e = 0; e < NN; ++e) Cf[e] = (float)el(C, e, k);
+    for (int n = 0; n < N; ++n)
+      for (int m = 0; m < N; ++m) {
+        double t = 0.;
+        for (int l = 0; l < N; ++l) t += Id[n * N + l] * (double)Cf[l * N + m];
+        gp[n * N + m] = (float)t;
+      }
+    for (int e = 0; e < NN; ++e) Gm[(size_t)k * NN + e] = gp[e];
+  }
+  // solve with the factorisation: rhs in Y[.. + N + n], result into X (add != 0: X += solution)
+  auto solve = [&](bool add) {
+    float yp[N];
+    for (int n = 0; n < N; ++n) yp[n] = 0.f;
+    for (int k = 0; k < nrow; ++k) {
+      float v[N];
+      for (int n = 0; n < N; ++n) {
+        float t = (float)Y[((size_t)k * 2 + 1) * N + n];
+        for (int l = 0; l < N; ++l) t -= (float)el(A, n * N + l, k) * yp[l];
+        v[n] = t;
+      }
+      for (int n = 0; n < N; ++n) {
+        double t = 0.;
+        for (int l = 0; l < N; ++l) t += Gi[(size_t)k * NN + n * N + l] * (double)v[l];
+        yp[n] = (float)t;
+        Y[(size_t)k * 2 * N + n] = t;
+      }
+    }
+    float xn[N];
+    for (int n = 0; n < N; ++n) xn[n] = 0.f;
+    for (int k = nrow - 1; k >= 0; --k) {
+      float xv[N];
+      for (int n = 0; n < N; ++n) {
+        float t = (float)Y[(size_t)k * 2 * N + n];
+        for (int l = 0; l < N; ++l) t -= Gm[(size_t)k * NN + n * N + l] * xn[l];
+        xv[n] = t;
+      }
+      for (int n = 0; n < N; ++n) {
+        double* xo = X + (size_t)n * pstride + base + k;
+        *xo = add ? *xo + (double)xv[n] : (double)xv[n];
+        xn[n] = xv[n];
+      }
+    }
+  };
+  for (int k = 0; k < nrow; ++k)
+    for (int n = 0; n < N; ++n) Y[((size_t)k * 2 + 1) * N + n] = el(F, n, k);
+  solve(false);
+  for (int sw = 0; sw < nsweep; ++sw) {
+    // FP64 residual r = F - (A x(k-1) + B x(k) + C x(k+1))
+    for (int k = 0; k < nrow; ++k)
+      for (int n = 0; n < N; ++n) {
+        double r = el(F, n, k);
+        for (int l = 0; l < N; ++l) {
+          if (k > 0) r -= el(A, n * N + l, k) * X[(size_t)l * pstride + base + k - 1];
+          r -= el(B, n * N + l, k) * X[(size_t)l * pstride + base + k];
+          if (k + 1 < nrow) r -= el(C, n * N + l, k) * X[(size_t)l * pstride + base + k + 1];
+        }
+        Y[((size_t)k * 2 + 1) * N + n] = r;
+      }
+    solve(true);
+  }
+}
+}  // namespace tgx
+
 extern "C" int tgx_launch_thomas_f64(const double *a, const double *b, const double *c,
                                      const double *f, double *x, double *w2s, int ldk, int k1,
                                      int k2, int col0, int ncol, hipStream_t s) {
@@ -421,4 +546,37 @@ extern "C" int tgx_launch_blktri3_f64(const double *p, const double *q, const do
                      f, x, gs, ldk, nrow, col0, ncol);
   TGX_HIP(hipGetLastError());
   return 0;
+}
+
+extern "C" int tgx_blk_mixed_dev(int nblk, const double *A, const double *B, const double *C,
+                                 const double *F, double *X, long pstride, int nk, int nrow,
+                                 const unsigned char *valid, long ncol, int nsweep, void *work,
+                                 void *stream) {
+  if (nblk < 1 || nblk > 3) return tgx_set_error(1, "tgx_blk_mixed_dev: block size %d not in 1..3", nblk);
+  if (nrow < 1 || nrow > nk || ncol < 0 || nsweep < 0 || nsweep > 8 || !work || !valid)
+    return tgx_set_error(1, "tgx_blk_mixed_dev: bad arguments");
+  if (ncol == 0) return 0;
+  const size_t nn = (size_t)nblk * nblk, per = (size_t)ncol * nrow;
+  float *gam = (float *)work;
+  double *ginv = (double *)((char *)work + ((per * nn * sizeof(float) + 255) / 256) * 256);
+  double *yv = ginv + per * nn;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)((ncol + 255) / 256)), block(256);
+  if (nblk == 1)
+    hipLaunchKernelGGL(tgx::blk_mixed_kernel<1>, grid, block, 0, s, A, B, C, F, X, pstride, nk, nrow, valid,
+                       ncol, nsweep, gam, ginv, yv);
+  else if (nblk == 2)
+    hipLaunchKernelGGL(tgx::blk_mixed_kernel<2>, grid, block, 0, s, A, B, C, F, X, pstride, nk, nrow, valid,
+                       ncol, nsweep, gam, ginv, yv);
+  else
+    hipLaunchKernelGGL(tgx::blk_mixed_kernel<3>, grid, block, 0, s, A, B, C, F, X, pstride, nk, nrow, valid,
+                       ncol, nsweep, gam, ginv, yv);
+  TGX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" unsigned long long tgx_blk_mixed_work_bytes(int nblk, int nrow, long ncol) {
+  const size_t nn = (size_t)nblk * nblk, per = (size_t)ncol * nrow;
+  return ((per * nn * sizeof(float) + 255) / 256) * 256 + per * nn * sizeof(double) +
+         per * 2 * nblk * sizeof(double);
 }

@@ -51,6 +51,16 @@ struct Geo {
   __device__ __forceinline__ size_t a2(int i, int j) const { return (size_t)j * ld + i; }
 };
 
+// Rows assembled by the Thomas column kernels, written only when the caller
+// asks for them (grid.capture != NULL): plane pl of the capture buffer, level
+// fastest per column.
+enum { CAP_DT = 0, CAP_OP = 4, CAP_MIN = 8, CAP_UV = 24, CAP_CO = 38 };
+__device__ __forceinline__ void cap_put(const tgx_ng_grid &g, const Geo &G, int pl, int i, int j, int k,
+                                        double v) {
+  size_t ncol = (size_t)G.nlond * G.nlatd, col = (size_t)j * G.nlond + i;
+  g.capture[((size_t)pl * ncol + col) * G.nk + k] = v;
+}
+
 static Geo make_geo(const tgx_ng_grid &g) {
   Geo G;
   G.nk = g.nk; G.nlon = g.nlon; G.nlat = g.nlat;
@@ -373,6 +383,7 @@ __global__ void __launch_bounds__(256) stencil_kernel(tgx_ng_grid g, Geo G, tgx_
 
 // =============================================================================
 // dt_ng.F90:54-213
+template <bool CAP>
 __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                  tgx_ng_out out, double *__restrict__ w2s,
                                                  const double *__restrict__ hadv_tn,
@@ -495,6 +506,10 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
       q = q - p;
       rhs = rhs - 2. * p * t_lbc;
       p = 0.;
+    }
+    if (CAP) {
+      cap_put(g, G, CAP_DT + 0, i, j, k, p); cap_put(g, G, CAP_DT + 1, i, j, k, q);
+      cap_put(g, G, CAP_DT + 2, i, j, k, r); cap_put(g, G, CAP_DT + 3, i, j, k, rhs);
     }
     // Thomas forward (trsolv_ng.F90:14-22)
     size_t pk = G.a3(k, i, j);
@@ -719,7 +734,7 @@ __global__ void __launch_bounds__(256) minor_src_kernel(tgx_ng_grid g, Geo G, tg
   }
 }
 
-template <int SA, int SB>
+template <int SA, int SB, bool CAP>
 __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     tgx_ng_out out, double *__restrict__ w2a,
                                                     double *__restrict__ w2b, MinorTerms MT) {
@@ -942,6 +957,11 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
         qc = pc - 2.;
         rc = 0.;
         fr = -GRAV * sc[q].rmx * fubc * dzp / (P0 * Tc[q].ax * AVO);
+      }
+      if (CAP) {
+        const int pl = CAP_MIN + 4 * SP[q];
+        cap_put(g, G, pl + 0, i, j, k, pc); cap_put(g, G, pl + 1, i, j, k, qc);
+        cap_put(g, G, pl + 2, i, j, k, rc); cap_put(g, G, pl + 3, i, j, k, fr);
       }
       // Thomas forward, rows 1..nk (minor_ng.F90:200)
       if (k == 0) {
@@ -1747,6 +1767,7 @@ __global__ void __launch_bounds__(256) o2o_kernel(tgx_ng_grid g, Geo G, tgx_ng_f
   ((double *)CT.fs[5])[p] = (pox1 - lox3) * RMO1 / xn;               // fs(k,2,0)
 }
 
+template <bool CAP>
 __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                    tgx_ng_out out, double *__restrict__ gs,
                                                    CompTerms CT) {
@@ -1922,6 +1943,16 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
           fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * epp[1])) -
                     rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * epp[2]));
         for (int n = 0; n < 9; ++n) rk[n] = 0.;
+      }
+      if (CAP) {
+        for (int r = 0; r < 3; ++r) {
+          for (int c = 0; c < 3; ++c) {
+            cap_put(g, G, CAP_CO + 0 + 3 * r + c, i, j, k, pk[r + 3 * c]);
+            cap_put(g, G, CAP_CO + 9 + 3 * r + c, i, j, k, qk[r + 3 * c]);
+            cap_put(g, G, CAP_CO + 18 + 3 * r + c, i, j, k, rk[r + 3 * c]);
+          }
+          cap_put(g, G, CAP_CO + 27 + r, i, j, k, fk[r]);
+        }
       }
       for (int m = 0; m < 3; ++m)
         for (int n = 0; n < 3; ++n)
@@ -2409,6 +2440,7 @@ __global__ void __launch_bounds__(256) oplus_chem_kernel(tgx_ng_grid g, Geo G, t
   }
 }
 
+template <bool CAP>
 __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     const double *__restrict__ op,
                                                     const double *__restrict__ optm1,
@@ -2548,6 +2580,10 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
       qcf = qcf - pcf;
       expl = expl - 2. * pcf * qop0 / (1.5 * os.oploss - 0.5 * ol1);
       pcf = 0.;
+    }
+    if (CAP) {
+      cap_put(g, G, CAP_OP + 0, i, j, k, pcf); cap_put(g, G, CAP_OP + 1, i, j, k, qcf);
+      cap_put(g, G, CAP_OP + 2, i, j, k, rcf); cap_put(g, G, CAP_OP + 3, i, j, k, expl);
     }
     // Thomas forward (trsolv_ng, rows 1..nk-1)
     if (k == 0) {
@@ -2780,6 +2816,7 @@ struct UvTerms {  // prelude planes
   const double *divx_ztmp, *divy_z, *unms, *vnms, *advu, *advv;
 };
 
+template <bool CAP>
 __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                   tgx_ng_out out, double *__restrict__ gs,
                                                   UvTerms UT) {
@@ -2895,6 +2932,16 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
     if (k == nk - 2) {  // :214-215
       qq11 = qq11 + r11; qq12 = qq12 + r12; qq21 = qq21 + r21; qq22 = qq22 + r22;
       r11 = 0.; r12 = 0.; r21 = 0.; r22 = 0.;
+    }
+    if (CAP) {
+      const double Ab[4] = {p11, p12, p21, p22}, Bb[4] = {qq11, qq12, qq21, qq22}, Cb[4] = {r11, r12, r21, r22};
+      for (int e = 0; e < 4; ++e) {
+        cap_put(g, G, CAP_UV + e, i, j, k, Ab[e]);
+        cap_put(g, G, CAP_UV + 4 + e, i, j, k, Bb[e]);
+        cap_put(g, G, CAP_UV + 8 + e, i, j, k, Cb[e]);
+      }
+      cap_put(g, G, CAP_UV + 12, i, j, k, s1);
+      cap_put(g, G, CAP_UV + 13, i, j, k, s2);
     }
     if (k == 0) {
       double det = qq11 * qq22 - qq12 * qq21;
@@ -3158,7 +3205,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   };
   switch (which) {
     case 0:  // dt: bufs = {w2, hadv_tn, smooth_tn_nm}
-      hipLaunchKernelGGL(dt_kernel, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
+      if (g.capture)
+        hipLaunchKernelGGL(dt_kernel<true>, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
+      else
+        hipLaunchKernelGGL(dt_kernel<false>, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
       break;
     case 20:
       hipLaunchKernelGGL(dt_wave_kernel, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
@@ -3178,10 +3228,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       MT.sloss[1] = D(12);
       MT.s0prod[1] = D(13);
       MT.sloss[3] = MT.s0prod[3] = nullptr;
-      if (which == 1)
-        hipLaunchKernelGGL((minor_kernel<2, 3>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      if (which == 1 && g.capture)
+        hipLaunchKernelGGL((minor_kernel<2, 3, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      else if (which == 1)
+        hipLaunchKernelGGL((minor_kernel<2, 3, false>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      else if (which == 2 && g.capture)
+        hipLaunchKernelGGL((minor_kernel<1, 0, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
       else if (which == 2)
-        hipLaunchKernelGGL((minor_kernel<1, 0>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+        hipLaunchKernelGGL((minor_kernel<1, 0, false>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
       else if (which == 18)
         hipLaunchKernelGGL((minor_wave_kernel<2, 3>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 19)
@@ -3224,8 +3278,12 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
         if (which == 5)
-          hipLaunchKernelGGL(oplus_stage3, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
-                             (const double *)D(19), (const double *)D(20), opout, optm1out);
+          if (g.capture)
+            hipLaunchKernelGGL(oplus_stage3<true>, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
+                               (const double *)D(19), (const double *)D(20), opout, optm1out);
+          else
+            hipLaunchKernelGGL(oplus_stage3<false>, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
+                               (const double *)D(19), (const double *)D(20), opout, optm1out);
         else
           hipLaunchKernelGGL(oplus_wave_stage3, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                              OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
@@ -3239,7 +3297,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 22: {  // duv stage 2: bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
       UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
       if (which == 7)
-        hipLaunchKernelGGL(duv_stage2, grid, block, 0, s, g, G, in, out, D(1), UT);
+        if (g.capture)
+          hipLaunchKernelGGL(duv_stage2<true>, grid, block, 0, s, g, G, in, out, D(1), UT);
+        else
+          hipLaunchKernelGGL(duv_stage2<false>, grid, block, 0, s, g, G, in, out, D(1), UT);
       else
         hipLaunchKernelGGL(duv_wave_stage2, gridw, block, 0, s, g, G, in, out, UT);
       break;
@@ -3249,7 +3310,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 23: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6]}
       CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6), {D(7), D(8), D(9), D(10), D(11), D(12)}};
       if (which == 8) {
-        hipLaunchKernelGGL(comp_kernel, grid, block, 0, s, g, G, in, out, D(0), CT);
+        if (g.capture)
+          hipLaunchKernelGGL(comp_kernel<true>, grid, block, 0, s, g, G, in, out, D(0), CT);
+        else
+          hipLaunchKernelGGL(comp_kernel<false>, grid, block, 0, s, g, G, in, out, D(0), CT);
       } else if (which == 23) {
         hipLaunchKernelGGL(comp_wave_kernel, gridw, block, 0, s, g, G, in, out, CT);
       } else {

@@ -25,7 +25,9 @@ GRID_FIELDS = [
     ("b", C.c_double * 9), ("fb", C.c_double * 3),
     ("expz", _P), ("difk", _P), ("dift", _P), ("xmue", _P), ("zpint", _P), ("zpmid", _P),
     ("racs", _P), ("cs", _P), ("tanphi", _P), ("cor", _P), ("glat", _P),
+    ("capture", _P),
 ]
+CAPTURE_PLANES = 68
 IN_3D = ("tn tn_nm un vn un_nm vn_nm o2 o2_nm o1 o1_nm he he_nm "
          "op op_nm n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm ti te ne o2p w z "
          "mbar barm xnmbar xnmbari scht schti n2 w_upd "
@@ -112,7 +114,7 @@ class NestState:
             (one wavefront per column) or "auto" (timed once per shape).
     """
 
-    def __init__(self, grid, fields, device="cuda", scheme="thomas"):
+    def __init__(self, grid, fields, device="cuda", scheme="thomas", capture=False):
         self.grid = grid
         nk, nlat, nlon = grid["nk"], grid["nlat"], grid["nlon"]
         self.nk, self.nlatd, self.nlond = nk, nlat + 4, nlon + 4
@@ -165,6 +167,11 @@ class NestState:
         self.o = o
         nbytes = _lib().tgx_ng_workspace_bytes(C.byref(self.g))
         self.work = torch.empty((nbytes + 7) // 8, **kw)
+        self.capture = None
+        if capture:  # the assembled column systems, written by the Thomas kernels
+            ncol = self.nlond * self.nlatd
+            self.capture = torch.zeros((CAPTURE_PLANES, ncol, nk), **kw)
+            self.g.capture = self.capture.data_ptr()
 
     # ---- layout helpers ------------------------------------------------------
     def _padlast(self, t):
@@ -223,6 +230,52 @@ class NestState:
         names = {1: "thomas", 2: "pcr"}
         return {r: names.get(_lib().tgx_ng_scheme_choice(C.byref(self.g), n), "?")
                 for n, r in enumerate(ROUTINES)}
+
+
+class NestedStep:
+    """One global step of the global grid and its nests in the reference's
+    recursion (advance_ng.F90:57-199): nest level i runs nstep_ng[i] sub-steps
+    per step of level i-1, so level i runs prod(nstep_ng[:i+1]) sub-steps per
+    global step.  Each domain's column step is the fused step on its own
+    device state (synthetic, independent inputs: the lateral coupling through
+    map_in/set_bndry is outside the column step).
+
+    run() launches the sequence eagerly on the current stream; capture() records
+    it once into a HIP graph and replay() launches the whole outer step as one
+    graph.  AUTO schemes are tuned by a first eager run before capture.
+    """
+
+    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda"):
+        self.states = [NestState(g, F, device=device, scheme=scheme) for g, F in domains]
+        self.nstep_ng = list(nstep_ng)
+        self.graph = None
+        self.order = []  # domain index of each sub-step, in launch order
+
+        def visit(level):
+            for _ in range(self.nstep_ng[level]):
+                self.order.append(level)
+                if level + 1 < len(self.states):
+                    visit(level + 1)
+        visit(0)
+
+    def substeps(self):
+        return {i: self.order.count(i) for i in range(len(self.states))}
+
+    def run(self):
+        for i in self.order:
+            self.states[i].step_serial()
+
+    def capture(self):
+        torch.cuda.synchronize()
+        self.run()  # tunes AUTO schemes and settles allocations before capture
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.run()
+        torch.cuda.synchronize()
+
+    def replay(self):
+        self.graph.replay()
 
 
 # Algorithmic HBM traffic of each routine: the distinct 3-D fields its fused
