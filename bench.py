@@ -132,6 +132,81 @@ def bench_nests(pkg, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_precision(pkg, args):
+    """FP32 elimination + FP64 refinement against the plain FP64 solvers on the
+    column systems the fused step assembles at 1.25 deg (288x144 x 113), 5%
+    stiff columns.  Per field: solve throughput of each variant and its error
+    relative to the FP64 device solution (the oracle comparison of the same
+    variants is tests/test_gpu_precision.py)."""
+    import importlib
+    import numpy as np
+    import torch
+    prec = importlib.import_module(pkg.__name__ + ".precision")
+    cfg = "g1p25" if args.config == "g0p625" else args.config
+    g = pkg.synth.make_grid(cfg, nstep_sub=1)
+    cap, valid, nk = prec.capture_systems(g, pkg.synth.make_state(g, device="cuda"))
+    vdev = torch.tensor(valid, device=cap.device)
+    ncol_valid = int(valid.sum())
+    fields = {}
+    tot = {"fp64": 0.0, 0: 0.0, 1: 0.0, 2: 0.0}
+    colev = 0
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(args.steps):
+            r = fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / args.steps, r
+
+    for fi, field in enumerate(prec.FIELDS):
+        n, _, drop = prec.FIELDS[field]
+        A, B, Cc, F = prec.split(cap, field)
+        prec.make_stiff(A, B, Cc, valid, 0.05, seed=0x7E6C0003 + fi)
+        nrow = nk - drop
+        ncol = B.shape[1]
+        if n == 1:
+            a, b, c, f = (t[0].contiguous() for t in (A, B, Cc, F))
+            x64 = torch.zeros_like(f)
+            ms64, _ = timed(lambda: pkg.tgx.trsolv_dev(a, b, c, f, x64, 0, nrow - 1, scheme=1))
+            x64 = x64[None]
+        else:
+            # (ncol, nk, N, N) column-major blocks for the FP64 block solvers
+            bl = [t.reshape(n, n, ncol, nk).permute(2, 3, 1, 0).contiguous() for t in (A, B, Cc)]
+            fr = F.permute(1, 2, 0).contiguous()
+            xb = torch.zeros_like(fr)
+            solver = pkg.tgx.blktri2_dev if n == 2 else pkg.tgx.blktri3_dev
+            ms64, _ = timed(lambda: solver(bl[0], bl[1], bl[2], fr, xb, nrow))
+            x64 = xb.permute(2, 0, 1).contiguous()
+        ref = x64.cpu().numpy()
+        rec = {"fp64_ms": round(ms64, 4)}
+        tot["fp64"] += ms64
+        for nsweep in (0, 1, 2):
+            ms, x = timed(lambda: prec.solve_mixed(field, A, B, Cc, F, vdev, nk, nsweep))
+            e = prec.errors(x.cpu().numpy(), ref, valid, np.zeros_like(valid, dtype=bool))["all"]
+            rec["sweeps_%d" % nsweep] = {"ms": round(ms, 4), "max_err": e[0], "mean_err": e[1]}
+            tot[nsweep] += ms
+        colev += ncol_valid * nk * n
+        fields[field] = rec
+    out = {
+        "metric": "Mcolumn-level solves/s + %HBM roofline, FP32+refine vs FP64 at <=1e-10 rel",
+        "value": round(colev / (tot[2] * 1e-3) / 1e6, 2), "unit": "Mcolumn-level solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(tot[2], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "column systems assembled by the fused step on synthetic state, 5% stiff columns",
+        "config": {"workload": "precision: %s column systems of all 11 unknowns, FP32 elimination + 2 FP64 "
+                               "refinement sweeps (plain solve, assembly not timed)" % cfg,
+                   "fp64_Mcolev_per_s": round(colev / (tot["fp64"] * 1e-3) / 1e6, 2),
+                   "fp32_only_Mcolev_per_s": round(colev / (tot[0] * 1e-3) / 1e6, 2),
+                   "fp32_1sweep_Mcolev_per_s": round(colev / (tot[1] * 1e-3) / 1e6, 2),
+                   "fields": fields},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +223,8 @@ def main():
                     help="run the independent routines concurrently on forked streams (default: the "
                          "reference's order on one stream, so each kernel's duration is its own)")
     ap.add_argument("--serial", action="store_true", help="(default) the reference's order on one stream")
+    ap.add_argument("--precision", action="store_true",
+                    help="FP32 + FP64-refinement solves against the FP64 solvers on the assembled systems")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
@@ -162,6 +239,8 @@ def main():
     torch.cuda.set_device(0)
     if args.config == "nests":
         return bench_nests(pkg, args)
+    if args.precision:
+        return bench_precision(pkg, args)
     g = pkg.synth.make_grid(args.config, nstep_sub=args.nstep_sub)
     print("# generating %s state on the GPU" % args.config, file=sys.stderr, flush=True)
     F = pkg.synth.make_state(g, device="cuda")

@@ -559,6 +559,267 @@ __global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fi
 #undef F3
 }
 
+// dt_ng with the horizontal stencils fused in: a block covers a TILE_X x
+// TILE_Y tile of columns; at each level the tile and its 2-point halo of TN,
+// UN, VN and TN_NM are staged through LDS (double-buffered, the next level's
+// loads issued with the column loads), the advection of TN (advec_ng) and the
+// Shapiro smoothing of TN_NM (smooth_ng, latitude pass once per point) are
+// evaluated from LDS, and the column is assembled and eliminated as in
+// dt_kernel.  Every thread of the block takes part in the staging and the
+// barriers; lateral-boundary and out-of-domain columns do not solve.
+template <bool CAP>
+__global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                      tgx_ng_out out, double *__restrict__ w2s) {
+  constexpr int TS = TILE_R * TILE_W, NF = 4;
+  __shared__ double tb[2][NF][TS];
+  __shared__ double tls[TILE_Y * TILE_W];
+  const int ntx = (G.nlond + TILE_X - 1) / TILE_X;
+  const int bb = tgx_xcd_block(gridDim.x);
+  const int i0 = (bb % ntx) * TILE_X, j0 = (bb / ntx) * TILE_Y;
+  const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
+  const bool active = i < G.nlond && j < G.nlatd;
+  const bool lat = active && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
+  const bool solve = active && !lat;
+  const int nk = G.nk;
+  const double dz = g.dz, dtx2inv = g.dtx2inv;
+  // tile slots of this thread: e = tid and tid + 256
+  int te[2];
+  bool tok[2];
+  size_t toff[2];
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {
+    const int e = (int)threadIdx.x + sl * 256;
+    const int ii = i0 - TILE_H + e % TILE_W, jj = j0 - TILE_H + e / TILE_W;
+    te[sl] = e;
+    tok[sl] = e < TS && ii >= 0 && ii < G.nlond && jj >= 0 && jj < G.nlatd;
+    toff[sl] = tok[sl] ? (size_t)jj * G.ld + ii : 0;
+  }
+  const double *tsrc[NF] = {in.tn, in.un, in.vn, in.tn_nm};
+  double stage[NF][2];
+  auto tile_load = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int sl = 0; sl < 2; ++sl)
+        if (tok[sl]) stage[f][sl] = tsrc[f][(size_t)k * G.P + toff[sl]];
+  };
+  auto tile_store = [&](int buf) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int sl = 0; sl < 2; ++sl)
+        if (tok[sl]) tb[buf][f][te[sl]] = stage[f][sl];
+  };
+  const int ic = active ? i : 0, jc = active ? j : 0;
+  const size_t c2 = G.a2(ic, jc);
+  double t_lbc = 0., tl = 0.;
+  if (active) {
+    t_lbc = in.t_lbc[c2];
+    tl = in.tlbc[c2];
+    out.tlbc_nm[c2] = tl;
+    out.tlbc[c2] = t_lbc;
+  }
+  if (lat) {
+    double v;
+    for (int k = 0; k < nk; ++k) {
+      lateral_value(G, in.lonb_tn, in.latb_tn, k, i, j, v);
+      size_t p = G.a3(k, i, j);
+      out.tn_nm[p] = DTSMOOTH_DIV2 * (in.tn_nm[p] + v) + DTSMOOTH * in.tn[p];
+      out.tn[p] = v > 100. ? v : 100.;
+    }
+  }
+#define F3(f, k) (f)[G.a3((k), ic, jc)]
+  struct DL { double w, cp, km, qtotal, schti, kt, un, vn; };
+  struct DS { double xn, o1, rkm12, mbar, scht, hdt, qji, cool_imp, cool_exp; };
+  auto load_dl = [&](int L, DL &D) {
+    size_t p = G.a3(L, ic, jc);
+    D.w = in.w_upd[p]; D.cp = in.cp[p]; D.km = in.km[p]; D.qtotal = in.qtotal[p];
+    D.schti = in.schti[p]; D.kt = in.kt[p]; D.un = in.un[p]; D.vn = in.vn[p];
+  };
+  auto load_ds = [&](int L, DS &S) {
+    size_t p = G.a3(L, ic, jc);
+    S.xn = in.xnmbar[p]; S.o1 = in.o1[p];
+    S.rkm12 = in.rkm12[p]; S.mbar = in.mbar[p]; S.scht = in.scht[p]; S.hdt = in.hdt[p];
+    S.qji = in.qji_tn[p]; S.cool_imp = in.cool_imp[p]; S.cool_exp = in.cool_exp[p];
+  };
+  auto gf = [&](const DL &D, double expzL, double diftL, double tni, double &gg, double &ff) {
+    double h = D.schti;
+    double rho = P0 * g.expzmid_inv / (h * GRAV);
+    rho = rho * expzL;
+    double gpart = h * h * rho * D.cp;
+    double gpart_a = gpart * diftL;
+    gg = GRAV * (D.kt + gpart_a) / (P0 * h * (dz * dz));
+    ff = GRAV * GRAV * (h * h) * rho / (tni * P0 * 2. * dz);
+    ff = ff * diftL;
+  };
+  const double ulbc = solve ? in.ulbc[c2] : 0., vlbc = solve ? in.vlbc[c2] : 0.;
+  const double racs = g.racs[jc];
+  DL dc = {}, dn = {};
+  double tn_m = 0., tn_c = 0., tn_p = 0.;
+  double gk = 0., fk = 0.;
+  if (solve) {
+    load_dl(0, dc);
+    tn_c = F3(in.tn, 0);
+    tn_p = F3(in.tn, 1);
+    gf(dc, g.expz[0], g.dift[0], tl, gk, fk);
+  }
+  double un_m = 0., vn_m = 0.;
+  double w1 = 0., w3 = 0., rprev = 0., dudz_prev = 0., dvdz_prev = 0.;
+  tile_load(0);
+  tile_store(0);
+  for (int k = 0; k <= nk - 2; ++k) {
+    DS ds = {};
+    double tn_pp = 0.;
+    if (k + 1 <= nk - 2) tile_load(k + 1);
+    if (solve) {
+      load_dl(k + 1, dn);
+      load_ds(k, ds);
+      tn_pp = k + 2 <= nk - 1 ? F3(in.tn, k + 2) : 0.;
+    }
+    const double expzk = g.expz[k], expzk1 = g.expz[k + 1], dift1 = g.dift[k + 1];
+    asm volatile("" ::: "memory");
+    __syncthreads();  // the level-k tiles are in buffer k&1
+    const int bk = k & 1;
+    const LdsPlane Ftn{tb[bk][0], i0 - TILE_H, j0 - TILE_H}, U{tb[bk][1], i0 - TILE_H, j0 - TILE_H},
+        V{tb[bk][2], i0 - TILE_H, j0 - TILE_H}, Fnm{tb[bk][3], i0 - TILE_H, j0 - TILE_H};
+    // smooth_ng.F90:283-294: latitude pass of TN_NM on the tile rows and their longitude halo
+    for (int e = threadIdx.x; e < TILE_W * TILE_Y; e += blockDim.x) {
+      int ii = i0 - TILE_H + e % TILE_W, jj = j0 + e / TILE_W;
+      if (ii >= 0 && ii < G.nlond && jj < G.nlatd) tls[e] = smooth_lat(Fnm, G, ii, jj, g.shapiro);
+    }
+    __syncthreads();
+    if (solve) {
+      // advec_ng of TN and the longitude pass of the smoothing (:296-307)
+      const double hadv = advec(Ftn, U, V, G, i, j, racs, g.dlamda, g.dphi);
+      double tsm;
+      {
+        const double *row = tls + (j - j0) * TILE_W + TILE_H - i0;
+        const double s_ = g.shapiro;
+        int L1 = G.nlond - 1;
+        bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+        int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
+        int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
+        double t0 = row[i];
+        double fm1 = (row[i1p] + row[i1m]) * 0.5;
+        double fm2 = (row[i2p] + row[i2m]) * 0.5;
+        double gen = t0 - s_ * (2. * fm2 - 8. * fm1 + 6. * t0);
+        double nr = t0 - s_ * (-8. * fm1 + 8. * t0);
+        tsm = edge ? t0 : (near ? nr : gen);
+      }
+      const double w_k = dc.w, w_k1 = dn.w;
+      // veradv (:57-65)
+      double raw_k = k == 0 ? (tn_c - t_lbc) * w_k * 2. : (tn_c - tn_m) * w_k;
+      double raw_k1 = (k + 1 <= nk - 2) ? (tn_p - tn_c) * w_k1 : 0.;
+      double veradv = (raw_k + raw_k1) * .5;
+      veradv = veradv / dz;
+      // interface means (:71-78)
+      double cpi = .5 * (dc.cp + dn.cp);
+      double kmi = .5 * (dc.km + dn.km);
+      double wi = .5 * (w_k + w_k1);
+      // cptn (:67, :80-83)
+      double advec_tn = hadv + veradv;
+      double cptn = cpi * (advec_tn - dtx2inv * tsm);
+      cptn = cptn * expzk;
+      // dudz (:85-94)
+      double dudz, dvdz;
+      if (k == 0) {
+        dudz = (dc.un + 1. / 3. * dn.un - 4. / 3. * ulbc) / dz;
+        dvdz = (dc.vn + 1. / 3. * dn.vn - 4. / 3. * vlbc) / dz;
+      } else if (k <= nk - 3) {
+        dudz = (dn.un - un_m) / (2. * dz);
+        dvdz = (dn.vn - vn_m) / (2. * dz);
+      } else {
+        dudz = dudz_prev / 3.;
+        dvdz = dvdz_prev / 3.;
+      }
+      dudz_prev = dudz;
+      dvdz_prev = dvdz;
+      // heating (:96-113)
+      double qsolar = .5 * (dc.qtotal + dn.qtotal);
+      double t = ds.xn * ds.o1 * RIO1;
+      double qrecomb = TSURPLUS * ds.rkm12 * (t * t) * AVO / ds.mbar;
+      double qm = GRAV * kmi / (P0 * ds.scht) * (dudz * dudz + dvdz * dvdz);
+      qm = qm / expzk;
+      double total_heat = qsolar + ds.hdt + qrecomb + ds.qji + qm;
+      double total_heat_a = total_heat * expzk;
+      cptn = cptn - total_heat_a;
+      // coefficients (:141-163)
+      double p = gk - fk, q, r;
+      double g1 = 0., f1 = 0.;
+      if (k <= nk - 3) {
+        double tni1 = .5 * (tn_c + tn_p);
+        gf(dn, expzk1, dift1, tni1, g1, f1);
+        q = -(g1 - f1 + gk + fk);
+        r = g1 + f1;
+      } else {
+        q = -gk - fk;
+        r = 0.;
+      }
+      double rhs = cptn;
+      double qpart = cpi * (dtx2inv + ds.cool_imp) + wi * GASK / ds.mbar;
+      rhs = rhs + ds.cool_exp;
+      double qpart_a = qpart * expzk;
+      q = q - qpart_a;
+      if (k == 0) {
+        q = q - p;
+        rhs = rhs - 2. * p * t_lbc;
+        p = 0.;
+      }
+      if (CAP) {
+        cap_put(g, G, CAP_DT + 0, i, j, k, p); cap_put(g, G, CAP_DT + 1, i, j, k, q);
+        cap_put(g, G, CAP_DT + 2, i, j, k, r); cap_put(g, G, CAP_DT + 3, i, j, k, rhs);
+      }
+      // Thomas forward (trsolv_ng.F90:14-22)
+      size_t pk = G.a3(k, ic, jc);
+      if (k == 0) {
+        w1 = q;
+        w3 = rhs / w1;
+      } else {
+        double w2 = rprev / w1;
+        w2s[G.a3(k - 1, ic, jc)] = w2;
+        w1 = q - p * w2;
+        w3 = (rhs - p * w3) / w1;
+      }
+      out.tn[pk] = w3;
+      rprev = r;
+      gk = g1; fk = f1;
+    }
+    tn_m = tn_c; tn_c = tn_p; tn_p = tn_pp;
+    un_m = dc.un; vn_m = dc.vn;
+    dc = dn;
+    if (k + 1 <= nk - 2) tile_store((k + 1) & 1);
+  }
+  if (!solve) return;
+  // back substitution (:165-167) fused with the output stage (:208-213); the
+  // next level's loads are in flight while this level is stored
+  struct DB { double x, w2, tnm, tn; };
+  auto load_db = [&](int k, DB &B) {
+    size_t p = G.a3(k, ic, jc);
+    B.x = out.tn[p]; B.w2 = k < nk - 2 ? w2s[p] : 0.; B.tnm = in.tn_nm[p]; B.tn = in.tn[p];
+  };
+  const double tnm_top = F3(in.tn_nm, nk - 1), tn_top = F3(in.tn, nk - 1);
+  double xn = w3;
+  DB bc, bn;
+  load_db(nk - 2, bc);
+  for (int k = nk - 2; k >= 0; --k) {
+    if (k > 0) load_db(k - 1, bn);
+    asm volatile("" ::: "memory");
+    double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
+    if (k == nk - 3) {
+      double xt = xn * xn / x;
+      size_t pt = G.a3(nk - 1, ic, jc);
+      out.tn_nm[pt] = DTSMOOTH_DIV2 * (tnm_top + xt) + DTSMOOTH * tn_top;
+      out.tn[pt] = xt > 100. ? xt : 100.;
+    }
+    size_t p = G.a3(k, ic, jc);
+    out.tn_nm[p] = DTSMOOTH_DIV2 * (bc.tnm + x) + DTSMOOTH * bc.tn;
+    out.tn[p] = x > 100. ? x : 100.;
+    xn = x;
+    bc = bn;
+  }
+#undef F3
+}
+
 // 3x3 inverse (matutil.F90:110-118), column-major, same term order as the oracle
 __device__ __forceinline__ void matinv3_ng(const double *A, double *Bm) {
 #define M3(r, c) A[((r) - 1) + 3 * ((c) - 1)]
@@ -3214,6 +3475,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       hipLaunchKernelGGL(dt_wave_kernel, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
                          (const double *)D(2));
       break;
+    case 24: {  // dt with the stencils fused: one block per column tile
+      dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
+      if (g.capture)
+        hipLaunchKernelGGL(dt_tile_kernel<true>, gridt, block, 0, s, g, G, in, out, D(0));
+      else
+        hipLaunchKernelGGL(dt_tile_kernel<false>, gridt, block, 0, s, g, G, in, out, D(0));
+      break;
+    }
     case 1:
     case 2:
     case 15:

@@ -132,7 +132,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
-       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W };
+       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -167,9 +167,13 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_DT, s);
   void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
+  if (!use_pcr(g, R_DT)) {  // Thomas: the stencils are fused into the column kernel
+    ProfScope kern(R_N + R_DT, s);
+    return tgx_ng_launch(K_DT_TILE, g, in, out, b, 0, s);
+  }
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_DT, s);
-  return tgx_ng_launch(use_pcr(g, R_DT) ? K_DT_W : K_DT, g, in, out, b, 0, s);
+  return tgx_ng_launch(K_DT_W, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
