@@ -82,6 +82,24 @@ def test_concurrent_step_equals_serial(pkg):
         assert torch.equal(a.out[n], b.out[n]), n
 
 
+@pytest.mark.parametrize("name,nsub", [("small", 2), ("tall", 1)])
+def test_routines_one_at_a_time_match_oracle(pkg, oracle, name, nsub):
+    """The five routines called one by one (each with its own pointwise
+    chemistry, as a model replacing them one at a time does) agree with the
+    oracle, as the fused step does with its single chemistry pass."""
+    import torch
+    g = pkg.synth.make_grid(name, nstep_sub=nsub)
+    F = pkg.synth.make_state(g)
+    ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
+    ns = pkg.step.NestState(g, F)
+    for r in ("oplus", "minors", "dt", "duv", "comp"):
+        getattr(ns, r)()
+    torch.cuda.synchronize()
+    for n in ("tn", "un", "vn", "op", "xiop2p", "xiop2d", "no", "n4s", "n2d", "ar", "o2", "o1", "he"):
+        err = field_err(ns.fortran(n), ref[n])
+        assert err <= TOL, (n, err)
+
+
 def test_auto_scheme_tunes_and_matches_oracle(pkg, oracle):
     """TGX_SCHEME_AUTO times both schemes on its first step, records a choice
     per routine, and its results stay within the FP64 bound."""

@@ -1276,6 +1276,169 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 
 
 // =============================================================================
+// All pointwise chemistry of one step that does not depend on this step's
+// solutions, in one level-parallel pass that reads every chemistry field once:
+//   comp_o2o_ng fs planes (comp_o2o_ng.F90:44-81, levels < nk-1);
+//   N2D loss and scaled production (comp_n2d_ng.F90:195-203);
+//   N4S loss (comp_n4s_ng.F90:134-137);
+//   NO and N4S production, and NO loss, split as a + n2d*b in this step's N2D
+//   (comp_no_ng.F90:48-57, comp_n4s_ng.F90:129-133), scaled by rmx/xnmbar;
+//   O+ xiop2p, xiop2d, loss and production (oplus_ng.F90:288-299).
+// All values are computed before any is stored, so shared fields load once.
+struct ChemOut {
+  double *fs[6];                       // comp_o2o fs planes
+  double *n2d_sloss, *n2d_s0prod;      // minor-species source planes of N2D
+  double *n4s_sloss;                   // N4S loss (independent of N2D)
+  double *lin[6];                      // N4S s0prod a, b; NO sloss a, b; NO s0prod a, b
+  double *oploss, *opprod;             // O+ loss and production
+};
+
+__global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                   tgx_ng_out out, ChemOut C) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  long npt = (long)G.nlond * G.nlatd * G.nk;
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int k = (int)(r / G.nlatd);
+  const int nk = G.nk;
+  const size_t p = G.a3(k, i, j);
+  const bool top = k == nk - 1;
+  const size_t pu = top ? p : p + G.P, pd = top ? p - G.P : p;  // level above / below for interpolation
+  const double xn = in.xnmbar[p], ne = in.ne[p], ne_u = in.ne[pu], ne_d = in.ne[pd];
+  const double o1 = in.o1[p], o2 = in.o2[p], n2 = in.n2[p], op = in.op[p], no = in.no[p], n4s = in.n4s[p];
+  const double n2d = in.n2d[p], o2p = in.o2p[p], nplus = in.nplus[p], n2p = in.n2p[p], nop = in.nop[p];
+  const double beta8 = in.beta8[p], beta8_o = in.beta8[top ? pd : pu];
+  // ---- comp_o2o_ng fs (levels < nk-1)
+  double fsv[6] = {0., 0., 0., 0., 0., 0.};
+  if (!top) {
+    double rji = 0.5 * (in.rj[p] + in.rj[pu]);
+    double qo2pi = 0.5 * (in.qo2p[p] + in.qo2p[pu]);
+    double qopi = 0.5 * (in.qop[p] + in.qop[pu]);
+    double beta8i = 0.5 * (beta8 + beta8_o);
+    double nei = sqrt(ne * ne_u);
+    double pox1 = xn * xn * (in.beta3[p] * n4s / RMN4S * no / RMNO + BETA6 * n2d / RMN2D * no / RMNO) +
+                  beta8i * no / RMNO * xn + xn * (RK4 * o2p * n4s / RMN4S + RK10 * op * n2d / RMN2D) +
+                  (in.ra1[p] * nop + 2. * in.ra2[p] * o2p) * nei;
+    double pox2 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + RK7 * nplus + 2. * rji;
+    double lox1 = 2. * in.rkm12[p] * xn / in.mbar[p];
+    double lox2 = in.rk3[p] * n2p + RK8 * nplus;
+    double lox3 = qopi;
+    double po21 = in.rkm12[p] * xn / in.mbar[p];
+    double po23 = RK5 * no / RMNO * o2p * xn;
+    double lo21 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + (RK6 + RK7) * nplus +
+                  RK9 * n2p + rji;
+    double lo22 = qo2pi;
+    fsv[0] = -lo21;
+    fsv[1] = xn * po21 * o1 * (RIO1 * RIO1) * RMO2;
+    fsv[2] = pox2 * RMO1 * RIO2;
+    fsv[3] = -lox2 - lox1 * o1 * RIO1 * xn;
+    fsv[4] = (po23 - lo22) * RMO2 / xn;
+    fsv[5] = (pox1 - lox3) * RMO1 / xn;
+  }
+  // ---- N2D (comp_n2d_ng.F90:195-203)
+  double n2d_sl, n2d_sp;
+  {
+    double qtefi, nei;
+    if (!top) {
+      qtefi = .5 * (in.qtef[p] + in.qtef[pu]);
+      nei = sqrt(ne * ne_u);
+    } else {
+      qtefi = 1.5 * in.qtef[p] - .5 * in.qtef[pd];
+      nei = sqrt(ne * ne * ne / ne_d);
+    }
+    n2d_sp = qtefi * BRN2D + in.rk3[p] * n2p * xn * o1 * RIO1 + (in.ra1[p] * nop * 0.85 + in.ra3[p] * n2p * 0.9) * nei;
+    n2d_sl = -(xn * (BETA2 * o2 * RIO2 + BETA4 * o1 * RIO1 + BETA6 * no * RINO) + BETA7 + in.beta5[p] * nei +
+               RK10 * op);
+  }
+  // ---- N4S (comp_n4s_ng.F90:122-138): production a + n2d*b, loss
+  double n4s_a, n4s_b, n4s_sl;
+  {
+    double qtefi, beta8i, nei1, nei2;
+    if (!top) {
+      qtefi = .5 * (in.qtef[p] + in.qtef[pu]);
+      beta8i = .5 * (beta8 + beta8_o);
+      nei1 = .5 * (ne + ne_u);
+      nei2 = sqrt(ne * ne_u);
+    } else {
+      qtefi = 0.;  // undefined in the reference (:128); reaches only f_rhs(nk), overwritten
+      beta8i = 1.5 * beta8 - .5 * beta8_o;
+      nei1 = 1.5 * ne - .5 * ne_d;
+      nei2 = sqrt(ne * ne * ne / ne_d);
+    }
+    n4s_a = qtefi * (1. - BRN2D) + xn * (beta8i * no * RINO) +
+            xn * (in.rk2[p] * op * n2 * RIN2 + RK6 * nplus * o2 * RIO2 + RK8 * nplus * o1 * RIO1) +
+            nei2 * (in.ra1[p] * nop * 0.15 + in.ra3[p] * n2p * 1.1);
+    n4s_b = xn * RIN2D * (xn * BETA4 * o1 * RIO1 + in.beta5[p] * nei1 + BETA7);
+    n4s_sl = -xn * (in.beta1[p] * o2 * RIO2 + in.beta3[p] * no * RINO + xn * in.beta17[p] * o1 * RIO1 * n2 * RIN2) -
+             RK4 * o2p;
+  }
+  // ---- NO (comp_no_ng.F90:48-57): production and loss a + n2d*b
+  double no_spa, no_spb, no_sla, no_slb;
+  {
+    no_spa = xn * xn * o2 * RIO2 * (in.beta1[p] * n4s * RIN4S) +
+             xn * xn * xn * in.beta17[p] * o1 * RIO1 * n2 * RIN2 * n4s * RIN4S;
+    no_spb = xn * xn * o2 * RIO2 * (BETA2 * RIN2D);
+    const double b9 = in.beta9[p], b9n = in.beta9n[p], b9o = in.beta9[top ? pd : pu], b9no = in.beta9n[top ? pd : pu];
+    double betai = !top ? .5 * (beta8 + b9 + b9n + beta8_o + b9o + b9no)
+                        : 1.5 * (beta8 + b9 + b9n) - .5 * (beta8_o + b9o + b9no);
+    no_sla = -xn * (in.beta3[p] * n4s * RIN4S) - RK5 * o2p - betai;
+    no_slb = -xn * (BETA6 * RIN2D);
+  }
+  // ---- O+ (oplus_ng.F90:288-299)
+  double qop2pi = !top ? .5 * (in.qop2p[p] + in.qop2p[pu]) : 1.5 * in.qop2p[p] - .5 * in.qop2p[pd];
+  double qop2di = !top ? .5 * (in.qop2d[p] + in.qop2d[pu]) : 1.5 * in.qop2d[p] - .5 * in.qop2d[pd];
+  double x2p = qop2pi / (xn * ((RK16 + RK17) * n2 * RIN2 + RK18 * o1 * RIO1) + (in.rk19[p] + in.rk20[p]) * ne + RK21 + RK22);
+  double x2d = (qop2di + (in.rk20[p] * ne + RK22) * x2p) /
+               (xn * (RK23 * n2 * RIN2 + RK24 * o1 * RIO1 + RK26 * o2 * RIO2) + in.rk25[p] * ne + RK27);
+  double oploss = xn * (in.rk1[p] * o2 * RIO2 + in.rk2[p] * n2 * RIN2 + RK10 * n2d * RIN2D);
+  double opprod = 0.;
+  if (!top) {
+    double qopi = .5 * (in.qop[p] + in.qop[pu]);
+    opprod = qopi + (in.rk19[p] * ne + RK21) * x2p + (in.rk25[p] * ne + RK27) * x2d + (RK18 * x2p + RK24 * x2d) * o1 * RIO1 * xn;
+  }
+  // ---- stores
+  if (!top)
+    for (int q = 0; q < 6; ++q) C.fs[q][p] = fsv[q];
+  C.n2d_sloss[p] = n2d_sl;
+  C.n2d_s0prod[p] = n2d_sp * RMN2D / xn;
+  C.n4s_sloss[p] = n4s_sl;
+  C.lin[0][p] = n4s_a * RMN4S / xn;
+  C.lin[1][p] = n4s_b * RMN4S / xn;
+  C.lin[2][p] = no_sla;
+  C.lin[3][p] = no_slb;
+  C.lin[4][p] = no_spa * RMNO / xn;
+  C.lin[5][p] = no_spb * RMNO / xn;
+  out.xiop2p[p] = x2p;
+  out.xiop2d[p] = x2d;
+  C.oploss[p] = oploss;
+  if (!top) C.opprod[p] = opprod;
+}
+
+// The NO and N4S sources of this step from the linear coefficients of
+// chem_kernel and this step's N2D (after the N2D solve).
+__global__ void __launch_bounds__(256) minor_lin_kernel(Geo G, tgx_ng_out out, ChemOut C,
+                                                        double *__restrict__ no_sloss,
+                                                        double *__restrict__ no_s0prod,
+                                                        double *__restrict__ n4s_s0prod) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  long npt = (long)G.nlond * G.nlatd * G.nk;
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int k = (int)(r / G.nlatd);
+  const size_t p = G.a3(k, i, j);
+  const double n2d = out.n2d[p];
+  n4s_s0prod[p] = C.lin[0][p] + n2d * C.lin[1][p];
+  no_sloss[p] = C.lin[2][p] + n2d * C.lin[3][p];
+  no_s0prod[p] = C.lin[4][p] + n2d * C.lin[5][p];
+}
+
+// =============================================================================
 // One wavefront per column.  Lane l holds level l ("lo") and level l+64
 // ("hi"), so a wave covers up to 128 levels; neighbouring levels come from
 // other lanes by ds_bpermute, and the column's tridiagonal systems are solved
@@ -3475,6 +3638,20 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       hipLaunchKernelGGL(dt_wave_kernel, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
                          (const double *)D(2));
       break;
+    case 25:
+    case 26: {  // bufs = {fs[6], n2d sloss, n2d s0prod, n4s sloss, lin[6], oploss, opprod,
+                //         no sloss, no s0prod, n4s s0prod}
+      ChemOut C;
+      for (int q = 0; q < 6; ++q) C.fs[q] = D(q);
+      C.n2d_sloss = D(6); C.n2d_s0prod = D(7); C.n4s_sloss = D(8);
+      for (int q = 0; q < 6; ++q) C.lin[q] = D(9 + q);
+      C.oploss = D(15); C.opprod = D(16);
+      if (which == 25)
+        hipLaunchKernelGGL(chem_kernel, gridp, block, 0, s, g, G, in, out, C);
+      else
+        hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C, D(17), D(18), D(19));
+      break;
+    }
     case 24: {  // dt with the stencils fused: one block per column tile
       dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
       if (g.capture)

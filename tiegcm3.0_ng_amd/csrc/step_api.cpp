@@ -33,7 +33,8 @@ enum {
   W_DT = 0, W_DT_T = 1,                  // dt ratios; 2 prelude planes
   W_MIN_A = 3, W_MIN_B = 4, W_MIN_T = 5, // minor ratios (two species per pass); 8 prelude planes
   W_MIN_S = 13,                          // minor sources: 2 species x (sloss, s0prod)
-  W_OP = 17,                             // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
+  W_MIN_L = 17,                          // NO / N4S sources linear in N2D: 6 coefficient planes
+  W_OP = 23,                             // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
@@ -132,7 +133,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
-       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE };
+       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -176,8 +177,50 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   return tgx_ng_launch(K_DT_W, g, in, out, b, 0, s);
 }
 
+namespace {
+// Buffers of chem_kernel / minor_lin_kernel (the step-wide chemistry pass).
+void chem_bufs(void *work, const tgx_ng_grid *g, void **b) {
+  for (int q = 0; q < 6; ++q) b[q] = plane(work, g, W_CO + 15 + q);  // comp fs planes
+  b[6] = plane(work, g, W_MIN_S + 0);                                // N2D sloss (then NO's)
+  b[7] = plane(work, g, W_MIN_S + 1);                                // N2D s0prod (then NO's)
+  b[8] = plane(work, g, W_MIN_S + 2);                                // N4S sloss
+  for (int q = 0; q < 6; ++q) b[9 + q] = plane(work, g, W_MIN_L + q);
+  b[15] = plane(work, g, W_OP_C);
+  b[16] = plane(work, g, W_OP_C + 1);
+  b[17] = plane(work, g, W_MIN_S + 0);  // minor_lin: NO sloss
+  b[18] = plane(work, g, W_MIN_S + 1);  // NO s0prod
+  b[19] = plane(work, g, W_MIN_S + 3);  // N4S s0prod
+}
+int run_chem(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+             hipStream_t s) {
+  void *b[20];
+  chem_bufs(work, g, b);
+  return tgx_ng_launch(K_CHEM, g, in, out, b, 0, s);
+}
+int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+               void *stream, bool chem_done);
+int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+              void *stream, bool chem_done);
+int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+             void *stream, bool chem_done);
+}  // namespace
+
 extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                              void *work, void *stream) {
+  return run_minors(g, in, out, work, stream, false);
+}
+extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                            void *work, void *stream) {
+  return run_oplus(g, in, out, work, stream, false);
+}
+extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                           void *work, void *stream) {
+  return run_comp(g, in, out, work, stream, false);
+}
+
+namespace {
+int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+               void *stream, bool chem_done) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_MINORS, s);
@@ -189,17 +232,24 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
   const bool pcr = use_pcr(g, R_MINORS);
-  if (int rc = tgx_ng_launch(K_MIN_SRC_A, g, in, out, b, 0, s)) return rc;
+  if (!chem_done)
+    if (int rc = tgx_ng_launch(K_MIN_SRC_A, g, in, out, b, 0, s)) return rc;
   {
     ProfScope kern(R_N + R_MINORS, s);  // the N2D + Ar column kernel
     if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
   }
-  if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
+  if (chem_done) {  // NO and N4S sources from the step's chemistry pass and this N2D
+    void *c[20];
+    chem_bufs(work, g, c);
+    if (int rc = tgx_ng_launch(K_MIN_LIN, g, in, out, c, 0, s)) return rc;
+  } else {
+    if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
+  }
   return tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s);
 }
 
-extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
-                            void *work, void *stream) {
+int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+              void *stream, bool chem_done) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_OPLUS, s);
@@ -222,7 +272,7 @@ extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const
     b[19] = oploss;
     b[20] = opprod;
     // the O+ chemistry does not depend on O+: once for all sub-cycles
-    if (sub == 0)
+    if (sub == 0 && !chem_done)
       if (int rc = tgx_ng_launch(K_OP_CHEM, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
@@ -250,8 +300,8 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   return tgx_ng_launch(use_pcr(g, R_DUV) ? K_UV2_W : K_UV2, g, in, out, b, 0, s);
 }
 
-extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
-                           void *work, void *stream) {
+int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+             void *stream, bool chem_done) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_COMP, s);
@@ -259,10 +309,12 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   b[0] = plane(work, g, W_CO);
   for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
-  if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
+  if (!chem_done)
+    if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
   ProfScope kern(R_N + R_COMP, s);
   return tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
 }
+}  // namespace
 
 // Concurrency of the step: oplus, the minor species, dt and comp read only
 // the step's inputs and write disjoint outputs and workspace planes, so they
@@ -359,13 +411,15 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   Fork *F = nullptr;
   if (int rc = get_fork(&F)) return rc;
   hipStream_t s0 = (hipStream_t)stream, sa = F->s[0], sb = F->s[1], sc = F->s[2];
+  // the pointwise chemistry of all routines in one pass, before the fork
+  if (int rc = run_chem(g, in, out, work, s0)) return rc;
   TGX_HIP(hipEventRecord(F->ev[0], s0));
   TGX_HIP(hipStreamWaitEvent(sa, F->ev[0], 0));
   TGX_HIP(hipStreamWaitEvent(sb, F->ev[0], 0));
   TGX_HIP(hipStreamWaitEvent(sc, F->ev[0], 0));
-  if (int rc = tgx_ng_oplus(g, in, out, work, sa)) return rc;    // stream a
-  if (int rc = tgx_ng_minors(g, in, out, work, sb)) return rc;   // stream b
-  if (int rc = tgx_ng_comp(g, in, out, work, sc)) return rc;     // stream c
+  if (int rc = run_oplus(g, in, out, work, sa, true)) return rc;    // stream a
+  if (int rc = run_minors(g, in, out, work, sb, true)) return rc;   // stream b
+  if (int rc = run_comp(g, in, out, work, sc, true)) return rc;     // stream c
   if (int rc = tgx_ng_dt(g, in, out, work, s0)) return rc;       // caller's stream
   TGX_HIP(hipEventRecord(F->ev[1], sa));
   TGX_HIP(hipStreamWaitEvent(s0, F->ev[1], 0));
@@ -383,9 +437,10 @@ extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
   if (int rc = check(g, work)) return rc;
   if (g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
     if (int rc = tgx_ng_autotune(g, in, out, work, stream)) return rc;
-  if (int rc = tgx_ng_oplus(g, in, out, work, stream)) return rc;
-  if (int rc = tgx_ng_minors(g, in, out, work, stream)) return rc;
+  if (int rc = run_chem(g, in, out, work, (hipStream_t)stream)) return rc;
+  if (int rc = run_oplus(g, in, out, work, stream, true)) return rc;
+  if (int rc = run_minors(g, in, out, work, stream, true)) return rc;
   if (int rc = tgx_ng_dt(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_duv(g, in, out, work, stream)) return rc;
-  return tgx_ng_comp(g, in, out, work, stream);
+  return run_comp(g, in, out, work, stream, true);
 }
