@@ -1293,42 +1293,60 @@ struct ChemOut {
   double *oploss, *opprod;             // O+ loss and production
 };
 
-__global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                   tgx_ng_out out, ChemOut C) {
-  int nb = gridDim.x;
-  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
-  long npt = (long)G.nlond * G.nlatd * G.nk;
-  if (t >= npt) return;
-  int i = (int)(t % G.nlond);
-  long r = t / G.nlond;
-  int j = (int)(r % G.nlatd);
-  int k = (int)(r / G.nlatd);
-  const int nk = G.nk;
-  const size_t p = G.a3(k, i, j);
-  const bool top = k == nk - 1;
-  const size_t pu = top ? p : p + G.P, pd = top ? p - G.P : p;  // level above / below for interpolation
-  const double xn = in.xnmbar[p], ne = in.ne[p], ne_u = in.ne[pu], ne_d = in.ne[pd];
-  const double o1 = in.o1[p], o2 = in.o2[p], n2 = in.n2[p], op = in.op[p], no = in.no[p], n4s = in.n4s[p];
-  const double n2d = in.n2d[p], o2p = in.o2p[p], nplus = in.nplus[p], n2p = in.n2p[p], nop = in.nop[p];
-  const double beta8 = in.beta8[p], beta8_o = in.beta8[top ? pd : pu];
+// Fields chem_kernel reads at two neighbouring levels (interface means, the
+// top level's extrapolation): one level of them is carried between levels.
+// Loads and stores at a 32-bit byte offset from a kernel-argument base: the
+// address stays one SGPR pair plus one VGPR (every level plane of the
+// largest grids is well under 4 GiB).
+__device__ __forceinline__ double ldo(const double *base, unsigned off) {
+  return *(const double *)((const char *)base + off);
+}
+__device__ __forceinline__ void sto(double *base, unsigned off, double v) {
+  *(double *)((char *)base + off) = v;
+}
+
+struct ChemPair { double ne, beta8, rj, qo2p, qop, qtef, beta9, beta9n, qop2p, qop2d; };
+__device__ __forceinline__ void chem_pair_load(const tgx_ng_fields &in, unsigned p, ChemPair &v) {
+  v.ne = ldo(in.ne, p); v.beta8 = ldo(in.beta8, p); v.rj = ldo(in.rj, p); v.qo2p = ldo(in.qo2p, p); v.qop = ldo(in.qop, p);
+  v.qtef = ldo(in.qtef, p); v.beta9 = ldo(in.beta9, p); v.beta9n = ldo(in.beta9n, p); v.qop2p = ldo(in.qop2p, p);
+  v.qop2d = ldo(in.qop2d, p);
+}
+
+// Number of level runs of chem_kernel: one thread walks about CHEM_RUN levels
+// of a column, upward, loading each level once.
+constexpr int CHEM_RUN = 8;
+__host__ __device__ inline int chem_runs(int nk) { return (nk + CHEM_RUN - 1) / CHEM_RUN; }
+
+// One level of the chemistry pass: c = the paired fields at this level, o =
+// at the level above (below at the top level).
+__device__ __forceinline__ void chem_level(const tgx_ng_fields &in, const tgx_ng_out &out, const ChemOut &C,
+                                           unsigned p, bool top, const ChemPair &c, const ChemPair &o) {
+  const double xn = ldo(in.xnmbar, p), ne = c.ne;
+  const double o1 = ldo(in.o1, p), o2 = ldo(in.o2, p), n2 = ldo(in.n2, p), op = ldo(in.op, p), no = ldo(in.no, p), n4s = ldo(in.n4s, p);
+  const double n2d = ldo(in.n2d, p), o2p = ldo(in.o2p, p), nplus = ldo(in.nplus, p), n2p = ldo(in.n2p, p), nop = ldo(in.nop, p);
+  const double rk1 = ldo(in.rk1, p), rk2 = ldo(in.rk2, p), rk3 = ldo(in.rk3, p), ra1 = ldo(in.ra1, p), ra3 = ldo(in.ra3, p);
+  const double beta1 = ldo(in.beta1, p), beta3 = ldo(in.beta3, p), beta5 = ldo(in.beta5, p), beta17 = ldo(in.beta17, p);
+  const double rkm12 = ldo(in.rkm12, p), mbar = ldo(in.mbar, p), ra2 = ldo(in.ra2, p);
+  const double rk19 = ldo(in.rk19, p), rk20 = ldo(in.rk20, p), rk25 = ldo(in.rk25, p);
+  const double beta8 = c.beta8, beta8_o = o.beta8;
   // ---- comp_o2o_ng fs (levels < nk-1)
   double fsv[6] = {0., 0., 0., 0., 0., 0.};
   if (!top) {
-    double rji = 0.5 * (in.rj[p] + in.rj[pu]);
-    double qo2pi = 0.5 * (in.qo2p[p] + in.qo2p[pu]);
-    double qopi = 0.5 * (in.qop[p] + in.qop[pu]);
+    double rji = 0.5 * (c.rj + o.rj);
+    double qo2pi = 0.5 * (c.qo2p + o.qo2p);
+    double qopi = 0.5 * (c.qop + o.qop);
     double beta8i = 0.5 * (beta8 + beta8_o);
-    double nei = sqrt(ne * ne_u);
-    double pox1 = xn * xn * (in.beta3[p] * n4s / RMN4S * no / RMNO + BETA6 * n2d / RMN2D * no / RMNO) +
+    double nei = sqrt(ne * o.ne);
+    double pox1 = xn * xn * (beta3 * n4s / RMN4S * no / RMNO + BETA6 * n2d / RMN2D * no / RMNO) +
                   beta8i * no / RMNO * xn + xn * (RK4 * o2p * n4s / RMN4S + RK10 * op * n2d / RMN2D) +
-                  (in.ra1[p] * nop + 2. * in.ra2[p] * o2p) * nei;
-    double pox2 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + RK7 * nplus + 2. * rji;
-    double lox1 = 2. * in.rkm12[p] * xn / in.mbar[p];
-    double lox2 = in.rk3[p] * n2p + RK8 * nplus;
+                  (ra1 * nop + 2. * ra2 * o2p) * nei;
+    double pox2 = xn * (beta1 * n4s / RMN4S + BETA2 * n2d / RMN2D) + rk1 * op + RK7 * nplus + 2. * rji;
+    double lox1 = 2. * rkm12 * xn / mbar;
+    double lox2 = rk3 * n2p + RK8 * nplus;
     double lox3 = qopi;
-    double po21 = in.rkm12[p] * xn / in.mbar[p];
+    double po21 = rkm12 * xn / mbar;
     double po23 = RK5 * no / RMNO * o2p * xn;
-    double lo21 = xn * (in.beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + in.rk1[p] * op + (RK6 + RK7) * nplus +
+    double lo21 = xn * (beta1 * n4s / RMN4S + BETA2 * n2d / RMN2D) + rk1 * op + (RK6 + RK7) * nplus +
                   RK9 * n2p + rji;
     double lo22 = qo2pi;
     fsv[0] = -lo21;
@@ -1343,14 +1361,14 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
   {
     double qtefi, nei;
     if (!top) {
-      qtefi = .5 * (in.qtef[p] + in.qtef[pu]);
-      nei = sqrt(ne * ne_u);
+      qtefi = .5 * (c.qtef + o.qtef);
+      nei = sqrt(ne * o.ne);
     } else {
-      qtefi = 1.5 * in.qtef[p] - .5 * in.qtef[pd];
-      nei = sqrt(ne * ne * ne / ne_d);
+      qtefi = 1.5 * c.qtef - .5 * o.qtef;
+      nei = sqrt(ne * ne * ne / o.ne);
     }
-    n2d_sp = qtefi * BRN2D + in.rk3[p] * n2p * xn * o1 * RIO1 + (in.ra1[p] * nop * 0.85 + in.ra3[p] * n2p * 0.9) * nei;
-    n2d_sl = -(xn * (BETA2 * o2 * RIO2 + BETA4 * o1 * RIO1 + BETA6 * no * RINO) + BETA7 + in.beta5[p] * nei +
+    n2d_sp = qtefi * BRN2D + rk3 * n2p * xn * o1 * RIO1 + (ra1 * nop * 0.85 + ra3 * n2p * 0.9) * nei;
+    n2d_sl = -(xn * (BETA2 * o2 * RIO2 + BETA4 * o1 * RIO1 + BETA6 * no * RINO) + BETA7 + beta5 * nei +
                RK10 * op);
   }
   // ---- N4S (comp_n4s_ng.F90:122-138): production a + n2d*b, loss
@@ -1358,63 +1376,95 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
   {
     double qtefi, beta8i, nei1, nei2;
     if (!top) {
-      qtefi = .5 * (in.qtef[p] + in.qtef[pu]);
+      qtefi = .5 * (c.qtef + o.qtef);
       beta8i = .5 * (beta8 + beta8_o);
-      nei1 = .5 * (ne + ne_u);
-      nei2 = sqrt(ne * ne_u);
+      nei1 = .5 * (ne + o.ne);
+      nei2 = sqrt(ne * o.ne);
     } else {
       qtefi = 0.;  // undefined in the reference (:128); reaches only f_rhs(nk), overwritten
       beta8i = 1.5 * beta8 - .5 * beta8_o;
-      nei1 = 1.5 * ne - .5 * ne_d;
-      nei2 = sqrt(ne * ne * ne / ne_d);
+      nei1 = 1.5 * ne - .5 * o.ne;
+      nei2 = sqrt(ne * ne * ne / o.ne);
     }
     n4s_a = qtefi * (1. - BRN2D) + xn * (beta8i * no * RINO) +
-            xn * (in.rk2[p] * op * n2 * RIN2 + RK6 * nplus * o2 * RIO2 + RK8 * nplus * o1 * RIO1) +
-            nei2 * (in.ra1[p] * nop * 0.15 + in.ra3[p] * n2p * 1.1);
-    n4s_b = xn * RIN2D * (xn * BETA4 * o1 * RIO1 + in.beta5[p] * nei1 + BETA7);
-    n4s_sl = -xn * (in.beta1[p] * o2 * RIO2 + in.beta3[p] * no * RINO + xn * in.beta17[p] * o1 * RIO1 * n2 * RIN2) -
-             RK4 * o2p;
+            xn * (rk2 * op * n2 * RIN2 + RK6 * nplus * o2 * RIO2 + RK8 * nplus * o1 * RIO1) +
+            nei2 * (ra1 * nop * 0.15 + ra3 * n2p * 1.1);
+    n4s_b = xn * RIN2D * (xn * BETA4 * o1 * RIO1 + beta5 * nei1 + BETA7);
+    n4s_sl = -xn * (beta1 * o2 * RIO2 + beta3 * no * RINO + xn * beta17 * o1 * RIO1 * n2 * RIN2) - RK4 * o2p;
   }
   // ---- NO (comp_no_ng.F90:48-57): production and loss a + n2d*b
   double no_spa, no_spb, no_sla, no_slb;
   {
-    no_spa = xn * xn * o2 * RIO2 * (in.beta1[p] * n4s * RIN4S) +
-             xn * xn * xn * in.beta17[p] * o1 * RIO1 * n2 * RIN2 * n4s * RIN4S;
+    no_spa = xn * xn * o2 * RIO2 * (beta1 * n4s * RIN4S) + xn * xn * xn * beta17 * o1 * RIO1 * n2 * RIN2 * n4s * RIN4S;
     no_spb = xn * xn * o2 * RIO2 * (BETA2 * RIN2D);
-    const double b9 = in.beta9[p], b9n = in.beta9n[p], b9o = in.beta9[top ? pd : pu], b9no = in.beta9n[top ? pd : pu];
-    double betai = !top ? .5 * (beta8 + b9 + b9n + beta8_o + b9o + b9no)
-                        : 1.5 * (beta8 + b9 + b9n) - .5 * (beta8_o + b9o + b9no);
-    no_sla = -xn * (in.beta3[p] * n4s * RIN4S) - RK5 * o2p - betai;
+    double betai = !top ? .5 * (beta8 + c.beta9 + c.beta9n + beta8_o + o.beta9 + o.beta9n)
+                        : 1.5 * (beta8 + c.beta9 + c.beta9n) - .5 * (beta8_o + o.beta9 + o.beta9n);
+    no_sla = -xn * (beta3 * n4s * RIN4S) - RK5 * o2p - betai;
     no_slb = -xn * (BETA6 * RIN2D);
   }
   // ---- O+ (oplus_ng.F90:288-299)
-  double qop2pi = !top ? .5 * (in.qop2p[p] + in.qop2p[pu]) : 1.5 * in.qop2p[p] - .5 * in.qop2p[pd];
-  double qop2di = !top ? .5 * (in.qop2d[p] + in.qop2d[pu]) : 1.5 * in.qop2d[p] - .5 * in.qop2d[pd];
-  double x2p = qop2pi / (xn * ((RK16 + RK17) * n2 * RIN2 + RK18 * o1 * RIO1) + (in.rk19[p] + in.rk20[p]) * ne + RK21 + RK22);
-  double x2d = (qop2di + (in.rk20[p] * ne + RK22) * x2p) /
-               (xn * (RK23 * n2 * RIN2 + RK24 * o1 * RIO1 + RK26 * o2 * RIO2) + in.rk25[p] * ne + RK27);
-  double oploss = xn * (in.rk1[p] * o2 * RIO2 + in.rk2[p] * n2 * RIN2 + RK10 * n2d * RIN2D);
+  double qop2pi = !top ? .5 * (c.qop2p + o.qop2p) : 1.5 * c.qop2p - .5 * o.qop2p;
+  double qop2di = !top ? .5 * (c.qop2d + o.qop2d) : 1.5 * c.qop2d - .5 * o.qop2d;
+  double x2p = qop2pi / (xn * ((RK16 + RK17) * n2 * RIN2 + RK18 * o1 * RIO1) + (rk19 + rk20) * ne + RK21 + RK22);
+  double x2d = (qop2di + (rk20 * ne + RK22) * x2p) /
+               (xn * (RK23 * n2 * RIN2 + RK24 * o1 * RIO1 + RK26 * o2 * RIO2) + rk25 * ne + RK27);
+  double oploss = xn * (rk1 * o2 * RIO2 + rk2 * n2 * RIN2 + RK10 * n2d * RIN2D);
   double opprod = 0.;
   if (!top) {
-    double qopi = .5 * (in.qop[p] + in.qop[pu]);
-    opprod = qopi + (in.rk19[p] * ne + RK21) * x2p + (in.rk25[p] * ne + RK27) * x2d + (RK18 * x2p + RK24 * x2d) * o1 * RIO1 * xn;
+    double qopi = .5 * (c.qop + o.qop);
+    opprod = qopi + (rk19 * ne + RK21) * x2p + (rk25 * ne + RK27) * x2d + (RK18 * x2p + RK24 * x2d) * o1 * RIO1 * xn;
   }
   // ---- stores
   if (!top)
-    for (int q = 0; q < 6; ++q) C.fs[q][p] = fsv[q];
-  C.n2d_sloss[p] = n2d_sl;
-  C.n2d_s0prod[p] = n2d_sp * RMN2D / xn;
-  C.n4s_sloss[p] = n4s_sl;
-  C.lin[0][p] = n4s_a * RMN4S / xn;
-  C.lin[1][p] = n4s_b * RMN4S / xn;
-  C.lin[2][p] = no_sla;
-  C.lin[3][p] = no_slb;
-  C.lin[4][p] = no_spa * RMNO / xn;
-  C.lin[5][p] = no_spb * RMNO / xn;
-  out.xiop2p[p] = x2p;
-  out.xiop2d[p] = x2d;
-  C.oploss[p] = oploss;
-  if (!top) C.opprod[p] = opprod;
+    for (int q = 0; q < 6; ++q) sto(C.fs[q], p, fsv[q]);
+  sto(C.n2d_sloss, p, n2d_sl);
+  sto(C.n2d_s0prod, p, n2d_sp * RMN2D / xn);
+  sto(C.n4s_sloss, p, n4s_sl);
+  sto(C.lin[0], p, n4s_a * RMN4S / xn);
+  sto(C.lin[1], p, n4s_b * RMN4S / xn);
+  sto(C.lin[2], p, no_sla);
+  sto(C.lin[3], p, no_slb);
+  sto(C.lin[4], p, no_spa * RMNO / xn);
+  sto(C.lin[5], p, no_spb * RMNO / xn);
+  sto(out.xiop2p, p, x2p);
+  sto(out.xiop2d, p, x2d);
+  sto(C.oploss, p, oploss);
+  if (!top) sto(C.opprod, p, opprod);
+}
+
+// The pointwise chemistry of the whole step (comp_o2o_ng fs, the N2D / N4S /
+// NO sources, O+ loss and production and xiop2p/xiop2d) in one pass.  A
+// thread takes a run of levels of one column, bottom to top, so the fields
+// read at two levels are loaded once each.
+__global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                   tgx_ng_out out, ChemOut C) {
+  int nb = gridDim.x;
+  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  const int nk = G.nk, nrun = chem_runs(nk);
+  long npt = (long)G.nlond * G.nlatd * nrun;
+  if (t >= npt) return;
+  int i = (int)(t % G.nlond);
+  long r = t / G.nlond;
+  int j = (int)(r % G.nlatd);
+  int run = (int)(r / G.nlatd);
+  const int k0 = (int)((long)run * nk / nrun), k1 = (int)((long)(run + 1) * nk / nrun);
+  ChemPair c, o, d;
+  const unsigned PB = (unsigned)(G.P * sizeof(double));  // level stride in bytes
+  chem_pair_load(in, (unsigned)(G.a3(k0, i, j) * sizeof(double)), c);
+  if (k0 == nk - 1)  // a run that is only the top level
+    chem_pair_load(in, (unsigned)(G.a3(k0 - 1, i, j) * sizeof(double)), d);
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k) {
+    const bool top = k == nk - 1;
+    const unsigned p = (unsigned)(G.a3(k, i, j) * sizeof(double));
+    if (!top)
+      chem_pair_load(in, p + PB, o);
+    else
+      o = d;
+    chem_level(in, out, C, p, top, c, o);
+    d = c;
+    c = o;
+  }
 }
 
 // The NO and N4S sources of this step from the linear coefficients of
@@ -3646,9 +3696,13 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       C.n2d_sloss = D(6); C.n2d_s0prod = D(7); C.n4s_sloss = D(8);
       for (int q = 0; q < 6; ++q) C.lin[q] = D(9 + q);
       C.oploss = D(15); C.opprod = D(16);
-      if (which == 25)
-        hipLaunchKernelGGL(chem_kernel, gridp, block, 0, s, g, G, in, out, C);
-      else
+      if (which == 25) {
+        if ((unsigned long long)G.nk * G.P * sizeof(double) >= (1ull << 32))
+          return tgx_set_error(1, "chem_kernel: a field of %d levels x %ld points exceeds 4 GiB", G.nk, (long)G.P);
+        long nt = (long)G.nlond * G.nlatd * chem_runs(G.nk);
+        dim3 gridc((unsigned)((nt + 255) / 256));
+        hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C);
+      } else
         hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C, D(17), D(18), D(19));
       break;
     }
