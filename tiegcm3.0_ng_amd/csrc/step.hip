@@ -170,6 +170,26 @@ __device__ __forceinline__ double advec(const F &f, const U &u, const U &v,
   return wk1 / dlamda + wk2 * RE_INV / dphi;
 }
 
+// advec with the constant factors folded and the divisions by the grid
+// spacings taken as products with their reciprocals (rdl = 1/dlamda,
+// rdp = 1/(re dphi)): a few ulp from advec, for the species the minor and
+// major-species kernels advect.
+template <class F, class U>
+__device__ __forceinline__ double advec_r(const F &f, const U &u, const U &v, const Geo &G, int i, int j,
+                                          double racs, double rdl, double rdp) {
+  constexpr double C43 = 4. / 3., C13 = 1. / 3.;
+  double ub1 = fmlon1(u, G, i, j), ub2 = fmlon2(u, G, i, j);
+  double dx1 = dflon1(f, G, i, j), dx2 = dflon2(f, G, i, j);
+  double wk1 = dx1 * C43 * ub1 - dx2 * C13 * ub2;
+  if (i <= 1 || i >= G.nlon + 2) wk1 = dx1 * ub1;
+  wk1 = wk1 * racs;
+  double vb1 = fmlat1(v, G, i, j), vb2 = fmlat2(v, G, i, j);
+  double dy1 = dflat1(f, G, i, j), dy2 = dflat2(f, G, i, j);
+  double wk2 = dy1 * C43 * vb1 - dy2 * C13 * vb2;
+  if (j <= 1 || j >= G.nlat + 2) wk2 = dy1 * vb1;
+  return wk1 * rdl + wk2 * rdp;
+}
+
 // smooth_ng.F90:283-294: latitude pass at (i, j); rows 0/J1 copy, rows 1/J1-1
 // use the 3-point form
 template <class F>
@@ -257,7 +277,7 @@ __device__ __forceinline__ bool lateral_value(const Geo &G, const double *lonb, 
 // =============================================================================
 // Horizontal-stencil prelude: one thread per (column, level), every term a
 // column kernel needs from neighbouring columns, written as a level plane.
-enum { ST_ADVEC = 0, ST_SMOOTH, ST_DIVX, ST_DIVY, ST_BDOTDH, ST_VDOTN };
+enum { ST_ADVEC = 0, ST_SMOOTH, ST_DIVX, ST_DIVY, ST_BDOTDH, ST_VDOTN, ST_ADVEC_R };
 struct StencilJob {
   int kind;
   const double *f;
@@ -269,117 +289,163 @@ struct StencilJobs {
   StencilJob job[8];
 };
 
-// Horizontal stencils of one level: a block takes a TILE_X x TILE_Y tile of
-// the plane and loads every job's input once into LDS with a 2-point halo
-// (coalesced rows), all jobs' loads in flight together, then evaluates the
-// stencil functions from LDS.  Halo points outside the domain are never read:
-// the stencils clamp at the domain edges.  Dynamic LDS: (jobs + 2) tiles and
-// one latitude-pass buffer.
-__global__ void __launch_bounds__(256) stencil_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                      StencilJobs J) {
+// Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
+// planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
+// into LDS with a 2-point halo (coalesced rows); halo points outside the
+// domain are never read (the stencils clamp at the domain edges).  The job
+// kinds are fixed at compile time.  While a level is evaluated from LDS, the
+// next level's tiles are already in flight into registers.  The latitude
+// passes of all smoothed fields share one barrier.  Dynamic LDS: one tile per
+// job (+ u, v when a job advects) and one latitude-pass buffer per smoothed
+// field.
+constexpr int STENCIL_RUN = 8;
+template <int... K>
+struct StencilKinds {
+  static constexpr int n = sizeof...(K);
+  static constexpr int kind[n] = {K...};
+  static constexpr bool adv() {
+    for (int q = 0; q < n; ++q)
+      if (kind[q] == ST_ADVEC || kind[q] == ST_ADVEC_R) return true;
+    return false;
+  }
+  static constexpr int nsmooth() {
+    int c = 0;
+    for (int q = 0; q < n; ++q) c += kind[q] == ST_SMOOTH;
+    return c;
+  }
+  static constexpr int ntiles() { return n + (adv() ? 2 : 0); }
+  static constexpr size_t lds_bytes() {
+    return ((size_t)ntiles() * TILE_R * TILE_W + (size_t)nsmooth() * TILE_Y * TILE_W) * sizeof(double);
+  }
+};
+
+template <class KS>
+__global__ void __launch_bounds__(256) stencil_run_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                          StencilJobs J) {
   extern __shared__ double smem[];
-  constexpr int TS = TILE_R * TILE_W;
+  constexpr int NJ = KS::n, NT = KS::ntiles(), NS = KS::nsmooth();
+  constexpr bool ADV = KS::adv();
+  constexpr int TS = TILE_R * TILE_W, TL = TILE_Y * TILE_W;
   const int ntx = (G.nlond + TILE_X - 1) / TILE_X, nty = (G.nlatd + TILE_Y - 1) / TILE_Y;
+  const int nrun = (G.nk + STENCIL_RUN - 1) / STENCIL_RUN;
   const int b = tgx_xcd_block(gridDim.x);
-  if (b >= ntx * nty * G.nk) return;  // uniform per block
-  const int bxi = b % ntx, rr = b / ntx, byj = rr % nty, k = rr / nty;
+  if (b >= ntx * nty * nrun) return;  // uniform per block
+  const int bxi = b % ntx, rr = b / ntx, byj = rr % nty, run = rr / nty;
+  const int k0 = run * STENCIL_RUN, k1 = min(k0 + STENCIL_RUN, G.nk);
   const int i0 = bxi * TILE_X, j0 = byj * TILE_Y;
   const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
   const bool inside = i < G.nlond && j < G.nlatd;
-  const size_t off = (size_t)k * G.P;
-  bool adv = false;
-  for (int n = 0; n < J.n; ++n) adv = adv || J.job[n].kind == ST_ADVEC;
-  double *tu = smem + (size_t)J.n * TS, *tv = tu + TS;
-  // every input tile of the block is requested before any is used: up to
-  // 2 x (8 + 2) loads per thread in flight
-  double ld[2][10];
+  const size_t c2 = (size_t)j * G.ld + i;
+  const double *src[NT];
+#pragma unroll
+  for (int t = 0; t < NJ; ++t) src[t] = J.job[t].f;
+  if (ADV) {
+    src[NT - 2] = in.un;
+    src[NT - 1] = in.vn;
+  }
+  // tile elements of this thread (a 12 x 36 tile: two per thread at most)
   bool okr[2];
   int er[2];
+  size_t qr[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int e = (int)threadIdx.x + r * 256;
     const int ii = i0 - TILE_H + e % TILE_W, jj = j0 - TILE_H + e / TILE_W;
     er[r] = e;
     okr[r] = e < TS && ii >= 0 && ii < G.nlond && jj >= 0 && jj < G.nlatd;
-    const size_t q = okr[r] ? off + (size_t)jj * G.ld + ii : off;
-#pragma unroll
-    for (int n = 0; n < 8; ++n)
-      if (n < J.n && okr[r]) ld[r][n] = J.job[n].f[q];
-    if (adv && okr[r]) {
-      ld[r][8] = in.un[q];
-      ld[r][9] = in.vn[q];
-    }
+    qr[r] = okr[r] ? (size_t)jj * G.ld + ii : 0;
   }
+  double ld[2][NT];
+  auto issue = [&](int k) {
+    const size_t off = (size_t)k * G.P;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    if (!okr[r]) continue;
+    for (int r = 0; r < 2; ++r)
+      if (okr[r]) {
 #pragma unroll
-    for (int n = 0; n < 8; ++n)
-      if (n < J.n) smem[n * TS + er[r]] = ld[r][n];
-    if (adv) {
-      tu[er[r]] = ld[r][8];
-      tv[er[r]] = ld[r][9];
-    }
-  }
-  __syncthreads();
-  const LdsPlane U{tu, i0 - TILE_H, j0 - TILE_H}, V{tv, i0 - TILE_H, j0 - TILE_H};
-  const size_t p = off + (size_t)j * G.ld + i, c2 = (size_t)j * G.ld + i;
-  double *tl = tv + TS;  // latitude pass of a smoothed field: TILE_Y rows x TILE_W
-  for (int n = 0; n < J.n; ++n) {
-    const StencilJob &jb = J.job[n];
-    const LdsPlane F{smem + n * TS, i0 - TILE_H, j0 - TILE_H};
-    if (jb.kind == ST_SMOOTH) {
-      // smooth_ng.F90:283-294 once per point of the tile rows and their
-      // longitude halo, then the longitude pass (:296-307) from LDS
-      for (int e = threadIdx.x; e < TILE_W * TILE_Y; e += blockDim.x) {
-        int ii = i0 - TILE_H + e % TILE_W, jj = j0 + e / TILE_W;
-        if (ii >= 0 && ii < G.nlond && jj < G.nlatd) tl[e] = smooth_lat(F, G, ii, jj, jb.s);
+        for (int t = 0; t < NT; ++t) ld[r][t] = src[t][off + qr[r]];
+      }
+  };
+  const double racs = inside ? g.racs[j] : 0., csj = inside ? g.cs[j] : 1.;
+  const double bx = inside ? in.bx[c2] : 0., by = inside ? in.by[c2] : 0., bm = inside ? in.bmod2[c2] : 0.;
+  const double rdl = 1. / g.dlamda, rdp = RE_INV / g.dphi;
+  double *tl = smem + (size_t)NT * TS;  // NS latitude-pass buffers
+  const LdsPlane U{smem + (NT - 2) * TS, i0 - TILE_H, j0 - TILE_H}, V{smem + (NT - 1) * TS, i0 - TILE_H, j0 - TILE_H};
+  issue(k0);
+  for (int k = k0; k < k1; ++k) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      if (okr[r]) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) smem[t * TS + er[r]] = ld[r][t];
+      }
+    __syncthreads();
+    if (k + 1 < k1) issue(k + 1);
+    asm volatile("" ::: "memory");
+    if (NS > 0) {  // smooth_ng.F90:283-294 for the tile rows and their longitude halo
+      int qs = 0;
+#pragma unroll
+      for (int q = 0; q < NJ; ++q) {
+        if (KS::kind[q] != ST_SMOOTH) continue;
+        const LdsPlane F{smem + q * TS, i0 - TILE_H, j0 - TILE_H};
+        for (int e = threadIdx.x; e < TL; e += blockDim.x) {
+          int ii = i0 - TILE_H + e % TILE_W, jj = j0 + e / TILE_W;
+          if (ii >= 0 && ii < G.nlond && jj < G.nlatd) tl[qs * TL + e] = smooth_lat(F, G, ii, jj, J.job[q].s);
+        }
+        ++qs;
       }
       __syncthreads();
-      if (inside) {
-        const double *row = tl + (j - j0) * TILE_W + TILE_H - i0;  // row[ii] = lat pass at (ii, j)
-        int L1 = G.nlond - 1;
-        bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
-        int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
-        int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
-        double t0 = row[i];
-        double fm1 = (row[i1p] + row[i1m]) * 0.5;
-        double fm2 = (row[i2p] + row[i2m]) * 0.5;
-        double gen = t0 - jb.s * (2. * fm2 - 8. * fm1 + 6. * t0);
-        double nr = t0 - jb.s * (-8. * fm1 + 8. * t0);
-        jb.out[p] = edge ? t0 : (near ? nr : gen);
-      }
-      __syncthreads();
-      continue;
     }
-    if (!inside) continue;
-    double v;
-    switch (jb.kind) {
-      case ST_ADVEC:
-        v = advec(F, U, V, G, i, j, g.racs[j], g.dlamda, g.dphi);
-        break;
-      case ST_DIVX:
-        v = divx(F, G, i, j);
-        break;
-      case ST_DIVY:
-        v = divy(F, G, i, j);
-        break;
-      case ST_BDOTDH:
-        v = bdotdh(F, G, i, j, in.bx[c2], in.by[c2], g.cs[j], g.dlamda, g.dphi);
-        break;
-      default: {  // ST_VDOTN: oplus_ng.F90:196-209, f = op/bmod2**2
-        double uii = k < G.nk - 1 ? .5 * (in.ui[p] + in.ui[p + G.P]) : 1.5 * in.ui[p] - .5 * in.ui[p - G.P];
-        double vii = k < G.nk - 1 ? .5 * (in.vi[p] + in.vi[p + G.P]) : 1.5 * in.vi[p] - .5 * in.vi[p - G.P];
-        double vx = uii * dflon1(F, G, i, j), vy = vii * dflat1(F, G, i, j);
-        vx = vx / g.cs[j];
-        double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
-        double bm = in.bmod2[c2];
-        v = vh * (bm * bm);
+    if (inside) {
+      const size_t p = (size_t)k * G.P + c2;
+      int qs = 0;
+#pragma unroll
+      for (int q = 0; q < NJ; ++q) {
+        const LdsPlane F{smem + q * TS, i0 - TILE_H, j0 - TILE_H};
+        double v;
+        if (KS::kind[q] == ST_SMOOTH) {  // :296-307 from the latitude pass
+          const double *row = tl + qs * TL + (j - j0) * TILE_W + TILE_H - i0;
+          ++qs;
+          const double sc = J.job[q].s;
+          int L1 = G.nlond - 1;
+          bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+          int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
+          int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
+          double t0 = row[i];
+          double fm1 = (row[i1p] + row[i1m]) * 0.5;
+          double fm2 = (row[i2p] + row[i2m]) * 0.5;
+          double gen = t0 - sc * (2. * fm2 - 8. * fm1 + 6. * t0);
+          double nr = t0 - sc * (-8. * fm1 + 8. * t0);
+          v = edge ? t0 : (near ? nr : gen);
+        } else if (KS::kind[q] == ST_ADVEC) {
+          v = advec(F, U, V, G, i, j, racs, g.dlamda, g.dphi);
+        } else if (KS::kind[q] == ST_ADVEC_R) {
+          v = advec_r(F, U, V, G, i, j, racs, rdl, rdp);
+        } else if (KS::kind[q] == ST_DIVX) {
+          v = divx(F, G, i, j);
+        } else if (KS::kind[q] == ST_DIVY) {
+          v = divy(F, G, i, j);
+        } else if (KS::kind[q] == ST_BDOTDH) {
+          v = bdotdh(F, G, i, j, bx, by, csj, g.dlamda, g.dphi);
+        } else {  // ST_VDOTN: oplus_ng.F90:196-209, f = op/bmod2**2
+          double uii = k < G.nk - 1 ? .5 * (in.ui[p] + in.ui[p + G.P]) : 1.5 * in.ui[p] - .5 * in.ui[p - G.P];
+          double vii = k < G.nk - 1 ? .5 * (in.vi[p] + in.vi[p + G.P]) : 1.5 * in.vi[p] - .5 * in.vi[p - G.P];
+          double vx = uii * dflon1(F, G, i, j), vy = vii * dflat1(F, G, i, j);
+          vx = vx / csj;
+          double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
+          v = vh * (bm * bm);
+        }
+        J.job[q].out[p] = v;
       }
     }
-    jb.out[p] = v;
+    __syncthreads();  // the next level overwrites the tiles and buffers
   }
 }
+using KindsOplus = StencilKinds<ST_BDOTDH, ST_BDOTDH, ST_BDOTDH, ST_VDOTN, ST_BDOTDH, ST_SMOOTH>;
+using KindsDt = StencilKinds<ST_ADVEC, ST_SMOOTH>;
+using KindsMinor =
+    StencilKinds<ST_SMOOTH, ST_SMOOTH, ST_SMOOTH, ST_SMOOTH, ST_ADVEC_R, ST_ADVEC_R, ST_ADVEC_R, ST_ADVEC_R>;
+using KindsComp = StencilKinds<ST_ADVEC_R, ST_ADVEC_R, ST_ADVEC_R, ST_SMOOTH, ST_SMOOTH, ST_SMOOTH>;
+using KindsDuv = StencilKinds<ST_DIVX, ST_DIVY, ST_SMOOTH, ST_SMOOTH, ST_ADVEC, ST_ADVEC>;
 
 // =============================================================================
 // dt_ng.F90:54-213
@@ -3650,11 +3716,17 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
 // ---------------------------------------------------------------------------
 // Launcher used by step_api.cpp.  bufs holds the workspace planes (and for
 // O+ the input/output state of the sub-step) in the order documented there.
-static size_t stencil_lds(const tgx::ng::StencilJobs &J) {
-  // one tile per job, the u and v tiles, and one latitude-pass buffer
-  return ((size_t)(J.n + 2) * tgx::ng::TILE_R * tgx::ng::TILE_W +
-          (size_t)tgx::ng::TILE_Y * tgx::ng::TILE_W) * sizeof(double);
+// Stencil prelude launch: a run of levels of one tile per block.
+template <class KS>
+static void launch_stencil_run(const tgx_ng_grid &g, const tgx::ng::Geo &G, const tgx_ng_fields &in,
+                               const tgx::ng::StencilJobs &J, hipStream_t s) {
+  using namespace tgx::ng;
+  const int ntx = (G.nlond + TILE_X - 1) / TILE_X, nty = (G.nlatd + TILE_Y - 1) / TILE_Y;
+  const int nrun = (G.nk + STENCIL_RUN - 1) / STENCIL_RUN;
+  hipLaunchKernelGGL(stencil_run_kernel<KS>, dim3((unsigned)(ntx * nty * nrun)), dim3(256), KS::lds_bytes(), s, g,
+                     G, in, J);
 }
+
 
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fields *inp,
                              const tgx_ng_out *outp, void *const *bufs, int sub, hipStream_t s) {
@@ -3667,10 +3739,9 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   int nb = (int)((ncol + 255) / 256);
   int nbp = (int)((ncol * G.nk + 255) / 256);
   // thread per column (grid), per column-level (gridp), wavefront per column
-  // (gridw), one stencil tile of one level per block (grids)
+  // (gridw)
   int nbw = (int)((ncol + 3) / 4);
-  int nbs = ((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y) * G.nk;
-  dim3 grid(nb), gridp(nbp), gridw(nbw), grids(nbs), block(256);
+  dim3 grid(nb), gridp(nbp), gridw(nbw), block(256);
   auto D = [&](int n) { return (double *)bufs[n]; };
   auto job = [](int kind, const double *f, double *o, double sh) {
     StencilJob jb;
@@ -3773,7 +3844,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         J.job[3] = job(ST_VDOTN, W.opbm2, D(16), 0.);
         J.job[4] = job(ST_BDOTDH, W.djbz, D(17), 0.);
         J.job[5] = job(ST_SMOOTH, optm1, D(18), g.shapiro / g.nstep_sub);
-        hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
+        launch_stencil_run<KindsOplus>(g, G, in, J, s);
       } else {
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
@@ -3827,7 +3898,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.n = 2;
       J.job[0] = job(ST_ADVEC, in.tn, D(1), 0.);
       J.job[1] = job(ST_SMOOTH, in.tn_nm, D(2), g.shapiro);
-      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
+      launch_stencil_run<KindsDt>(g, G, in, J, s);
       break;
     }
     case 11: {  // minors prelude (species 0..3 = NO, N4S, N2D, AR)
@@ -3839,7 +3910,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         J.job[q] = job(ST_SMOOTH, fm[q], D(2 + q), g.shapiro);
         J.job[4 + q] = job(ST_ADVEC, fc[q], D(6 + q), 0.);
       }
-      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
+      launch_stencil_run<KindsMinor>(g, G, in, J, s);
       break;
     }
     case 12: {  // comp prelude
@@ -3851,7 +3922,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.job[3] = job(ST_SMOOTH, in.o2_nm, D(4), g.shapiro);
       J.job[4] = job(ST_SMOOTH, in.o1_nm, D(5), g.shapiro);
       J.job[5] = job(ST_SMOOTH, in.he_nm, D(6), g.shapiro);
-      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
+      launch_stencil_run<KindsComp>(g, G, in, J, s);
       break;
     }
     case 13: {  // duv prelude (after stage 1 wrote ztmp)
@@ -3863,7 +3934,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.job[3] = job(ST_SMOOTH, in.vn_nm, D(5), g.shapiro);
       J.job[4] = job(ST_ADVEC, in.un, D(6), 0.);
       J.job[5] = job(ST_ADVEC, in.vn, D(7), 0.);
-      hipLaunchKernelGGL(stencil_kernel, grids, block, stencil_lds(J), s, g, G, in, J);
+      launch_stencil_run<KindsDuv>(g, G, in, J, s);
       break;
     }
     default:
