@@ -4,14 +4,17 @@
 Workload (default): the 0.625 deg global grid, 576 x 288 columns x 113 levels,
 all 11 unknowns per column per step (TN; UN+VN as the 2x2 block; O+; NO,
 N4S, N2D, Ar; O2+O+He as the 3x3 block), one O+ sub-cycle, FP64, seeded
-synthetic fields resident in HBM.  A step is one pass of the whole column
-step (dynamics_ng.F90 order) over that state.
+synthetic fields resident in HBM.  A step is one call of tgx_ng_step, the
+whole column step over that state: the routines of dynamics_ng.F90 with the
+independent ones on forked streams, bitwise equal to the reference's order
+(--serial times that order on one stream instead).
 
 Prints one JSON line:
   value   = Mcolumn-level solves/s over the interior columns (whole job),
   roofline: the column kernel with the longest launch: its algorithmic bytes
             (the planes it must read and write once) / its hipEvent-timed
-            duration, against the 8 TB/s HBM peak; traffic from the
+            duration (in a few serial steps after the timed region, where
+            it runs alone), against the 8 TB/s HBM peak; traffic from the
             committed rocprofv3 --pmc summary of that kernel when present,
   cpu_baseline: the C oracle (a restatement of the reference) on one host
             core over a bounded sub-grid of the same workload.
@@ -220,9 +223,11 @@ def main():
     ap.add_argument("--scheme", default="auto", choices=("thomas", "pcr", "auto"),
                     help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
     ap.add_argument("--concurrent", action="store_true",
-                    help="run the independent routines concurrently on forked streams (default: the "
-                         "reference's order on one stream, so each kernel's duration is its own)")
-    ap.add_argument("--serial", action="store_true", help="(default) the reference's order on one stream")
+                    help="(default) tgx_ng_step: the independent routines on forked streams")
+    ap.add_argument("--serial", action="store_true",
+                    help="time tgx_ng_step_serial: the reference's order on one stream")
+    ap.add_argument("--roofline-steps", type=int, default=5,
+                    help="serial steps after the timed region that time each column kernel alone")
     ap.add_argument("--precision", action="store_true",
                     help="FP32 + FP64-refinement solves against the FP64 solvers on the assembled systems")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -262,7 +267,8 @@ def main():
               file=sys.stderr, flush=True)
         del a, b
 
-    step_fn = ns.step if args.concurrent else ns.step_serial
+    serial = args.serial or bool(args.only)
+    step_fn = ns.step_serial if serial else ns.step
     if args.only:
         step_fn = calls[args.only]
     for _ in range(args.warmup):  # the first AUTO step times both schemes
@@ -282,6 +288,22 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     rt = {r: prof[r][0] / max(prof[r][1], 1) for r in routines}
     kt = {r: kern[r][0] / max(kern[r][1], 1) for r in routines}
+    serial_ms = ms_per_step if serial else None
+    if not serial:
+        # On forked streams the kernels share the GPU, so their event times
+        # are not their own: a few steps in the reference's order on one
+        # stream time each routine and column kernel alone.
+        nrs = max(1, args.roofline_steps)
+        pkg.step.profile_begin(nrs * max(1, args.nstep_sub))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for s in range(nrs):
+            ns.step_serial()
+        torch.cuda.synchronize()
+        serial_ms = (time.perf_counter() - t2) * 1e3 / nrs
+        prof, kern = pkg.step.profile_end()
+        rt = {r: prof[r][0] / max(prof[r][1], 1) for r in routines}
+        kt = {r: kern[r][0] / max(kern[r][1], 1) for r in routines}
 
     nk, cols = g["nk"], g["nlon"] * g["nlat"]
     cols_all = (g["nlon"] + 4) * (g["nlat"] + 4)
@@ -315,14 +337,18 @@ def main():
                                "nstep_sub=%d, fused assembly+solve FP64" % (
                                    args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
                    "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64",
-                   "schedule": "concurrent" if args.concurrent else "serial", "scheme": args.scheme,
+                   "schedule": "serial" if serial else "concurrent", "scheme": args.scheme,
                    "schemes": schemes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": kname, "alg_bytes_per_launch": alg, "launch_ms": round(kt[dom], 4)},
+                     "kernel": kname, "alg_bytes_per_launch": alg, "launch_ms": round(kt[dom], 4),
+                     "timing": "hipEvents around each launch, %s" % (
+                         "inside the timed region" if serial else
+                         "%d serial steps after the timed region" % max(1, args.roofline_steps))},
         "routine_ms": {r: round(rt[r], 4) for r in routines},
         "column_kernel_ms": {r: round(kt[r], 4) for r in routines},
         "step_alg_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "serial_ms_per_step": round(serial_ms, 4),
     }
     if not args.no_cpu_baseline and not args.only:
         print("# cpu baseline", file=sys.stderr, flush=True)

@@ -298,7 +298,7 @@ struct StencilJobs {
 // passes of all smoothed fields share one barrier.  Dynamic LDS: one tile per
 // job (+ u, v when a job advects) and one latitude-pass buffer per smoothed
 // field.
-constexpr int STENCIL_RUN = 8;
+constexpr int STENCIL_RUN = 3;
 template <int... K>
 struct StencilKinds {
   static constexpr int n = sizeof...(K);
@@ -970,6 +970,9 @@ struct MinorSpecies {  // per species and level
 struct MinorTerms {  // prelude planes by species id: smoothed fcomp_tm1, advection of
                      // fcomp, and the loss / scaled production of minor_src_kernel
   const double *ftm1s[4], *hadvec[4], *sloss[4], *s0prod[4];
+  // NO / N4S sources as chem_kernel's linear forms in this step's N2D
+  // (minor_kernel<.., LIN = true>): lin[0..5] and the solved N2D
+  const double *lin[6], *n2d;
 };
 
 // Production and loss of species s at level L of column (i, j): the pointwise
@@ -1061,7 +1064,7 @@ __global__ void __launch_bounds__(256) minor_src_kernel(tgx_ng_grid g, Geo G, tg
   }
 }
 
-template <int SA, int SB, bool CAP>
+template <int SA, int SB, bool CAP, bool LIN = false>
 __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     tgx_ng_out out, double *__restrict__ w2a,
                                                     double *__restrict__ w2b, MinorTerms MT) {
@@ -1124,7 +1127,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   const double xmbari = 1. / (bo2 * RIO2 + bo1 * RIO1 + (1. - bo2 - bo1 - bhe) * RIN2 + bhe * RIHE);
 
   struct MR { double o1, o2, he, mbar, tn, barm, w; };  // shared fields at one level
-  struct SR { double ftm1s, hadvec, sloss, s0prod; };     // one species at one level
+  struct SR { double ftm1s, hadvec, sloss, s0prod, la, lb, n2d; };  // one species at one level
   auto load_mr = [&](int L, MR &R) {
     size_t p = G.a3(L, i, j);
     R.o1 = in.o1[p]; R.o2 = in.o2[p]; R.he = in.he[p]; R.mbar = in.mbar[p]; R.tn = in.tn[p];
@@ -1137,6 +1140,13 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
       Q[q].ftm1s = MT.ftm1s[SP[q]][p];
       Q[q].hadvec = MT.hadvec[SP[q]][p];
       if (SP[q] == 3) { Q[q].sloss = 0.; Q[q].s0prod = 0.; }
+      else if (LIN && SP[q] == 0) {  // NO: sloss = lin2 + n2d lin3, s0prod = lin4 + n2d lin5
+        Q[q].sloss = MT.lin[2][p]; Q[q].la = MT.lin[3][p];
+        Q[q].s0prod = MT.lin[4][p]; Q[q].lb = MT.lin[5][p]; Q[q].n2d = MT.n2d[p];
+      } else if (LIN && SP[q] == 1) {  // N4S: s0prod = lin0 + n2d lin1
+        Q[q].sloss = MT.sloss[1][p];
+        Q[q].s0prod = MT.lin[0][p]; Q[q].lb = MT.lin[1][p]; Q[q].n2d = MT.n2d[p];
+      }
       else { Q[q].sloss = MT.sloss[SP[q]][p]; Q[q].s0prod = MT.s0prod[SP[q]][p]; }
     }
   };
@@ -1197,6 +1207,12 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     T.thdiff = ex - c.alfax * dtnidz / (dzp * S.tni);
     T.sloss = R.sloss;
     T.s0prod = R.s0prod;
+    if (LIN && SP[q] == 0) {
+      T.sloss = R.sloss + R.n2d * R.la;
+      T.s0prod = R.s0prod + R.n2d * R.lb;
+    } else if (LIN && SP[q] == 1) {
+      T.s0prod = R.s0prod + R.n2d * R.lb;
+    }
     T.ftm1s = R.ftm1s;
     T.hadvec = R.hadvec;
   };
@@ -3790,9 +3806,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 15:
     case 16:
     case 18:
-    case 19: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
+    case 19:
+    case 27: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
                 // NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO after N2D's solve
-      MinorTerms MT;
+      MinorTerms MT{};
       for (int q = 0; q < 4; ++q) { MT.ftm1s[q] = D(2 + q); MT.hadvec[q] = D(6 + q); }
       MT.sloss[0] = MT.sloss[2] = D(10);
       MT.s0prod[0] = MT.s0prod[2] = D(11);
@@ -3803,7 +3820,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL((minor_kernel<2, 3, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
       else if (which == 1)
         hipLaunchKernelGGL((minor_kernel<2, 3, false>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
-      else if (which == 2 && g.capture)
+      else if (which == 27) {  // bufs 14..19: chem_kernel's lin planes
+        for (int q = 0; q < 6; ++q) MT.lin[q] = D(14 + q);
+        MT.n2d = out.n2d;
+        if (g.capture)
+          hipLaunchKernelGGL((minor_kernel<1, 0, true, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+        else
+          hipLaunchKernelGGL((minor_kernel<1, 0, false, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+      } else if (which == 2 && g.capture)
         hipLaunchKernelGGL((minor_kernel<1, 0, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
       else if (which == 2)
         hipLaunchKernelGGL((minor_kernel<1, 0, false>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);

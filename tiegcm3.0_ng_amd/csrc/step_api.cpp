@@ -133,7 +133,8 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
-       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN };
+       K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
+       K_MIN_B_LIN };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -237,6 +238,13 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   {
     ProfScope kern(R_N + R_MINORS, s);  // the N2D + Ar column kernel
     if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
+  }
+  if (chem_done && !pcr) {  // NO and N4S sources formed in the column kernel from the
+                            // step's chemistry pass and this N2D
+    void *b2[20];
+    for (int q = 0; q < 14; ++q) b2[q] = b[q];
+    for (int q = 0; q < 6; ++q) b2[14 + q] = plane(work, g, W_MIN_L + q);
+    return tgx_ng_launch(K_MIN_B_LIN, g, in, out, b2, 0, s);
   }
   if (chem_done) {  // NO and N4S sources from the step's chemistry pass and this N2D
     void *c[20];
