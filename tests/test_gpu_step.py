@@ -100,6 +100,25 @@ def test_routines_one_at_a_time_match_oracle(pkg, oracle, name, nsub):
         assert err <= TOL, (n, err)
 
 
+@pytest.mark.parametrize("scheme,calc_helium", [("thomas", 1), ("pcr", 1), ("thomas", 0)])
+def test_comp_n2_extrapolation_matches_oracle(pkg, oracle, scheme, calc_helium):
+    """A thin N2 top (N2 < 0.01 crossed inside every interior column) runs the
+    post pass's N2 extrapolation above the crossing (comp_ng.F90:419-431)."""
+    import torch
+    g = pkg.synth.make_grid("tall", nstep_sub=1, n2_top=0.003, calc_helium=calc_helium)
+    F = pkg.synth.make_state(g)
+    ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
+    n2 = 1. - ref["o2"] - ref["o1"] - ref["he"]
+    assert (n2[2:-2, 2:-2, -2:] < 0.01).all()
+    for fn in ("step", "step_serial"):
+        ns = pkg.step.NestState(g, F, scheme=scheme)
+        getattr(ns, fn)()
+        torch.cuda.synchronize()
+        for n in ("o2", "o1", "he", "o2_nm", "o1_nm", "he_nm"):
+            err = field_err(ns.fortran(n), ref[n])
+            assert err <= TOL, (fn, n, err)
+
+
 def test_auto_scheme_tunes_and_matches_oracle(pkg, oracle):
     """TGX_SCHEME_AUTO times both schemes on its first step, records a choice
     per routine, and its results stay within the FP64 bound."""

@@ -2280,6 +2280,7 @@ struct CompTerms {  // prelude planes: advection of O2, O, He, smoothed *_nm, an
                     // fs(:, isp, m) of comp_o2o_ng as fs01 fs02 fs11 fs12 fs00 fs10
   const double *o2adv, *o1adv, *headv, *o2s, *o1s, *hes;
   const double *fs[6];
+  double *zs;  // comp_kernel: the forward solution z, 3 planes (O2, O, He)
 };
 
 // fs(k, isp, m) of comp_o2o_ng.F90:44-81 at the levels comp_ng uses (k <
@@ -2526,69 +2527,129 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       for (int n = 0; n < 9; ++n) gama[n] = ng[n];
       for (int n = 0; n < 9; ++n) gs[(size_t)n * GP + G.a3(k, i, j)] = ng[n];
       for (int isp = 0; isp < 3; ++isp) zz[isp] = nz[isp];
-      out.o2[G.a3(k, i, j)] = nz[0];
-      out.o1[G.a3(k, i, j)] = nz[1];
-      out.he[G.a3(k, i, j)] = nz[2];
+      for (int isp = 0; isp < 3; ++isp) CT.zs[(size_t)isp * GP + G.a3(k, i, j)] = nz[isp];
       Qc = Qn;
       r1 = r2;
     }
-    // :317-332 back substitution (the next level's loads are in flight while
-    // this level is computed and stored) and the closed-form top level
-    double u1[3] = {0., 0., 0.};
-    struct BS { double z[3], gm[9]; };
+    // :317-332 back substitution (top down), the closed-form top level and,
+    // as each level's solution is known, the post pass :386-457 (time
+    // smoothing, N2 above the O-dominated top, clamps, normalisation).  The
+    // lowest level with N2 < 0.01 is known only at the bottom; the levels
+    // above it are then normalised again with the extrapolated N2, their
+    // solution recomputed by the same back substitution.
+    struct BS { double z[3], gm[9], o2nm, o1nm, henm, o2, o1, he, xn; };
     auto load_bs = [&](int k, BS &B) {
       size_t p = G.a3(k, i, j);
-      B.z[0] = out.o2[p]; B.z[1] = out.o1[p]; B.z[2] = out.he[p];
+      for (int isp = 0; isp < 3; ++isp) B.z[isp] = CT.zs[(size_t)isp * GP + p];
       for (int n = 0; n < 9; ++n) B.gm[n] = gs[(size_t)n * GP + p];
+      B.o2nm = in.o2_nm[p]; B.o1nm = in.o1_nm[p]; B.henm = in.he_nm[p];
+      B.o2 = in.o2[p]; B.o1 = in.o1[p]; B.he = in.he[p];
+      B.xn = in.xnmbar[p];
     };
-    BS bc, bn;
-    load_bs(nk - 2, bc);
-    for (int k = nk - 2; k >= 0; --k) {
-      if (k > 0) load_bs(k - 1, bn);
-      asm volatile("" ::: "memory");
-      size_t p = G.a3(k, i, j);
-      double u[3];
+    auto back = [&](const BS &B, const double *u1, double *u) {
       for (int isp = 0; isp < 3; ++isp) {
-        double uu = bc.z[isp];
-        for (int m = 0; m < 3; ++m) uu = uu - bc.gm[isp + 3 * m] * u1[m];
+        double uu = B.z[isp];
+        for (int m = 0; m < 3; ++m) uu = uu - B.gm[isp + 3 * m] * u1[m];
         u[isp] = uu;
       }
-      out.o2[p] = u[0]; out.o1[p] = u[1]; out.he[p] = u[2];
-      for (int isp = 0; isp < 3; ++isp) u1[isp] = u[isp];
+    };
+    auto top_of = [&](const double *u, double *t) {  // :317-332 from the solution at nk-2
+      for (int isp = 0; isp < 3; ++isp) t[isp] = (1. + .5 * epp[isp] * dz) / (1. - .5 * epp[isp] * dz) * u[isp];
+      t[1] = t[1] + (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * epp[1]));
+      t[2] = t[2] + (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * epp[2]));
+    };
+    auto put_norm = [&](size_t p, double o2u, double o1u, double heu, double n2) {
+      double a = o2u > small ? o2u : small, b = o1u > small ? o1u : small;
+      double c = heu > small ? heu : small, d = n2 > small ? n2 : small;
+      double norm = a + b + c + d;
+      out.o2[p] = a / norm;
+      out.o1[p] = b / norm;
+      out.he[p] = c / norm;
+    };
+    int kf = -1;  // lowest level so far with N2 < 0.01
+    double n2kf = 0., xnkf = 0., n2b = 0., xnb = 0.;
+    auto post = [&](int k, const double *u, const BS &B) {
+      size_t p = G.a3(k, i, j);
+      const double o2u = u[0], o1u = u[1], heu = g.calc_helium == 0 ? g.pshelb : u[2];
+      double o2nm = DTSMOOTH_DIV2 * (B.o2nm + o2u) + DTSMOOTH * B.o2;
+      double o1nm = DTSMOOTH_DIV2 * (B.o1nm + o1u) + DTSMOOTH * B.o1;
+      double henm = DTSMOOTH_DIV2 * (B.henm + heu) + DTSMOOTH * B.he;
+      double n2 = 1. - o2u - o1u - heu;
+      double n2nm = 1. - o2nm - o1nm - henm;
+      const bool lo = n2 < 0.01, below = !lo && kf == k + 1;  // (selects keep these in registers)
+      n2kf = lo ? n2 : n2kf;
+      xnkf = lo ? B.xn : xnkf;
+      kf = lo ? k : kf;
+      n2b = below ? n2 : n2b;
+      xnb = below ? B.xn : xnb;
+      put_norm(p, o2u, o1u, heu, n2);
+      double an = o2nm > small ? o2nm : small, bn = o1nm > small ? o1nm : small;
+      double cn = henm > small ? henm : small, dn = n2nm > small ? n2nm : small;
+      double normn = an + bn + cn + dn;
+      out.o2_nm[p] = an / normn;
+      out.o1_nm[p] = bn / normn;
+      out.he_nm[p] = cn / normn;
+    };
+    {
+      BS bc, bn, bt;
+      load_bs(nk - 2, bc);
+      load_bs(nk - 1, bt);
+      load_bs(nk - 3, bn);
+      asm volatile("" ::: "memory");
+      const double zero[3] = {0., 0., 0.};
+      double u1[3], u[3];
+      back(bc, zero, u1);  // level nk-2
+      double ut[3];
+      top_of(u1, ut);
+      post(nk - 1, ut, bt);
+      post(nk - 2, u1, bc);
       bc = bn;
+      for (int k = nk - 3; k >= 0; --k) {
+        if (k > 0) load_bs(k - 1, bn);
+        asm volatile("" ::: "memory");
+        back(bc, u1, u);
+        post(k, u, bc);
+        for (int isp = 0; isp < 3; ++isp) u1[isp] = u[isp];
+        bc = bn;
+      }
     }
-    double top[3];
-    for (int isp = 0; isp < 3; ++isp)
-      top[isp] = (1. + .5 * epp[isp] * dz) / (1. - .5 * epp[isp] * dz) * (isp == 0 ? F3(out.o2, nk - 2) : isp == 1 ? F3(out.o1, nk - 2) : F3(out.he, nk - 2));
-    top[1] = top[1] + (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * epp[1]));
-    top[2] = top[2] + (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * epp[2]));
-    size_t pt = G.a3(nk - 1, i, j);
-    out.o2[pt] = top[0]; out.o1[pt] = top[1]; out.he[pt] = top[2];
-    if (g.calc_helium == 0)
-      for (int k = 0; k < nk; ++k) out.he[G.a3(k, i, j)] = g.pshelb;
+    if (kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
+      const double zmtop = g.zpmid[kf];
+      const double n2top = log(n2kf * xnkf);
+      const double n2top1 = log(n2b * xnb);
+      const double dn2dz = (n2top - n2top1) * rdz;
+      BS bc;
+      const double zero[3] = {0., 0., 0.};
+      double u1[3], u[3];
+      load_bs(nk - 2, bc);
+      back(bc, zero, u1);
+      double t[3];
+      top_of(u1, t);
+      for (int k = nk - 1; k > kf; --k) {
+        if (k < nk - 2) {
+          load_bs(k, bc);
+          back(bc, u1, u);
+          for (int isp = 0; isp < 3; ++isp) u1[isp] = u[isp];
+        }
+        double uk[3];
+        for (int isp = 0; isp < 3; ++isp) uk[isp] = k == nk - 1 ? t[isp] : u1[isp];
+        const size_t p = G.a3(k, i, j);
+        const double xn = in.xnmbar[p];
+        const double n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / xn;
+        put_norm(p, uk[0], uk[1], g.calc_helium == 0 ? g.pshelb : uk[2], n2);
+      }
+    }
+    return;
   }
-  // :415-457 time smoothing, N2 above the O-dominated top, clamps,
-  // normalisation; the next level's loads are issued before this level's stores
+  // lateral-boundary columns: :415-457 on the Dirichlet values
   bool found = false, extrap = false;
   double n2top = 0., dn2dz = 0., zmtop = 0., n2prev = 0., xnprev = 0.;
-  struct PS { double o2u, o1u, heu, o2nm, o1nm, henm, o2, o1, he, xn; };
-  auto load_ps = [&](int k, PS &Q) {
-    size_t p = G.a3(k, i, j);
-    Q.o2u = out.o2[p]; Q.o1u = out.o1[p]; Q.heu = out.he[p];
-    Q.o2nm = in.o2_nm[p]; Q.o1nm = in.o1_nm[p]; Q.henm = in.he_nm[p];
-    Q.o2 = in.o2[p]; Q.o1 = in.o1[p]; Q.he = in.he[p];
-    Q.xn = in.xnmbar[p];
-  };
-  PS pc, pn;
-  load_ps(0, pc);
   for (int k = 0; k < nk; ++k) {
-    if (k + 1 < nk) load_ps(k + 1, pn);
-    asm volatile("" ::: "memory");
     size_t p = G.a3(k, i, j);
-    double o2u = pc.o2u, o1u = pc.o1u, heu = pc.heu;
-    double o2nm = DTSMOOTH_DIV2 * (pc.o2nm + o2u) + DTSMOOTH * pc.o2;
-    double o1nm = DTSMOOTH_DIV2 * (pc.o1nm + o1u) + DTSMOOTH * pc.o1;
-    double henm = DTSMOOTH_DIV2 * (pc.henm + heu) + DTSMOOTH * pc.he;
+    const double o2u = out.o2[p], o1u = out.o1[p], heu = out.he[p], xn = in.xnmbar[p];
+    double o2nm = DTSMOOTH_DIV2 * (in.o2_nm[p] + o2u) + DTSMOOTH * in.o2[p];
+    double o1nm = DTSMOOTH_DIV2 * (in.o1_nm[p] + o1u) + DTSMOOTH * in.o1[p];
+    double henm = DTSMOOTH_DIV2 * (in.he_nm[p] + heu) + DTSMOOTH * in.he[p];
     double n2 = 1. - o2u - o1u - heu;
     double n2nm = 1. - o2nm - o1nm - henm;
     if (!found) {
@@ -2597,16 +2658,16 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
         if (k + 1 < nk && k >= 1) {
           extrap = true;
           zmtop = g.zpmid[k];
-          n2top = log(n2 * pc.xn);
+          n2top = log(n2 * xn);
           double n2top1 = log(n2prev * xnprev);
           dn2dz = (n2top - n2top1) * rdz;
         }
       }
     } else if (extrap) {
-      n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / pc.xn;
+      n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / xn;
     }
     n2prev = n2;
-    xnprev = pc.xn;
+    xnprev = xn;
     double a = o2u > small ? o2u : small, b = o1u > small ? o1u : small;
     double c = heu > small ? heu : small, d = n2 > small ? n2 : small;
     double norm = a + b + c + d;
@@ -2619,7 +2680,6 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     out.o2_nm[p] = an / normn;
     out.o1_nm[p] = bn / normn;
     out.he_nm[p] = cn / normn;
-    pc = pn;
   }
 #undef F3
 }
@@ -3902,9 +3962,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     }
     case 8:
     case 14:
-    case 23: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6]}
-      CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6), {D(7), D(8), D(9), D(10), D(11), D(12)}};
+    case 23: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6], zs (comp_kernel)}
+      CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6), {D(7), D(8), D(9), D(10), D(11), D(12)}, nullptr};
       if (which == 8) {
+        CT.zs = D(13);
         if (g.capture)
           hipLaunchKernelGGL(comp_kernel<true>, grid, block, 0, s, g, G, in, out, D(0), CT);
         else

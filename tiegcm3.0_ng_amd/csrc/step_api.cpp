@@ -38,8 +38,8 @@ enum {
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
-  W_CO = W_UV + 11,                      // comp: 9 gamma, 6 prelude planes, 6 fs planes
-  W_NPLANES = W_CO + 21
+  W_CO = W_UV + 11,                      // comp: 9 gamma, 6 prelude, 6 fs, 3 forward-solution planes
+  W_NPLANES = W_CO + 24
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -313,9 +313,10 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_COMP, s);
-  void *b[13];
+  void *b[14];
   b[0] = plane(work, g, W_CO);
   for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
+  b[13] = plane(work, g, W_CO + 21);  // forward solution z (3 planes)
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
   if (!chem_done)
     if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng

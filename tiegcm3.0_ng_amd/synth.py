@@ -66,7 +66,10 @@ def bndcmp(dz):
     return b, fb
 
 
-def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor=1):
+def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor=1, n2_top=None):
+    """Grid constants of configuration `name`.  n2_top: if set, make_state
+    shifts O in the upper levels so that N2 = 1-O-O2-He falls to n2_top there
+    (the comp post pass then extrapolates N2 above its 0.01 level)."""
     nlon, nlat, nk, dlev, step, default_step, region = CONFIGS[name]
     nlond, nlatd = nlon + 4, nlat + 4
     dtr = math.pi / 180.
@@ -98,7 +101,7 @@ def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor
         cor=[2. * OMEGA * math.sin(x * dtr) for x in glat], racs=[1. / (RE * c) for c in cs],
         nstep_sub=nstep_sub, calc_helium=calc_helium, colfac=colfac, opdiffcap=opdiffcap,
         difhor=difhor, b=b, fb=fb, pshelb=0.1154E-5, ar_glbm=[1.2e-3, 1.1e-3],
-        step=step, seed=SEED_BASE + sorted(CONFIGS).index(name),
+        step=step, seed=SEED_BASE + sorted(CONFIGS).index(name), n2_top=n2_top,
     )
 
 
@@ -137,6 +140,12 @@ def make_state(grid, device="cpu", seed=None):
     o1 = (.01 + .92 / (1. + torch.exp(-zmid / 1.2))) * (1. + .02 * h2) * (1. + n3(1e-3))
     o2 = (.22 * torch.exp(-s / 3.) * (1. + .03 * h1) + .002) * (1. + n3(1e-3))
     he = 5e-6 * torch.exp(s / 1.6) * (1. + .1 * h1) * (1. + n3(1e-3))
+    if grid.get("n2_top") is not None:  # O takes up the N2 of the upper levels
+        # (and He's share too when He is not solved: comp then sets He to its
+        # lower-boundary value, so N2 = 1-O-O2 there)
+        top = torch.sigmoid((zmid - 3.5) / 0.5)
+        he_share = he if grid["calc_helium"] != 0 else 0.
+        o1 = o1 + top * (1. - o2 - o1 - he_share - grid["n2_top"])
     n2 = torch.clamp(1. - o2 - o1 - he, min=5e-3)
     F["o1"], F["o2"], F["he"], F["n2"] = o1, o2, he, n2
     for nm in ("o1", "o2", "he"):
