@@ -315,7 +315,7 @@ def main():
     dom = max(routines, key=lambda r: kt[r])
     names = pkg.step.COLUMN_KERNELS[dom][0]
     kname = names[1] if schemes[dom] == "pcr" else names[0]
-    alg = pkg.step.column_kernel_bytes_per_column_level(dom) * cols_all * nk
+    alg = pkg.step.column_kernel_bytes_per_column_level(dom, schemes[dom]) * cols_all * nk
     achieved = alg / (kt[dom] * 1e-3) / 1e9
     traffic = traffic_from_profiles(kname, args.config, schemes[dom])
     step_bytes = sum(pkg.step.alg_bytes_per_column_level(r) * cols_all * nk * (nsub if r == "oplus" else 1)

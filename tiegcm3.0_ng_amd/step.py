@@ -315,8 +315,9 @@ def alg_bytes_per_column_level(routine):
 # The column kernel of each routine: its name per scheme and the distinct
 # planes it must read and write once per launch (8 bytes per column-level
 # each).  Horizontal-stencil and chemistry terms arrive as planes from the
-# routine's level-parallel kernels; Thomas ratio / gamma planes are scratch,
-# not algorithmic.
+# routine's level-parallel kernels (except dt under Thomas: dt_tile_kernel
+# evaluates its stencils itself; its PCR kernel reads the two stencil planes,
+# PCR_EXTRA); Thomas ratio / gamma planes are scratch, not algorithmic.
 COLUMN_KERNELS = {
     "comp": (("comp_kernel", "comp_wave_kernel"),
              "o2 o1 he mbar tn w fs01 fs02 fs11 fs12 fs00 fs10 o2adv o1adv headv o2s o1s hes "
@@ -326,8 +327,8 @@ COLUMN_KERNELS = {
                "o1 o2 he mbar tn barm w_upd ftm1s_n2d hadvec_n2d ftm1s_ar hadvec_ar sloss_n2d "
                "s0prod_n2d n2d n2d_nm ar ar_nm".split(),
                "n2d n2d_nm ar ar_nm".split()),
-    "dt": (("dt_kernel", "dt_wave_kernel"),
-           "tn w_upd cp km un vn qtotal schti kt hadv_tn smooth_tn_nm xnmbar o1 rkm12 mbar scht "
+    "dt": (("dt_tile_kernel", "dt_wave_kernel"),
+           "tn w_upd cp km un vn qtotal schti kt xnmbar o1 rkm12 mbar scht "
            "hdt qji_tn cool_imp cool_exp tn_nm".split(),
            "tn tn_nm".split()),
     "duv": (("duv_stage2", "duv_wave_stage2"),
@@ -341,9 +342,13 @@ COLUMN_KERNELS = {
 }
 
 
-def column_kernel_bytes_per_column_level(routine):
+PCR_EXTRA = {"dt": ("hadv_tn", "smooth_tn_nm")}
+
+
+def column_kernel_bytes_per_column_level(routine, scheme="thomas"):
     _, ins, outs = COLUMN_KERNELS[routine]
-    return 8 * (len(set(ins)) + len(set(outs)))
+    extra = PCR_EXTRA.get(routine, ()) if scheme == "pcr" else ()
+    return 8 * (len(set(ins) | set(extra)) + len(set(outs)))
 
 
 def run_oracle_step(oracle, grid, Fo):
