@@ -287,7 +287,16 @@ struct StencilJob {
 struct StencilJobs {
   int n;
   StencilJob job[8];
+  const double *aux[3];  // STC_COMP: the fs(:,:,0) planes the right-hand sides take
 };
+// How a prelude stores its jobs: each job its own plane (STC_NONE), or the
+// smoothed and advected planes of one species combined into the part of the
+// right-hand side they form, in the reference's order of operations:
+//   STC_MINOR (jobs 0-3 smooth fcomp_tm1, 4-7 advect fcomp, minor_ng.F90:219):
+//     out[s] = smooth[s]*dtx2inv - advec[s]
+//   STC_COMP (jobs 0-2 advect O2/O/He, 3-5 smooth *_nm, comp_ng.F90:191-193):
+//     out[s] = smooth[s]*dtx2inv - (advec[s] - fs(s,0))   (He: no fs term)
+enum { STC_NONE = 0, STC_MINOR, STC_COMP };
 
 // Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
 // planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
@@ -319,7 +328,7 @@ struct StencilKinds {
   }
 };
 
-template <class KS>
+template <class KS, int COMB>
 __global__ void __launch_bounds__(256) stencil_run_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                           StencilJobs J) {
   extern __shared__ double smem[];
@@ -398,6 +407,7 @@ __global__ void __launch_bounds__(256) stencil_run_kernel(tgx_ng_grid g, Geo G, 
     if (inside) {
       const size_t p = (size_t)k * G.P + c2;
       int qs = 0;
+      double vv[NJ];
 #pragma unroll
       for (int q = 0; q < NJ; ++q) {
         const LdsPlane F{smem + q * TS, i0 - TILE_H, j0 - TILE_H};
@@ -434,7 +444,18 @@ __global__ void __launch_bounds__(256) stencil_run_kernel(tgx_ng_grid g, Geo G, 
           double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
           v = vh * (bm * bm);
         }
-        J.job[q].out[p] = v;
+        if (COMB == STC_NONE)
+          J.job[q].out[p] = v;
+        else
+          vv[q] = v;
+      }
+      if (COMB == STC_MINOR) {
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) J.job[sp].out[p] = vv[sp] * g.dtx2inv - vv[4 + sp];
+      } else if (COMB == STC_COMP) {
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp)
+          J.job[sp].out[p] = vv[3 + sp] * g.dtx2inv - (vv[sp] - (sp < 2 ? J.aux[sp][p] : 0.));
       }
     }
     __syncthreads();  // the next level overwrites the tiles and buffers
@@ -964,12 +985,13 @@ struct MinorShared {   // level quantities shared by the species (minor_ng.F90:6
   double pso1, pso2, pshe, dmdz, do1dz, do2dz, dhedz, tni, barm, inv[9], q;
 };
 struct MinorSpecies {  // per species and level
-  double ax, thdiff, sloss, s0prod, ftm1s, hadvec;
+  double ax, thdiff, sloss, s0prod, rhs;
 };
 
-struct MinorTerms {  // prelude planes by species id: smoothed fcomp_tm1, advection of
-                     // fcomp, and the loss / scaled production of minor_src_kernel
-  const double *ftm1s[4], *hadvec[4], *sloss[4], *s0prod[4];
+struct MinorTerms {  // planes by species id: the prelude's right-hand-side part
+                     // smooth(fcomp_tm1)*dtx2inv - advec(fcomp), and the loss /
+                     // scaled production of minor_src_kernel
+  const double *rhs[4], *sloss[4], *s0prod[4];
   // NO / N4S sources as chem_kernel's linear forms in this step's N2D
   // (minor_kernel<.., LIN = true>): lin[0..5] and the solved N2D
   const double *lin[6], *n2d;
@@ -1127,7 +1149,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
   const double xmbari = 1. / (bo2 * RIO2 + bo1 * RIO1 + (1. - bo2 - bo1 - bhe) * RIN2 + bhe * RIHE);
 
   struct MR { double o1, o2, he, mbar, tn, barm, w; };  // shared fields at one level
-  struct SR { double ftm1s, hadvec, sloss, s0prod, la, lb, n2d; };  // one species at one level
+  struct SR { double rhs, sloss, s0prod, la, lb, n2d; };  // one species at one level
   auto load_mr = [&](int L, MR &R) {
     size_t p = G.a3(L, i, j);
     R.o1 = in.o1[p]; R.o2 = in.o2[p]; R.he = in.he[p]; R.mbar = in.mbar[p]; R.tn = in.tn[p];
@@ -1137,8 +1159,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     size_t p = G.a3(L, i, j);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      Q[q].ftm1s = MT.ftm1s[SP[q]][p];
-      Q[q].hadvec = MT.hadvec[SP[q]][p];
+      Q[q].rhs = MT.rhs[SP[q]][p];
       if (SP[q] == 3) { Q[q].sloss = 0.; Q[q].s0prod = 0.; }
       else if (LIN && SP[q] == 0) {  // NO: sloss = lin2 + n2d lin3, s0prod = lin4 + n2d lin5
         Q[q].sloss = MT.lin[2][p]; Q[q].la = MT.lin[3][p];
@@ -1213,8 +1234,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     } else if (LIN && SP[q] == 1) {
       T.s0prod = R.s0prod + R.n2d * R.lb;
     }
-    T.ftm1s = R.ftm1s;
-    T.hadvec = R.hadvec;
+    T.rhs = R.rhs;
   };
   // lower-boundary data flbc(:,:,1:3) of each species (comp_*_ng.F90)
   double l1[2], l2[2], l3[2];
@@ -1288,7 +1308,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
         pc = molp * rdz - expzk * (eddyp * dfac + .5 * wic) * rdz;
         rc = molr * rdz - expzk * (eddyr * dfac - .5 * wic) * rdz;
         qc = -molq * rdz + expzk * (eddyq * dfac * rdz + dtx2inv - Tc[q].sloss);
-        fr = Tc[q].ftm1s * dtx2inv - Tc[q].hadvec + Tc[q].s0prod;
+        fr = Tc[q].rhs + Tc[q].s0prod;
         fr = fr * expzk;
         if (k == 0) {
           qc = qc + pc * (l1[q] + .5 * l2[q] * dzp) / (l1[q] - .5 * l2[q] * dzp);
@@ -2017,7 +2037,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
   wave_dn(S[0].tni, S[1].tni, lane, tnim[0], tnim[1]);
   wave_up(S[0].tni, S[1].tni, lane, tnip[0], tnip[1]);
   // species terms at level L
-  double ax[2][2], thd[2][2], slo[2][2], s0p[2][2], fts[2][2], hadv[2][2];
+  double ax[2][2], thd[2][2], slo[2][2], s0p[2][2], rhs[2][2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int L = Lr[h];
@@ -2044,8 +2064,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
       ax[h][q] = -(Q.barm / (TAU * RMN2) * pow(T00 / Q.tni, 0.25) /
                    (cq.phix[2] + cq.salfax[0] * Q.pso2 + cq.salfax[1] * Q.pso1 + cq.salfax[2] * Q.pshe));
       thd[h][q] = ex - cq.alfax * dtnidz / (dzp * Q.tni);
-      fts[h][q] = MT.ftm1s[SP[q]][p];
-      hadv[h][q] = MT.hadvec[SP[q]][p];
+      rhs[h][q] = MT.rhs[SP[q]][p];
       if (SP[q] == 3) { slo[h][q] = 0.; s0p[h][q] = 0.; }
       else { slo[h][q] = MT.sloss[SP[q]][p]; s0p[h][q] = MT.s0prod[SP[q]][p]; }
     }
@@ -2088,7 +2107,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
         pc = molp / dzp - expzk * (eddyp * dfac + .5 * wic) / dzp;
         rc = molr / dzp - expzk * (eddyr * dfac - .5 * wic) / dzp;
         qc = -molq / dzp + expzk * (eddyq * dfac / dzp + dtx2inv - slo[h][q]);
-        fr = fts[h][q] * dtx2inv - hadv[h][q] + s0p[h][q];
+        fr = rhs[h][q] + s0p[h][q];
         fr = fr * expzk;
         if (L == 0) {
           qc = qc + pc * (l1[q] + .5 * l2[q] * dzp) / (l1[q] - .5 * l2[q] * dzp);
@@ -2278,7 +2297,7 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
 // comp_o2o_ng.F90:44-81 (fused) + comp_ng.F90:71-457
 struct CompTerms {  // prelude planes: advection of O2, O, He, smoothed *_nm, and
                     // fs(:, isp, m) of comp_o2o_ng as fs01 fs02 fs11 fs12 fs00 fs10
-  const double *o2adv, *o1adv, *headv, *o2s, *o1s, *hes;
+  const double *rhs[3];  // the prelude's smooth(*_nm)*dtx2inv - (advec - fs(:,:,0)) of O2, O, He
   const double *fs[6];
   double *zs;  // comp_kernel: the forward solution z, 3 planes (O2, O, He)
 };
@@ -2362,7 +2381,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     // Fields and level constants read at the level being eliminated only.
     struct LS {
       double fs[2][3];
-      double o2adv, o1adv, headv, o2s, o1s, hes, hdo2, hdo1, hdhe, expz, difk0, difk1;
+      double rhs[3], hdo2, hdo1, hdhe, expz, difk0, difk1;
     };
     auto load_lr = [&](int L, LR &R) {
       size_t p = G.a3(L, i, j);
@@ -2372,9 +2391,8 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     auto load_ls = [&](int L, LS &S) {
       size_t p = G.a3(L, i, j);
       S.fs[0][1] = CT.fs[0][p]; S.fs[0][2] = CT.fs[1][p]; S.fs[1][1] = CT.fs[2][p];
-      S.fs[1][2] = CT.fs[3][p]; S.fs[0][0] = CT.fs[4][p]; S.fs[1][0] = CT.fs[5][p];
-      S.o2adv = CT.o2adv[p]; S.o1adv = CT.o1adv[p]; S.headv = CT.headv[p];
-      S.o2s = CT.o2s[p]; S.o1s = CT.o1s[p]; S.hes = CT.hes[p];
+      S.fs[1][2] = CT.fs[3][p];
+      S.rhs[0] = CT.rhs[0][p]; S.rhs[1] = CT.rhs[1][p]; S.rhs[2] = CT.rhs[2][p];
       S.hdo2 = in.hdo2[p]; S.hdo1 = in.hdo1[p]; S.hdhe = in.hdhe[p];
       S.expz = g.expz[L]; S.difk0 = g.difk[L]; S.difk1 = g.difk[L + 1];
     };
@@ -2475,9 +2493,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
           rk[e] = molr[e] * rdz - expzk * delta * rdz * (eddyr * dfac - .5 * wik);
           qk[e] = -molq[e] * rdz + expzk * delta * (eddyq * dfac * rdz + dtx2inv) - expzk * fsv;
         }
-      fk[0] = expzk * (ls.o2s * dtx2inv - (ls.o2adv - fs[0][0]) + ls.hdo2);
-      fk[1] = expzk * (ls.o1s * dtx2inv - (ls.o1adv - fs[1][0]) + ls.hdo1);
-      fk[2] = expzk * (ls.hes * dtx2inv - (ls.headv - 0.) + ls.hdhe);
+      fk[0] = expzk * (ls.rhs[0] + ls.hdo2);
+      fk[1] = expzk * (ls.rhs[1] + ls.hdo1);
+      fk[2] = expzk * (ls.rhs[2] + ls.hdhe);
       if (k == 0) {
         for (int m = 0; m < 3; ++m)
           for (int n = 0; n < 3; ++n)
@@ -2808,7 +2826,7 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       const double expzk = g.expz[k];
       double fs[2][3];
       fs[0][1] = CT.fs[0][p]; fs[0][2] = CT.fs[1][p]; fs[1][1] = CT.fs[2][p];
-      fs[1][2] = CT.fs[3][p]; fs[0][0] = CT.fs[4][p]; fs[1][0] = CT.fs[5][p];
+      fs[1][2] = CT.fs[3][p];
       const double *akm = ak[h], *epm = ep[h], *akpp = akp[h], *eppp = epp[h];
       const double wks3 = wks[h], wks4 = wksp[h];
       double pk[9], rk[9], qk[9], fk[3];
@@ -2828,9 +2846,9 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
           rk[e] = molr / dz - expzk * delta / dz * (eddyr * dfac - .5 * wik);
           qk[e] = -molq / dz + expzk * delta * (eddyq * dfac / dz + dtx2inv) - expzk * fsv;
         }
-      fk[0] = expzk * (CT.o2s[p] * dtx2inv - (CT.o2adv[p] - fs[0][0]) + in.hdo2[p]);
-      fk[1] = expzk * (CT.o1s[p] * dtx2inv - (CT.o1adv[p] - fs[1][0]) + in.hdo1[p]);
-      fk[2] = expzk * (CT.hes[p] * dtx2inv - (CT.headv[p] - 0.) + in.hdhe[p]);
+      fk[0] = expzk * (CT.rhs[0][p] + in.hdo2[p]);
+      fk[1] = expzk * (CT.rhs[1][p] + in.hdo1[p]);
+      fk[2] = expzk * (CT.rhs[2][p] + in.hdhe[p]);
       if (k == 0) {
         for (int m = 0; m < 3; ++m)
           for (int n = 0; n < 3; ++n)
@@ -3793,14 +3811,14 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
 // Launcher used by step_api.cpp.  bufs holds the workspace planes (and for
 // O+ the input/output state of the sub-step) in the order documented there.
 // Stencil prelude launch: a run of levels of one tile per block.
-template <class KS>
+template <class KS, int COMB = tgx::ng::STC_NONE>
 static void launch_stencil_run(const tgx_ng_grid &g, const tgx::ng::Geo &G, const tgx_ng_fields &in,
                                const tgx::ng::StencilJobs &J, hipStream_t s) {
   using namespace tgx::ng;
   const int ntx = (G.nlond + TILE_X - 1) / TILE_X, nty = (G.nlatd + TILE_Y - 1) / TILE_Y;
   const int nrun = (G.nk + STENCIL_RUN - 1) / STENCIL_RUN;
-  hipLaunchKernelGGL(stencil_run_kernel<KS>, dim3((unsigned)(ntx * nty * nrun)), dim3(256), KS::lds_bytes(), s, g,
-                     G, in, J);
+  hipLaunchKernelGGL((stencil_run_kernel<KS, COMB>), dim3((unsigned)(ntx * nty * nrun)), dim3(256), KS::lds_bytes(),
+                     s, g, G, in, J);
 }
 
 
@@ -3870,7 +3888,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 27: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
                 // NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO after N2D's solve
       MinorTerms MT{};
-      for (int q = 0; q < 4; ++q) { MT.ftm1s[q] = D(2 + q); MT.hadvec[q] = D(6 + q); }
+      for (int q = 0; q < 4; ++q) MT.rhs[q] = D(2 + q);
       MT.sloss[0] = MT.sloss[2] = D(10);
       MT.s0prod[0] = MT.s0prod[2] = D(11);
       MT.sloss[1] = D(12);
@@ -3920,7 +3938,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       } else if (which == 4) {
         hipLaunchKernelGGL(oplus_stage2, grid, block, 0, s, g, G, in, W);
       } else if (which == 9) {
-        StencilJobs J;
+        StencilJobs J{};
         J.n = 6;
         J.job[0] = job(ST_BDOTDH, W.bdopj, D(13), 0.);
         J.job[1] = job(ST_BDOTDH, W.diffj, D(14), 0.);
@@ -3962,8 +3980,8 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     }
     case 8:
     case 14:
-    case 23: {  // comp: bufs = {gamma, o2adv, o1adv, headv, o2s, o1s, hes, fs[6], zs (comp_kernel)}
-      CompTerms CT{D(1), D(2), D(3), D(4), D(5), D(6), {D(7), D(8), D(9), D(10), D(11), D(12)}, nullptr};
+    case 23: {  // comp: bufs = {gamma, rhs[3], (unused x3), fs[6], zs (comp_kernel)}
+      CompTerms CT{{D(1), D(2), D(3)}, {D(7), D(8), D(9), D(10), D(11), D(12)}, nullptr};
       if (which == 8) {
         CT.zs = D(13);
         if (g.capture)
@@ -3979,7 +3997,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       break;
     }
     case 10: {  // dt prelude: bufs = {w2, hadv_tn, smooth_tn_nm}
-      StencilJobs J;
+      StencilJobs J{};
       J.n = 2;
       J.job[0] = job(ST_ADVEC, in.tn, D(1), 0.);
       J.job[1] = job(ST_SMOOTH, in.tn_nm, D(2), g.shapiro);
@@ -3987,19 +4005,19 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       break;
     }
     case 11: {  // minors prelude (species 0..3 = NO, N4S, N2D, AR)
-      StencilJobs J;
+      StencilJobs J{};
       J.n = 8;
       const double *fc[4] = {in.no, in.n4s, in.n2d, in.ar};
       const double *fm[4] = {in.no_nm, in.n4s_nm, in.n2d_nm, in.ar_nm};
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < 4; ++q) {  // outputs: the right-hand-side part of each species
         J.job[q] = job(ST_SMOOTH, fm[q], D(2 + q), g.shapiro);
-        J.job[4 + q] = job(ST_ADVEC, fc[q], D(6 + q), 0.);
+        J.job[4 + q] = job(ST_ADVEC_R, fc[q], nullptr, 0.);
       }
-      launch_stencil_run<KindsMinor>(g, G, in, J, s);
+      launch_stencil_run<KindsMinor, STC_MINOR>(g, G, in, J, s);
       break;
     }
     case 12: {  // comp prelude
-      StencilJobs J;
+      StencilJobs J{};
       J.n = 6;
       J.job[0] = job(ST_ADVEC, in.o2, D(1), 0.);
       J.job[1] = job(ST_ADVEC, in.o1, D(2), 0.);
@@ -4007,11 +4025,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.job[3] = job(ST_SMOOTH, in.o2_nm, D(4), g.shapiro);
       J.job[4] = job(ST_SMOOTH, in.o1_nm, D(5), g.shapiro);
       J.job[5] = job(ST_SMOOTH, in.he_nm, D(6), g.shapiro);
-      launch_stencil_run<KindsComp>(g, G, in, J, s);
+      J.aux[0] = D(11);  // fs(:,1,0), fs(:,2,0) of comp_o2o_ng
+      J.aux[1] = D(12);
+      J.aux[2] = nullptr;
+      launch_stencil_run<KindsComp, STC_COMP>(g, G, in, J, s);
       break;
     }
     case 13: {  // duv prelude (after stage 1 wrote ztmp)
-      StencilJobs J;
+      StencilJobs J{};
       J.n = 6;
       J.job[0] = job(ST_DIVX, D(0), D(2), 0.);
       J.job[1] = job(ST_DIVY, in.z, D(3), 0.);

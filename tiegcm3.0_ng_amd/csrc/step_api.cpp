@@ -317,9 +317,11 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   b[0] = plane(work, g, W_CO);
   for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
   b[13] = plane(work, g, W_CO + 21);  // forward solution z (3 planes)
-  if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
+  // comp_o2o_ng's fs planes first: the prelude folds fs(:,:,0) into the
+  // right-hand sides
   if (!chem_done)
-    if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;  // comp_o2o_ng
+    if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;
+  if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_COMP, s);
   return tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
 }

@@ -319,12 +319,12 @@ def alg_bytes_per_column_level(routine):
 # evaluates its stencils itself; its PCR kernel reads the two stencil planes,
 # PCR_EXTRA); Thomas ratio / gamma planes are scratch, not algorithmic.
 COLUMN_KERNELS = {
-    "comp": (("comp_kernel", "comp_wave_kernel"),
-             "o2 o1 he mbar tn w fs01 fs02 fs11 fs12 fs00 fs10 o2adv o1adv headv o2s o1s hes "
+    "comp": (("comp_kernel", "comp_wave_kernel"),  # rhs_*: the prelude's smoothed/advected part
+             "o2 o1 he mbar tn w fs01 fs02 fs11 fs12 rhs_o2 rhs_o1 rhs_he "
              "hdo2 hdo1 hdhe o2_nm o1_nm he_nm xnmbar".split(),
              "o2 o1 he o2_nm o1_nm he_nm".split()),
     "minors": (("minor_kernel<2, 3>", "minor_wave_kernel<2, 3>"),  # N2D + Ar
-               "o1 o2 he mbar tn barm w_upd ftm1s_n2d hadvec_n2d ftm1s_ar hadvec_ar sloss_n2d "
+               "o1 o2 he mbar tn barm w_upd rhs_n2d rhs_ar sloss_n2d "
                "s0prod_n2d n2d n2d_nm ar ar_nm".split(),
                "n2d n2d_nm ar ar_nm".split()),
     "dt": (("dt_tile_kernel", "dt_wave_kernel"),
