@@ -119,6 +119,30 @@ def test_comp_n2_extrapolation_matches_oracle(pkg, oracle, scheme, calc_helium):
             assert err <= TOL, (fn, n, err)
 
 
+ALL_FIELDS = ("op", "op_nm", "xiop2p", "xiop2d", "Fe", "Fn", "n2d", "n2d_nm", "ar", "ar_nm", "n4s", "n4s_nm",
+              "no", "no_nm", "tn", "tn_nm", "un", "un_nm", "vn", "vn_nm", "o2", "o2_nm", "o1", "o1_nm", "he",
+              "he_nm")
+
+
+@pytest.mark.parametrize("scheme", ["thomas", "pcr"])
+def test_step_run_switches_match_oracle(pkg, oracle, scheme):
+    """The run switches away from their defaults: no helium solve
+    (calc_helium=0), the O+ diffusion cap binding on the upper levels
+    (opdiffcap=1e8, oplus_ng.F90:145-148), no horizontal-diffusion latitude
+    factor (difhor=0), another O-N2 collision factor, two O+ sub-cycles."""
+    import torch
+    g = pkg.synth.make_grid("small", nstep_sub=2, calc_helium=0, colfac=1.7, opdiffcap=1e8, difhor=0)
+    F = pkg.synth.make_state(g)
+    ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
+    for fn in ("step", "step_serial"):
+        ns = pkg.step.NestState(g, F, scheme=scheme)
+        getattr(ns, fn)()
+        torch.cuda.synchronize()
+        for n in ALL_FIELDS:
+            err = field_err(ns.fortran(n), ref[n])
+            assert err <= TOL, (fn, n, err)
+
+
 def test_auto_scheme_tunes_and_matches_oracle(pkg, oracle):
     """TGX_SCHEME_AUTO times both schemes on its first step, records a choice
     per routine, and its results stay within the FP64 bound."""
