@@ -3651,6 +3651,325 @@ __global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_f
 #undef F3
 }
 
+// duv_ng in one pass per column tile: the vertical integral ztmp of stage 1
+// (:73-80), the horizontal stencils (divergence of ztmp and z for the
+// pressure gradient, Shapiro smoothing of UN_NM/VN_NM, advection of UN/VN)
+// and the 2x2 block elimination of stage 2, level by level.  A block covers a
+// TILE_X x TILE_Y tile; at level k the tile and its 2-point halo of UN, VN,
+// UN_NM, VN_NM (level k) and of z and ztmp (level k+1) are staged through
+// double-buffered LDS.  ztmp is integrated upward for every tile and halo
+// point by the thread that stages it (the halo points are integrated by each
+// block that needs them, in the same arithmetic as stage 1).  Every thread of
+// the block takes part in the staging and the barriers; lateral-boundary and
+// out-of-domain columns do not solve.
+template <bool CAP>
+__global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                       tgx_ng_out out, double *__restrict__ gs) {
+  constexpr int TS = TILE_R * TILE_W, NF = 6, TL = TILE_Y * TILE_W;
+  enum { T_UN = 0, T_VN, T_UNM, T_VNM, T_Z, T_ZT };
+  __shared__ double tb[2][NF][TS];
+  __shared__ double tls[2][TL];  // latitude passes of UN_NM, VN_NM
+  const int ntx = (G.nlond + TILE_X - 1) / TILE_X;
+  const int bb = tgx_xcd_block(gridDim.x);
+  const int i0 = (bb % ntx) * TILE_X, j0 = (bb / ntx) * TILE_Y;
+  const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
+  const bool active = i < G.nlond && j < G.nlatd;
+  const bool lat = active && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
+  const bool solve = active && !lat;
+  const int nk = G.nk;
+  const double dz = g.dz, dtx2inv = g.dtx2inv, wt = 0.225;
+  // tile slots of this thread: e = tid and tid + 256
+  int te[2];
+  bool tok[2];
+  size_t toff[2];
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {
+    const int e = (int)threadIdx.x + sl * 256;
+    const int ii = i0 - TILE_H + e % TILE_W, jj = j0 - TILE_H + e / TILE_W;
+    te[sl] = e;
+    tok[sl] = e < TS && ii >= 0 && ii < G.nlond && jj >= 0 && jj < G.nlatd;
+    toff[sl] = tok[sl] ? (size_t)jj * G.ld + ii : 0;
+  }
+  // staged values of level k: UN, VN, UN_NM, VN_NM at k, z at k+1, and the
+  // stage-1 inputs at k from which ztmp(k+1) follows
+  double st[2][NF], zt[2] = {0., 0.};
+  auto tile_load = [&](int k) {
+    const size_t o = (size_t)k * G.P, o1 = o + G.P;
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl)
+      if (tok[sl]) {
+        const size_t q = o + toff[sl];
+        st[sl][T_UN] = in.un[q]; st[sl][T_VN] = in.vn[q];
+        st[sl][T_UNM] = in.un_nm[q]; st[sl][T_VNM] = in.vn_nm[q];
+        st[sl][T_Z] = in.z[o1 + toff[sl]];
+        // duv_stage1 at this point: ztmp(k+1) = ztmp(k) + dz/dzgrav*tbar/mbar
+        const double tn = in.tn[q];
+        const double tbar = (tn * (-2.) + in.tn_nm[q] + out.tn[q]) * wt + tn;
+        const double dztbar = dz / DZGRAV * tbar / in.mbar[q];
+        st[sl][T_ZT] = dztbar + zt[sl];
+      }
+  };
+  auto tile_store = [&](int buf) {
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl)
+      if (tok[sl]) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) tb[buf][f][te[sl]] = st[sl][f];
+        zt[sl] = st[sl][T_ZT];
+      }
+  };
+  const int ic = active ? i : 0, jc = active ? j : 0;
+  const size_t c2 = G.a2(ic, jc);
+  if (active) {
+    out.ulbc_nm[c2] = in.ulbc[c2];
+    out.vlbc_nm[c2] = in.vlbc[c2];
+    out.ulbc[c2] = in.u_lbc[c2];
+    out.vlbc[c2] = in.v_lbc[c2];
+  }
+  if (lat) {
+    double vu, vv;
+    for (int k = 0; k < nk; ++k) {
+      size_t p = G.a3(k, i, j);
+      lateral_value(G, in.lonb_un, in.latb_un, k, i, j, vu);
+      lateral_value(G, in.lonb_vn, in.latb_vn, k, i, j, vv);
+      out.un[p] = vu;
+      out.vn[p] = vv;
+      out.un_nm[p] = DTSMOOTH_DIV2 * (in.un_nm[p] + vu) + DTSMOOTH * in.un[p];
+      out.vn_nm[p] = DTSMOOTH_DIV2 * (in.vn_nm[p] + vv) + DTSMOOTH * in.vn[p];
+    }
+  }
+#define F3(f, k) (f)[G.a3((k), ic, jc)]
+  const double racs = g.racs[jc], cor = g.cor[jc], tanphi = g.tanphi[jc];
+  const LdsPlane P0z{tb[1][T_Z], i0 - TILE_H, j0 - TILE_H};
+  // level 0 of z (= ztmp(0)) for the divergences at level 0
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl)
+    if (tok[sl]) {
+      const double z0 = in.z[toff[sl]];
+      tb[1][T_Z][te[sl]] = z0;
+      zt[sl] = z0;
+    }
+  tile_load(0);
+  __syncthreads();
+  double divx_c = 0., divy_c = 0.;
+  if (solve) {
+    divx_c = divx(P0z, G, i, j);
+    divy_c = divy(P0z, G, i, j);
+  }
+  tile_store(0);  // buffer 0; buffer 1 is read above before the next barrier
+  // stage-2 fields at one level (as in duv_stage2; the divergences come from
+  // the tiles)
+  struct UL { double w, lxx, lyy, lxy, lyx, ui, vi, km, schti, scht; };
+  struct US { double Fe, Fn, hdu, hdv; };
+  auto load_ul = [&](int L, UL &U) {
+    size_t p = G.a3(L, ic, jc);
+    U.w = in.w_upd[p]; U.lxx = in.lxx[p]; U.lyy = in.lyy[p]; U.lxy = in.lxy[p]; U.lyx = in.lyx[p];
+    U.ui = in.ui[p]; U.vi = in.vi[p]; U.km = in.km[p]; U.schti = in.schti[p]; U.scht = in.scht[p];
+  };
+  auto load_us = [&](int L, US &S) {
+    size_t p = G.a3(L, ic, jc);
+    S.Fe = out.Fe[p]; S.Fn = out.Fn[p]; S.hdu = in.hdu[p]; S.hdv = in.hdv[p];
+  };
+  auto e1b = [&](double scht, int L) -> double {
+    double e = P0 * scht / GRAV;
+    return e * g.expz[L] * g.xmue[L];
+  };
+  auto gg = [&](const UL &U, double evf) { return GRAV * (U.km + evf) / (P0 * U.schti * (dz * dz)); };
+  const double ulbc = solve ? in.u_lbc[c2] : 0., vlbc = solve ? in.v_lbc[c2] : 0.;
+  UL ua = {}, ub = {};
+  double e1_m = 0., e1_c = 0., gk = 0.;
+  if (solve) {
+    load_ul(0, ua);
+    load_ul(1, ub);
+    e1_c = e1b(ua.scht, 0);
+    const double e1_1 = e1b(ub.scht, 1);
+    gk = gg(ua, e1_c * e1_c / sqrt(e1_c * e1_1));
+  }
+  double be11 = 0, be12 = 0, be21 = 0, be22 = 0, y1 = 0, y2 = 0;
+  double r11p = 0, r12p = 0, r21p = 0, r22p = 0;
+  const size_t NP = (size_t)nk * G.P;
+  for (int k = 0; k <= nk - 2; ++k) {
+    US us = {};
+    if (k + 1 <= nk - 2) tile_load(k + 1);
+    if (solve) {
+      if (k > 0) load_ul(k + 1, ub);
+      load_us(k, us);
+    }
+    asm volatile("" ::: "memory");
+    __syncthreads();  // the level-k tiles are in buffer k&1
+    const int bk = k & 1;
+    const LdsPlane Tun{tb[bk][T_UN], i0 - TILE_H, j0 - TILE_H}, Tvn{tb[bk][T_VN], i0 - TILE_H, j0 - TILE_H},
+        Tunm{tb[bk][T_UNM], i0 - TILE_H, j0 - TILE_H}, Tvnm{tb[bk][T_VNM], i0 - TILE_H, j0 - TILE_H},
+        Tz{tb[bk][T_Z], i0 - TILE_H, j0 - TILE_H}, Tzt{tb[bk][T_ZT], i0 - TILE_H, j0 - TILE_H};
+    // smooth_ng.F90:283-294: latitude passes on the tile rows and their longitude halo
+    for (int e = threadIdx.x; e < TL; e += blockDim.x) {
+      int ii = i0 - TILE_H + e % TILE_W, jj = j0 + e / TILE_W;
+      if (ii >= 0 && ii < G.nlond && jj < G.nlatd) {
+        tls[0][e] = smooth_lat(Tunm, G, ii, jj, g.shapiro);
+        tls[1][e] = smooth_lat(Tvnm, G, ii, jj, g.shapiro);
+      }
+    }
+    __syncthreads();
+    if (solve) {
+      // :296-307 longitude passes, advec_ng of UN and VN, and the divergences at k+1
+      double sm[2];
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        const double *row = tls[f] + (j - j0) * TILE_W + TILE_H - i0;
+        const double s_ = g.shapiro;
+        int L1 = G.nlond - 1;
+        bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+        int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
+        int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
+        double t0 = row[i];
+        double fm1 = (row[i1p] + row[i1m]) * 0.5;
+        double fm2 = (row[i2p] + row[i2m]) * 0.5;
+        double gen = t0 - s_ * (2. * fm2 - 8. * fm1 + 6. * t0);
+        double nr = t0 - s_ * (-8. * fm1 + 8. * t0);
+        sm[f] = edge ? t0 : (near ? nr : gen);
+      }
+      const double unms = sm[0], vnms = sm[1];
+      const double advu = advec(Tun, Tun, Tvn, G, i, j, racs, g.dlamda, g.dphi);
+      const double advv = advec(Tvn, Tun, Tvn, G, i, j, racs, g.dlamda, g.dphi);
+      const double divx_n = divx(Tzt, G, i, j), divy_n = divy(Tz, G, i, j);
+      const double un_k = pv(Tun, G, i, j);
+      size_t p = G.a3(k, i, j);
+      double e1_n = k + 1 <= nk - 2 ? e1b(ub.scht, k + 1) : sqrt(e1_c * e1_c * e1_c / e1_m);
+      double gk1 = gg(ub, sqrt(e1_c * e1_n));
+      double g_1 = gk1;
+      const double expzk = g.expz[k];
+      double wi = .5 * (ua.w + ub.w);
+      double dwdz = wi / (2. * dz);
+      dwdz = dwdz * expzk;
+      double lxxi = .5 * (ua.lxx + ub.lxx), lyyi = .5 * (ua.lyy + ub.lyy);
+      double lxyi = .5 * (ua.lxy + ub.lxy), lyxi = .5 * (ua.lyx + ub.lyx);
+      double uii = .5 * (ua.ui + ub.ui), vii = .5 * (ua.vi + ub.vi);
+      double udc = us.Fe + lxxi * uii + lxyi * vii;
+      double vdc = us.Fn + lyyi * vii - lyxi * uii;
+      double zl = (divx_c + divx_n) * .5;
+      zl = zl * racs;
+      zl = zl * GRAV / g.dlamda;
+      double zp = (divy_c + divy_n) * .5;
+      zp = zp * GRAV * RE_INV / g.dphi;
+      double s1 = dtx2inv * unms + us.hdu + udc - advu - zl;
+      double s2 = dtx2inv * vnms + us.hdv + vdc - advv - zp;
+      s1 = s1 * expzk;
+      s2 = s2 * expzk;
+      double p11 = -(gk + dwdz), p12 = 0., p21 = 0., p22 = p11;
+      double r11 = -(g_1 - dwdz), r12 = 0., r21 = 0., r22 = r11;
+      double q11 = gk + g_1;
+      double qa = dtx2inv + lxxi;
+      qa = qa * expzk;
+      double qq11 = q11 + qa;
+      qa = dtx2inv + lyyi;
+      qa = qa * expzk;
+      double qq22 = q11 + qa;
+      double q12 = cor + un_k / RE * tanphi;
+      double q21 = q12;
+      qa = q12 - lxyi;
+      qa = qa * expzk;
+      double qq12 = -qa;
+      qa = q21 - lyxi;
+      qa = qa * expzk;
+      double qq21 = qa;
+      if (k == 0) {  // :207-212
+        qq11 = qq11 - p11; qq12 = qq12 - p12; qq21 = qq21 - p21; qq22 = qq22 - p22;
+        s1 = s1 - 2. * (p11 * ulbc + p12 * vlbc);
+        s2 = s2 - 2. * (p21 * ulbc + p22 * vlbc);
+        p11 = 0.; p12 = 0.; p21 = 0.; p22 = 0.;
+      }
+      if (k == nk - 2) {  // :214-215
+        qq11 = qq11 + r11; qq12 = qq12 + r12; qq21 = qq21 + r21; qq22 = qq22 + r22;
+        r11 = 0.; r12 = 0.; r21 = 0.; r22 = 0.;
+      }
+      if (CAP) {
+        const double Ab[4] = {p11, p12, p21, p22}, Bb[4] = {qq11, qq12, qq21, qq22}, Cb[4] = {r11, r12, r21, r22};
+        for (int e = 0; e < 4; ++e) {
+          cap_put(g, G, CAP_UV + e, i, j, k, Ab[e]);
+          cap_put(g, G, CAP_UV + 4 + e, i, j, k, Bb[e]);
+          cap_put(g, G, CAP_UV + 8 + e, i, j, k, Cb[e]);
+        }
+        cap_put(g, G, CAP_UV + 12, i, j, k, s1);
+        cap_put(g, G, CAP_UV + 13, i, j, k, s2);
+      }
+      if (k == 0) {
+        double det = qq11 * qq22 - qq12 * qq21;
+        be11 = qq22 / det; be12 = -qq12 / det; be21 = -qq21 / det; be22 = qq11 / det;
+        y1 = be11 * s1 + be12 * s2;
+        y2 = be21 * s1 + be22 * s2;
+      } else {
+        double g11 = be11 * r11p + be12 * r21p;
+        double g12 = be11 * r12p + be12 * r22p;
+        double g21 = be21 * r11p + be22 * r21p;
+        double g22 = be21 * r12p + be22 * r22p;
+        size_t pm = G.a3(k - 1, i, j);
+        gs[pm] = g11; gs[pm + NP] = g21; gs[pm + 2 * NP] = g12; gs[pm + 3 * NP] = g22;
+        double h11 = qq11 - p11 * g11 - p12 * g21;
+        double h12 = qq12 - p11 * g12 - p12 * g22;
+        double h21 = qq21 - p21 * g11 - p22 * g21;
+        double h22 = qq22 - p21 * g12 - p22 * g22;
+        double dk = h11 * h22 - h12 * h21;
+        be11 = h22 / dk; be12 = -h12 / dk; be21 = -h21 / dk; be22 = h11 / dk;
+        double x1 = s1 - p11 * y1 - p12 * y2;
+        double x2 = s2 - p21 * y1 - p22 * y2;
+        y1 = be11 * x1 + be12 * x2;
+        y2 = be21 * x1 + be22 * x2;
+      }
+      out.un[p] = y1;
+      out.vn[p] = y2;
+      r11p = r11; r12p = r12; r21p = r21; r22p = r22;
+      gk = gk1;
+      e1_m = e1_c;
+      e1_c = e1_n;
+      ua = ub;
+      divx_c = divx_n;
+      divy_c = divy_n;
+    }
+    if (k + 1 <= nk - 2) tile_store((k + 1) & 1);
+  }
+  if (!solve) return;
+  // back substitution (:244-250) and outputs (:320-321), as in duv_stage2
+  struct UB { double x1, x2, g11, g21, g12, g22, unm, un, vnm, vn; };
+  auto load_ub = [&](int k, UB &B) {
+    size_t p = G.a3(k, i, j);
+    B.x1 = out.un[p]; B.x2 = out.vn[p];
+    if (k < nk - 2) { B.g11 = gs[p]; B.g21 = gs[p + NP]; B.g12 = gs[p + 2 * NP]; B.g22 = gs[p + 3 * NP]; }
+    else { B.g11 = B.g21 = B.g12 = B.g22 = 0.; }
+    B.unm = in.un_nm[p]; B.un = in.un[p]; B.vnm = in.vn_nm[p]; B.vn = in.vn[p];
+  };
+  const double unm_top = F3(in.un_nm, nk - 1), un_top = F3(in.un, nk - 1);
+  const double vnm_top = F3(in.vn_nm, nk - 1), vn_top = F3(in.vn, nk - 1);
+  double xn1 = y1, xn2 = y2;
+  UB bc, bn;
+  load_ub(nk - 2, bc);
+  for (int k = nk - 2; k >= 0; --k) {
+    if (k > 0) load_ub(k - 1, bn);
+    asm volatile("" ::: "memory");
+    double x1, x2;
+    if (k == nk - 2) { x1 = xn1; x2 = xn2; }
+    else {
+      x1 = bc.x1 - bc.g11 * xn1 - bc.g12 * xn2;
+      x2 = bc.x2 - bc.g21 * xn1 - bc.g22 * xn2;
+    }
+    if (k == nk - 3) {
+      size_t pt = G.a3(nk - 1, i, j);
+      double u = 2 * xn1 - x1, vv = 2 * xn2 - x2;
+      out.un[pt] = u;
+      out.vn[pt] = vv;
+      out.un_nm[pt] = DTSMOOTH_DIV2 * (unm_top + u) + DTSMOOTH * un_top;
+      out.vn_nm[pt] = DTSMOOTH_DIV2 * (vnm_top + vv) + DTSMOOTH * vn_top;
+    }
+    size_t p = G.a3(k, i, j);
+    out.un[p] = x1;
+    out.vn[p] = x2;
+    out.un_nm[p] = DTSMOOTH_DIV2 * (bc.unm + x1) + DTSMOOTH * bc.un;
+    out.vn_nm[p] = DTSMOOTH_DIV2 * (bc.vnm + x2) + DTSMOOTH * bc.vn;
+    xn1 = x1; xn2 = x2;
+    bc = bn;
+  }
+#undef F3
+}
+
 // duv_stage2 with one wavefront per column: the 2x2 block rows 0..nk-2 by
 // block cyclic reduction, the top level extrapolated (:250).
 __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
@@ -3869,6 +4188,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C);
       } else
         hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C, D(17), D(18), D(19));
+      break;
+    }
+    case 28: {  // duv with stage 1 and the stencils fused: bufs = {ztmp (unused), gamma}
+      dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
+      if (g.capture)
+        hipLaunchKernelGGL(duv_tile_kernel<true>, gridt, block, 0, s, g, G, in, out, D(1));
+      else
+        hipLaunchKernelGGL(duv_tile_kernel<false>, gridt, block, 0, s, g, G, in, out, D(1));
       break;
     }
     case 24: {  // dt with the stencils fused: one block per column tile

@@ -134,7 +134,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
-       K_MIN_B_LIN };
+       K_MIN_B_LIN, K_UV_TILE };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -302,10 +302,14 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   b[0] = plane(work, g, W_UV);
   b[1] = plane(work, g, W_UV + 1);
   for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 5 + q);
+  if (!use_pcr(g, R_DUV)) {  // Thomas: stage 1 and the stencils are fused into the column kernel
+    ProfScope kern(R_N + R_DUV, s);
+    return tgx_ng_launch(K_UV_TILE, g, in, out, b, 0, s);
+  }
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_DUV, s);
-  return tgx_ng_launch(use_pcr(g, R_DUV) ? K_UV2_W : K_UV2, g, in, out, b, 0, s);
+  return tgx_ng_launch(K_UV2_W, g, in, out, b, 0, s);
 }
 
 int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
