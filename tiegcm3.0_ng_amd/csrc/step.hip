@@ -3266,6 +3266,290 @@ __global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng
 #undef F3
 }
 
+// oplus_ng for one sub-step in one pass per column tile: the horizontal
+// stencils of the O+ prelude (b.grad of bdotdh_opj, diffj, bvel and dj*bz,
+// the vdotn term, the Shapiro smoothing of O+ at n-1) and the assembly and
+// elimination of oplus_stage3, level by level.  A block covers a TILE_X x
+// TILE_Y tile; at level k the tile and its 2-point halo of the six stencil
+// inputs are staged through double-buffered LDS.  Every thread of the block
+// takes part in the staging and the barriers; lateral-boundary and
+// out-of-domain columns do not solve.
+template <bool CAP>
+__global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                         const double *__restrict__ op,
+                                                         const double *__restrict__ optm1,
+                                                         const double *__restrict__ lonb,
+                                                         const double *__restrict__ latb, OpWork W,
+                                                         const double *__restrict__ oploss_p,
+                                                         const double *__restrict__ opprod_p,
+                                                         double *__restrict__ opout,
+                                                         double *__restrict__ optm1out) {
+  constexpr int TS = TILE_R * TILE_W, NF = 6, TL = TILE_Y * TILE_W;
+  enum { T_BDOPJ = 0, T_DIFFJ, T_BVEL, T_OPBM2, T_DJBZ, T_OPTM1 };
+  __shared__ double tb[2][NF][TS];
+  __shared__ double tls[TL];  // latitude pass of O+ at n-1
+  const int ntx = (G.nlond + TILE_X - 1) / TILE_X;
+  const int bb = tgx_xcd_block(gridDim.x);
+  const int i0 = (bb % ntx) * TILE_X, j0 = (bb / ntx) * TILE_Y;
+  const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
+  const bool active = i < G.nlond && j < G.nlatd;
+  const bool lat = active && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
+  const bool solve = active && !lat;
+  const int nk = G.nk, nsub = g.nstep_sub;
+  const double dz = g.dz, dtx2inv = g.dtx2inv;
+  int te[2];
+  bool tok[2];
+  size_t toff[2];
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {
+    const int e = (int)threadIdx.x + sl * 256;
+    const int ii = i0 - TILE_H + e % TILE_W, jj = j0 - TILE_H + e / TILE_W;
+    te[sl] = e;
+    tok[sl] = e < TS && ii >= 0 && ii < G.nlond && jj >= 0 && jj < G.nlatd;
+    toff[sl] = tok[sl] ? (size_t)jj * G.ld + ii : 0;
+  }
+  const double *tsrc[NF] = {W.bdopj, W.diffj, W.bvel, W.opbm2, W.djbz, optm1};
+  double stage[NF][2];
+  auto tile_load = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int sl = 0; sl < 2; ++sl)
+        if (tok[sl]) stage[f][sl] = tsrc[f][(size_t)k * G.P + toff[sl]];
+  };
+  auto tile_store = [&](int buf) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int sl = 0; sl < 2; ++sl)
+        if (tok[sl]) tb[buf][f][te[sl]] = stage[f][sl];
+  };
+  const int ic = active ? i : 0, jc = active ? j : 0;
+  const size_t c2 = G.a2(ic, jc);
+#define F3(f, k) (f)[G.a3((k), ic, jc)]
+  if (lat) {
+    double v;
+    for (int k = 0; k < nk; ++k) {
+      size_t p = G.a3(k, i, j);
+      lateral_value(G, lonb, latb, k, i, j, v);
+      double nm = DTSMOOTH * op[p] + DTSMOOTH_DIV2 * (optm1[p] + v);
+      opout[p] = v > 0. ? v : 0.;
+      optm1out[p] = nm > 0. ? nm : 0.;
+    }
+  }
+  const double bz = in.bz[c2], cs = g.cs[jc];
+  const double bxc = in.bx[c2], byc = in.by[c2], bm = in.bmod2[c2];
+  const double ssm = g.shapiro / g.nstep_sub;
+  double opflux = 0., dvb = 1.;
+  if (solve) {
+    const double pi = 4.0 * atan(1.0), rtd = 180. / pi;
+    {  // :77-91
+      const double phid = 2.0e8, phin = -2.0e8, ppolar = 0.;
+      double rl = in.rlatm[c2], a = 0.;
+      if (fabs(rl) >= pi / 4.5) a = 1.;
+      if (fabs(rl) < pi / 4.5 && fabs(rl) > pi / 18) a = .5 * (1. + cos(fabs(rl) * 6. - pi * 4. / 3.));
+      if (fabs(rl) <= pi / 18) a = 0.;
+      double fed = phid * a, fen = phin * a, chi = in.chi[c2];
+      if (chi >= 0.5 * pi) opflux = fen; else opflux = fed;
+      if ((chi * rtd - 80.) * (chi * rtd - 100.) < 0.)
+        opflux = .5 * (fed + fen) + .5 * (fed - fen) * cos(pi * (chi * rtd - 80.) / 20.);
+      if (fabs(rl) >= pi / 3.) opflux = opflux + ppolar;
+    }
+    {  // :94-104 (df_2d of bx and of cs*by)
+      double dbxdx = dflon1(in.bx, G, i, j);
+      auto csby = [&](int jj) { return g.cs[jj] * in.by[G.a2(i, jj)]; };
+      int J1 = G.nlatd - 1;
+      double dcsbydy;
+      if (j == 0) dcsbydy = csby(1) - csby(0);
+      else if (j == J1) dcsbydy = csby(J1) - csby(J1 - 1);
+      else dcsbydy = (csby(j + 1) - csby(j - 1)) / 2.;
+      double vv = dbxdx / g.dlamda + dcsbydy / g.dphi;
+      vv = vv / cs;
+      dvb = (vv + 2. * bz) / RE;
+    }
+  }
+  const double dvbz = dvb / bz;
+  const double gmr = GRAV * RMOP / (2. * GASK);
+  struct OL { double scht, hdz, tp1, dj, bdotu, bdopj, wi; };
+  auto load_ol = [&](int L, OL &Q) {
+    size_t p = G.a3(L, ic, jc);
+    Q.scht = in.scht[p];
+    Q.hdz = 1. / (Q.scht * dz);
+    Q.tp1 = 0.5 * (in.ti[p] + in.te[p]);
+    Q.dj = W.dj[p]; Q.bdotu = W.bdotu[p]; Q.bdopj = W.bdopj[p]; Q.wi = in.wi[p];
+  };
+  OL qa = {0, 0, 0, 0, 0, 0, 0}, qb = {}, qc = {};
+  double ol1 = 0., qop0 = 0.;
+  if (solve) {
+    load_ol(0, qb);
+    load_ol(1, qc);
+    ol1 = F3(oploss_p, 1);
+    qop0 = F3(in.qop, 0);  // lower boundary (:301-303)
+  }
+  double w1 = 0., w3 = 0., rprev = 0.;
+  tile_load(0);
+  tile_store(0);
+  for (int k = 0; k <= nk - 2; ++k) {
+    size_t p = G.a3(k, ic, jc);
+    double oploss = 0., opprod = 0., ui0 = 0., ui1 = 0., vi0 = 0., vi1 = 0.;
+    if (k + 1 <= nk - 2) tile_load(k + 1);
+    if (solve) {
+      if (k > 0) load_ol(k + 1, qc);
+      oploss = oploss_p[p];
+      opprod = opprod_p[p];
+      ui0 = in.ui[p]; ui1 = in.ui[p + G.P]; vi0 = in.vi[p]; vi1 = in.vi[p + G.P];
+    }
+    asm volatile("" ::: "memory");
+    __syncthreads();  // the level-k tiles are in buffer k&1
+    const int bk = k & 1;
+    const LdsPlane Tbdopj{tb[bk][T_BDOPJ], i0 - TILE_H, j0 - TILE_H},
+        Tdiffj{tb[bk][T_DIFFJ], i0 - TILE_H, j0 - TILE_H}, Tbvel{tb[bk][T_BVEL], i0 - TILE_H, j0 - TILE_H},
+        Topbm2{tb[bk][T_OPBM2], i0 - TILE_H, j0 - TILE_H}, Tdjbz{tb[bk][T_DJBZ], i0 - TILE_H, j0 - TILE_H},
+        Toptm1{tb[bk][T_OPTM1], i0 - TILE_H, j0 - TILE_H};
+    // smooth_ng.F90:283-294: latitude pass of O+ at n-1 on the tile rows and their longitude halo
+    for (int e = threadIdx.x; e < TL; e += blockDim.x) {
+      int ii = i0 - TILE_H + e % TILE_W, jj = j0 + e / TILE_W;
+      if (ii >= 0 && ii < G.nlond && jj < G.nlatd) tls[e] = smooth_lat(Toptm1, G, ii, jj, ssm);
+    }
+    __syncthreads();
+    if (solve) {
+      // the prelude's terms at (i, j, k) (oplus_ng.F90:172-211)
+      const double bd_diff = bdotdh(Tbdopj, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
+      const double bd_diffj = bdotdh(Tdiffj, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
+      const double bd_bvel = bdotdh(Tbvel, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
+      const double bd_djbz = bdotdh(Tdjbz, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
+      const double djbz = pv(Tdjbz, G, i, j);
+      double vdotn;
+      {  // oplus_ng.F90:196-209
+        double uii = .5 * (ui0 + ui1);
+        double vii = .5 * (vi0 + vi1);
+        double vx = uii * dflon1(Topbm2, G, i, j), vy = vii * dflat1(Topbm2, G, i, j);
+        vx = vx / cs;
+        double vh = 1. / RE * (vx / g.dlamda + vy / g.dphi);
+        vdotn = vh * (bm * bm);
+      }
+      double opts2;
+      {  // smooth_ng.F90:296-307
+        const double *row = tls + (j - j0) * TILE_W + TILE_H - i0;
+        int L1 = G.nlond - 1;
+        bool edge = (i == 0 || i == L1), near = (i == 1 || i == L1 - 1);
+        int i1p = i < L1 ? i + 1 : L1, i1m = i > 0 ? i - 1 : 0;
+        int i2p = i + 2 <= L1 ? i + 2 : L1, i2m = i >= 2 ? i - 2 : 0;
+        double t0 = row[i];
+        double fm1 = (row[i1p] + row[i1m]) * 0.5;
+        double fm2 = (row[i2p] + row[i2m]) * 0.5;
+        double gen = t0 - ssm * (2. * fm2 - 8. * fm1 + 6. * t0);
+        double nr = t0 - ssm * (-8. * fm1 + 8. * t0);
+        opts2 = edge ? t0 : (near ? nr : gen);
+      }
+      const bool top = k == nk - 2;
+      // explicit terms (:181-211, :250)
+      double dbd;
+      if (k == 0) dbd = qc.bdopj - qb.bdopj;
+      else if (k <= nk - 3) dbd = (qc.bdopj - qa.bdopj) / 2.;
+      else dbd = qb.bdopj - qa.bdopj;
+      double bdzdvb = bz / (qb.scht * dz) * dbd + dvb * qb.bdopj;
+      double bd_op = djbz * bd_diffj;
+      double diffexp = bdzdvb + bd_diff + bd_op;
+      double expl = -1. * diffexp;
+      expl = expl + vdotn + bd_bvel;
+      expl = expl - opts2 * dtx2inv * nsub;
+      // diffusion, drift and wind coefficients (:239-278)
+      double divbz = dvbz + bd_djbz / (qb.dj * (bz * bz));
+      double hm = qb.hdz - 0.5 * divbz, hp = qb.hdz + 0.5 * divbz;
+      hm = hm * (bz * bz);
+      hp = hp * (bz * bz);
+      double hdzi_k = k == 0 ? 1.5 * qb.hdz - 0.5 * qc.hdz : 0.5 * (qa.hdz + qb.hdz);
+      double hdzi_k1 = top ? 1.5 * qb.hdz - 0.5 * qa.hdz : 0.5 * (qb.hdz + qc.hdz);
+      double dji = k == 0 ? 1.5 * qb.dj - 0.5 * qc.dj : 0.5 * (qa.dj + qb.dj);
+      double dji1 = top ? 1.5 * qb.dj - 0.5 * qa.dj : 0.5 * (qb.dj + qc.dj);
+      double t0 = k == 0 ? 2. * (2. * qb.tp1 - qc.tp1) * hdzi_k - gmr : 2. * qa.tp1 * hdzi_k - gmr;
+      double t01 = 2. * qb.tp1 * hdzi_k1 - gmr;
+      double t1 = 2. * qb.tp1 * hdzi_k + gmr;
+      double t11 = top ? 2. * (2. * qb.tp1 - qa.tp1) * hdzi_k1 + gmr : 2. * qc.tp1 * hdzi_k1 + gmr;
+      double diffp = hm * dji * t0;
+      double diffq = -(hp * dji1 * t01 + hm * dji * t1);
+      double diffr = hp * dji1 * t11;
+      double hdzk = qb.hdz;
+      double wii = .5 * (qb.wi + qc.wi);
+      double driftp = wii * 0.5 * hdzk;
+      double driftq = -wii * 6. / RE;
+      double driftr = -wii * 0.5 * hdzk;
+      double windp = k == 0 ? bz * (2. * qb.bdotu - qc.bdotu) * 0.5 * hdzk : bz * qa.bdotu * 0.5 * hdzk;
+      double windq = -qb.bdotu * dvbz * bz;
+      double windr = !top ? -bz * qc.bdotu * 0.5 * hdzk : bz * (2. * qb.bdotu - qa.bdotu) * 0.5 * hdzk;
+      double pcf = diffp + driftp + windp;
+      double qcf = diffq + driftq + windq - dtx2inv * nsub;
+      double rcf = diffr + driftr + windr;
+      if (top) {  // upper boundary (:280-286)
+        double djn = dji1;
+        double ubcb = -(bz * bz) * djn * t01;
+        double ubca = -(bz * bz) * djn * t11;
+        qcf = qcf + ubcb / ubca * rcf;
+        expl = expl - opflux * rcf / ubca;
+        rcf = 0.;
+      }
+      qcf = qcf - oploss;
+      expl = expl - opprod;
+      if (k == 0) {  // lower boundary (:301-303)
+        qcf = qcf - pcf;
+        expl = expl - 2. * pcf * qop0 / (1.5 * oploss - 0.5 * ol1);
+        pcf = 0.;
+      }
+      if (CAP) {
+        cap_put(g, G, CAP_OP + 0, i, j, k, pcf); cap_put(g, G, CAP_OP + 1, i, j, k, qcf);
+        cap_put(g, G, CAP_OP + 2, i, j, k, rcf); cap_put(g, G, CAP_OP + 3, i, j, k, expl);
+      }
+      // Thomas forward (trsolv_ng, rows 1..nk-1)
+      if (k == 0) {
+        w1 = qcf;
+        w3 = expl / w1;
+      } else {
+        double w2 = rprev / w1;
+        W.w2[G.a3(k - 1, i, j)] = w2;
+        w1 = qcf - pcf * w2;
+        w3 = (expl - pcf * w3) / w1;
+      }
+      opout[p] = w3;
+      rprev = rcf;
+      qa = qb;
+      qb = qc;
+    }
+    if (k + 1 <= nk - 2) tile_store((k + 1) & 1);
+  }
+  if (!solve) return;
+  // back substitution, top extrapolation (:307) and outputs (:337-340); the
+  // next level's loads are in flight while this level is stored
+  struct OB { double x, w2, op, optm1; };
+  auto load_ob = [&](int k, OB &B) {
+    size_t p = G.a3(k, ic, jc);
+    B.x = opout[p]; B.w2 = k < nk - 2 ? W.w2[p] : 0.; B.op = op[p]; B.optm1 = optm1[p];
+  };
+  const double op_top = F3(op, nk - 1), optm1_top = F3(optm1, nk - 1);
+  double xn = w3;
+  OB bc, bn;
+  load_ob(nk - 2, bc);
+  for (int k = nk - 2; k >= 0; --k) {
+    if (k > 0) load_ob(k - 1, bn);
+    asm volatile("" ::: "memory");
+    double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
+    if (k == nk - 3) {
+      double xt = xn * xn / x;
+      size_t pt = G.a3(nk - 1, ic, jc);
+      double nm = DTSMOOTH * op_top + DTSMOOTH_DIV2 * (optm1_top + xt);
+      opout[pt] = xt > 0. ? xt : 0.;
+      optm1out[pt] = nm > 0. ? nm : 0.;
+    }
+    size_t p = G.a3(k, ic, jc);
+    double nm = DTSMOOTH * bc.op + DTSMOOTH_DIV2 * (bc.optm1 + x);
+    opout[p] = x > 0. ? x : 0.;
+    optm1out[p] = nm > 0. ? nm : 0.;
+    xn = x;
+    bc = bn;
+  }
+#undef F3
+}
+
 // oplus_stage3 with one wavefront per column: rows 0..nk-2 by parallel
 // cyclic reduction, the top level extrapolated (:307).
 __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
@@ -4251,7 +4535,8 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 5:
     case 9:
     case 17:
-    case 21: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
+    case 21:
+    case 29: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
                 //           optm1out, bd_diff bd_diffj bd_bvel vdotn bd_djbz opts2, oploss, opprod}
       OpWork W{D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8)};
       const double *op = D(9), *optm1 = D(10);
@@ -4277,7 +4562,15 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       } else {
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
-        if (which == 5)
+        dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
+        if (which == 29)  // stencils fused: one block per column tile
+          if (g.capture)
+            hipLaunchKernelGGL(oplus_tile_kernel<true>, gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+                               (const double *)D(19), (const double *)D(20), opout, optm1out);
+          else
+            hipLaunchKernelGGL(oplus_tile_kernel<false>, gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+                               (const double *)D(19), (const double *)D(20), opout, optm1out);
+        else if (which == 5)
           if (g.capture)
             hipLaunchKernelGGL(oplus_stage3<true>, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
                                (const double *)D(19), (const double *)D(20), opout, optm1out);

@@ -134,7 +134,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
-       K_MIN_B_LIN, K_UV_TILE };
+       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -284,9 +284,11 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
       if (int rc = tgx_ng_launch(K_OP_CHEM, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
-    if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
+    const bool pcr = use_pcr(g, R_OPLUS);
+    if (pcr)  // Thomas: the stencils are fused into the column kernel
+      if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
     ProfScope kern(R_N + R_OPLUS, s);
-    if (int rc = tgx_ng_launch(use_pcr(g, R_OPLUS) ? K_OP3_W : K_OP3, g, in, out, b, sub, s)) return rc;
+    if (int rc = tgx_ng_launch(pcr ? K_OP3_W : K_OP_TILE, g, in, out, b, sub, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
   }
