@@ -314,10 +314,12 @@ def alg_bytes_per_column_level(routine):
 
 # The column kernel of each routine: its name per scheme and the distinct
 # planes it must read and write once per launch (8 bytes per column-level
-# each).  Horizontal-stencil and chemistry terms arrive as planes from the
-# routine's level-parallel kernels (except dt under Thomas: dt_tile_kernel
-# evaluates its stencils itself; its PCR kernel reads the two stencil planes,
-# PCR_EXTRA); Thomas ratio / gamma planes are scratch, not algorithmic.
+# each).  Chemistry terms and (for the species) the horizontal-stencil
+# right-hand-side parts arrive as planes from the routine's level-parallel
+# kernels; the dt, duv and O+ Thomas kernels evaluate their stencils (and
+# duv its vertical integral) themselves from tiles of the fields, while their
+# PCR kernels read the stencil planes (PCR_INS).  Thomas ratio / gamma planes
+# are scratch, not algorithmic.
 COLUMN_KERNELS = {
     "comp": (("comp_kernel", "comp_wave_kernel"),  # rhs_*: the prelude's smoothed/advected part
              "o2 o1 he mbar tn w fs01 fs02 fs11 fs12 rhs_o2 rhs_o1 rhs_he "
@@ -331,24 +333,30 @@ COLUMN_KERNELS = {
            "tn w_upd cp km un vn qtotal schti kt xnmbar o1 rkm12 mbar scht "
            "hdt qji_tn cool_imp cool_exp tn_nm".split(),
            "tn tn_nm".split()),
-    "duv": (("duv_stage2", "duv_wave_stage2"),
-            "w_upd lxx lyy lxy lyx ui vi divx_ztmp divy_z km schti scht Fe Fn unms vnms advu advv "
-            "hdu hdv un un_nm vn_nm vn".split(),
+    "duv": (("duv_tile_kernel", "duv_wave_stage2"),  # duv_ng.F90:33-55 (tn_upd = this step's TN)
+            "w_upd lxx lyy lxy lyx ui vi km schti scht Fe Fn hdu hdv un vn un_nm vn_nm "
+            "z tn tn_nm tn_upd mbar".split(),
             "un vn un_nm vn_nm".split()),
-    "oplus": (("oplus_stage3", "oplus_wave_stage3"),  # per O+ sub-step
-              "scht ti te dj bdotu bdopj wi bd_diff bd_diffj djbz vdotn bd_bvel opts2 bd_djbz "
-              "oploss opprod op optm1".split(),
+    "oplus": (("oplus_tile_kernel", "oplus_wave_stage3"),  # per O+ sub-step
+              "scht ti te dj bdotu wi bdopj diffj bvel opbm2 djbz optm1 ui vi "
+              "oploss opprod op".split(),
               "op op_nm".split()),
 }
-
-
-PCR_EXTRA = {"dt": ("hadv_tn", "smooth_tn_nm")}
+PCR_INS = {
+    "dt": "tn w_upd cp km un vn qtotal schti kt hadv_tn smooth_tn_nm xnmbar o1 rkm12 mbar scht "
+          "hdt qji_tn cool_imp cool_exp tn_nm".split(),
+    "duv": "w_upd lxx lyy lxy lyx ui vi divx_ztmp divy_z km schti scht Fe Fn unms vnms advu advv "
+           "hdu hdv un un_nm vn_nm vn".split(),
+    "oplus": "scht ti te dj bdotu bdopj wi bd_diff bd_diffj djbz vdotn bd_bvel opts2 bd_djbz "
+             "oploss opprod op optm1".split(),
+}
 
 
 def column_kernel_bytes_per_column_level(routine, scheme="thomas"):
     _, ins, outs = COLUMN_KERNELS[routine]
-    extra = PCR_EXTRA.get(routine, ()) if scheme == "pcr" else ()
-    return 8 * (len(set(ins) | set(extra)) + len(set(outs)))
+    if scheme == "pcr":
+        ins = PCR_INS.get(routine, ins)
+    return 8 * (len(set(ins)) + len(set(outs)))
 
 
 def run_oracle_step(oracle, grid, Fo):
