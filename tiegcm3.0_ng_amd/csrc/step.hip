@@ -469,190 +469,13 @@ using KindsComp = StencilKinds<ST_ADVEC_R, ST_ADVEC_R, ST_ADVEC_R, ST_SMOOTH, ST
 using KindsDuv = StencilKinds<ST_DIVX, ST_DIVY, ST_SMOOTH, ST_SMOOTH, ST_ADVEC, ST_ADVEC>;
 
 // =============================================================================
-// dt_ng.F90:54-213
-template <bool CAP>
-__global__ void __launch_bounds__(256) dt_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                 tgx_ng_out out, double *__restrict__ w2s,
-                                                 const double *__restrict__ hadv_tn,
-                                                 const double *__restrict__ smooth_tn_nm) {
-  int i, j;
-  if (!column_of(G, i, j)) return;
-  const int nk = G.nk;
-  const double dz = g.dz, dtx2inv = g.dtx2inv;
-  const size_t c2 = G.a2(i, j);
-  const double t_lbc = in.t_lbc[c2], tl = in.tlbc[c2];
-  out.tlbc_nm[c2] = tl;
-  out.tlbc[c2] = t_lbc;
-  double v;
-  if (lateral_value(G, in.lonb_tn, in.latb_tn, 0, i, j, v)) {
-    for (int k = 0; k < nk; ++k) {
-      lateral_value(G, in.lonb_tn, in.latb_tn, k, i, j, v);
-      size_t p = G.a3(k, i, j);
-      out.tn_nm[p] = DTSMOOTH_DIV2 * (in.tn_nm[p] + v) + DTSMOOTH * in.tn[p];
-      out.tn[p] = v > 100. ? v : 100.;
-    }
-    return;
-  }
-#define F3(f, k) (f)[G.a3((k), i, j)]
-  // Fields used at two neighbouring levels, carried one level in registers;
-  // every load of a level is issued before its arithmetic.
-  struct DL { double w, cp, km, qtotal, schti, kt, un, vn; };
-  struct DS { double hadv, tsm, xn, o1, rkm12, mbar, scht, hdt, qji, cool_imp, cool_exp; };
-  auto load_dl = [&](int L, DL &D) {
-    size_t p = G.a3(L, i, j);
-    D.w = in.w_upd[p]; D.cp = in.cp[p]; D.km = in.km[p]; D.qtotal = in.qtotal[p];
-    D.schti = in.schti[p]; D.kt = in.kt[p]; D.un = in.un[p]; D.vn = in.vn[p];
-  };
-  auto load_ds = [&](int L, DS &S) {
-    size_t p = G.a3(L, i, j);
-    S.hadv = hadv_tn[p]; S.tsm = smooth_tn_nm[p]; S.xn = in.xnmbar[p]; S.o1 = in.o1[p];
-    S.rkm12 = in.rkm12[p]; S.mbar = in.mbar[p]; S.scht = in.scht[p]; S.hdt = in.hdt[p];
-    S.qji = in.qji_tn[p]; S.cool_imp = in.cool_imp[p]; S.cool_exp = in.cool_exp[p];
-  };
-  // g(L), f(L) of dt_ng.F90:121-139 from the level-L fields
-  auto gf = [&](const DL &D, double expzL, double diftL, double tni, double &gg, double &ff) {
-    double h = D.schti;
-    double rho = P0 * g.expzmid_inv / (h * GRAV);
-    rho = rho * expzL;
-    double gpart = h * h * rho * D.cp;
-    double gpart_a = gpart * diftL;
-    gg = GRAV * (D.kt + gpart_a) / (P0 * h * (dz * dz));
-    ff = GRAV * GRAV * (h * h) * rho / (tni * P0 * 2. * dz);
-    ff = ff * diftL;
-  };
-  const double ulbc = in.ulbc[c2], vlbc = in.vlbc[c2];
-  DL dc, dn;
-  load_dl(0, dc);
-  double tn_m = 0., tn_c = F3(in.tn, 0), tn_p = F3(in.tn, 1);
-  double un_m = 0., vn_m = 0.;
-  double gk, fk;
-  gf(dc, g.expz[0], g.dift[0], tl, gk, fk);
-  double w1 = 0., w3 = 0., rprev = 0., dudz_prev = 0., dvdz_prev = 0.;
-  for (int k = 0; k <= nk - 2; ++k) {
-    DS ds;
-    load_dl(k + 1, dn);
-    load_ds(k, ds);
-    double tn_pp = k + 2 <= nk - 1 ? F3(in.tn, k + 2) : 0.;
-    const double expzk = g.expz[k], expzk1 = g.expz[k + 1], dift1 = g.dift[k + 1];
-    asm volatile("" ::: "memory");
-    const double w_k = dc.w, w_k1 = dn.w;
-    // veradv (:57-65)
-    double raw_k = k == 0 ? (tn_c - t_lbc) * w_k * 2. : (tn_c - tn_m) * w_k;
-    double raw_k1 = (k + 1 <= nk - 2) ? (tn_p - tn_c) * w_k1 : 0.;
-    double veradv = (raw_k + raw_k1) * .5;
-    veradv = veradv / dz;
-    // interface means (:71-78)
-    double cpi = .5 * (dc.cp + dn.cp);
-    double kmi = .5 * (dc.km + dn.km);
-    double wi = .5 * (w_k + w_k1);
-    // cptn (:67, :80-83)
-    double advec_tn = ds.hadv + veradv;
-    double cptn = cpi * (advec_tn - dtx2inv * ds.tsm);
-    cptn = cptn * expzk;
-    // dudz (:85-94)
-    double dudz, dvdz;
-    if (k == 0) {
-      dudz = (dc.un + 1. / 3. * dn.un - 4. / 3. * ulbc) / dz;
-      dvdz = (dc.vn + 1. / 3. * dn.vn - 4. / 3. * vlbc) / dz;
-    } else if (k <= nk - 3) {
-      dudz = (dn.un - un_m) / (2. * dz);
-      dvdz = (dn.vn - vn_m) / (2. * dz);
-    } else {
-      dudz = dudz_prev / 3.;
-      dvdz = dvdz_prev / 3.;
-    }
-    dudz_prev = dudz;
-    dvdz_prev = dvdz;
-    // heating (:96-113)
-    double qsolar = .5 * (dc.qtotal + dn.qtotal);
-    double t = ds.xn * ds.o1 * RIO1;
-    double qrecomb = TSURPLUS * ds.rkm12 * (t * t) * AVO / ds.mbar;
-    double qm = GRAV * kmi / (P0 * ds.scht) * (dudz * dudz + dvdz * dvdz);
-    qm = qm / expzk;
-    double total_heat = qsolar + ds.hdt + qrecomb + ds.qji + qm;
-    double total_heat_a = total_heat * expzk;
-    cptn = cptn - total_heat_a;
-    // coefficients (:141-163)
-    double p = gk - fk, q, r;
-    double g1 = 0., f1 = 0.;
-    if (k <= nk - 3) {
-      double tni1 = .5 * (tn_c + tn_p);
-      gf(dn, expzk1, dift1, tni1, g1, f1);
-      q = -(g1 - f1 + gk + fk);
-      r = g1 + f1;
-    } else {
-      q = -gk - fk;
-      r = 0.;
-    }
-    double rhs = cptn;
-    double qpart = cpi * (dtx2inv + ds.cool_imp) + wi * GASK / ds.mbar;
-    rhs = rhs + ds.cool_exp;
-    double qpart_a = qpart * expzk;
-    q = q - qpart_a;
-    if (k == 0) {
-      q = q - p;
-      rhs = rhs - 2. * p * t_lbc;
-      p = 0.;
-    }
-    if (CAP) {
-      cap_put(g, G, CAP_DT + 0, i, j, k, p); cap_put(g, G, CAP_DT + 1, i, j, k, q);
-      cap_put(g, G, CAP_DT + 2, i, j, k, r); cap_put(g, G, CAP_DT + 3, i, j, k, rhs);
-    }
-    // Thomas forward (trsolv_ng.F90:14-22)
-    size_t pk = G.a3(k, i, j);
-    if (k == 0) {
-      w1 = q;
-      w3 = rhs / w1;
-    } else {
-      double w2 = rprev / w1;
-      w2s[G.a3(k - 1, i, j)] = w2;
-      w1 = q - p * w2;
-      w3 = (rhs - p * w3) / w1;
-    }
-    out.tn[pk] = w3;
-    rprev = r;
-    gk = g1; fk = f1;
-    tn_m = tn_c; tn_c = tn_p; tn_p = tn_pp;
-    un_m = dc.un; vn_m = dc.vn;
-    dc = dn;
-  }
-  // back substitution (:165-167) fused with the output stage (:208-213); the
-  // next level's loads are in flight while this level is stored
-  struct DB { double x, w2, tnm, tn; };
-  auto load_db = [&](int k, DB &B) {
-    size_t p = G.a3(k, i, j);
-    B.x = out.tn[p]; B.w2 = k < nk - 2 ? w2s[p] : 0.; B.tnm = in.tn_nm[p]; B.tn = in.tn[p];
-  };
-  const double tnm_top = F3(in.tn_nm, nk - 1), tn_top = F3(in.tn, nk - 1);
-  double xn = w3;
-  DB bc, bn;
-  load_db(nk - 2, bc);
-  for (int k = nk - 2; k >= 0; --k) {
-    if (k > 0) load_db(k - 1, bn);
-    asm volatile("" ::: "memory");
-    double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
-    if (k == nk - 3) {
-      double xt = xn * xn / x;
-      size_t pt = G.a3(nk - 1, i, j);
-      out.tn_nm[pt] = DTSMOOTH_DIV2 * (tnm_top + xt) + DTSMOOTH * tn_top;
-      out.tn[pt] = xt > 100. ? xt : 100.;
-    }
-    size_t p = G.a3(k, i, j);
-    out.tn_nm[p] = DTSMOOTH_DIV2 * (bc.tnm + x) + DTSMOOTH * bc.tn;
-    out.tn[p] = x > 100. ? x : 100.;
-    xn = x;
-    bc = bn;
-  }
-#undef F3
-}
-
-// dt_ng with the horizontal stencils fused in: a block covers a TILE_X x
+// dt_ng.F90:54-213 with the horizontal stencils fused in: a block covers a TILE_X x
 // TILE_Y tile of columns; at each level the tile and its 2-point halo of TN,
 // UN, VN and TN_NM are staged through LDS (double-buffered, the next level's
 // loads issued with the column loads), the advection of TN (advec_ng) and the
 // Shapiro smoothing of TN_NM (smooth_ng, latitude pass once per point) are
-// evaluated from LDS, and the column is assembled and eliminated as in
-// dt_kernel.  Every thread of the block takes part in the staging and the
+// evaluated from LDS, and the column is assembled and eliminated by Thomas
+// (trsolv_ng).  Every thread of the block takes part in the staging and the
 // barriers; lateral-boundary and out-of-domain columns do not solve.
 template <bool CAP>
 __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
@@ -3074,202 +2897,10 @@ __global__ void __launch_bounds__(256) oplus_chem_kernel(tgx_ng_grid g, Geo G, t
   }
 }
 
-template <bool CAP>
-__global__ void __launch_bounds__(256) oplus_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                    const double *__restrict__ op,
-                                                    const double *__restrict__ optm1,
-                                                    const double *__restrict__ lonb,
-                                                    const double *__restrict__ latb, OpWork W,
-                                                    OpTerms OT,
-                                                    const double *__restrict__ oploss_p,
-                                                    const double *__restrict__ opprod_p,
-                                                    double *__restrict__ opout,
-                                                    double *__restrict__ optm1out) {
-  int i, j;
-  if (!column_of(G, i, j)) return;
-  const int nk = G.nk, nsub = g.nstep_sub;
-  const double dz = g.dz, dtx2inv = g.dtx2inv;
-  const size_t c2 = G.a2(i, j);
-#define F3(f, k) (f)[G.a3((k), i, j)]
-  double v;
-  if (lateral_value(G, lonb, latb, 0, i, j, v)) {
-    for (int k = 0; k < nk; ++k) {
-      size_t p = G.a3(k, i, j);
-      lateral_value(G, lonb, latb, k, i, j, v);
-      double nm = DTSMOOTH * op[p] + DTSMOOTH_DIV2 * (optm1[p] + v);
-      opout[p] = v > 0. ? v : 0.;
-      optm1out[p] = nm > 0. ? nm : 0.;
-    }
-    return;
-  }
-  const double pi = 4.0 * atan(1.0), rtd = 180. / pi;
-  const double bz = in.bz[c2], cs = g.cs[j];
-  double opflux;
-  {  // :77-91
-    const double phid = 2.0e8, phin = -2.0e8, ppolar = 0.;
-    double rl = in.rlatm[c2], a = 0.;
-    if (fabs(rl) >= pi / 4.5) a = 1.;
-    if (fabs(rl) < pi / 4.5 && fabs(rl) > pi / 18) a = .5 * (1. + cos(fabs(rl) * 6. - pi * 4. / 3.));
-    if (fabs(rl) <= pi / 18) a = 0.;
-    double fed = phid * a, fen = phin * a, chi = in.chi[c2];
-    if (chi >= 0.5 * pi) opflux = fen; else opflux = fed;
-    if ((chi * rtd - 80.) * (chi * rtd - 100.) < 0.)
-      opflux = .5 * (fed + fen) + .5 * (fed - fen) * cos(pi * (chi * rtd - 80.) / 20.);
-    if (fabs(rl) >= pi / 3.) opflux = opflux + ppolar;
-  }
-  double dvb;
-  {  // :94-104 (df_2d of bx and of cs*by)
-    double dbxdx = dflon1(in.bx, G, i, j);
-    auto csby = [&](int jj) { return g.cs[jj] * in.by[G.a2(i, jj)]; };
-    int J1 = G.nlatd - 1;
-    double dcsbydy;
-    if (j == 0) dcsbydy = csby(1) - csby(0);
-    else if (j == J1) dcsbydy = csby(J1) - csby(J1 - 1);
-    else dcsbydy = (csby(j + 1) - csby(j - 1)) / 2.;
-    double vv = dbxdx / g.dlamda + dcsbydy / g.dphi;
-    vv = vv / cs;
-    dvb = (vv + 2. * bz) / RE;
-  }
-  const double dvbz = dvb / bz;
-  const double gmr = GRAV * RMOP / (2. * GASK);
-  // Fields used at up to three neighbouring levels, in a window A = k-1,
-  // B = k, C = k+1 carried in registers; each level's loads are issued
-  // before its arithmetic.  hdz = 1/(scht dz), tp1 = (ti+te)/2 (:163-178).
-  struct OL { double scht, hdz, tp1, dj, bdotu, bdopj, wi; };
-  struct OS { double bd_diff, bd_diffj, djbz, vdotn, bd_bvel, opts2, bd_djbz, oploss, opprod; };
-  auto load_ol = [&](int L, OL &Q) {
-    size_t p = G.a3(L, i, j);
-    Q.scht = in.scht[p];
-    Q.hdz = 1. / (Q.scht * dz);
-    Q.tp1 = 0.5 * (in.ti[p] + in.te[p]);
-    Q.dj = W.dj[p]; Q.bdotu = W.bdotu[p]; Q.bdopj = W.bdopj[p]; Q.wi = in.wi[p];
-  };
-  auto load_os = [&](int L, OS &S) {
-    size_t p = G.a3(L, i, j);
-    S.bd_diff = OT.bd_diff[p]; S.bd_diffj = OT.bd_diffj[p]; S.djbz = W.djbz[p]; S.vdotn = OT.vdotn[p];
-    S.bd_bvel = OT.bd_bvel[p]; S.opts2 = OT.opts2[p]; S.bd_djbz = OT.bd_djbz[p];
-    S.oploss = oploss_p[p]; S.opprod = opprod_p[p];
-  };
-  OL qa = {0, 0, 0, 0, 0, 0, 0}, qb, qc;
-  load_ol(0, qb);
-  load_ol(1, qc);
-  const double ol1 = F3(oploss_p, 1), qop0 = F3(in.qop, 0);  // lower boundary (:301-303)
-  double w1 = 0., w3 = 0., rprev = 0.;
-  for (int k = 0; k <= nk - 2; ++k) {
-    size_t p = G.a3(k, i, j);
-    OS os;
-    if (k > 0) load_ol(k + 1, qc);
-    load_os(k, os);
-    asm volatile("" ::: "memory");
-    const bool top = k == nk - 2;
-    // explicit terms (:181-211, :250)
-    double dbd;
-    if (k == 0) dbd = qc.bdopj - qb.bdopj;
-    else if (k <= nk - 3) dbd = (qc.bdopj - qa.bdopj) / 2.;
-    else dbd = qb.bdopj - qa.bdopj;
-    double bdzdvb = bz / (qb.scht * dz) * dbd + dvb * qb.bdopj;
-    double bd_op = os.djbz * os.bd_diffj;
-    double diffexp = bdzdvb + os.bd_diff + bd_op;
-    double expl = -1. * diffexp;
-    expl = expl + os.vdotn + os.bd_bvel;
-    expl = expl - os.opts2 * dtx2inv * nsub;
-    // diffusion, drift and wind coefficients (:239-278)
-    double divbz = dvbz + os.bd_djbz / (qb.dj * (bz * bz));
-    double hm = qb.hdz - 0.5 * divbz, hp = qb.hdz + 0.5 * divbz;
-    hm = hm * (bz * bz);
-    hp = hp * (bz * bz);
-    double hdzi_k = k == 0 ? 1.5 * qb.hdz - 0.5 * qc.hdz : 0.5 * (qa.hdz + qb.hdz);
-    double hdzi_k1 = top ? 1.5 * qb.hdz - 0.5 * qa.hdz : 0.5 * (qb.hdz + qc.hdz);
-    double dji = k == 0 ? 1.5 * qb.dj - 0.5 * qc.dj : 0.5 * (qa.dj + qb.dj);
-    double dji1 = top ? 1.5 * qb.dj - 0.5 * qa.dj : 0.5 * (qb.dj + qc.dj);
-    double t0 = k == 0 ? 2. * (2. * qb.tp1 - qc.tp1) * hdzi_k - gmr : 2. * qa.tp1 * hdzi_k - gmr;
-    double t01 = 2. * qb.tp1 * hdzi_k1 - gmr;
-    double t1 = 2. * qb.tp1 * hdzi_k + gmr;
-    double t11 = top ? 2. * (2. * qb.tp1 - qa.tp1) * hdzi_k1 + gmr : 2. * qc.tp1 * hdzi_k1 + gmr;
-    double diffp = hm * dji * t0;
-    double diffq = -(hp * dji1 * t01 + hm * dji * t1);
-    double diffr = hp * dji1 * t11;
-    double hdzk = qb.hdz;
-    double wii = .5 * (qb.wi + qc.wi);
-    double driftp = wii * 0.5 * hdzk;
-    double driftq = -wii * 6. / RE;
-    double driftr = -wii * 0.5 * hdzk;
-    double windp = k == 0 ? bz * (2. * qb.bdotu - qc.bdotu) * 0.5 * hdzk : bz * qa.bdotu * 0.5 * hdzk;
-    double windq = -qb.bdotu * dvbz * bz;
-    double windr = !top ? -bz * qc.bdotu * 0.5 * hdzk : bz * (2. * qb.bdotu - qa.bdotu) * 0.5 * hdzk;
-    double pcf = diffp + driftp + windp;
-    double qcf = diffq + driftq + windq - dtx2inv * nsub;
-    double rcf = diffr + driftr + windr;
-    if (top) {  // upper boundary (:280-286)
-      double djn = dji1;
-      double ubcb = -(bz * bz) * djn * t01;
-      double ubca = -(bz * bz) * djn * t11;
-      qcf = qcf + ubcb / ubca * rcf;
-      expl = expl - opflux * rcf / ubca;
-      rcf = 0.;
-    }
-    qcf = qcf - os.oploss;
-    expl = expl - os.opprod;
-    if (k == 0) {  // lower boundary (:301-303)
-      qcf = qcf - pcf;
-      expl = expl - 2. * pcf * qop0 / (1.5 * os.oploss - 0.5 * ol1);
-      pcf = 0.;
-    }
-    if (CAP) {
-      cap_put(g, G, CAP_OP + 0, i, j, k, pcf); cap_put(g, G, CAP_OP + 1, i, j, k, qcf);
-      cap_put(g, G, CAP_OP + 2, i, j, k, rcf); cap_put(g, G, CAP_OP + 3, i, j, k, expl);
-    }
-    // Thomas forward (trsolv_ng, rows 1..nk-1)
-    if (k == 0) {
-      w1 = qcf;
-      w3 = expl / w1;
-    } else {
-      double w2 = rprev / w1;
-      W.w2[G.a3(k - 1, i, j)] = w2;
-      w1 = qcf - pcf * w2;
-      w3 = (expl - pcf * w3) / w1;
-    }
-    opout[p] = w3;
-    rprev = rcf;
-    qa = qb;
-    qb = qc;
-  }
-  // back substitution, top extrapolation (:307) and outputs (:337-340); the
-  // next level's loads are in flight while this level is stored
-  struct OB { double x, w2, op, optm1; };
-  auto load_ob = [&](int k, OB &B) {
-    size_t p = G.a3(k, i, j);
-    B.x = opout[p]; B.w2 = k < nk - 2 ? W.w2[p] : 0.; B.op = op[p]; B.optm1 = optm1[p];
-  };
-  const double op_top = F3(op, nk - 1), optm1_top = F3(optm1, nk - 1);
-  double xn = w3;
-  OB bc, bn;
-  load_ob(nk - 2, bc);
-  for (int k = nk - 2; k >= 0; --k) {
-    if (k > 0) load_ob(k - 1, bn);
-    asm volatile("" ::: "memory");
-    double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
-    if (k == nk - 3) {
-      double xt = xn * xn / x;
-      size_t pt = G.a3(nk - 1, i, j);
-      double nm = DTSMOOTH * op_top + DTSMOOTH_DIV2 * (optm1_top + xt);
-      opout[pt] = xt > 0. ? xt : 0.;
-      optm1out[pt] = nm > 0. ? nm : 0.;
-    }
-    size_t p = G.a3(k, i, j);
-    double nm = DTSMOOTH * bc.op + DTSMOOTH_DIV2 * (bc.optm1 + x);
-    opout[p] = x > 0. ? x : 0.;
-    optm1out[p] = nm > 0. ? nm : 0.;
-    xn = x;
-    bc = bn;
-  }
-#undef F3
-}
-
 // oplus_ng for one sub-step in one pass per column tile: the horizontal
 // stencils of the O+ prelude (b.grad of bdotdh_opj, diffj, bvel and dj*bz,
 // the vdotn term, the Shapiro smoothing of O+ at n-1) and the assembly and
-// elimination of oplus_stage3, level by level.  A block covers a TILE_X x
+// elimination of the O+ column (:172-340), level by level.  A block covers a TILE_X x
 // TILE_Y tile; at level k the tile and its 2-point halo of the six stencil
 // inputs are staged through double-buffered LDS.  Every thread of the block
 // takes part in the staging and the barriers; lateral-boundary and
@@ -3550,7 +3181,7 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
 #undef F3
 }
 
-// oplus_stage3 with one wavefront per column: rows 0..nk-2 by parallel
+// The O+ column (:172-340) with one wavefront per column: rows 0..nk-2 by parallel
 // cyclic reduction, the top level extrapolated (:307).
 __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                          const double *__restrict__ op,
@@ -3734,207 +3365,6 @@ struct UvTerms {  // prelude planes
   const double *divx_ztmp, *divy_z, *unms, *vnms, *advu, *advv;
 };
 
-template <bool CAP>
-__global__ void __launch_bounds__(256) duv_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                  tgx_ng_out out, double *__restrict__ gs,
-                                                  UvTerms UT) {
-  int i, j;
-  if (!column_of(G, i, j)) return;
-  const int nk = G.nk;
-  const double dz = g.dz, dtx2inv = g.dtx2inv;
-  const size_t c2 = G.a2(i, j);
-  out.ulbc_nm[c2] = in.ulbc[c2];
-  out.vlbc_nm[c2] = in.vlbc[c2];
-  out.ulbc[c2] = in.u_lbc[c2];
-  out.vlbc[c2] = in.v_lbc[c2];
-  double v;
-  if (lateral_value(G, in.lonb_un, in.latb_un, 0, i, j, v)) {
-    for (int k = 0; k < nk; ++k) {
-      size_t p = G.a3(k, i, j);
-      double vu, vv;
-      lateral_value(G, in.lonb_un, in.latb_un, k, i, j, vu);
-      lateral_value(G, in.lonb_vn, in.latb_vn, k, i, j, vv);
-      out.un[p] = vu;
-      out.vn[p] = vv;
-      out.un_nm[p] = DTSMOOTH_DIV2 * (in.un_nm[p] + vu) + DTSMOOTH * in.un[p];
-      out.vn_nm[p] = DTSMOOTH_DIV2 * (in.vn_nm[p] + vv) + DTSMOOTH * in.vn[p];
-    }
-    return;
-  }
-#define F3(f, k) (f)[G.a3((k), i, j)]
-  const double racs = g.racs[j], cor = g.cor[j], tanphi = g.tanphi[j];
-  // Fields averaged to interfaces (iavg) or used at two levels, carried one
-  // level in registers; each level's loads are issued before its arithmetic.
-  struct UL { double w, lxx, lyy, lxy, lyx, ui, vi, divx, divy, km, schti, scht; };
-  struct US { double Fe, Fn, unms, vnms, advu, advv, hdu, hdv, un, expz; };
-  auto load_ul = [&](int L, UL &U) {
-    size_t p = G.a3(L, i, j);
-    U.w = in.w_upd[p]; U.lxx = in.lxx[p]; U.lyy = in.lyy[p]; U.lxy = in.lxy[p]; U.lyx = in.lyx[p];
-    U.ui = in.ui[p]; U.vi = in.vi[p]; U.divx = UT.divx_ztmp[p]; U.divy = UT.divy_z[p];
-    U.km = in.km[p]; U.schti = in.schti[p]; U.scht = in.scht[p];
-  };
-  auto load_us = [&](int L, US &S) {
-    size_t p = G.a3(L, i, j);
-    S.Fe = out.Fe[p]; S.Fn = out.Fn[p]; S.unms = UT.unms[p]; S.vnms = UT.vnms[p];
-    S.advu = UT.advu[p]; S.advv = UT.advv[p]; S.hdu = in.hdu[p]; S.hdv = in.hdv[p]; S.un = in.un[p];
-    S.expz = g.expz[L];
-  };
-  // eddy viscosity (:143-152): e1b(L) from the level-L scale height
-  auto e1b = [&](double scht, int L) -> double {
-    double e = P0 * scht / GRAV;
-    return e * g.expz[L] * g.xmue[L];
-  };
-  auto gg = [&](const UL &U, double evf) { return GRAV * (U.km + evf) / (P0 * U.schti * (dz * dz)); };
-  double ulbc = in.u_lbc[c2], vlbc = in.v_lbc[c2];
-  UL ua, ub;
-  load_ul(0, ua);
-  load_ul(1, ub);
-  double e1_m = 0., e1_c = e1b(ua.scht, 0);            // e1 at levels k-1 and k
-  const double e1_1 = e1b(ub.scht, 1);                  // nk >= 4: level 1 is interior
-  double gk = gg(ua, e1_c * e1_c / sqrt(e1_c * e1_1));  // evf(0)
-  // block Thomas state (duv_ng.F90:218-243): beta(k-1), yy(k-1), rr(k-1)
-  double be11 = 0, be12 = 0, be21 = 0, be22 = 0, y1 = 0, y2 = 0;
-  double r11p = 0, r12p = 0, r21p = 0, r22p = 0;
-  const size_t NP = (size_t)nk * G.P;
-  for (int k = 0; k <= nk - 2; ++k) {
-    size_t p = G.a3(k, i, j);
-    US us;
-    if (k > 0) load_ul(k + 1, ub);
-    load_us(k, us);
-    asm volatile("" ::: "memory");
-    // e1(k+1): interior levels directly, the top one extrapolated (:148-150)
-    double e1_n = k + 1 <= nk - 2 ? e1b(ub.scht, k + 1) : sqrt(e1_c * e1_c * e1_c / e1_m);
-    double gk1 = gg(ub, sqrt(e1_c * e1_n));
-    double g_1 = gk1;  // k <= nk-2
-    const double expzk = us.expz;
-    double wi = .5 * (ua.w + ub.w);
-    double dwdz = wi / (2. * dz);
-    dwdz = dwdz * expzk;
-    double lxxi = .5 * (ua.lxx + ub.lxx), lyyi = .5 * (ua.lyy + ub.lyy);
-    double lxyi = .5 * (ua.lxy + ub.lxy), lyxi = .5 * (ua.lyx + ub.lyx);
-    double uii = .5 * (ua.ui + ub.ui), vii = .5 * (ua.vi + ub.vi);
-    double udc = us.Fe + lxxi * uii + lxyi * vii;
-    double vdc = us.Fn + lyyi * vii - lyxi * uii;
-    double zl = (ua.divx + ub.divx) * .5;
-    zl = zl * racs;
-    zl = zl * GRAV / g.dlamda;
-    double zp = (ua.divy + ub.divy) * .5;
-    zp = zp * GRAV * RE_INV / g.dphi;
-    double s1 = dtx2inv * us.unms + us.hdu + udc - us.advu - zl;
-    double s2 = dtx2inv * us.vnms + us.hdv + vdc - us.advv - zp;
-    s1 = s1 * expzk;
-    s2 = s2 * expzk;
-    double p11 = -(gk + dwdz), p12 = 0., p21 = 0., p22 = p11;
-    double r11 = -(g_1 - dwdz), r12 = 0., r21 = 0., r22 = r11;
-    double q11 = gk + g_1;
-    double qa = dtx2inv + lxxi;
-    qa = qa * expzk;
-    double qq11 = q11 + qa;
-    qa = dtx2inv + lyyi;
-    qa = qa * expzk;
-    double qq22 = q11 + qa;
-    double q12 = cor + us.un / RE * tanphi;
-    double q21 = q12;
-    qa = q12 - lxyi;
-    qa = qa * expzk;
-    double qq12 = -qa;
-    qa = q21 - lyxi;
-    qa = qa * expzk;
-    double qq21 = qa;
-    if (k == 0) {  // :207-212
-      qq11 = qq11 - p11; qq12 = qq12 - p12; qq21 = qq21 - p21; qq22 = qq22 - p22;
-      s1 = s1 - 2. * (p11 * ulbc + p12 * vlbc);
-      s2 = s2 - 2. * (p21 * ulbc + p22 * vlbc);
-      p11 = 0.; p12 = 0.; p21 = 0.; p22 = 0.;
-    }
-    if (k == nk - 2) {  // :214-215
-      qq11 = qq11 + r11; qq12 = qq12 + r12; qq21 = qq21 + r21; qq22 = qq22 + r22;
-      r11 = 0.; r12 = 0.; r21 = 0.; r22 = 0.;
-    }
-    if (CAP) {
-      const double Ab[4] = {p11, p12, p21, p22}, Bb[4] = {qq11, qq12, qq21, qq22}, Cb[4] = {r11, r12, r21, r22};
-      for (int e = 0; e < 4; ++e) {
-        cap_put(g, G, CAP_UV + e, i, j, k, Ab[e]);
-        cap_put(g, G, CAP_UV + 4 + e, i, j, k, Bb[e]);
-        cap_put(g, G, CAP_UV + 8 + e, i, j, k, Cb[e]);
-      }
-      cap_put(g, G, CAP_UV + 12, i, j, k, s1);
-      cap_put(g, G, CAP_UV + 13, i, j, k, s2);
-    }
-    if (k == 0) {
-      double det = qq11 * qq22 - qq12 * qq21;
-      be11 = qq22 / det; be12 = -qq12 / det; be21 = -qq21 / det; be22 = qq11 / det;
-      y1 = be11 * s1 + be12 * s2;
-      y2 = be21 * s1 + be22 * s2;
-    } else {
-      double g11 = be11 * r11p + be12 * r21p;
-      double g12 = be11 * r12p + be12 * r22p;
-      double g21 = be21 * r11p + be22 * r21p;
-      double g22 = be21 * r12p + be22 * r22p;
-      size_t pm = G.a3(k - 1, i, j);
-      gs[pm] = g11; gs[pm + NP] = g21; gs[pm + 2 * NP] = g12; gs[pm + 3 * NP] = g22;
-      double h11 = qq11 - p11 * g11 - p12 * g21;
-      double h12 = qq12 - p11 * g12 - p12 * g22;
-      double h21 = qq21 - p21 * g11 - p22 * g21;
-      double h22 = qq22 - p21 * g12 - p22 * g22;
-      double dk = h11 * h22 - h12 * h21;
-      be11 = h22 / dk; be12 = -h12 / dk; be21 = -h21 / dk; be22 = h11 / dk;
-      double x1 = s1 - p11 * y1 - p12 * y2;
-      double x2 = s2 - p21 * y1 - p22 * y2;
-      y1 = be11 * x1 + be12 * x2;
-      y2 = be21 * x1 + be22 * x2;
-    }
-    out.un[p] = y1;
-    out.vn[p] = y2;
-    r11p = r11; r12p = r12; r21p = r21; r22p = r22;
-    gk = gk1;
-    e1_m = e1_c;
-    e1_c = e1_n;
-    ua = ub;
-  }
-  // back substitution (:244-250) and outputs (:320-321); the next level's
-  // loads are in flight while this level is stored
-  struct UB { double x1, x2, g11, g21, g12, g22, unm, un, vnm, vn; };
-  auto load_ub = [&](int k, UB &B) {
-    size_t p = G.a3(k, i, j);
-    B.x1 = out.un[p]; B.x2 = out.vn[p];
-    if (k < nk - 2) { B.g11 = gs[p]; B.g21 = gs[p + NP]; B.g12 = gs[p + 2 * NP]; B.g22 = gs[p + 3 * NP]; }
-    else { B.g11 = B.g21 = B.g12 = B.g22 = 0.; }
-    B.unm = in.un_nm[p]; B.un = in.un[p]; B.vnm = in.vn_nm[p]; B.vn = in.vn[p];
-  };
-  const double unm_top = F3(in.un_nm, nk - 1), un_top = F3(in.un, nk - 1);
-  const double vnm_top = F3(in.vn_nm, nk - 1), vn_top = F3(in.vn, nk - 1);
-  double xn1 = y1, xn2 = y2;
-  UB bc, bn;
-  load_ub(nk - 2, bc);
-  for (int k = nk - 2; k >= 0; --k) {
-    if (k > 0) load_ub(k - 1, bn);
-    asm volatile("" ::: "memory");
-    double x1, x2;
-    if (k == nk - 2) { x1 = xn1; x2 = xn2; }
-    else {
-      x1 = bc.x1 - bc.g11 * xn1 - bc.g12 * xn2;
-      x2 = bc.x2 - bc.g21 * xn1 - bc.g22 * xn2;
-    }
-    if (k == nk - 3) {
-      size_t pt = G.a3(nk - 1, i, j);
-      double u = 2 * xn1 - x1, vv = 2 * xn2 - x2;
-      out.un[pt] = u;
-      out.vn[pt] = vv;
-      out.un_nm[pt] = DTSMOOTH_DIV2 * (unm_top + u) + DTSMOOTH * un_top;
-      out.vn_nm[pt] = DTSMOOTH_DIV2 * (vnm_top + vv) + DTSMOOTH * vn_top;
-    }
-    size_t p = G.a3(k, i, j);
-    out.un[p] = x1;
-    out.vn[p] = x2;
-    out.un_nm[p] = DTSMOOTH_DIV2 * (bc.unm + x1) + DTSMOOTH * bc.un;
-    out.vn_nm[p] = DTSMOOTH_DIV2 * (bc.vnm + x2) + DTSMOOTH * bc.vn;
-    xn1 = x1; xn2 = x2;
-    bc = bn;
-  }
-#undef F3
-}
-
 // duv_ng in one pass per column tile: the vertical integral ztmp of stage 1
 // (:73-80), the horizontal stencils (divergence of ztmp and z for the
 // pressure gradient, Shapiro smoothing of UN_NM/VN_NM, advection of UN/VN)
@@ -4041,8 +3471,7 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
     divy_c = divy(P0z, G, i, j);
   }
   tile_store(0);  // buffer 0; buffer 1 is read above before the next barrier
-  // stage-2 fields at one level (as in duv_stage2; the divergences come from
-  // the tiles)
+  // fields of the column at one level (the divergences come from the tiles)
   struct UL { double w, lxx, lyy, lxy, lyx, ui, vi, km, schti, scht; };
   struct US { double Fe, Fn, hdu, hdv; };
   auto load_ul = [&](int L, UL &U) {
@@ -4212,7 +3641,7 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
     if (k + 1 <= nk - 2) tile_store((k + 1) & 1);
   }
   if (!solve) return;
-  // back substitution (:244-250) and outputs (:320-321), as in duv_stage2
+  // back substitution (:244-250) and outputs (:320-321)
   struct UB { double x1, x2, g11, g21, g12, g22, unm, un, vnm, vn; };
   auto load_ub = [&](int k, UB &B) {
     size_t p = G.a3(k, i, j);
@@ -4254,7 +3683,7 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
 #undef F3
 }
 
-// duv_stage2 with one wavefront per column: the 2x2 block rows 0..nk-2 by
+// duv_ng stage 2 with one wavefront per column: the 2x2 block rows 0..nk-2 by
 // block cyclic reduction, the top level extrapolated (:250).
 __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                        tgx_ng_out out, UvTerms UT) {
@@ -4446,12 +3875,6 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     return jb;
   };
   switch (which) {
-    case 0:  // dt: bufs = {w2, hadv_tn, smooth_tn_nm}
-      if (g.capture)
-        hipLaunchKernelGGL(dt_kernel<true>, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
-      else
-        hipLaunchKernelGGL(dt_kernel<false>, grid, block, 0, s, g, G, in, out, D(0), D(1), D(2));
-      break;
     case 20:
       hipLaunchKernelGGL(dt_wave_kernel, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
                          (const double *)D(2));
@@ -4532,7 +3955,6 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     }
     case 3:
     case 4:
-    case 5:
     case 9:
     case 17:
     case 21:
@@ -4570,13 +3992,6 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
           else
             hipLaunchKernelGGL(oplus_tile_kernel<false>, gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                                (const double *)D(19), (const double *)D(20), opout, optm1out);
-        else if (which == 5)
-          if (g.capture)
-            hipLaunchKernelGGL(oplus_stage3<true>, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
-                               (const double *)D(19), (const double *)D(20), opout, optm1out);
-          else
-            hipLaunchKernelGGL(oplus_stage3<false>, grid, block, 0, s, g, G, in, op, optm1, lonb, latb, W, OT,
-                               (const double *)D(19), (const double *)D(20), opout, optm1out);
         else
           hipLaunchKernelGGL(oplus_wave_stage3, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                              OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
@@ -4586,16 +4001,9 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 6:  // duv stage 1: bufs = {ztmp}
       hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
       break;
-    case 7:
-    case 22: {  // duv stage 2: bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
+    case 22: {  // duv stage 2 (PCR): bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
       UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
-      if (which == 7)
-        if (g.capture)
-          hipLaunchKernelGGL(duv_stage2<true>, grid, block, 0, s, g, G, in, out, D(1), UT);
-        else
-          hipLaunchKernelGGL(duv_stage2<false>, grid, block, 0, s, g, G, in, out, D(1), UT);
-      else
-        hipLaunchKernelGGL(duv_wave_stage2, gridw, block, 0, s, g, G, in, out, UT);
+      hipLaunchKernelGGL(duv_wave_stage2, gridw, block, 0, s, g, G, in, out, UT);
       break;
     }
     case 8:

@@ -131,6 +131,8 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
   return 0;
 }
 
+// Kernel ids of tgx_ng_launch (0, 5 and 7 are retired: the Thomas dt, O+ and
+// duv column kernels are the tile kernels K_DT_TILE, K_OP_TILE, K_UV_TILE).
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
