@@ -101,7 +101,7 @@ def bench_nests(pkg, args):
     for nm in names:
         g = pkg.synth.make_grid(nm, nstep_sub=args.nstep_sub)
         doms.append((g, pkg.synth.make_state(g, device="cuda")))
-    ns = pkg.step.NestedStep(doms, [1, 2, 5], scheme=args.scheme)
+    ns = pkg.step.NestedStep(doms, [1, 2, 5], scheme=args.scheme, forked=not args.serial)
     ns.capture()
     for _ in range(args.warmup):
         ns.replay()
@@ -126,7 +126,8 @@ def bench_nests(pkg, args):
                                "(NSTEP_NG=5 per nest-1 step), 57 levels, %d unknowns, one outer step per "
                                "HIP graph replay" % unknowns,
                    "substeps": {names[i]: n for i, n in sub.items()}, "column_levels_per_step": colev,
-                   "scheme": args.scheme, "schemes": ns.states[0].schemes()},
+                   "scheme": args.scheme, "schemes": ns.states[0].schemes(),
+                   "schedule": "serial" if args.serial else "concurrent"},
         "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole outer step (graph replay)", "alg_bytes_per_launch": alg,

@@ -175,7 +175,8 @@ def test_profile_hooks_count_calls(pkg):
         assert kn == 2 and 0.0 < km <= ms, (r, kern[r], prof[r])
 
 
-def test_nested_outer_step_graph_replay_matches_eager(pkg):
+@pytest.mark.parametrize("forked", [False, True], ids=["serial", "forked"])
+def test_nested_outer_step_graph_replay_matches_eager(pkg, forked):
     """The global grid and two nests stepped in the reference's recursion
     (level i runs prod(nstep_ng[:i+1]) sub-steps) captured into one HIP graph:
     every replay reproduces the eager outer step bit for bit, with eager work
@@ -185,7 +186,7 @@ def test_nested_outer_step_graph_replay_matches_eager(pkg):
     for name, nsub in (("small", 1), ("tiny", 2), ("tall", 1)):
         g = pkg.synth.make_grid(name, nstep_sub=nsub)
         doms.append((g, pkg.synth.make_state(g)))
-    ns = pkg.step.NestedStep(doms, [1, 2, 3])
+    ns = pkg.step.NestedStep(doms, [1, 2, 3], forked=forked)
     assert ns.substeps() == {0: 1, 1: 2, 2: 6}
     ns.run()
     torch.cuda.synchronize()

@@ -243,11 +243,14 @@ class NestedStep:
     run() launches the sequence eagerly on the current stream; capture() records
     it once into a HIP graph and replay() launches the whole outer step as one
     graph.  AUTO schemes are tuned by a first eager run before capture.
+    forked: each domain step is tgx_ng_step (routines on forked streams,
+    joined before the next sub-step) rather than tgx_ng_step_serial.
     """
 
-    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda"):
+    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda", forked=True):
         self.states = [NestState(g, F, device=device, scheme=scheme) for g, F in domains]
         self.nstep_ng = list(nstep_ng)
+        self.forked = forked
         self.graph = None
         self.order = []  # domain index of each sub-step, in launch order
 
@@ -263,7 +266,10 @@ class NestedStep:
 
     def run(self):
         for i in self.order:
-            self.states[i].step_serial()
+            if self.forked:
+                self.states[i].step()
+            else:
+                self.states[i].step_serial()
 
     def capture(self):
         torch.cuda.synchronize()
