@@ -61,7 +61,7 @@ def cpu_baseline(pkg, config, nstep_sub, budget_s=20.0):
     Fo = pkg.synth.to_fortran_layout(pkg.synth.make_state(g))
     cols = g["nlon"] * g["nlat"]
     reps, t_total = 0, 0.0
-    while t_total < budget_s and reps < 50:
+    while t_total < budget_s and reps < 2000:
         t0 = time.perf_counter()
         pkg.step.run_oracle_step(oracle, g, Fo)
         t_total += time.perf_counter() - t0
@@ -219,7 +219,7 @@ def main():
     ap.add_argument("--config", default="g0p625")
     ap.add_argument("--nstep-sub", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
     ap.add_argument("--scheme", default="auto", choices=("thomas", "pcr", "auto"),
                     help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
