@@ -204,3 +204,41 @@ def test_nested_outer_step_graph_replay_matches_eager(pkg, forked):
                 assert torch.equal(t, eager[d][n]), (rep, d, n)
         ns.run()  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_step_from_two_host_threads(pkg):
+    """Two host threads stepping two domains at once, each on its own torch
+    stream, give the outputs of the same steps taken one after the other."""
+    import threading
+    import torch
+    doms = []
+    for name in ("small", "tiny"):
+        g = pkg.synth.make_grid(name, nstep_sub=2)
+        doms.append((g, pkg.synth.make_state(g)))
+    ref = [pkg.step.NestState(g, F) for g, F in doms]
+    for st in ref:
+        st.step()
+    torch.cuda.synchronize()
+    par = [pkg.step.NestState(g, F) for g, F in doms]
+    errs = []
+
+    def work(st):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(3):
+                    st.step(stream=s.cuda_stream)
+            s.synchronize()
+        except Exception as e:  # reported in the main thread
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(st,)) for st in par]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    torch.cuda.synchronize()
+    for a, b in zip(ref, par):
+        for n in a.out:
+            assert torch.equal(a.out[n], b.out[n]), n

@@ -340,19 +340,21 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
 // run on four streams forked from the caller's stream; duv needs dt's TN and
 // oplus' Fe/Fn and joins both.  All streams have the default priority.
 namespace {
+// The fork streams and events belong to the calling host thread (one set per
+// thread and device): calls from several threads never share an event, and
+// calls from one thread are ordered through its streams.  They live until the
+// process ends.
 struct Fork {
   hipStream_t s[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev[8] = {};
 };
-std::mutex g_fork_mu;
-std::map<int, Fork> g_fork;
+thread_local std::map<int, Fork> t_fork;
 
 int get_fork(Fork **out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return tgx_set_error(3, "no HIP device: %s", hipGetErrorString(e));
-  std::lock_guard<std::mutex> lk(g_fork_mu);
-  Fork &f = g_fork[dev];
+  Fork &f = t_fork[dev];
   if (!f.s[0]) {
     for (int i = 0; i < 3; ++i) TGX_HIP(hipStreamCreateWithFlags(&f.s[i], hipStreamNonBlocking));
     for (int i = 0; i < 8; ++i) TGX_HIP(hipEventCreateWithFlags(&f.ev[i], hipEventDisableTiming));
