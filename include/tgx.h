@@ -12,9 +12,10 @@
  *     0 on success or a TGX_ERR_* code (message in tgx_last_error()).
  *  3. The fused column step on device-resident model state (tgx_step.h).
  *
- * Device entry points that need scratch (Thomas ratios, block gammas) share
- * one library-owned buffer per device: calls on one device must be ordered
- * (one stream, or synchronised streams).
+ * Device entry points that need scratch (Thomas ratios, block gammas) use a
+ * library-owned buffer per calling host thread, device and stream, so calls
+ * from several host threads, or on several streams, may overlap.  The
+ * buffers are held until the process ends.
  */
 #ifndef TGX_H
 #define TGX_H

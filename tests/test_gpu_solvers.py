@@ -163,3 +163,59 @@ def test_error_paths_do_not_abort(pkg):
     a = np.zeros((3, 5))
     with pytest.raises(pkg.tgx.TgxError):
         pkg.tgx.trsolv(a, a, a, a, a.copy(), 1, 5, 0, 9, 1, 3, 3)
+
+
+def test_solvers_from_two_host_threads_and_two_streams(pkg):
+    """Thomas and block solves (both use library scratch) queued from two host
+    threads at once, and from one thread on two streams without a wait in
+    between, give bitwise the results of the same solves run one by one."""
+    import threading
+    import torch
+    from test_oracle_solvers import random_blocks, to_colmajor
+    rng = np.random.default_rng(21)
+    jobs = []
+    for ncol, nk in ((6000, 113), (3000, 57)):
+        a, b, c, f = config1_system(rng, ncol, nk)
+        P, Q, R, F = random_blocks(rng, ncol // 4, 29, 3)
+        jobs.append(([torch_dev(v) for v in (a, b, c, f)], nk,
+                     [torch_dev(v) for v in (to_colmajor(P), to_colmajor(Q), to_colmajor(R),
+                                             np.ascontiguousarray(F))], ncol // 4))
+
+    def run(job, stream=None):
+        (sys_, nk, blk, nb) = job
+        x = torch.zeros(sys_[0].shape, dtype=torch.float64, device="cuda")
+        y = torch.zeros((nb, 29, 3), dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            pkg.tgx.trsolv_dev(*sys_, x, 0, nk - 2, scheme=1, stream=stream)
+            pkg.tgx.blktri3_dev(*blk, y, 28, stream=stream)
+        return x, y
+
+    ref = [run(j) for j in jobs]
+    torch.cuda.synchronize()
+
+    out, errs = [None, None], []
+
+    def work(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                out[i] = run(jobs[i], stream=s.cuda_stream)
+            s.synchronize()
+        except Exception as e:  # reported in the main thread
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(s1):
+        o1 = run(jobs[0], stream=s1.cuda_stream)
+    with torch.cuda.stream(s2):
+        o2 = run(jobs[1], stream=s2.cuda_stream)
+    torch.cuda.synchronize()
+    for got in (out, [o1, o2]):
+        for (rx, ry), (gx, gy) in zip(ref, got):
+            assert torch.equal(rx, gx) and torch.equal(ry, gy)

@@ -45,19 +45,27 @@ extern "C" int tgx_launch_blktri2_f64(const double *, const double *, const doub
 extern "C" int tgx_launch_blktri3_f64(const double *, const double *, const double *, const double *,
                                       double *, double *, int, int, int, int, hipStream_t);
 
-// ---- per-device workspace ----------------------------------------------------
-// Scratch for the device entry points (Thomas ratios, block gammas) and
-// staging buffers for the host drop-ins.  Grown on demand, kept per device.
+// ---- per-thread workspace ------------------------------------------------------
+// Scratch for the device entry points (Thomas ratios, block gammas) is kept
+// per calling host thread, device and stream: work on one stream runs in
+// order, so a buffer reused by the next call on the same stream is never
+// read by a kernel still in flight, and calls from other threads or on other
+// streams get buffers of their own.  Staging buffers and the copy stream of
+// the host drop-ins (which return only when their copies are done) are kept
+// per host thread and device.  Buffers are grown on demand and held until the
+// process ends.
 namespace {
-struct DevSpace {
-  void *scratch = nullptr;
-  size_t scratch_bytes = 0;
-  void *stage = nullptr;
-  size_t stage_bytes = 0;
+struct Buf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+struct Stage {
+  Buf buf;
   hipStream_t stream = nullptr;
 };
-std::mutex g_mu;
-std::map<int, DevSpace> g_dev;
+std::mutex g_mu;  // guards the autotuner's table
+thread_local std::map<std::pair<int, hipStream_t>, Buf> t_scratch;
+thread_local std::map<int, Stage> t_stage;
 
 int current_device(int *dev) {
   hipError_t e = hipGetDevice(dev);
@@ -65,42 +73,44 @@ int current_device(int *dev) {
   return 0;
 }
 
-int grow(void **p, size_t *have, size_t need) {
-  if (*have >= need) return 0;
-  if (*p) {
-    hipError_t e = hipFree(*p);
+// Grows b to at least need bytes.  The old buffer may still be read by work
+// queued on stream s, so s is drained before it is freed.
+int grow(Buf *b, size_t need, hipStream_t s) {
+  if (b->bytes >= need) return 0;
+  if (b->p) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return tgx_set_error(2, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    e = hipFree(b->p);
     if (e != hipSuccess) return tgx_set_error(2, "hipFree: %s", hipGetErrorString(e));
-    *p = nullptr;
-    *have = 0;
+    b->p = nullptr;
+    b->bytes = 0;
   }
   size_t sz = need + need / 4 + 4096;
-  hipError_t e = hipMalloc(p, sz);
+  hipError_t e = hipMalloc(&b->p, sz);
   if (e != hipSuccess) return tgx_set_error(2, "hipMalloc(%zu): %s", sz, hipGetErrorString(e));
-  *have = sz;
+  b->bytes = sz;
   return 0;
 }
 
-int get_scratch(size_t bytes, void **out) {
+int get_scratch(size_t bytes, hipStream_t s, void **out) {
   int dev;
   if (int rc = current_device(&dev)) return rc;
-  std::lock_guard<std::mutex> lk(g_mu);
-  DevSpace &d = g_dev[dev];
-  if (int rc = grow(&d.scratch, &d.scratch_bytes, bytes)) return rc;
-  *out = d.scratch;
+  Buf &b = t_scratch[{dev, s}];
+  if (int rc = grow(&b, bytes, s)) return rc;
+  *out = b.p;
   return 0;
 }
 
 int get_stage(size_t bytes, void **out, hipStream_t *s) {
   int dev;
   if (int rc = current_device(&dev)) return rc;
-  std::lock_guard<std::mutex> lk(g_mu);
-  DevSpace &d = g_dev[dev];
+  Stage &d = t_stage[dev];
   if (!d.stream) {
     hipError_t e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
     if (e != hipSuccess) return tgx_set_error(2, "hipStreamCreate: %s", hipGetErrorString(e));
   }
-  if (int rc = grow(&d.stage, &d.stage_bytes, bytes)) return rc;
-  *out = d.stage;
+  if (int rc = grow(&d.buf, bytes, d.stream)) return rc;
+  *out = d.buf.p;
   *s = d.stream;
   return 0;
 }
@@ -122,7 +132,7 @@ int run_scalar(int scheme, const double *a, const double *b, const double *c, co
   if (scheme == TGX_SCHEME_PCR) return tgx_launch_pcr(a, b, c, f, x, ldk, k1, k2, col0, ncol, 0, 0, s);
   void *w2 = nullptr;
   size_t need = sizeof(double) * (size_t)ldk * (size_t)(col0 + ncol);
-  if (int rc = get_scratch(need, &w2)) return rc;
+  if (int rc = get_scratch(need, s, &w2)) return rc;
   return tgx_launch_thomas_f64(a, b, c, f, x, (double *)w2, ldk, k1, k2, col0, ncol, s);
 }
 
@@ -201,7 +211,7 @@ extern "C" int tgx_blktri2_dev_f64(const double *a, const double *b, const doubl
   if (ldk <= 0 || nrow < 1 || nrow > ldk || col0 < 0 || ncol < 0)
     return tgx_set_error(1, "tgx_blktri2_dev_f64: bad shape");
   void *gs = nullptr;
-  if (int rc = get_scratch(sizeof(double) * 4 * (size_t)ldk * (size_t)(col0 + ncol), &gs)) return rc;
+  if (int rc = get_scratch(sizeof(double) * 4 * (size_t)ldk * (size_t)(col0 + ncol), (hipStream_t)stream, &gs)) return rc;
   return tgx_launch_blktri2_f64(a, b, c, f, x, (double *)gs, ldk, nrow, col0, ncol, (hipStream_t)stream);
 }
 
@@ -211,7 +221,7 @@ extern "C" int tgx_blktri3_dev_f64(const double *p, const double *q, const doubl
   if (ldk <= 0 || nrow < 1 || nrow > ldk || col0 < 0 || ncol < 0)
     return tgx_set_error(1, "tgx_blktri3_dev_f64: bad shape");
   void *gs = nullptr;
-  if (int rc = get_scratch(sizeof(double) * 9 * (size_t)ldk * (size_t)(col0 + ncol), &gs)) return rc;
+  if (int rc = get_scratch(sizeof(double) * 9 * (size_t)ldk * (size_t)(col0 + ncol), (hipStream_t)stream, &gs)) return rc;
   return tgx_launch_blktri3_f64(p, q, r, f, x, (double *)gs, ldk, nrow, col0, ncol, (hipStream_t)stream);
 }
 
