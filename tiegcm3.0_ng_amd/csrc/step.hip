@@ -2122,7 +2122,6 @@ struct CompTerms {  // prelude planes: advection of O2, O, He, smoothed *_nm, an
                     // fs(:, isp, m) of comp_o2o_ng as fs01 fs02 fs11 fs12 fs00 fs10
   const double *rhs[3];  // the prelude's smooth(*_nm)*dtx2inv - (advec - fs(:,:,0)) of O2, O, He
   const double *fs[6];
-  double *zs;  // comp_kernel: the forward solution z, 3 planes (O2, O, He)
 };
 
 // fs(k, isp, m) of comp_o2o_ng.F90:44-81 at the levels comp_ng uses (k <
@@ -2273,7 +2272,11 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
     double gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
     double alpha22 = 0, alpha33 = 0, alpha23 = 0, alpha32 = 0, flx00 = 0;
-    const size_t GP = (size_t)nk * G.P;
+    // The elimination's gamma (9) and forward solution z (3) of level k are
+    // stored as one tile per wavefront: 12 runs of 64 doubles, one per value,
+    // lane fastest, so a wavefront's values of a level share a 6 KiB block.
+    const long col = (long)j * G.nlond + i, ngw = ((long)G.nlond * G.nlatd + 63) / 64;
+    auto tile = [&](int k, int n) { return (((size_t)k * ngw + (col >> 6)) * 12 + n) * 64 + (col & 63); };
     for (int k = 0; k < nk - 1; ++k) {
       // every load of this level is issued before any of its arithmetic, so
       // the level costs one memory round trip
@@ -2366,9 +2369,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       for (int m = 0; m < 3; ++m)
         for (int isp = 0; isp < 3; ++isp) nz[isp] = nz[isp] + wkm1[isp + 3 * m] * wkv1[m];
       for (int n = 0; n < 9; ++n) gama[n] = ng[n];
-      for (int n = 0; n < 9; ++n) gs[(size_t)n * GP + G.a3(k, i, j)] = ng[n];
+      for (int n = 0; n < 9; ++n) gs[tile(k, n)] = ng[n];
       for (int isp = 0; isp < 3; ++isp) zz[isp] = nz[isp];
-      for (int isp = 0; isp < 3; ++isp) CT.zs[(size_t)isp * GP + G.a3(k, i, j)] = nz[isp];
+      for (int isp = 0; isp < 3; ++isp) gs[tile(k, 9 + isp)] = nz[isp];
       Qc = Qn;
       r1 = r2;
     }
@@ -2381,8 +2384,8 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     struct BS { double z[3], gm[9], o2nm, o1nm, henm, o2, o1, he, xn; };
     auto load_bs = [&](int k, BS &B) {
       size_t p = G.a3(k, i, j);
-      for (int isp = 0; isp < 3; ++isp) B.z[isp] = CT.zs[(size_t)isp * GP + p];
-      for (int n = 0; n < 9; ++n) B.gm[n] = gs[(size_t)n * GP + p];
+      for (int isp = 0; isp < 3; ++isp) B.z[isp] = gs[tile(k, 9 + isp)];
+      for (int n = 0; n < 9; ++n) B.gm[n] = gs[tile(k, n)];
       B.o2nm = in.o2_nm[p]; B.o1nm = in.o1_nm[p]; B.henm = in.he_nm[p];
       B.o2 = in.o2[p]; B.o1 = in.o1[p]; B.he = in.he[p];
       B.xn = in.xnmbar[p];
@@ -4008,10 +4011,9 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     }
     case 8:
     case 14:
-    case 23: {  // comp: bufs = {gamma, rhs[3], (unused x3), fs[6], zs (comp_kernel)}
-      CompTerms CT{{D(1), D(2), D(3)}, {D(7), D(8), D(9), D(10), D(11), D(12)}, nullptr};
+    case 23: {  // comp: bufs = {gamma and z tiles, rhs[3], (unused x3), fs[6]}
+      CompTerms CT{{D(1), D(2), D(3)}, {D(7), D(8), D(9), D(10), D(11), D(12)}};
       if (which == 8) {
-        CT.zs = D(13);
         if (g.capture)
           hipLaunchKernelGGL(comp_kernel<true>, grid, block, 0, s, g, G, in, out, D(0), CT);
         else

@@ -38,7 +38,7 @@ enum {
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
-  W_CO = W_UV + 11,                      // comp: 9 gamma, 6 prelude, 6 fs, 3 forward-solution planes
+  W_CO = W_UV + 11,                      // comp: 6 prelude, 6 fs, 12 elimination planes (last)
   W_NPLANES = W_CO + 24
 };
 
@@ -66,7 +66,11 @@ double *plane(void *work, const tgx_ng_grid *g, int idx) {
 
 extern "C" unsigned long long tgx_ng_workspace_bytes(const tgx_ng_grid *g) {
   if (!g) return 0;
-  return (unsigned long long)W_NPLANES * plane_elems(g) * sizeof(double);
+  // the comp elimination planes are the workspace's last and are stored in
+  // tiles of 64 columns, which can overrun 12 planes by less than 64 columns
+  // per level and plane
+  return ((unsigned long long)W_NPLANES * plane_elems(g) + 12ull * 64ull * (unsigned long long)g->nk) *
+         sizeof(double);
 }
 
 // Optional per-routine timing: hipEvent pairs recorded on the stream each
@@ -183,7 +187,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
 namespace {
 // Buffers of chem_kernel / minor_lin_kernel (the step-wide chemistry pass).
 void chem_bufs(void *work, const tgx_ng_grid *g, void **b) {
-  for (int q = 0; q < 6; ++q) b[q] = plane(work, g, W_CO + 15 + q);  // comp fs planes
+  for (int q = 0; q < 6; ++q) b[q] = plane(work, g, W_CO + 6 + q);  // comp fs planes
   b[6] = plane(work, g, W_MIN_S + 0);                                // N2D sloss (then NO's)
   b[7] = plane(work, g, W_MIN_S + 1);                                // N2D s0prod (then NO's)
   b[8] = plane(work, g, W_MIN_S + 2);                                // N4S sloss
@@ -322,9 +326,9 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_COMP, s);
   void *b[14];
-  b[0] = plane(work, g, W_CO);
-  for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + 9 + q);
-  b[13] = plane(work, g, W_CO + 21);  // forward solution z (3 planes)
+  b[0] = plane(work, g, W_CO + 12);  // Thomas gamma and forward solution, tiled per wavefront
+  for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + q);
+  b[13] = nullptr;
   // comp_o2o_ng's fs planes first: the prelude folds fs(:,:,0) into the
   // right-hand sides
   if (!chem_done)
