@@ -45,10 +45,19 @@ __device__ __forceinline__ double PHI(int r, int c) {
 struct Geo {
   int nk, nlon, nlat, nlond, nlatd, ld;
   size_t P;
+  long ngw;  // groups of 64 columns (j * nlond + i) in a tiled workspace region
   __device__ __forceinline__ size_t a3(int k, int i, int j) const {
     return (size_t)k * P + (size_t)j * ld + i;
   }
   __device__ __forceinline__ size_t a2(int i, int j) const { return (size_t)j * ld + i; }
+  // Element of value n (of nv) at level k of column (i, j) in a tiled
+  // workspace region: per level and group of 64 consecutive columns, nv runs
+  // of 64 doubles, column fastest, so the values a wavefront of a column
+  // kernel reads or writes at one level share one block of nv * 512 bytes.
+  __device__ __forceinline__ size_t tile(int k, int i, int j, int n, int nv) const {
+    const long c = (long)j * nlond + i;
+    return (((size_t)k * ngw + (size_t)(c >> 6)) * nv + n) * 64 + (size_t)(c & 63);
+  }
 };
 
 // Rows assembled by the Thomas column kernels, written only when the caller
@@ -66,6 +75,7 @@ static Geo make_geo(const tgx_ng_grid &g) {
   G.nk = g.nk; G.nlon = g.nlon; G.nlat = g.nlat;
   G.nlond = g.nlon + 4; G.nlatd = g.nlat + 4; G.ld = g.ld;
   G.P = (size_t)g.ld * (size_t)(g.nlat + 4);
+  G.ngw = ((long)G.nlond * G.nlatd + 63) / 64;
   return G;
 }
 
@@ -288,6 +298,7 @@ struct StencilJobs {
   int n;
   StencilJob job[8];
   const double *aux[3];  // STC_COMP: the fs(:,:,0) planes the right-hand sides take
+  double *ci;            // STC_COMP: the tiled comp inputs (values 0-2 are written)
 };
 // How a prelude stores its jobs: each job its own plane (STC_NONE), or the
 // smoothed and advected planes of one species combined into the part of the
@@ -297,6 +308,9 @@ struct StencilJobs {
 //   STC_COMP (jobs 0-2 advect O2/O/He, 3-5 smooth *_nm, comp_ng.F90:191-193):
 //     out[s] = smooth[s]*dtx2inv - (advec[s] - fs(s,0))   (He: no fs term)
 enum { STC_NONE = 0, STC_MINOR, STC_COMP };
+// Tiled comp inputs: the right-hand-side parts of O2, O, He (0-2) and
+// fs(:,1,1), fs(:,1,2), fs(:,2,1), fs(:,2,2) of comp_o2o_ng (3-6).
+constexpr int CI_NV = 7;
 
 // Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
 // planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
@@ -455,7 +469,7 @@ __global__ void __launch_bounds__(256) stencil_run_kernel(tgx_ng_grid g, Geo G, 
       } else if (COMB == STC_COMP) {
 #pragma unroll
         for (int sp = 0; sp < 3; ++sp)
-          J.job[sp].out[p] = vv[3 + sp] * g.dtx2inv - (vv[sp] - (sp < 2 ? J.aux[sp][p] : 0.));
+          J.ci[G.tile(k, i, j, sp, CI_NV)] = vv[3 + sp] * g.dtx2inv - (vv[sp] - (sp < 2 ? J.aux[sp][p] : 0.));
       }
     }
     __syncthreads();  // the next level overwrites the tiles and buffers
@@ -1211,7 +1225,8 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 //   O+ xiop2p, xiop2d, loss and production (oplus_ng.F90:288-299).
 // All values are computed before any is stored, so shared fields load once.
 struct ChemOut {
-  double *fs[6];                       // comp_o2o fs planes
+  double *ci;                          // tiled comp inputs (fs(:,1:2,1:2) written)
+  double *fs0[2];                      // fs(:,1,0), fs(:,2,0) planes of comp_o2o
   double *n2d_sloss, *n2d_s0prod;      // minor-species source planes of N2D
   double *n4s_sloss;                   // N4S loss (independent of N2D)
   double *lin[6];                      // N4S s0prod a, b; NO sloss a, b; NO s0prod a, b
@@ -1245,7 +1260,8 @@ __host__ __device__ inline int chem_runs(int nk) { return (nk + CHEM_RUN - 1) / 
 // One level of the chemistry pass: c = the paired fields at this level, o =
 // at the level above (below at the top level).
 __device__ __forceinline__ void chem_level(const tgx_ng_fields &in, const tgx_ng_out &out, const ChemOut &C,
-                                           unsigned p, bool top, const ChemPair &c, const ChemPair &o) {
+                                           unsigned p, size_t tci, bool top, const ChemPair &c,
+                                           const ChemPair &o) {
   const double xn = ldo(in.xnmbar, p), ne = c.ne;
   const double o1 = ldo(in.o1, p), o2 = ldo(in.o2, p), n2 = ldo(in.n2, p), op = ldo(in.op, p), no = ldo(in.no, p), n4s = ldo(in.n4s, p);
   const double n2d = ldo(in.n2d, p), o2p = ldo(in.o2p, p), nplus = ldo(in.nplus, p), n2p = ldo(in.n2p, p), nop = ldo(in.nop, p);
@@ -1340,8 +1356,11 @@ __device__ __forceinline__ void chem_level(const tgx_ng_fields &in, const tgx_ng
     opprod = qopi + (rk19 * ne + RK21) * x2p + (rk25 * ne + RK27) * x2d + (RK18 * x2p + RK24 * x2d) * o1 * RIO1 * xn;
   }
   // ---- stores
-  if (!top)
-    for (int q = 0; q < 6; ++q) sto(C.fs[q], p, fsv[q]);
+  if (!top) {
+    for (int q = 0; q < 4; ++q) C.ci[tci + (size_t)(3 + q) * 64] = fsv[q];
+    sto(C.fs0[0], p, fsv[4]);
+    sto(C.fs0[1], p, fsv[5]);
+  }
   sto(C.n2d_sloss, p, n2d_sl);
   sto(C.n2d_s0prod, p, n2d_sp * RMN2D / xn);
   sto(C.n4s_sloss, p, n4s_sl);
@@ -1386,7 +1405,7 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       chem_pair_load(in, p + PB, o);
     else
       o = d;
-    chem_level(in, out, C, p, top, c, o);
+    chem_level(in, out, C, p, G.tile(k, i, j, 0, CI_NV), top, c, o);
     d = c;
     c = o;
   }
@@ -2118,10 +2137,12 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
 
 // =============================================================================
 // comp_o2o_ng.F90:44-81 (fused) + comp_ng.F90:71-457
-struct CompTerms {  // prelude planes: advection of O2, O, He, smoothed *_nm, and
-                    // fs(:, isp, m) of comp_o2o_ng as fs01 fs02 fs11 fs12 fs00 fs10
-  const double *rhs[3];  // the prelude's smooth(*_nm)*dtx2inv - (advec - fs(:,:,0)) of O2, O, He
-  const double *fs[6];
+struct CompTerms {
+  // tiled comp inputs (CI_NV values): the prelude's smooth(*_nm)*dtx2inv -
+  // (advec - fs(:,:,0)) of O2, O, He, then fs(:,1,1), fs(:,1,2), fs(:,2,1),
+  // fs(:,2,2) of comp_o2o_ng
+  double *ci;
+  double *fs0[2];  // fs(:,1,0), fs(:,2,0) planes (read by the prelude)
 };
 
 // fs(k, isp, m) of comp_o2o_ng.F90:44-81 at the levels comp_ng uses (k <
@@ -2157,12 +2178,12 @@ __global__ void __launch_bounds__(256) o2o_kernel(tgx_ng_grid g, Geo G, tgx_ng_f
                 RK9 * in.n2p[p] + rji;
   double lo22 = qo2pi;
   double o1 = in.o1[p];
-  ((double *)CT.fs[0])[p] = -lo21;                                   // fs(k,1,1)
-  ((double *)CT.fs[1])[p] = xn * po21 * o1 * (RIO1 * RIO1) * RMO2;   // fs(k,1,2)
-  ((double *)CT.fs[2])[p] = pox2 * RMO1 * RIO2;                      // fs(k,2,1)
-  ((double *)CT.fs[3])[p] = -lox2 - lox1 * o1 * RIO1 * xn;           // fs(k,2,2)
-  ((double *)CT.fs[4])[p] = (po23 - lo22) * RMO2 / xn;               // fs(k,1,0)
-  ((double *)CT.fs[5])[p] = (pox1 - lox3) * RMO1 / xn;               // fs(k,2,0)
+  CT.ci[G.tile(k, i, j, 3, CI_NV)] = -lo21;                                  // fs(k,1,1)
+  CT.ci[G.tile(k, i, j, 4, CI_NV)] = xn * po21 * o1 * (RIO1 * RIO1) * RMO2;  // fs(k,1,2)
+  CT.ci[G.tile(k, i, j, 5, CI_NV)] = pox2 * RMO1 * RIO2;                     // fs(k,2,1)
+  CT.ci[G.tile(k, i, j, 6, CI_NV)] = -lox2 - lox1 * o1 * RIO1 * xn;          // fs(k,2,2)
+  CT.fs0[0][p] = (po23 - lo22) * RMO2 / xn;                                  // fs(k,1,0)
+  CT.fs0[1][p] = (pox1 - lox3) * RMO1 / xn;                                  // fs(k,2,0)
 }
 
 template <bool CAP>
@@ -2212,9 +2233,10 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     };
     auto load_ls = [&](int L, LS &S) {
       size_t p = G.a3(L, i, j);
-      S.fs[0][1] = CT.fs[0][p]; S.fs[0][2] = CT.fs[1][p]; S.fs[1][1] = CT.fs[2][p];
-      S.fs[1][2] = CT.fs[3][p];
-      S.rhs[0] = CT.rhs[0][p]; S.rhs[1] = CT.rhs[1][p]; S.rhs[2] = CT.rhs[2][p];
+      const size_t t = G.tile(L, i, j, 0, CI_NV);
+      S.rhs[0] = CT.ci[t]; S.rhs[1] = CT.ci[t + 64]; S.rhs[2] = CT.ci[t + 128];
+      S.fs[0][1] = CT.ci[t + 192]; S.fs[0][2] = CT.ci[t + 256]; S.fs[1][1] = CT.ci[t + 320];
+      S.fs[1][2] = CT.ci[t + 384];
       S.hdo2 = in.hdo2[p]; S.hdo1 = in.hdo1[p]; S.hdhe = in.hdhe[p];
       S.expz = g.expz[L]; S.difk0 = g.difk[L]; S.difk1 = g.difk[L + 1];
     };
@@ -2272,11 +2294,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
     double gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
     double alpha22 = 0, alpha33 = 0, alpha23 = 0, alpha32 = 0, flx00 = 0;
-    // The elimination's gamma (9) and forward solution z (3) of level k are
-    // stored as one tile per wavefront: 12 runs of 64 doubles, one per value,
-    // lane fastest, so a wavefront's values of a level share a 6 KiB block.
-    const long col = (long)j * G.nlond + i, ngw = ((long)G.nlond * G.nlatd + 63) / 64;
-    auto tile = [&](int k, int n) { return (((size_t)k * ngw + (col >> 6)) * 12 + n) * 64 + (col & 63); };
+    // The elimination's gamma (9) and forward solution z (3) of level k, in
+    // a tiled region of 12 values.
+    auto tile = [&](int k, int n) { return G.tile(k, i, j, n, 12); };
     for (int k = 0; k < nk - 1; ++k) {
       // every load of this level is issued before any of its arithmetic, so
       // the level costs one memory round trip
@@ -2651,8 +2671,9 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       size_t p = G.a3(k, i, j);
       const double expzk = g.expz[k];
       double fs[2][3];
-      fs[0][1] = CT.fs[0][p]; fs[0][2] = CT.fs[1][p]; fs[1][1] = CT.fs[2][p];
-      fs[1][2] = CT.fs[3][p];
+      const size_t t = G.tile(k, i, j, 0, CI_NV);
+      fs[0][1] = CT.ci[t + 192]; fs[0][2] = CT.ci[t + 256]; fs[1][1] = CT.ci[t + 320];
+      fs[1][2] = CT.ci[t + 384];
       const double *akm = ak[h], *epm = ep[h], *akpp = akp[h], *eppp = epp[h];
       const double wks3 = wks[h], wks4 = wksp[h];
       double pk[9], rk[9], qk[9], fk[3];
@@ -2672,9 +2693,9 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
           rk[e] = molr / dz - expzk * delta / dz * (eddyr * dfac - .5 * wik);
           qk[e] = -molq / dz + expzk * delta * (eddyq * dfac / dz + dtx2inv) - expzk * fsv;
         }
-      fk[0] = expzk * (CT.rhs[0][p] + in.hdo2[p]);
-      fk[1] = expzk * (CT.rhs[1][p] + in.hdo1[p]);
-      fk[2] = expzk * (CT.rhs[2][p] + in.hdhe[p]);
+      fk[0] = expzk * (CT.ci[t] + in.hdo2[p]);
+      fk[1] = expzk * (CT.ci[t + 64] + in.hdo1[p]);
+      fk[2] = expzk * (CT.ci[t + 128] + in.hdhe[p]);
       if (k == 0) {
         for (int m = 0; m < 3; ++m)
           for (int n = 0; n < 3; ++n)
@@ -3883,10 +3904,12 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
                          (const double *)D(2));
       break;
     case 25:
-    case 26: {  // bufs = {fs[6], n2d sloss, n2d s0prod, n4s sloss, lin[6], oploss, opprod,
-                //         no sloss, no s0prod, n4s s0prod}
+    case 26: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), n2d sloss, n2d s0prod,
+                //         n4s sloss, lin[6], oploss, opprod, no sloss, no s0prod, n4s s0prod}
       ChemOut C;
-      for (int q = 0; q < 6; ++q) C.fs[q] = D(q);
+      C.ci = D(0);
+      C.fs0[0] = D(1);
+      C.fs0[1] = D(2);
       C.n2d_sloss = D(6); C.n2d_s0prod = D(7); C.n4s_sloss = D(8);
       for (int q = 0; q < 6; ++q) C.lin[q] = D(9 + q);
       C.oploss = D(15); C.opprod = D(16);
@@ -4011,8 +4034,8 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     }
     case 8:
     case 14:
-    case 23: {  // comp: bufs = {gamma and z tiles, rhs[3], (unused x3), fs[6]}
-      CompTerms CT{{D(1), D(2), D(3)}, {D(7), D(8), D(9), D(10), D(11), D(12)}};
+    case 23: {  // comp: bufs = {elimination tiles (12 values), comp input tiles (7), fs(:,1,0), fs(:,2,0)}
+      CompTerms CT{D(1), {D(2), D(3)}};
       if (which == 8) {
         if (g.capture)
           hipLaunchKernelGGL(comp_kernel<true>, grid, block, 0, s, g, G, in, out, D(0), CT);
@@ -4049,15 +4072,16 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 12: {  // comp prelude
       StencilJobs J{};
       J.n = 6;
-      J.job[0] = job(ST_ADVEC, in.o2, D(1), 0.);
-      J.job[1] = job(ST_ADVEC, in.o1, D(2), 0.);
-      J.job[2] = job(ST_ADVEC, in.he, D(3), 0.);
-      J.job[3] = job(ST_SMOOTH, in.o2_nm, D(4), g.shapiro);
-      J.job[4] = job(ST_SMOOTH, in.o1_nm, D(5), g.shapiro);
-      J.job[5] = job(ST_SMOOTH, in.he_nm, D(6), g.shapiro);
-      J.aux[0] = D(11);  // fs(:,1,0), fs(:,2,0) of comp_o2o_ng
-      J.aux[1] = D(12);
+      J.job[0] = job(ST_ADVEC, in.o2, nullptr, 0.);  // combined into J.ci
+      J.job[1] = job(ST_ADVEC, in.o1, nullptr, 0.);
+      J.job[2] = job(ST_ADVEC, in.he, nullptr, 0.);
+      J.job[3] = job(ST_SMOOTH, in.o2_nm, nullptr, g.shapiro);
+      J.job[4] = job(ST_SMOOTH, in.o1_nm, nullptr, g.shapiro);
+      J.job[5] = job(ST_SMOOTH, in.he_nm, nullptr, g.shapiro);
+      J.aux[0] = D(2);  // fs(:,1,0), fs(:,2,0) of comp_o2o_ng
+      J.aux[1] = D(3);
       J.aux[2] = nullptr;
+      J.ci = D(1);
       launch_stencil_run<KindsComp, STC_COMP>(g, G, in, J, s);
       break;
     }

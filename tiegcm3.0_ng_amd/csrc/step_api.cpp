@@ -38,8 +38,15 @@ enum {
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
-  W_CO = W_UV + 11,                      // comp: 6 prelude, 6 fs, 12 elimination planes (last)
-  W_NPLANES = W_CO + 24
+  W_CO = W_UV + 11,                      // comp: fs(:,1,0), fs(:,2,0)
+  W_NPLANES = W_CO + 2
+};
+// Tiled regions after the planes, in units of nk * ngw * 64 doubles (ngw
+// groups of 64 columns; the kernels' Geo::tile gives the element order).
+enum {
+  T_CI = 0,        // comp inputs: 3 right-hand-side parts, 4 fs values
+  T_CE = T_CI + 7, // comp elimination: 9 gamma, 3 forward solution
+  T_N = T_CE + 12
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -62,14 +69,20 @@ int check(const tgx_ng_grid *g, const void *work) {
 double *plane(void *work, const tgx_ng_grid *g, int idx) {
   return (double *)work + (size_t)idx * plane_elems(g);
 }
+
+size_t tile_elems(const tgx_ng_grid *g) {
+  const size_t ncol = (size_t)(g->nlon + 4) * (size_t)(g->nlat + 4);
+  return (size_t)g->nk * ((ncol + 63) / 64) * 64;
+}
+
+double *tiles(void *work, const tgx_ng_grid *g, int idx) {
+  return (double *)work + (size_t)W_NPLANES * plane_elems(g) + (size_t)idx * tile_elems(g);
+}
 }  // namespace
 
 extern "C" unsigned long long tgx_ng_workspace_bytes(const tgx_ng_grid *g) {
   if (!g) return 0;
-  // the comp elimination planes are the workspace's last and are stored in
-  // tiles of 64 columns, which can overrun 12 planes by less than 64 columns
-  // per level and plane
-  return ((unsigned long long)W_NPLANES * plane_elems(g) + 12ull * 64ull * (unsigned long long)g->nk) *
+  return ((unsigned long long)W_NPLANES * plane_elems(g) + (unsigned long long)T_N * tile_elems(g)) *
          sizeof(double);
 }
 
@@ -187,7 +200,10 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
 namespace {
 // Buffers of chem_kernel / minor_lin_kernel (the step-wide chemistry pass).
 void chem_bufs(void *work, const tgx_ng_grid *g, void **b) {
-  for (int q = 0; q < 6; ++q) b[q] = plane(work, g, W_CO + 6 + q);  // comp fs planes
+  b[0] = tiles(work, g, T_CI);                                       // comp fs(:,1:2,1:2)
+  b[1] = plane(work, g, W_CO + 0);                                   // comp fs(:,1,0)
+  b[2] = plane(work, g, W_CO + 1);                                   // comp fs(:,2,0)
+  b[3] = b[4] = b[5] = nullptr;
   b[6] = plane(work, g, W_MIN_S + 0);                                // N2D sloss (then NO's)
   b[7] = plane(work, g, W_MIN_S + 1);                                // N2D s0prod (then NO's)
   b[8] = plane(work, g, W_MIN_S + 2);                                // N4S sloss
@@ -325,10 +341,11 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_COMP, s);
-  void *b[14];
-  b[0] = plane(work, g, W_CO + 12);  // Thomas gamma and forward solution, tiled per wavefront
-  for (int q = 0; q < 12; ++q) b[1 + q] = plane(work, g, W_CO + q);
-  b[13] = nullptr;
+  void *b[4];
+  b[0] = tiles(work, g, T_CE);  // Thomas gamma and forward solution
+  b[1] = tiles(work, g, T_CI);  // right-hand-side parts and fs(:,1:2,1:2)
+  b[2] = plane(work, g, W_CO + 0);
+  b[3] = plane(work, g, W_CO + 1);
   // comp_o2o_ng's fs planes first: the prelude folds fs(:,:,0) into the
   // right-hand sides
   if (!chem_done)
