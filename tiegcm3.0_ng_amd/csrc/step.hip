@@ -298,7 +298,8 @@ struct StencilJobs {
   int n;
   StencilJob job[8];
   const double *aux[3];  // STC_COMP: the fs(:,:,0) planes the right-hand sides take
-  double *ci;            // STC_COMP: the tiled comp inputs (values 0-2 are written)
+  double *ci;            // STC_COMP: the tiled comp inputs (values 0-2 are written);
+                         // STC_MINOR: the tiled minor right-hand-side parts (MR_NV values)
 };
 // How a prelude stores its jobs: each job its own plane (STC_NONE), or the
 // smoothed and advected planes of one species combined into the part of the
@@ -311,6 +312,9 @@ enum { STC_NONE = 0, STC_MINOR, STC_COMP };
 // Tiled comp inputs: the right-hand-side parts of O2, O, He (0-2) and
 // fs(:,1,1), fs(:,1,2), fs(:,2,1), fs(:,2,2) of comp_o2o_ng (3-6).
 constexpr int CI_NV = 7;
+// Tiled minor-species right-hand-side parts, one value per species id
+// (NO, N4S, N2D, AR), and the Thomas ratios of the two species of a pass.
+constexpr int MR_NV = 4, MW_NV = 2;
 
 // Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
 // planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
@@ -465,7 +469,7 @@ __global__ void __launch_bounds__(256) stencil_run_kernel(tgx_ng_grid g, Geo G, 
       }
       if (COMB == STC_MINOR) {
 #pragma unroll
-        for (int sp = 0; sp < 4; ++sp) J.job[sp].out[p] = vv[sp] * g.dtx2inv - vv[4 + sp];
+        for (int sp = 0; sp < 4; ++sp) J.ci[G.tile(k, i, j, sp, MR_NV)] = vv[sp] * g.dtx2inv - vv[4 + sp];
       } else if (COMB == STC_COMP) {
 #pragma unroll
         for (int sp = 0; sp < 3; ++sp)
@@ -825,10 +829,12 @@ struct MinorSpecies {  // per species and level
   double ax, thdiff, sloss, s0prod, rhs;
 };
 
-struct MinorTerms {  // planes by species id: the prelude's right-hand-side part
-                     // smooth(fcomp_tm1)*dtx2inv - advec(fcomp), and the loss /
-                     // scaled production of minor_src_kernel
-  const double *rhs[4], *sloss[4], *s0prod[4];
+struct MinorTerms {  // the prelude's right-hand-side parts smooth(fcomp_tm1)*dtx2inv
+                     // - advec(fcomp), tiled (MR_NV values by species id), and
+                     // planes by species id of the loss / scaled production of
+                     // minor_src_kernel
+  const double *rt;
+  const double *sloss[4], *s0prod[4];
   // NO / N4S sources as chem_kernel's linear forms in this step's N2D
   // (minor_kernel<.., LIN = true>): lin[0..5] and the solved N2D
   const double *lin[6], *n2d;
@@ -925,8 +931,8 @@ __global__ void __launch_bounds__(256) minor_src_kernel(tgx_ng_grid g, Geo G, tg
 
 template <int SA, int SB, bool CAP, bool LIN = false>
 __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                    tgx_ng_out out, double *__restrict__ w2a,
-                                                    double *__restrict__ w2b, MinorTerms MT) {
+                                                    tgx_ng_out out, double *__restrict__ w2t,
+                                                    MinorTerms MT) {
   // Column solve of two species (minor_ng.F90:60-237); their sources come
   // from minor_src_kernel.  Each level's loads are issued together ahead of
   // its arithmetic, with a window of three levels of the shared fields.
@@ -996,7 +1002,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     size_t p = G.a3(L, i, j);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      Q[q].rhs = MT.rhs[SP[q]][p];
+      Q[q].rhs = MT.rt[G.tile(L, i, j, SP[q], MR_NV)];
       if (SP[q] == 3) { Q[q].sloss = 0.; Q[q].s0prod = 0.; }
       else if (LIN && SP[q] == 0) {  // NO: sloss = lin2 + n2d lin3, s0prod = lin4 + n2d lin5
         Q[q].sloss = MT.lin[2][p]; Q[q].la = MT.lin[3][p];
@@ -1112,7 +1118,6 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 #pragma unroll
   for (int q = 0; q < 2; ++q) species_at(q, Sc, tni_1 - Sc.tni, Y[q], Tc[q]);
   double w1[2] = {0., 0.}, w3[2] = {0., 0.}, rprev[2] = {0., 0.};
-  double *w2s[2] = {w2a, w2b};
   double wic = .5 * (R0.w + R1.w);
   for (int k = 0; k < nk; ++k) {
     if (k + 2 <= nk - 1) load_mr(k + 2, R2);
@@ -1169,7 +1174,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
         w3[q] = fr / w1[q];
       } else {
         double w2 = rprev[q] / w1[q];
-        w2s[q][G.a3(k - 1, i, j)] = w2;
+        w2t[G.tile(k - 1, i, j, q, MW_NV)] = w2;
         w1[q] = qc - pc * w2;
         w3[q] = (fr - pc * w3[q]) / w1[q];
       }
@@ -1191,7 +1196,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     auto load_bl = [&](int k, BL &B) {
       size_t p = G.a3(k, i, j);
       B.x = outs[q][p];
-      B.w2 = k < nk - 1 ? w2s[q][p] : 0.;
+      B.w2 = k < nk - 1 ? w2t[G.tile(k, i, j, q, MW_NV)] : 0.;
       B.f = fc[q][p];
       B.fm = fctm1[q][p];
     };
@@ -1906,7 +1911,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
       ax[h][q] = -(Q.barm / (TAU * RMN2) * pow(T00 / Q.tni, 0.25) /
                    (cq.phix[2] + cq.salfax[0] * Q.pso2 + cq.salfax[1] * Q.pso1 + cq.salfax[2] * Q.pshe));
       thd[h][q] = ex - cq.alfax * dtnidz / (dzp * Q.tni);
-      rhs[h][q] = MT.rhs[SP[q]][p];
+      rhs[h][q] = MT.rt[G.tile(Lc, i, j, SP[q], MR_NV)];
       if (SP[q] == 3) { slo[h][q] = 0.; s0p[h][q] = 0.; }
       else { slo[h][q] = MT.sloss[SP[q]][p]; s0p[h][q] = MT.s0prod[SP[q]][p]; }
     }
@@ -3945,30 +3950,31 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 16:
     case 18:
     case 19:
-    case 27: {  // minors: bufs = {w2a, w2b, ftm1s[4], hadvec[4] (species 0..3), sloss/s0prod of
-                // NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO after N2D's solve
+    case 27: {  // minors: bufs = {ratio tiles (MW_NV), right-hand-side tiles (MR_NV), (unused x8),
+                // sloss/s0prod of NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO
+                // after N2D's solve
       MinorTerms MT{};
-      for (int q = 0; q < 4; ++q) MT.rhs[q] = D(2 + q);
+      MT.rt = D(1);
       MT.sloss[0] = MT.sloss[2] = D(10);
       MT.s0prod[0] = MT.s0prod[2] = D(11);
       MT.sloss[1] = D(12);
       MT.s0prod[1] = D(13);
       MT.sloss[3] = MT.s0prod[3] = nullptr;
       if (which == 1 && g.capture)
-        hipLaunchKernelGGL((minor_kernel<2, 3, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+        hipLaunchKernelGGL((minor_kernel<2, 3, true>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 1)
-        hipLaunchKernelGGL((minor_kernel<2, 3, false>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+        hipLaunchKernelGGL((minor_kernel<2, 3, false>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 27) {  // bufs 14..19: chem_kernel's lin planes
         for (int q = 0; q < 6; ++q) MT.lin[q] = D(14 + q);
         MT.n2d = out.n2d;
         if (g.capture)
-          hipLaunchKernelGGL((minor_kernel<1, 0, true, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+          hipLaunchKernelGGL((minor_kernel<1, 0, true, true>), grid, block, 0, s, g, G, in, out, D(0), MT);
         else
-          hipLaunchKernelGGL((minor_kernel<1, 0, false, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+          hipLaunchKernelGGL((minor_kernel<1, 0, false, true>), grid, block, 0, s, g, G, in, out, D(0), MT);
       } else if (which == 2 && g.capture)
-        hipLaunchKernelGGL((minor_kernel<1, 0, true>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+        hipLaunchKernelGGL((minor_kernel<1, 0, true>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 2)
-        hipLaunchKernelGGL((minor_kernel<1, 0, false>), grid, block, 0, s, g, G, in, out, D(0), D(1), MT);
+        hipLaunchKernelGGL((minor_kernel<1, 0, false>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 18)
         hipLaunchKernelGGL((minor_wave_kernel<2, 3>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 19)
@@ -4062,10 +4068,11 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       J.n = 8;
       const double *fc[4] = {in.no, in.n4s, in.n2d, in.ar};
       const double *fm[4] = {in.no_nm, in.n4s_nm, in.n2d_nm, in.ar_nm};
-      for (int q = 0; q < 4; ++q) {  // outputs: the right-hand-side part of each species
-        J.job[q] = job(ST_SMOOTH, fm[q], D(2 + q), g.shapiro);
+      for (int q = 0; q < 4; ++q) {  // combined into the right-hand-side part of each species
+        J.job[q] = job(ST_SMOOTH, fm[q], nullptr, g.shapiro);
         J.job[4 + q] = job(ST_ADVEC_R, fc[q], nullptr, 0.);
       }
+      J.ci = D(1);
       launch_stencil_run<KindsMinor, STC_MINOR>(g, G, in, J, s);
       break;
     }

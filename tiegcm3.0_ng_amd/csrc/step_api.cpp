@@ -31,10 +31,9 @@ namespace {
 // Workspace planes, each nk * ld * (nlat+4) doubles.
 enum {
   W_DT = 0, W_DT_T = 1,                  // dt ratios; 2 prelude planes
-  W_MIN_A = 3, W_MIN_B = 4, W_MIN_T = 5, // minor ratios (two species per pass); 8 prelude planes
-  W_MIN_S = 13,                          // minor sources: 2 species x (sloss, s0prod)
-  W_MIN_L = 17,                          // NO / N4S sources linear in N2D: 6 coefficient planes
-  W_OP = 23,                             // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
+  W_MIN_S = 3,                           // minor sources: 2 species x (sloss, s0prod)
+  W_MIN_L = W_MIN_S + 4,                 // NO / N4S sources linear in N2D: 6 coefficient planes
+  W_OP = W_MIN_L + 6,                    // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
@@ -46,7 +45,9 @@ enum {
 enum {
   T_CI = 0,        // comp inputs: 3 right-hand-side parts, 4 fs values
   T_CE = T_CI + 7, // comp elimination: 9 gamma, 3 forward solution
-  T_N = T_CE + 12
+  T_MR = T_CE + 12, // minor species: right-hand-side parts of NO, N4S, N2D, AR
+  T_MW = T_MR + 4,  // minor species: Thomas ratios of the two species of a pass
+  T_N = T_MW + 2
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -248,9 +249,9 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_MINORS, s);
   void *b[14];
-  b[0] = plane(work, g, W_MIN_A);
-  b[1] = plane(work, g, W_MIN_B);
-  for (int q = 0; q < 8; ++q) b[2 + q] = plane(work, g, W_MIN_T + q);
+  b[0] = tiles(work, g, T_MW);
+  b[1] = tiles(work, g, T_MR);
+  for (int q = 2; q < 10; ++q) b[q] = nullptr;
   for (int q = 0; q < 4; ++q) b[10 + q] = plane(work, g, W_MIN_S + q);
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
