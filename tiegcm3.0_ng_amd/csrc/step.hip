@@ -46,6 +46,7 @@ struct Geo {
   int nk, nlon, nlat, nlond, nlatd, ld;
   size_t P;
   long ngw;  // groups of 64 columns (j * nlond + i) in a tiled workspace region
+  long nwt;  // wavefronts of a tile-kernel launch (4 per column tile)
   __device__ __forceinline__ size_t a3(int k, int i, int j) const {
     return (size_t)k * P + (size_t)j * ld + i;
   }
@@ -57,6 +58,13 @@ struct Geo {
   __device__ __forceinline__ size_t tile(int k, int i, int j, int n, int nv) const {
     const long c = (long)j * nlond + i;
     return (((size_t)k * ngw + (size_t)(c >> 6)) * nv + n) * 64 + (size_t)(c & 63);
+  }
+  // The same for the calling thread of a tile kernel (one block of 256 per
+  // TGX_TILE_X x TGX_TILE_Y column tile): per level and wavefront of the
+  // launch, nv runs of 64 doubles, lane fastest.
+  __device__ __forceinline__ size_t wtile(int k, int n, int nv) const {
+    const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    return (((size_t)k * nwt + w) * nv + n) * 64 + (threadIdx.x & 63);
   }
 };
 
@@ -76,6 +84,7 @@ static Geo make_geo(const tgx_ng_grid &g) {
   G.nlond = g.nlon + 4; G.nlatd = g.nlat + 4; G.ld = g.ld;
   G.P = (size_t)g.ld * (size_t)(g.nlat + 4);
   G.ngw = ((long)G.nlond * G.nlatd + 63) / 64;
+  G.nwt = 4L * ((G.nlond + TGX_TILE_X - 1) / TGX_TILE_X) * ((G.nlatd + TGX_TILE_Y - 1) / TGX_TILE_Y);
   return G;
 }
 
@@ -88,7 +97,7 @@ static Geo make_geo(const tgx_ng_grid &g) {
 __device__ __forceinline__ double pv(const double *f, const Geo &G, int i, int j) {
   return f[(size_t)j * G.ld + i];
 }
-constexpr int TILE_X = 32, TILE_Y = 8, TILE_H = 2;
+constexpr int TILE_X = TGX_TILE_X, TILE_Y = TGX_TILE_Y, TILE_H = 2;
 constexpr int TILE_W = TILE_X + 2 * TILE_H, TILE_R = TILE_Y + 2 * TILE_H;
 struct LdsPlane {
   const double *t;  // TILE_R x TILE_W, row-major
@@ -315,6 +324,11 @@ constexpr int CI_NV = 7;
 // Tiled minor-species right-hand-side parts, one value per species id
 // (NO, N4S, N2D, AR), and the Thomas ratios of the two species of a pass.
 constexpr int MR_NV = 4, MW_NV = 2;
+// Tiled 2x2 block Thomas values of the wind solve: gamma (g11, g21, g12, g22)
+// and the forward solution (u, v).
+constexpr int UW_NV = 6;
+// Tiled Thomas values of the TN solve: the ratio and the forward solution.
+constexpr int DW_NV = 2;
 
 // Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
 // planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
@@ -698,17 +712,16 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
         cap_put(g, G, CAP_DT + 2, i, j, k, r); cap_put(g, G, CAP_DT + 3, i, j, k, rhs);
       }
       // Thomas forward (trsolv_ng.F90:14-22)
-      size_t pk = G.a3(k, ic, jc);
       if (k == 0) {
         w1 = q;
         w3 = rhs / w1;
       } else {
         double w2 = rprev / w1;
-        w2s[G.a3(k - 1, ic, jc)] = w2;
+        w2s[G.wtile(k - 1, 0, DW_NV)] = w2;
         w1 = q - p * w2;
         w3 = (rhs - p * w3) / w1;
       }
-      out.tn[pk] = w3;
+      w2s[G.wtile(k, 1, DW_NV)] = w3;  // forward solution
       rprev = r;
       gk = g1; fk = f1;
     }
@@ -723,7 +736,8 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
   struct DB { double x, w2, tnm, tn; };
   auto load_db = [&](int k, DB &B) {
     size_t p = G.a3(k, ic, jc);
-    B.x = out.tn[p]; B.w2 = k < nk - 2 ? w2s[p] : 0.; B.tnm = in.tn_nm[p]; B.tn = in.tn[p];
+    const size_t t = G.wtile(k, 0, DW_NV);
+    B.x = w2s[t + 64]; B.w2 = k < nk - 2 ? w2s[t] : 0.; B.tnm = in.tn_nm[p]; B.tn = in.tn[p];
   };
   const double tnm_top = F3(in.tn_nm, nk - 1), tn_top = F3(in.tn, nk - 1);
   double xn = w3;
@@ -3529,7 +3543,6 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
   }
   double be11 = 0, be12 = 0, be21 = 0, be22 = 0, y1 = 0, y2 = 0;
   double r11p = 0, r12p = 0, r21p = 0, r22p = 0;
-  const size_t NP = (size_t)nk * G.P;
   for (int k = 0; k <= nk - 2; ++k) {
     US us = {};
     if (k + 1 <= nk - 2) tile_load(k + 1);
@@ -3644,8 +3657,8 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
         double g12 = be11 * r12p + be12 * r22p;
         double g21 = be21 * r11p + be22 * r21p;
         double g22 = be21 * r12p + be22 * r22p;
-        size_t pm = G.a3(k - 1, i, j);
-        gs[pm] = g11; gs[pm + NP] = g21; gs[pm + 2 * NP] = g12; gs[pm + 3 * NP] = g22;
+        const size_t tm = G.wtile(k - 1, 0, UW_NV);
+        gs[tm] = g11; gs[tm + 64] = g21; gs[tm + 128] = g12; gs[tm + 192] = g22;
         double h11 = qq11 - p11 * g11 - p12 * g21;
         double h12 = qq12 - p11 * g12 - p12 * g22;
         double h21 = qq21 - p21 * g11 - p22 * g21;
@@ -3657,8 +3670,9 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
         y1 = be11 * x1 + be12 * x2;
         y2 = be21 * x1 + be22 * x2;
       }
-      out.un[p] = y1;
-      out.vn[p] = y2;
+      const size_t ty = G.wtile(k, 4, UW_NV);  // forward solution
+      gs[ty] = y1;
+      gs[ty + 64] = y2;
       r11p = r11; r12p = r12; r21p = r21; r22p = r22;
       gk = gk1;
       e1_m = e1_c;
@@ -3674,8 +3688,9 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
   struct UB { double x1, x2, g11, g21, g12, g22, unm, un, vnm, vn; };
   auto load_ub = [&](int k, UB &B) {
     size_t p = G.a3(k, i, j);
-    B.x1 = out.un[p]; B.x2 = out.vn[p];
-    if (k < nk - 2) { B.g11 = gs[p]; B.g21 = gs[p + NP]; B.g12 = gs[p + 2 * NP]; B.g22 = gs[p + 3 * NP]; }
+    const size_t t = G.wtile(k, 0, UW_NV);
+    B.x1 = gs[t + 256]; B.x2 = gs[t + 320];
+    if (k < nk - 2) { B.g11 = gs[t]; B.g21 = gs[t + 64]; B.g12 = gs[t + 128]; B.g22 = gs[t + 192]; }
     else { B.g11 = B.g21 = B.g12 = B.g22 = 0.; }
     B.unm = in.un_nm[p]; B.un = in.un[p]; B.vnm = in.vn_nm[p]; B.vn = in.vn[p];
   };
@@ -3928,7 +3943,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C, D(17), D(18), D(19));
       break;
     }
-    case 28: {  // duv with stage 1 and the stencils fused: bufs = {ztmp (unused), gamma}
+    case 28: {  // duv with stage 1 and the stencils fused: bufs = {ztmp (unused), Thomas tiles (UW_NV)}
       dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
       if (g.capture)
         hipLaunchKernelGGL(duv_tile_kernel<true>, gridt, block, 0, s, g, G, in, out, D(1));
@@ -3936,7 +3951,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL(duv_tile_kernel<false>, gridt, block, 0, s, g, G, in, out, D(1));
       break;
     }
-    case 24: {  // dt with the stencils fused: one block per column tile
+    case 24: {  // dt with the stencils fused, one block per column tile: bufs = {Thomas tiles (DW_NV)}
       dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
       if (g.capture)
         hipLaunchKernelGGL(dt_tile_kernel<true>, gridt, block, 0, s, g, G, in, out, D(0));

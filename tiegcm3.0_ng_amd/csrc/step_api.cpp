@@ -30,14 +30,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_field
 namespace {
 // Workspace planes, each nk * ld * (nlat+4) doubles.
 enum {
-  W_DT = 0, W_DT_T = 1,                  // dt ratios; 2 prelude planes
-  W_MIN_S = 3,                           // minor sources: 2 species x (sloss, s0prod)
+  W_DT_T = 0,                            // dt: 2 prelude planes (PCR)
+  W_MIN_S = 2,                           // minor sources: 2 species x (sloss, s0prod)
   W_MIN_L = W_MIN_S + 4,                 // NO / N4S sources linear in N2D: 6 coefficient planes
   W_OP = W_MIN_L + 6,                    // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
   W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
-  W_UV = W_OP_C + 2,                     // wind: ztmp, 4 gamma, 6 prelude planes
-  W_CO = W_UV + 11,                      // comp: fs(:,1,0), fs(:,2,0)
+  W_UV = W_OP_C + 2,                     // wind: ztmp, 6 prelude planes
+  W_CO = W_UV + 7,                       // comp: fs(:,1,0), fs(:,2,0)
   W_NPLANES = W_CO + 2
 };
 // Tiled regions after the planes, in units of nk * ngw * 64 doubles (ngw
@@ -47,7 +47,9 @@ enum {
   T_CE = T_CI + 7, // comp elimination: 9 gamma, 3 forward solution
   T_MR = T_CE + 12, // minor species: right-hand-side parts of NO, N4S, N2D, AR
   T_MW = T_MR + 4,  // minor species: Thomas ratios of the two species of a pass
-  T_N = T_MW + 2
+  T_UW = T_MW + 2,  // winds: 2x2 block gamma and forward solution
+  T_DW = T_UW + 6,  // TN: Thomas ratio and forward solution
+  T_N = T_DW + 2
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -71,9 +73,13 @@ double *plane(void *work, const tgx_ng_grid *g, int idx) {
   return (double *)work + (size_t)idx * plane_elems(g);
 }
 
+// One value of a tiled region: nk levels of either 64-column groups (column
+// kernels) or the wavefronts of a tile-kernel launch, whichever is larger.
 size_t tile_elems(const tgx_ng_grid *g) {
-  const size_t ncol = (size_t)(g->nlon + 4) * (size_t)(g->nlat + 4);
-  return (size_t)g->nk * ((ncol + 63) / 64) * 64;
+  const size_t nlond = (size_t)(g->nlon + 4), nlatd = (size_t)(g->nlat + 4);
+  const size_t ncol = (nlond * nlatd + 63) / 64 * 64;
+  const size_t nwt = 256 * ((nlond + TGX_TILE_X - 1) / TGX_TILE_X) * ((nlatd + TGX_TILE_Y - 1) / TGX_TILE_Y);
+  return (size_t)g->nk * (ncol > nwt ? ncol : nwt);
 }
 
 double *tiles(void *work, const tgx_ng_grid *g, int idx) {
@@ -188,7 +194,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_DT, s);
-  void *b[3] = {plane(work, g, W_DT), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
+  void *b[3] = {tiles(work, g, T_DW), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
   if (!use_pcr(g, R_DT)) {  // Thomas: the stencils are fused into the column kernel
     ProfScope kern(R_N + R_DT, s);
     return tgx_ng_launch(K_DT_TILE, g, in, out, b, 0, s);
@@ -325,8 +331,8 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   ProfScope prof(R_DUV, s);
   void *b[8];
   b[0] = plane(work, g, W_UV);
-  b[1] = plane(work, g, W_UV + 1);
-  for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 5 + q);
+  b[1] = tiles(work, g, T_UW);  // Thomas gamma and forward solution (K_UV_TILE)
+  for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 1 + q);
   if (!use_pcr(g, R_DUV)) {  // Thomas: stage 1 and the stencils are fused into the column kernel
     ProfScope kern(R_N + R_DUV, s);
     return tgx_ng_launch(K_UV_TILE, g, in, out, b, 0, s);

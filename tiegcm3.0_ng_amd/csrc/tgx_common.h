@@ -5,6 +5,10 @@
 #include <stdint.h>
 
 #define TGX_WAVE 64
+// Column tile of the tile kernels and stencil preludes: a block of 256
+// threads takes TGX_TILE_X x TGX_TILE_Y columns.
+#define TGX_TILE_X 32
+#define TGX_TILE_Y 8
 
 // Record a HIP failure in the library's error slot and return the status code.
 int tgx_set_error(int code, const char *fmt, ...);
