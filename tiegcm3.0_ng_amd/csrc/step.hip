@@ -329,6 +329,8 @@ constexpr int MR_NV = 4, MW_NV = 2;
 constexpr int UW_NV = 6;
 // Tiled Thomas values of the TN solve: the ratio and the forward solution.
 constexpr int DW_NV = 2;
+// Tiled Thomas values of the O+ solve: the ratio and the forward solution.
+constexpr int OW_NV = 2;
 
 // Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
 // planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
@@ -3180,11 +3182,11 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
         w3 = expl / w1;
       } else {
         double w2 = rprev / w1;
-        W.w2[G.a3(k - 1, i, j)] = w2;
+        W.w2[G.wtile(k - 1, 0, OW_NV)] = w2;
         w1 = qcf - pcf * w2;
         w3 = (expl - pcf * w3) / w1;
       }
-      opout[p] = w3;
+      W.w2[G.wtile(k, 1, OW_NV)] = w3;  // forward solution
       rprev = rcf;
       qa = qb;
       qb = qc;
@@ -3197,7 +3199,8 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
   struct OB { double x, w2, op, optm1; };
   auto load_ob = [&](int k, OB &B) {
     size_t p = G.a3(k, ic, jc);
-    B.x = opout[p]; B.w2 = k < nk - 2 ? W.w2[p] : 0.; B.op = op[p]; B.optm1 = optm1[p];
+    const size_t t = G.wtile(k, 0, OW_NV);
+    B.x = W.w2[t + 64]; B.w2 = k < nk - 2 ? W.w2[t] : 0.; B.op = op[p]; B.optm1 = optm1[p];
   };
   const double op_top = F3(op, nk - 1), optm1_top = F3(optm1, nk - 1);
   double xn = w3;

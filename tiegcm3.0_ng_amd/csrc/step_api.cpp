@@ -33,8 +33,8 @@ enum {
   W_DT_T = 0,                            // dt: 2 prelude planes (PCR)
   W_MIN_S = 2,                           // minor sources: 2 species x (sloss, s0prod)
   W_MIN_L = W_MIN_S + 4,                 // NO / N4S sources linear in N2D: 6 coefficient planes
-  W_OP = W_MIN_L + 6,                    // O+: 9 intermediates, 2 ping-pong, 6 prelude planes
-  W_OP_X = W_OP + 9, W_OP_XNM = W_OP + 10, W_OP_T = W_OP + 11,
+  W_OP = W_MIN_L + 6,                    // O+: 8 intermediates, 2 ping-pong, 6 prelude planes
+  W_OP_X = W_OP + 8, W_OP_XNM = W_OP + 9, W_OP_T = W_OP + 10,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 6 prelude planes
   W_CO = W_UV + 7,                       // comp: fs(:,1,0), fs(:,2,0)
@@ -49,7 +49,8 @@ enum {
   T_MW = T_MR + 4,  // minor species: Thomas ratios of the two species of a pass
   T_UW = T_MW + 2,  // winds: 2x2 block gamma and forward solution
   T_DW = T_UW + 6,  // TN: Thomas ratio and forward solution
-  T_N = T_DW + 2
+  T_OW = T_DW + 2,  // O+: Thomas ratio and forward solution
+  T_N = T_OW + 2
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -300,7 +301,8 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
     bool to_out = ((nsub - 1 - sub) % 2) == 0;
     double *op_o = to_out ? out->op : tmp_op, *nm_o = to_out ? out->op_nm : tmp_nm;
     void *b[21];
-    for (int q = 0; q < 9; ++q) b[q] = plane(work, g, W_OP + q);
+    for (int q = 0; q < 8; ++q) b[q] = plane(work, g, W_OP + q);
+    b[8] = tiles(work, g, T_OW);  // Thomas ratio and forward solution (K_OP_TILE)
     b[9] = (void *)op_in;
     b[10] = (void *)nm_in;
     b[11] = op_o;
