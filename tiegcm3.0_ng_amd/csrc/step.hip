@@ -324,6 +324,12 @@ constexpr int CI_NV = 7;
 // Tiled minor-species right-hand-side parts, one value per species id
 // (NO, N4S, N2D, AR), and the Thomas ratios of the two species of a pass.
 constexpr int MR_NV = 4, MW_NV = 2;
+// Tiled minor-species sources: loss and scaled production of NO or N2D
+// (values 0, 1; N2D's until its solve, then NO's) and of N4S (2, 3), and the
+// six coefficients of the NO / N4S sources linear in N2D.
+constexpr int MS_NV = 4, ML_NV = 6;
+__host__ __device__ constexpr int ms_loss(int species) { return species == 1 ? 2 : 0; }
+__host__ __device__ constexpr int ms_prod(int species) { return species == 1 ? 3 : 1; }
 // Tiled 2x2 block Thomas values of the wind solve: gamma (g11, g21, g12, g22)
 // and the forward solution (u, v).
 constexpr int UW_NV = 6;
@@ -850,10 +856,11 @@ struct MinorTerms {  // the prelude's right-hand-side parts smooth(fcomp_tm1)*dt
                      // planes by species id of the loss / scaled production of
                      // minor_src_kernel
   const double *rt;
-  const double *sloss[4], *s0prod[4];
+  double *st;  // tiled sources (MS_NV values, ms_loss / ms_prod by species)
   // NO / N4S sources as chem_kernel's linear forms in this step's N2D
-  // (minor_kernel<.., LIN = true>): lin[0..5] and the solved N2D
-  const double *lin[6], *n2d;
+  // (minor_kernel<.., LIN = true>): the tiled coefficients (ML_NV) and the
+  // solved N2D
+  const double *lt, *n2d;
 };
 
 // Production and loss of species s at level L of column (i, j): the pointwise
@@ -939,9 +946,9 @@ __global__ void __launch_bounds__(256) minor_src_kernel(tgx_ng_grid g, Geo G, tg
     if (SP[q] == 3) continue;
     double sl, sp;
     minor_sources(G, in, out.n2d, SP[q], k, i, j, sl, sp);
-    double *dl = (double *)MT.sloss[SP[q]], *dp = (double *)MT.s0prod[SP[q]];
-    dl[p] = sl;
-    dp[p] = sp * species_const(SP[q]).rmx / in.xnmbar[p];
+    const size_t ts = G.tile(k, i, j, 0, MS_NV);
+    MT.st[ts + ms_loss(SP[q]) * 64] = sl;
+    MT.st[ts + ms_prod(SP[q]) * 64] = sp * species_const(SP[q]).rmx / in.xnmbar[p];
   }
 }
 
@@ -1019,15 +1026,16 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       Q[q].rhs = MT.rt[G.tile(L, i, j, SP[q], MR_NV)];
+      const size_t ts = G.tile(L, i, j, 0, MS_NV), tl = G.tile(L, i, j, 0, ML_NV);
       if (SP[q] == 3) { Q[q].sloss = 0.; Q[q].s0prod = 0.; }
       else if (LIN && SP[q] == 0) {  // NO: sloss = lin2 + n2d lin3, s0prod = lin4 + n2d lin5
-        Q[q].sloss = MT.lin[2][p]; Q[q].la = MT.lin[3][p];
-        Q[q].s0prod = MT.lin[4][p]; Q[q].lb = MT.lin[5][p]; Q[q].n2d = MT.n2d[p];
+        Q[q].sloss = MT.lt[tl + 128]; Q[q].la = MT.lt[tl + 192];
+        Q[q].s0prod = MT.lt[tl + 256]; Q[q].lb = MT.lt[tl + 320]; Q[q].n2d = MT.n2d[p];
       } else if (LIN && SP[q] == 1) {  // N4S: s0prod = lin0 + n2d lin1
-        Q[q].sloss = MT.sloss[1][p];
-        Q[q].s0prod = MT.lin[0][p]; Q[q].lb = MT.lin[1][p]; Q[q].n2d = MT.n2d[p];
+        Q[q].sloss = MT.st[ts + ms_loss(1) * 64];
+        Q[q].s0prod = MT.lt[tl]; Q[q].lb = MT.lt[tl + 64]; Q[q].n2d = MT.n2d[p];
       }
-      else { Q[q].sloss = MT.sloss[SP[q]][p]; Q[q].s0prod = MT.s0prod[SP[q]][p]; }
+      else { Q[q].sloss = MT.st[ts + ms_loss(SP[q]) * 64]; Q[q].s0prod = MT.st[ts + ms_prod(SP[q]) * 64]; }
     }
   };
   // :66-126 interface quantities at L from levels A = L-1 and B = L (L >= 1),
@@ -1248,9 +1256,8 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
 struct ChemOut {
   double *ci;                          // tiled comp inputs (fs(:,1:2,1:2) written)
   double *fs0[2];                      // fs(:,1,0), fs(:,2,0) planes of comp_o2o
-  double *n2d_sloss, *n2d_s0prod;      // minor-species source planes of N2D
-  double *n4s_sloss;                   // N4S loss (independent of N2D)
-  double *lin[6];                      // N4S s0prod a, b; NO sloss a, b; NO s0prod a, b
+  double *st;                          // tiled minor sources: N2D loss, s0prod, N4S loss
+  double *lt;                          // tiled N4S s0prod a, b; NO sloss a, b; NO s0prod a, b
   double *oploss, *opprod;             // O+ loss and production
 };
 
@@ -1281,8 +1288,8 @@ __host__ __device__ inline int chem_runs(int nk) { return (nk + CHEM_RUN - 1) / 
 // One level of the chemistry pass: c = the paired fields at this level, o =
 // at the level above (below at the top level).
 __device__ __forceinline__ void chem_level(const tgx_ng_fields &in, const tgx_ng_out &out, const ChemOut &C,
-                                           unsigned p, size_t tci, bool top, const ChemPair &c,
-                                           const ChemPair &o) {
+                                           unsigned p, size_t tci, size_t tms, size_t tml, bool top,
+                                           const ChemPair &c, const ChemPair &o) {
   const double xn = ldo(in.xnmbar, p), ne = c.ne;
   const double o1 = ldo(in.o1, p), o2 = ldo(in.o2, p), n2 = ldo(in.n2, p), op = ldo(in.op, p), no = ldo(in.no, p), n4s = ldo(in.n4s, p);
   const double n2d = ldo(in.n2d, p), o2p = ldo(in.o2p, p), nplus = ldo(in.nplus, p), n2p = ldo(in.n2p, p), nop = ldo(in.nop, p);
@@ -1382,15 +1389,16 @@ __device__ __forceinline__ void chem_level(const tgx_ng_fields &in, const tgx_ng
     sto(C.fs0[0], p, fsv[4]);
     sto(C.fs0[1], p, fsv[5]);
   }
-  sto(C.n2d_sloss, p, n2d_sl);
-  sto(C.n2d_s0prod, p, n2d_sp * RMN2D / xn);
-  sto(C.n4s_sloss, p, n4s_sl);
-  sto(C.lin[0], p, n4s_a * RMN4S / xn);
-  sto(C.lin[1], p, n4s_b * RMN4S / xn);
-  sto(C.lin[2], p, no_sla);
-  sto(C.lin[3], p, no_slb);
-  sto(C.lin[4], p, no_spa * RMNO / xn);
-  sto(C.lin[5], p, no_spb * RMNO / xn);
+  const size_t ts = tms, tl = tml;
+  C.st[ts + 0 * 64] = n2d_sl;
+  C.st[ts + 1 * 64] = n2d_sp * RMN2D / xn;
+  C.st[ts + 2 * 64] = n4s_sl;
+  C.lt[tl + 0 * 64] = n4s_a * RMN4S / xn;
+  C.lt[tl + 1 * 64] = n4s_b * RMN4S / xn;
+  C.lt[tl + 2 * 64] = no_sla;
+  C.lt[tl + 3 * 64] = no_slb;
+  C.lt[tl + 4 * 64] = no_spa * RMNO / xn;
+  C.lt[tl + 5 * 64] = no_spb * RMNO / xn;
   sto(out.xiop2p, p, x2p);
   sto(out.xiop2d, p, x2d);
   sto(C.oploss, p, oploss);
@@ -1426,7 +1434,8 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       chem_pair_load(in, p + PB, o);
     else
       o = d;
-    chem_level(in, out, C, p, G.tile(k, i, j, 0, CI_NV), top, c, o);
+    chem_level(in, out, C, p, G.tile(k, i, j, 0, CI_NV), G.tile(k, i, j, 0, MS_NV), G.tile(k, i, j, 0, ML_NV),
+               top, c, o);
     d = c;
     c = o;
   }
@@ -1434,10 +1443,7 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
 
 // The NO and N4S sources of this step from the linear coefficients of
 // chem_kernel and this step's N2D (after the N2D solve).
-__global__ void __launch_bounds__(256) minor_lin_kernel(Geo G, tgx_ng_out out, ChemOut C,
-                                                        double *__restrict__ no_sloss,
-                                                        double *__restrict__ no_s0prod,
-                                                        double *__restrict__ n4s_s0prod) {
+__global__ void __launch_bounds__(256) minor_lin_kernel(Geo G, tgx_ng_out out, ChemOut C) {
   int nb = gridDim.x;
   long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
   long npt = (long)G.nlond * G.nlatd * G.nk;
@@ -1448,9 +1454,10 @@ __global__ void __launch_bounds__(256) minor_lin_kernel(Geo G, tgx_ng_out out, C
   int k = (int)(r / G.nlatd);
   const size_t p = G.a3(k, i, j);
   const double n2d = out.n2d[p];
-  n4s_s0prod[p] = C.lin[0][p] + n2d * C.lin[1][p];
-  no_sloss[p] = C.lin[2][p] + n2d * C.lin[3][p];
-  no_s0prod[p] = C.lin[4][p] + n2d * C.lin[5][p];
+  const size_t ts = G.tile(k, i, j, 0, MS_NV), tl = G.tile(k, i, j, 0, ML_NV);
+  C.st[ts + ms_prod(1) * 64] = C.lt[tl] + n2d * C.lt[tl + 64];
+  C.st[ts + ms_loss(0) * 64] = C.lt[tl + 128] + n2d * C.lt[tl + 192];
+  C.st[ts + ms_prod(0) * 64] = C.lt[tl + 256] + n2d * C.lt[tl + 320];
 }
 
 // =============================================================================
@@ -1928,8 +1935,9 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
                    (cq.phix[2] + cq.salfax[0] * Q.pso2 + cq.salfax[1] * Q.pso1 + cq.salfax[2] * Q.pshe));
       thd[h][q] = ex - cq.alfax * dtnidz / (dzp * Q.tni);
       rhs[h][q] = MT.rt[G.tile(Lc, i, j, SP[q], MR_NV)];
+      const size_t ts = G.tile(Lc, i, j, 0, MS_NV);
       if (SP[q] == 3) { slo[h][q] = 0.; s0p[h][q] = 0.; }
-      else { slo[h][q] = MT.sloss[SP[q]][p]; s0p[h][q] = MT.s0prod[SP[q]][p]; }
+      else { slo[h][q] = MT.st[ts + ms_loss(SP[q]) * 64]; s0p[h][q] = MT.st[ts + ms_prod(SP[q]) * 64]; }
     }
   }
   // level L+1 terms from the next lane
@@ -3927,14 +3935,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
                          (const double *)D(2));
       break;
     case 25:
-    case 26: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), n2d sloss, n2d s0prod,
-                //         n4s sloss, lin[6], oploss, opprod, no sloss, no s0prod, n4s s0prod}
+    case 26: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), minor source tiles,
+                //         (unused x2), linear-coefficient tiles, (unused x5), oploss, opprod}
       ChemOut C;
       C.ci = D(0);
       C.fs0[0] = D(1);
       C.fs0[1] = D(2);
-      C.n2d_sloss = D(6); C.n2d_s0prod = D(7); C.n4s_sloss = D(8);
-      for (int q = 0; q < 6; ++q) C.lin[q] = D(9 + q);
+      C.st = D(6);
+      C.lt = D(9);
       C.oploss = D(15); C.opprod = D(16);
       if (which == 25) {
         if ((unsigned long long)G.nk * G.P * sizeof(double) >= (1ull << 32))
@@ -3943,7 +3951,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         dim3 gridc((unsigned)((nt + 255) / 256));
         hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C);
       } else
-        hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C, D(17), D(18), D(19));
+        hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C);
       break;
     }
     case 28: {  // duv with stage 1 and the stencils fused: bufs = {ztmp (unused), Thomas tiles (UW_NV)}
@@ -3969,21 +3977,17 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 18:
     case 19:
     case 27: {  // minors: bufs = {ratio tiles (MW_NV), right-hand-side tiles (MR_NV), (unused x8),
-                // sloss/s0prod of NO or N2D, sloss/s0prod of N4S}; N2D's planes are reused by NO
-                // after N2D's solve
+                // source tiles (MS_NV), (unused x3)}; N2D's source values are reused by NO after
+                // N2D's solve
       MinorTerms MT{};
       MT.rt = D(1);
-      MT.sloss[0] = MT.sloss[2] = D(10);
-      MT.s0prod[0] = MT.s0prod[2] = D(11);
-      MT.sloss[1] = D(12);
-      MT.s0prod[1] = D(13);
-      MT.sloss[3] = MT.s0prod[3] = nullptr;
+      MT.st = D(10);
       if (which == 1 && g.capture)
         hipLaunchKernelGGL((minor_kernel<2, 3, true>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 1)
         hipLaunchKernelGGL((minor_kernel<2, 3, false>), grid, block, 0, s, g, G, in, out, D(0), MT);
-      else if (which == 27) {  // bufs 14..19: chem_kernel's lin planes
-        for (int q = 0; q < 6; ++q) MT.lin[q] = D(14 + q);
+      else if (which == 27) {  // buf 14: chem_kernel's linear-coefficient tiles
+        MT.lt = D(14);
         MT.n2d = out.n2d;
         if (g.capture)
           hipLaunchKernelGGL((minor_kernel<1, 0, true, true>), grid, block, 0, s, g, G, in, out, D(0), MT);

@@ -31,9 +31,7 @@ namespace {
 // Workspace planes, each nk * ld * (nlat+4) doubles.
 enum {
   W_DT_T = 0,                            // dt: 2 prelude planes (PCR)
-  W_MIN_S = 2,                           // minor sources: 2 species x (sloss, s0prod)
-  W_MIN_L = W_MIN_S + 4,                 // NO / N4S sources linear in N2D: 6 coefficient planes
-  W_OP = W_MIN_L + 6,                    // O+: 8 intermediates, 2 ping-pong, 6 prelude planes
+  W_OP = 2,                              // O+: 8 intermediates, 2 ping-pong, 6 prelude planes
   W_OP_X = W_OP + 8, W_OP_XNM = W_OP + 9, W_OP_T = W_OP + 10,
   W_OP_C = W_OP_T + 6,                   // O+ loss and production (once per step)
   W_UV = W_OP_C + 2,                     // wind: ztmp, 6 prelude planes
@@ -50,7 +48,9 @@ enum {
   T_UW = T_MW + 2,  // winds: 2x2 block gamma and forward solution
   T_DW = T_UW + 6,  // TN: Thomas ratio and forward solution
   T_OW = T_DW + 2,  // O+: Thomas ratio and forward solution
-  T_N = T_OW + 2
+  T_MS = T_OW + 2,  // minor species: loss and scaled production of NO or N2D, and of N4S
+  T_ML = T_MS + 4,  // minor species: NO / N4S source coefficients linear in N2D
+  T_N = T_ML + 6
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -212,15 +212,13 @@ void chem_bufs(void *work, const tgx_ng_grid *g, void **b) {
   b[1] = plane(work, g, W_CO + 0);                                   // comp fs(:,1,0)
   b[2] = plane(work, g, W_CO + 1);                                   // comp fs(:,2,0)
   b[3] = b[4] = b[5] = nullptr;
-  b[6] = plane(work, g, W_MIN_S + 0);                                // N2D sloss (then NO's)
-  b[7] = plane(work, g, W_MIN_S + 1);                                // N2D s0prod (then NO's)
-  b[8] = plane(work, g, W_MIN_S + 2);                                // N4S sloss
-  for (int q = 0; q < 6; ++q) b[9 + q] = plane(work, g, W_MIN_L + q);
+  b[6] = tiles(work, g, T_MS);  // N2D loss, s0prod (then NO's), N4S loss, s0prod
+  b[7] = b[8] = nullptr;
+  b[9] = tiles(work, g, T_ML);  // NO / N4S source coefficients
+  for (int q = 10; q < 15; ++q) b[q] = nullptr;
   b[15] = plane(work, g, W_OP_C);
   b[16] = plane(work, g, W_OP_C + 1);
-  b[17] = plane(work, g, W_MIN_S + 0);  // minor_lin: NO sloss
-  b[18] = plane(work, g, W_MIN_S + 1);  // NO s0prod
-  b[19] = plane(work, g, W_MIN_S + 3);  // N4S s0prod
+  b[17] = b[18] = b[19] = nullptr;
 }
 int run_chem(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
              hipStream_t s) {
@@ -259,7 +257,8 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   b[0] = tiles(work, g, T_MW);
   b[1] = tiles(work, g, T_MR);
   for (int q = 2; q < 10; ++q) b[q] = nullptr;
-  for (int q = 0; q < 4; ++q) b[10 + q] = plane(work, g, W_MIN_S + q);
+  b[10] = tiles(work, g, T_MS);
+  for (int q = 11; q < 14; ++q) b[q] = nullptr;
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
   const bool pcr = use_pcr(g, R_MINORS);
@@ -273,7 +272,8 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
                             // step's chemistry pass and this N2D
     void *b2[20];
     for (int q = 0; q < 14; ++q) b2[q] = b[q];
-    for (int q = 0; q < 6; ++q) b2[14 + q] = plane(work, g, W_MIN_L + q);
+    b2[14] = tiles(work, g, T_ML);
+    for (int q = 15; q < 20; ++q) b2[q] = nullptr;
     return tgx_ng_launch(K_MIN_B_LIN, g, in, out, b2, 0, s);
   }
   if (chem_done) {  // NO and N4S sources from the step's chemistry pass and this N2D
