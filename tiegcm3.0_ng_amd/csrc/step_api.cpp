@@ -11,18 +11,6 @@
 #include "../../include/tgx_step.h"
 #include "tgx_common.h"
 
-namespace tgx {
-namespace ng {
-struct Geo {
-  int nk, nlon, nlat, nlond, nlatd, ld;
-  size_t P;
-};
-struct OpWork {
-  double *dj, *bdotu, *bvel, *tpop, *diffj, *djbz, *opbm2, *bdopj, *w2;
-};
-}  // namespace ng
-}  // namespace tgx
-
 // launchers defined in step.hip
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_fields *in,
                              const tgx_ng_out *out, void *const *bufs, int sub, hipStream_t s);
