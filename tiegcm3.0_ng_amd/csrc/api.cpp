@@ -156,7 +156,7 @@ int time_scheme(int scheme, const double *a, const double *b, const double *c, c
 
 extern "C" int tgx_autotune_choice(int ldk, int k1, int k2, int ncol) {
   int dev = 0;
-  hipGetDevice(&dev);
+  if (hipGetDevice(&dev) != hipSuccess) return TGX_SCHEME_AUTO;  // no device: nothing tuned
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_tune.find({dev, ldk, k2 - k1 + 1, col_bucket(ncol)});
   return it == g_tune.end() ? TGX_SCHEME_AUTO : it->second;
@@ -174,7 +174,7 @@ extern "C" int tgx_trsolv_dev_f64(const double *a, const double *b, const double
   if (scheme == TGX_SCHEME_PCR && k2 - k1 + 1 > 256) scheme = TGX_SCHEME_THOMAS;
   if (scheme == TGX_SCHEME_AUTO) {
     int dev = 0;
-    hipGetDevice(&dev);
+    if (int rc = current_device(&dev)) return rc;
     auto key = std::make_tuple(dev, ldk, k2 - k1 + 1, col_bucket(ncol));
     int choice = TGX_SCHEME_AUTO;
     {
