@@ -26,8 +26,8 @@ enum {
   W_CO = W_UV + 7,                       // comp: fs(:,1,0), fs(:,2,0)
   W_NPLANES = W_CO + 2
 };
-// Tiled regions after the planes, in units of nk * ngw * 64 doubles (ngw
-// groups of 64 columns; the kernels' Geo::tile gives the element order).
+// Tiled regions after the planes, in units of tile_elems() doubles per value
+// (the kernels' Geo::tile and Geo::wtile give the element order).
 enum {
   T_CI = 0,        // comp inputs: 3 right-hand-side parts, 4 fs values
   T_CE = T_CI + 7, // comp elimination: 9 gamma, 3 forward solution
