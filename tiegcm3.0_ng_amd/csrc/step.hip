@@ -10,6 +10,8 @@
 //
 // Arithmetic follows the reference's evaluation order term by term (the
 // library is built with FP contraction off), citing the _ng sources.
+#include <cstdlib>
+
 #include "tgx_common.h"
 #include "../../include/tgx.h"
 #include "../../include/tgx_step.h"
@@ -2842,7 +2844,10 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
 // oplus_ng.F90, sub-step stage 1 (every column, ghosts included): the
 // pointwise and vertical quantities whose horizontal derivatives are needed
 // (:106-170).  Writes dj, bdotu, bvel, tp_op, diffj, djbz, op/bmod2**2 to the
-// workspace and Fe, Fn to the outputs.
+// workspace and Fe, Fn to the outputs.  A thread walks one of nrun runs of
+// levels of one column (the vertical derivative of tp_op carries three
+// levels): the whole column on large grids, short runs on small nest
+// domains so that they still fill the chip (oplus_runs).
 struct OpWork {
   double *dj, *bdotu, *bvel, *tpop, *diffj, *djbz, *opbm2, *bdopj, *w2;
 };
@@ -2852,10 +2857,17 @@ struct OpTerms {  // prelude planes of one sub-step
 
 __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     const double *__restrict__ op, OpWork W,
-                                                    double *__restrict__ Fe, double *__restrict__ Fn) {
-  int i, j;
-  if (!column_of(G, i, j)) return;
+                                                    double *__restrict__ Fe, double *__restrict__ Fn,
+                                                    int nrun) {
+  const int nb = gridDim.x;
+  const long tid = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
   const int nk = G.nk;
+  if (tid >= (long)G.nlond * G.nlatd * nrun) return;
+  const int i = (int)(tid % G.nlond);
+  const long r = tid / G.nlond;
+  const int j = (int)(r % G.nlatd);
+  const int run = (int)(r / G.nlatd);
+  const int k0 = (int)((long)run * nk / nrun), k1 = (int)((long)(run + 1) * nk / nrun);
   const double dz = g.dz;
   const size_t c2 = G.a2(i, j);
   const double bx = in.bx[c2], by = in.by[c2], bz = in.bz[c2], bm = in.bmod2[c2];
@@ -2863,8 +2875,8 @@ __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng
   const double sndec = in.sndec[c2], csdec = in.csdec[c2];
 #define F3(f, k) (f)[G.a3((k), i, j)]
   auto tp_at = [&](int L) { return (F3(in.te, L) + F3(in.ti, L)) * F3(op, L); };
-  double tpm = 0., tpc = tp_at(0), tpn = tp_at(1);
-  for (int k = 0; k < nk; ++k) {
+  double tpm = k0 > 0 ? tp_at(k0 - 1) : 0., tpc = tp_at(k0), tpn = k0 + 1 < nk ? tp_at(k0 + 1) : tpc;
+  for (int k = k0; k < k1; ++k) {
     size_t p = G.a3(k, i, j);
     double wni = k < nk - 1 ? .5 * (in.w[p] + in.w[p + G.P]) : 1.5 * in.w[p] - .5 * in.w[p - G.P];
     double tr = 0.5 * (in.tn[p] + in.ti[p]);
@@ -2901,16 +2913,34 @@ __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng
 #undef F3
 }
 
-// stage 2 (every column): bdotdh_opj = bdotdh(tp_op) * dj (:175-176)
-__global__ void __launch_bounds__(256) oplus_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in, OpWork W) {
-  int i, j;
-  if (!column_of(G, i, j)) return;
+// stage 2 (every column, nrun runs of levels as stage 1): bdotdh_opj =
+// bdotdh(tp_op) * dj (:175-176)
+__global__ void __launch_bounds__(256) oplus_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in, OpWork W,
+                                                    int nrun) {
+  const int nb = gridDim.x;
+  const long tid = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  if (tid >= (long)G.nlond * G.nlatd * nrun) return;
+  const int i = (int)(tid % G.nlond);
+  const long r = tid / G.nlond;
+  const int j = (int)(r % G.nlatd);
+  const int run = (int)(r / G.nlatd);
+  const int k0 = (int)((long)run * G.nk / nrun), k1 = (int)((long)(run + 1) * G.nk / nrun);
   const size_t c2 = G.a2(i, j);
   const double bx = in.bx[c2], by = in.by[c2], cs = g.cs[j];
-  for (int k = 0; k < G.nk; ++k) {
-    size_t off = (size_t)k * G.P, p = G.a3(k, i, j);
+  for (int k = k0; k < k1; ++k) {
+    const size_t off = (size_t)k * G.P, p = G.a3(k, i, j);
     W.bdopj[p] = bdotdh(W.tpop + off, G, i, j, bx, by, cs, g.dlamda, g.dphi) * W.dj[p];
   }
+}
+// Runs of levels per column for the O+ stages: the whole column once a grid
+// has enough columns to fill the chip (one column loop was measured faster
+// there), else runs of about CHEM_RUN levels (stage 1) and single levels
+// (stage 2).  TGX_OPLUS_RUNS=1 in the environment forces the whole-column
+// loop on every grid (both forms give bitwise the same values).
+__host__ inline int oplus_runs(long ncol, int nk, int stage) {
+  const char *force = getenv("TGX_OPLUS_RUNS");
+  if (ncol >= 65536 || (force && force[0] == '1' && force[1] == 0)) return 1;
+  return stage == 1 ? chem_runs(nk) : nk;
 }
 
 // stage 3: assembly, solve and outputs of one sub-step (:172-340)
@@ -4022,9 +4052,13 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL(oplus_chem_kernel, gridp, block, 0, s, g, G, in, D(19), D(20), out.xiop2p,
                            out.xiop2d);
       } else if (which == 3) {
-        hipLaunchKernelGGL(oplus_stage1, grid, block, 0, s, g, G, in, op, W, out.Fe, out.Fn);
+        const int nrun = oplus_runs(ncol, G.nk, 1);
+        dim3 grid1((unsigned)((ncol * nrun + 255) / 256));
+        hipLaunchKernelGGL(oplus_stage1, grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, nrun);
       } else if (which == 4) {
-        hipLaunchKernelGGL(oplus_stage2, grid, block, 0, s, g, G, in, W);
+        const int nrun = oplus_runs(ncol, G.nk, 2);
+        dim3 grid2((unsigned)((ncol * nrun + 255) / 256));
+        hipLaunchKernelGGL(oplus_stage2, grid2, block, 0, s, g, G, in, W, nrun);
       } else if (which == 9) {
         StencilJobs J{};
         J.n = 6;
