@@ -1483,6 +1483,25 @@ __device__ __forceinline__ void wave_up(double lo, double hi, int lane, double &
   uhi = __shfl(hi, src, 64);
 }
 
+// The same for NH rows per lane: NH = 1 when the column has at most 64
+// levels (lane l holds level l only; hi is neither read nor written).
+template <int NH>
+__device__ __forceinline__ void wave_dn_h(double lo, double hi, int lane, double &dlo, double &dhi) {
+  if constexpr (NH == 2) {
+    wave_dn(lo, hi, lane, dlo, dhi);
+  } else {
+    dlo = __shfl(lo, (lane - 1) & 63, 64);
+  }
+}
+template <int NH>
+__device__ __forceinline__ void wave_up_h(double lo, double hi, int lane, double &ulo, double &uhi) {
+  if constexpr (NH == 2) {
+    wave_up(lo, hi, lane, ulo, uhi);
+  } else {
+    ulo = __shfl(lo, (lane + 1) & 63, 64);
+  }
+}
+
 struct TriRow { double a, b, c, f; };  // a x(k-1) + b x(k) + c x(k+1) = f
 
 __device__ __forceinline__ TriRow tri_ident() { return TriRow{0., 1., 0., 0.}; }
@@ -1749,6 +1768,18 @@ __device__ __forceinline__ void wave_pcr3(Nrm3 &lo, Nrm3 &hi, int lane) {
     nrm3_update(hi, sm, hi, lo, offer_m_hi, true, sp, hi, hi, true, hi_p_ok, nhi);
     lo = nlo;
     hi = nhi;
+  }
+}
+
+// The same reduction for one row per lane (at most 64 rows): partners past
+// either end of the wave are absent.
+__device__ __forceinline__ void wave_pcr3_1(Nrm3 &lo, int lane) {
+#pragma unroll 1
+  for (int d = 1; d < 64; d <<= 1) {
+    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
+    Nrm3 nlo;
+    nrm3_update(lo, sm, lo, lo, true, lane >= d, sp, lo, lo, true, lane + d < 64, nlo);
+    lo = nlo;
   }
 }
 
@@ -2583,6 +2614,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
 // 0..nk-2 by block cyclic reduction, the closed-form top level (:317-332),
 // then the post pass (:415-457) with the N2 extrapolation level found by a
 // wave ballot.
+template <int NH>
 __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                         tgx_ng_out out, CompTerms CT) {
   int i, j;
@@ -2593,12 +2625,12 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
   const size_t c2 = G.a2(i, j);
   const double fhe = g.calc_helium == 0 ? 0. : in.flx_he[c2];
   if (lane == 0) out.flx_he[c2] = fhe;
-  double x[2][3];  // O2, O, He at this lane's levels before the post pass
+  double x[2][3] = {};  // O2, O, He at this lane's levels before the post pass
   double v;
   const bool bnd = lateral_value(G, in.lonb_o2, in.latb_o2, 0, i, j, v);
   if (bnd) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int Lc = Lr[h] < nk ? Lr[h] : nk - 1;
       lateral_value(G, in.lonb_o2, in.latb_o2, Lc, i, j, x[h][0]);
       lateral_value(G, in.lonb_o1, in.latb_o1, Lc, i, j, x[h][1]);
@@ -2615,22 +2647,23 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
     const double embar0 =
         1. / (ps0[0] * RIO2 + ps0[1] * RIO1 + ps0[2] * RIHE + (1. - ps0[0] - ps0[1] - ps0[2]) * RIN2);
     struct LR { double o2, o1, he, mbar, tn, w; };
-    LR R[2], A[2];
+    LR R[2] = {}, A[2] = {};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
       R[h].o2 = in.o2[p]; R[h].o1 = in.o1[p]; R[h].he = in.he[p]; R[h].mbar = in.mbar[p];
       R[h].tn = in.tn[p]; R[h].w = in.w[p];
     }
-    wave_dn(R[0].o2, R[1].o2, lane, A[0].o2, A[1].o2);
-    wave_dn(R[0].o1, R[1].o1, lane, A[0].o1, A[1].o1);
-    wave_dn(R[0].he, R[1].he, lane, A[0].he, A[1].he);
-    wave_dn(R[0].mbar, R[1].mbar, lane, A[0].mbar, A[1].mbar);
-    wave_dn(R[0].tn, R[1].tn, lane, A[0].tn, A[1].tn);
+    wave_dn_h<NH>(R[0].o2, R[1].o2, lane, A[0].o2, A[1].o2);
+    wave_dn_h<NH>(R[0].o1, R[1].o1, lane, A[0].o1, A[1].o1);
+    wave_dn_h<NH>(R[0].he, R[1].he, lane, A[0].he, A[1].he);
+    wave_dn_h<NH>(R[0].mbar, R[1].mbar, lane, A[0].mbar, A[1].mbar);
+    wave_dn_h<NH>(R[0].tn, R[1].tn, lane, A[0].tn, A[1].tn);
     // interface L (between levels L-1 and L): ak = matinv3(alpha)*wks1, ep, wks
-    double ak[2][9], ep[2][3], wks[2], a22[2], a33[2], a23[2], a32[2], flx[2];
+    double ak[2][9] = {}, ep[2][3] = {}, wks[2] = {}, a22[2] = {}, a33[2] = {}, a23[2] = {}, a32[2] = {},
+           flx[2] = {};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int L = Lr[h];
       double o2i, o1i, hei, embari, tni, dembardz, dtndz, df[3];
       if (L == 0) {  // :80-97
@@ -2681,17 +2714,17 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
     }
     // the interface above (L+1) from the next lane
     double akp[2][9], epp[2][3], wksp[2], wp[2], a22p[2], a33p[2], a23p[2], a32p[2];
-    for (int n = 0; n < 9; ++n) wave_up(ak[0][n], ak[1][n], lane, akp[0][n], akp[1][n]);
-    for (int n = 0; n < 3; ++n) wave_up(ep[0][n], ep[1][n], lane, epp[0][n], epp[1][n]);
-    wave_up(wks[0], wks[1], lane, wksp[0], wksp[1]);
-    wave_up(R[0].w, R[1].w, lane, wp[0], wp[1]);
-    wave_up(a22[0], a22[1], lane, a22p[0], a22p[1]);
-    wave_up(a33[0], a33[1], lane, a33p[0], a33p[1]);
-    wave_up(a23[0], a23[1], lane, a23p[0], a23p[1]);
-    wave_up(a32[0], a32[1], lane, a32p[0], a32p[1]);
-    Nrm3 rows[2];
+    for (int n = 0; n < 9; ++n) wave_up_h<NH>(ak[0][n], ak[1][n], lane, akp[0][n], akp[1][n]);
+    for (int n = 0; n < 3; ++n) wave_up_h<NH>(ep[0][n], ep[1][n], lane, epp[0][n], epp[1][n]);
+    wave_up_h<NH>(wks[0], wks[1], lane, wksp[0], wksp[1]);
+    wave_up_h<NH>(R[0].w, R[1].w, lane, wp[0], wp[1]);
+    wave_up_h<NH>(a22[0], a22[1], lane, a22p[0], a22p[1]);
+    wave_up_h<NH>(a33[0], a33[1], lane, a33p[0], a33p[1]);
+    wave_up_h<NH>(a23[0], a23[1], lane, a23p[0], a23p[1]);
+    wave_up_h<NH>(a32[0], a32[1], lane, a32p[0], a32p[1]);
+    Nrm3 rows[2] = {};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int k = Lr[h];
       Nrm3 &N3 = rows[h];
       if (k > nk - 2) {
@@ -2760,14 +2793,17 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       blk_mm<3>(Bi, Cr, N3.C);
       blk_mv<3>(Bi, fk, N3.F);
     }
-    wave_pcr3(rows[0], rows[1], lane);
+    if constexpr (NH == 2)
+      wave_pcr3(rows[0], rows[1], lane);
+    else
+      wave_pcr3_1(rows[0], lane);
     // the closed-form top level (:317-332) from level nk-2
     double xm[2][3], flxm[2];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) wave_dn(rows[0].F[n], rows[1].F[n], lane, xm[0][n], xm[1][n]);
-    wave_dn(flx[0], flx[1], lane, flxm[0], flxm[1]);
+    for (int n = 0; n < 3; ++n) wave_dn_h<NH>(rows[0].F[n], rows[1].F[n], lane, xm[0][n], xm[1][n]);
+    wave_dn_h<NH>(flx[0], flx[1], lane, flxm[0], flxm[1]);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       for (int n = 0; n < 3; ++n) x[h][n] = rows[h].F[n];
       if (Lr[h] == nk - 1) {
         const double *e_ = ep[h];  // ep of the top interface
@@ -2782,9 +2818,9 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
     }
   }
   // :415-457 time smoothing, N2 above the O-dominated top, clamps, normalisation
-  double n2[2], xn[2];
+  double n2[2] = {}, xn[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
     xn[h] = in.xnmbar[p];
     n2[h] = 1. - x[h][0] - x[h][1] - x[h][2];
@@ -2813,7 +2849,7 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
     }
   }
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int k = Lr[h];
     if (k >= nk) continue;
     size_t p = G.a3(k, i, j);
@@ -4104,7 +4140,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         else
           hipLaunchKernelGGL(comp_kernel<false>, grid, block, 0, s, g, G, in, out, D(0), CT);
       } else if (which == 23) {
-        hipLaunchKernelGGL(comp_wave_kernel, gridw, block, 0, s, g, G, in, out, CT);
+        if (G.nk <= 64)
+          hipLaunchKernelGGL(comp_wave_kernel<1>, gridw, block, 0, s, g, G, in, out, CT);
+        else
+          hipLaunchKernelGGL(comp_wave_kernel<2>, gridw, block, 0, s, g, G, in, out, CT);
       } else {
         dim3 grido((unsigned)((ncol * (G.nk - 1) + 255) / 256));
         hipLaunchKernelGGL(o2o_kernel, grido, block, 0, s, g, G, in, CT);
