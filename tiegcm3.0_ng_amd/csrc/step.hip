@@ -1553,6 +1553,22 @@ __device__ __forceinline__ void wave_pcr(TriRow (&lo)[NS], TriRow (&hi)[NS], int
   }
 }
 
+// The same for one row per lane (systems of at most 64 rows): partners past
+// either end of the wave are absent.
+template <int NS>
+__device__ __forceinline__ void wave_pcr1(TriRow (&lo)[NS], int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
+    const bool has_m = lane >= d, has_p = lane + d < 64;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      TriRow Lm = tri_shfl(lo[q], sm), Lp = tri_shfl(lo[q], sp);
+      lo[q] = tri_reduce(lo[q], has_m ? Lm : tri_ident(), has_p ? Lp : tri_ident());
+    }
+  }
+}
+
 // Block rows A x(k-1) + B x(k) + C x(k+1) = F with N x N blocks, row-major
 // (e = n*N + m is row n, column m).
 template <int N>
@@ -1796,7 +1812,7 @@ __device__ __forceinline__ bool wave_column_of(const Geo &G, int &i, int &j) {
 
 // minor_ng.F90:60-237 for species SA and SB, one wavefront per column, the
 // solve by parallel cyclic reduction over the nk rows.
-template <int SA, int SB>
+template <int SA, int SB, int NH>
 __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                          tgx_ng_out out, MinorTerms MT) {
   int i, j;
@@ -1821,7 +1837,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
   double v;
   if (lateral_value(G, in.lonb_no, in.latb_no, 0, i, j, v)) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int L = Lr[h];
       if (L >= nk) continue;
       size_t p = G.a3(L, i, j);
@@ -1882,24 +1898,24 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
   // own-level fields (levels beyond the column read the top level; their
   // rows become identity rows)
   struct MR { double o1, o2, he, mbar, tn, barm, w; };
-  MR R[2];
+  MR R[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
     R[h].o1 = in.o1[p]; R[h].o2 = in.o2[p]; R[h].he = in.he[p]; R[h].mbar = in.mbar[p];
     R[h].tn = in.tn[p]; R[h].barm = in.barm[p]; R[h].w = in.w_upd[p];
   }
   MR A[2];  // level L-1
-  wave_dn(R[0].o1, R[1].o1, lane, A[0].o1, A[1].o1);
-  wave_dn(R[0].o2, R[1].o2, lane, A[0].o2, A[1].o2);
-  wave_dn(R[0].he, R[1].he, lane, A[0].he, A[1].he);
-  wave_dn(R[0].mbar, R[1].mbar, lane, A[0].mbar, A[1].mbar);
-  wave_dn(R[0].tn, R[1].tn, lane, A[0].tn, A[1].tn);
-  wave_dn(R[0].w, R[1].w, lane, A[0].w, A[1].w);
+  wave_dn_h<NH>(R[0].o1, R[1].o1, lane, A[0].o1, A[1].o1);
+  wave_dn_h<NH>(R[0].o2, R[1].o2, lane, A[0].o2, A[1].o2);
+  wave_dn_h<NH>(R[0].he, R[1].he, lane, A[0].he, A[1].he);
+  wave_dn_h<NH>(R[0].mbar, R[1].mbar, lane, A[0].mbar, A[1].mbar);
+  wave_dn_h<NH>(R[0].tn, R[1].tn, lane, A[0].tn, A[1].tn);
+  wave_dn_h<NH>(R[0].w, R[1].w, lane, A[0].w, A[1].w);
   // :66-126 interface quantities at level L
-  MinorShared S[2];
+  MinorShared S[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     MinorShared &Q = S[h];
     if (L == 0) {
@@ -1937,12 +1953,12 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
     matinv3_ng(al, Q.inv);
   }
   double tnim[2], tnip[2];
-  wave_dn(S[0].tni, S[1].tni, lane, tnim[0], tnim[1]);
-  wave_up(S[0].tni, S[1].tni, lane, tnip[0], tnip[1]);
+  wave_dn_h<NH>(S[0].tni, S[1].tni, lane, tnim[0], tnim[1]);
+  wave_up_h<NH>(S[0].tni, S[1].tni, lane, tnip[0], tnip[1]);
   // species terms at level L
-  double ax[2][2], thd[2][2], slo[2][2], s0p[2][2], rhs[2][2];
+  double ax[2][2] = {}, thd[2][2] = {}, slo[2][2] = {}, s0p[2][2] = {}, rhs[2][2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     const int Lc = L < nk ? L : nk - 1;
     size_t p = G.a3(Lc, i, j);
@@ -1976,22 +1992,22 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
   // level L+1 terms from the next lane
   double axn[2][2], thdn[2][2], dmn[2], wn[2];
   {
-    double dm0 = S[0].dmdz / S[0].barm, dm1 = S[1].dmdz / S[1].barm;
-    wave_up(dm0, dm1, lane, dmn[0], dmn[1]);
-    wave_up(R[0].w, R[1].w, lane, wn[0], wn[1]);
+    double dm0 = S[0].dmdz / S[0].barm, dm1 = NH == 2 ? S[1].dmdz / S[1].barm : 0.;
+    wave_up_h<NH>(dm0, dm1, lane, dmn[0], dmn[1]);
+    wave_up_h<NH>(R[0].w, R[1].w, lane, wn[0], wn[1]);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       double u0, u1;
-      wave_up(ax[0][q], ax[1][q], lane, u0, u1);
+      wave_up_h<NH>(ax[0][q], ax[1][q], lane, u0, u1);
       axn[0][q] = u0; axn[1][q] = u1;
-      wave_up(thd[0][q], thd[1][q], lane, u0, u1);
+      wave_up_h<NH>(thd[0][q], thd[1][q], lane, u0, u1);
       thdn[0][q] = u0; thdn[1][q] = u1;
     }
   }
   // rows of the two systems (:160-198), identity beyond the column
-  TriRow rlo[2], rhi[2];
+  TriRow rlo[2] = {}, rhi[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     const double dmdzc = S[h].dmdz / S[h].barm;
     const int Lc = L < nk ? L : nk - 1;
@@ -2028,10 +2044,13 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
       if (h == 0) rlo[q] = t; else rhi[q] = t;
     }
   }
-  wave_pcr<2>(rlo, rhi, lane);
+  if constexpr (NH == 2)
+    wave_pcr<2>(rlo, rhi, lane);
+  else
+    wave_pcr1<2>(rlo, lane);
   // outputs (:234-237)
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     if (L >= nk) continue;
     size_t p = G.a3(L, i, j);
@@ -4063,10 +4082,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL((minor_kernel<1, 0, true>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 2)
         hipLaunchKernelGGL((minor_kernel<1, 0, false>), grid, block, 0, s, g, G, in, out, D(0), MT);
+      else if (which == 18 && G.nk <= 64)
+        hipLaunchKernelGGL((minor_wave_kernel<2, 3, 1>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 18)
-        hipLaunchKernelGGL((minor_wave_kernel<2, 3>), gridw, block, 0, s, g, G, in, out, MT);
+        hipLaunchKernelGGL((minor_wave_kernel<2, 3, 2>), gridw, block, 0, s, g, G, in, out, MT);
+      else if (which == 19 && G.nk <= 64)
+        hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 19)
-        hipLaunchKernelGGL((minor_wave_kernel<1, 0>), gridw, block, 0, s, g, G, in, out, MT);
+        hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 15)
         hipLaunchKernelGGL((minor_src_kernel<2, 3>), gridp, block, 0, s, g, G, in, out, MT);
       else
