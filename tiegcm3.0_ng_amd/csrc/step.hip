@@ -1706,6 +1706,24 @@ __device__ __forceinline__ void wave_block_pcr(BlkRow<N> &lo, BlkRow<N> &hi, int
   blk_mv<N>(Bi, hi.F, xhi);
 }
 
+// The same for one row per lane (at most 64 rows): partners past either end
+// of the wave are absent.
+template <int N>
+__device__ __forceinline__ void wave_block_pcr1(BlkRow<N> &lo, int lane, double *xlo) {
+#pragma unroll 1
+  for (int d = 1; d < 64; d <<= 1) {
+    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
+    const bool has_m = lane >= d, has_p = lane + d < 64;
+    BlkNorm<N> nlo = blk_normalize<N>(lo);
+    BlkNorm<N> Lm = blk_shfl<N>(nlo, sm), Lp = blk_shfl<N>(nlo, sp);
+    const BlkNorm<N> Z = blk_zero<N>();
+    blk_reduce<N>(lo, blk_sel<N>(has_m, Lm, Z), blk_sel<N>(has_p, Lp, Z));
+  }
+  double Bi[N * N];
+  blk_inv<N>(lo.B, Bi);
+  blk_mv<N>(Bi, lo.F, xlo);
+}
+
 struct Nrm3 { double A[9], C[9], F[3]; };
 
 // dst = (ok ? the source lane's (pick_a ? a : b) : 0), element by element
@@ -2068,6 +2086,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
 
 // dt_ng.F90:54-213, one wavefront per column: rows 0..nk-2 solved by
 // parallel cyclic reduction, the top level extrapolated (:167).
+template <int NH>
 __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                       tgx_ng_out out,
                                                       const double *__restrict__ hadv_tn,
@@ -2086,7 +2105,7 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
   double v;
   if (lateral_value(G, in.lonb_tn, in.latb_tn, 0, i, j, v)) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int L = Lr[h];
       if (L >= nk) continue;
       lateral_value(G, in.lonb_tn, in.latb_tn, L, i, j, v);
@@ -2097,29 +2116,29 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
     return;
   }
   struct DL { double tn, w, cp, km, un, vn, qtotal, schti, kt; };
-  DL R[2];
+  DL R[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
     R[h].tn = in.tn[p]; R[h].w = in.w_upd[p]; R[h].cp = in.cp[p]; R[h].km = in.km[p];
     R[h].un = in.un[p]; R[h].vn = in.vn[p]; R[h].qtotal = in.qtotal[p]; R[h].schti = in.schti[p];
     R[h].kt = in.kt[p];
   }
   double tnm[2], tnp[2], unm[2], unp[2], vnm[2], vnp[2], wp[2], cpp[2], kmp[2], qtp[2];
-  wave_dn(R[0].tn, R[1].tn, lane, tnm[0], tnm[1]);
-  wave_up(R[0].tn, R[1].tn, lane, tnp[0], tnp[1]);
-  wave_dn(R[0].un, R[1].un, lane, unm[0], unm[1]);
-  wave_up(R[0].un, R[1].un, lane, unp[0], unp[1]);
-  wave_dn(R[0].vn, R[1].vn, lane, vnm[0], vnm[1]);
-  wave_up(R[0].vn, R[1].vn, lane, vnp[0], vnp[1]);
-  wave_up(R[0].w, R[1].w, lane, wp[0], wp[1]);
-  wave_up(R[0].cp, R[1].cp, lane, cpp[0], cpp[1]);
-  wave_up(R[0].km, R[1].km, lane, kmp[0], kmp[1]);
-  wave_up(R[0].qtotal, R[1].qtotal, lane, qtp[0], qtp[1]);
+  wave_dn_h<NH>(R[0].tn, R[1].tn, lane, tnm[0], tnm[1]);
+  wave_up_h<NH>(R[0].tn, R[1].tn, lane, tnp[0], tnp[1]);
+  wave_dn_h<NH>(R[0].un, R[1].un, lane, unm[0], unm[1]);
+  wave_up_h<NH>(R[0].un, R[1].un, lane, unp[0], unp[1]);
+  wave_dn_h<NH>(R[0].vn, R[1].vn, lane, vnm[0], vnm[1]);
+  wave_up_h<NH>(R[0].vn, R[1].vn, lane, vnp[0], vnp[1]);
+  wave_up_h<NH>(R[0].w, R[1].w, lane, wp[0], wp[1]);
+  wave_up_h<NH>(R[0].cp, R[1].cp, lane, cpp[0], cpp[1]);
+  wave_up_h<NH>(R[0].km, R[1].km, lane, kmp[0], kmp[1]);
+  wave_up_h<NH>(R[0].qtotal, R[1].qtotal, lane, qtp[0], qtp[1]);
   // g(L), f(L) of dt_ng.F90:121-139 at this lane's levels
-  double gg[2], ff[2];
+  double gg[2] = {}, ff[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h], Lc = L < nk ? L : nk - 1;
     double tni = L == 0 ? tl : .5 * (tnm[h] + R[h].tn);
     double hh = R[h].schti;
@@ -2132,12 +2151,12 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
     ff[h] = f * g.dift[Lc];
   }
   double ggp[2], ffp[2];
-  wave_up(gg[0], gg[1], lane, ggp[0], ggp[1]);
-  wave_up(ff[0], ff[1], lane, ffp[0], ffp[1]);
+  wave_up_h<NH>(gg[0], gg[1], lane, ggp[0], ggp[1]);
+  wave_up_h<NH>(ff[0], ff[1], lane, ffp[0], ffp[1]);
   // dudz, dvdz (:85-94); the top row takes a third of the row below's
-  double dudz[2], dvdz[2];
+  double dudz[2] = {}, dvdz[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     if (L == 0) {
       dudz[h] = (R[h].un + 1. / 3. * unp[h] - 4. / 3. * in.ulbc[c2]) / dz;
@@ -2148,11 +2167,11 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
     }
   }
   double dudzm[2], dvdzm[2];
-  wave_dn(dudz[0], dudz[1], lane, dudzm[0], dudzm[1]);
-  wave_dn(dvdz[0], dvdz[1], lane, dvdzm[0], dvdzm[1]);
-  TriRow rows[2][1];
+  wave_dn_h<NH>(dudz[0], dudz[1], lane, dudzm[0], dudzm[1]);
+  wave_dn_h<NH>(dvdz[0], dvdz[1], lane, dvdzm[0], dvdzm[1]);
+  TriRow rows[2][1] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     if (L > nk - 2) { rows[h][0] = tri_ident(); continue; }
     size_t p = G.a3(L, i, j);
@@ -2200,12 +2219,15 @@ __global__ void __launch_bounds__(256) dt_wave_kernel(tgx_ng_grid g, Geo G, tgx_
     rows[h][0] = TriRow{pc, qc, rc, rhs};
   }
   TriRow lo[1] = {rows[0][0]}, hi[1] = {rows[1][0]};
-  wave_pcr<1>(lo, hi, lane);
-  double x[2] = {lo[0].f / lo[0].b, hi[0].f / hi[0].b}, xm1[2], xm2[2];
-  wave_dn(x[0], x[1], lane, xm1[0], xm1[1]);
-  wave_dn(xm1[0], xm1[1], lane, xm2[0], xm2[1]);
+  if constexpr (NH == 2)
+    wave_pcr<1>(lo, hi, lane);
+  else
+    wave_pcr1<1>(lo, lane);
+  double x[2] = {lo[0].f / lo[0].b, NH == 2 ? hi[0].f / hi[0].b : 0.}, xm1[2], xm2[2];
+  wave_dn_h<NH>(x[0], x[1], lane, xm1[0], xm1[1]);
+  wave_dn_h<NH>(xm1[0], xm1[1], lane, xm2[0], xm2[1]);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     if (L >= nk) continue;
     double xv = L == nk - 1 ? xm1[h] * xm1[h] / xm2[h] : x[h];  // :167
@@ -3322,6 +3344,7 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
 
 // The O+ column (:172-340) with one wavefront per column: rows 0..nk-2 by parallel
 // cyclic reduction, the top level extrapolated (:307).
+template <int NH>
 __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                          const double *__restrict__ op,
                                                          const double *__restrict__ optm1,
@@ -3341,7 +3364,7 @@ __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, t
   double v;
   if (lateral_value(G, lonb, latb, 0, i, j, v)) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int L = Lr[h];
       if (L >= nk) continue;
       size_t p = G.a3(L, i, j);
@@ -3383,9 +3406,9 @@ __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, t
   const double dvbz = dvb / bz;
   const double gmr = GRAV * RMOP / (2. * GASK);
   struct OL { double scht, hdz, tp1, dj, bdotu, bdopj, wi; };
-  OL B[2], A[2], Cn[2];  // this lane's level, the one below and the one above
+  OL B[2] = {}, A[2] = {}, Cn[2] = {};  // this lane's level, the one below and the one above
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     size_t p = G.a3(Lr[h] < nk ? Lr[h] : nk - 1, i, j);
     B[h].scht = in.scht[p];
     B[h].hdz = 1. / (B[h].scht * dz);
@@ -3393,14 +3416,14 @@ __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, t
     B[h].dj = W.dj[p]; B[h].bdotu = W.bdotu[p]; B[h].bdopj = W.bdopj[p]; B[h].wi = in.wi[p];
   }
 #define TGX_NBR(fld)                                                          \
-  wave_dn(B[0].fld, B[1].fld, lane, A[0].fld, A[1].fld);                      \
-  wave_up(B[0].fld, B[1].fld, lane, Cn[0].fld, Cn[1].fld);
+  wave_dn_h<NH>(B[0].fld, B[1].fld, lane, A[0].fld, A[1].fld);                      \
+  wave_up_h<NH>(B[0].fld, B[1].fld, lane, Cn[0].fld, Cn[1].fld);
   TGX_NBR(hdz) TGX_NBR(tp1) TGX_NBR(dj) TGX_NBR(bdotu) TGX_NBR(bdopj) TGX_NBR(wi)
 #undef TGX_NBR
   const double ol1 = oploss_p[G.a3(1, i, j)], qop0 = in.qop[c2];  // lower boundary (:301-303)
-  TriRow lo[1], hi[1];
+  TriRow lo[1] = {}, hi[1] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int k = Lr[h];
     TriRow t = tri_ident();
     if (k <= nk - 2) {
@@ -3463,12 +3486,15 @@ __global__ void __launch_bounds__(256) oplus_wave_stage3(tgx_ng_grid g, Geo G, t
     }
     if (h == 0) lo[0] = t; else hi[0] = t;
   }
-  wave_pcr<1>(lo, hi, lane);
-  double x[2] = {lo[0].f / lo[0].b, hi[0].f / hi[0].b}, xm1[2], xm2[2];
-  wave_dn(x[0], x[1], lane, xm1[0], xm1[1]);
-  wave_dn(xm1[0], xm1[1], lane, xm2[0], xm2[1]);
+  if constexpr (NH == 2)
+    wave_pcr<1>(lo, hi, lane);
+  else
+    wave_pcr1<1>(lo, lane);
+  double x[2] = {lo[0].f / lo[0].b, NH == 2 ? hi[0].f / hi[0].b : 0.}, xm1[2], xm2[2];
+  wave_dn_h<NH>(x[0], x[1], lane, xm1[0], xm1[1]);
+  wave_dn_h<NH>(xm1[0], xm1[1], lane, xm2[0], xm2[1]);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     if (L >= nk) continue;
     double xv = L == nk - 1 ? xm1[h] * xm1[h] / xm2[h] : x[h];
@@ -3825,6 +3851,7 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
 
 // duv_ng stage 2 with one wavefront per column: the 2x2 block rows 0..nk-2 by
 // block cyclic reduction, the top level extrapolated (:250).
+template <int NH>
 __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                        tgx_ng_out out, UvTerms UT) {
   int i, j;
@@ -3842,7 +3869,7 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
   double v;
   if (lateral_value(G, in.lonb_un, in.latb_un, 0, i, j, v)) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int L = Lr[h];
       if (L >= nk) continue;
       size_t p = G.a3(L, i, j);
@@ -3858,10 +3885,10 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
   }
   const double racs = g.racs[j], cor = g.cor[j], tanphi = g.tanphi[j];
   struct UL { double w, lxx, lyy, lxy, lyx, ui, vi, divx, divy, km, schti; };
-  UL U[2], Un[2];
+  UL U[2] = {}, Un[2] = {};
   double e1r[2];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int Lc = Lr[h] < nk ? Lr[h] : nk - 1;
     size_t p = G.a3(Lc, i, j);
     U[h].w = in.w_upd[p]; U[h].lxx = in.lxx[p]; U[h].lyy = in.lyy[p]; U[h].lxy = in.lxy[p];
@@ -3870,29 +3897,29 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
     double e = P0 * in.scht[p] / GRAV;  // eddy viscosity (:143-152)
     e1r[h] = e * g.expz[Lc] * g.xmue[Lc];
   }
-#define TGX_UP(fld) wave_up(U[0].fld, U[1].fld, lane, Un[0].fld, Un[1].fld);
+#define TGX_UP(fld) wave_up_h<NH>(U[0].fld, U[1].fld, lane, Un[0].fld, Un[1].fld);
   TGX_UP(w) TGX_UP(lxx) TGX_UP(lyy) TGX_UP(lxy) TGX_UP(lyx) TGX_UP(ui) TGX_UP(vi) TGX_UP(divx) TGX_UP(divy)
 #undef TGX_UP
   // e1 at the top level from the two below (:148-150)
   double e1m[2], e1mm[2], e1[2];
-  wave_dn(e1r[0], e1r[1], lane, e1m[0], e1m[1]);
-  wave_dn(e1m[0], e1m[1], lane, e1mm[0], e1mm[1]);
+  wave_dn_h<NH>(e1r[0], e1r[1], lane, e1m[0], e1m[1]);
+  wave_dn_h<NH>(e1m[0], e1m[1], lane, e1mm[0], e1mm[1]);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) e1[h] = Lr[h] == nk - 1 ? sqrt(e1m[h] * e1m[h] * e1m[h] / e1mm[h]) : e1r[h];
+  for (int h = 0; h < NH; ++h) e1[h] = Lr[h] == nk - 1 ? sqrt(e1m[h] * e1m[h] * e1m[h] / e1mm[h]) : e1r[h];
   double e1d[2], e1u[2];
-  wave_dn(e1[0], e1[1], lane, e1d[0], e1d[1]);
-  wave_up(e1[0], e1[1], lane, e1u[0], e1u[1]);
+  wave_dn_h<NH>(e1[0], e1[1], lane, e1d[0], e1d[1]);
+  wave_up_h<NH>(e1[0], e1[1], lane, e1u[0], e1u[1]);
   double gg[2], ggp[2];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     double evf = Lr[h] >= 1 ? sqrt(e1d[h] * e1[h]) : e1[h] * e1[h] / sqrt(e1[h] * e1u[h]);
     gg[h] = GRAV * (U[h].km + evf) / (P0 * U[h].schti * (dz * dz));
   }
-  wave_up(gg[0], gg[1], lane, ggp[0], ggp[1]);
+  wave_up_h<NH>(gg[0], gg[1], lane, ggp[0], ggp[1]);
   const double ulbc = in.u_lbc[c2], vlbc = in.v_lbc[c2];
-  BlkRow<2> rw[2];
+  BlkRow<2> rw[2] = {};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int k = Lr[h];
     BlkRow<2> &R = rw[h];
     if (k > nk - 2) {
@@ -3954,16 +3981,19 @@ __global__ void __launch_bounds__(256) duv_wave_stage2(tgx_ng_grid g, Geo G, tgx
     R.C[0] = r11; R.C[1] = r12; R.C[2] = r21; R.C[3] = r22;
     R.F[0] = s1; R.F[1] = s2;
   }
-  double xlo[2], xhi[2];
-  wave_block_pcr<2>(rw[0], rw[1], lane, xlo, xhi);
+  double xlo[2] = {}, xhi[2] = {};
+  if constexpr (NH == 2)
+    wave_block_pcr<2>(rw[0], rw[1], lane, xlo, xhi);
+  else
+    wave_block_pcr1<2>(rw[0], lane, xlo);
   double x1[2] = {xlo[0], xhi[0]}, x2[2] = {xlo[1], xhi[1]};
   double x1m[2], x1mm[2], x2m[2], x2mm[2];
-  wave_dn(x1[0], x1[1], lane, x1m[0], x1m[1]);
-  wave_dn(x1m[0], x1m[1], lane, x1mm[0], x1mm[1]);
-  wave_dn(x2[0], x2[1], lane, x2m[0], x2m[1]);
-  wave_dn(x2m[0], x2m[1], lane, x2mm[0], x2mm[1]);
+  wave_dn_h<NH>(x1[0], x1[1], lane, x1m[0], x1m[1]);
+  wave_dn_h<NH>(x1m[0], x1m[1], lane, x1mm[0], x1mm[1]);
+  wave_dn_h<NH>(x2[0], x2[1], lane, x2m[0], x2m[1]);
+  wave_dn_h<NH>(x2m[0], x2m[1], lane, x2mm[0], x2mm[1]);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NH; ++h) {
     const int L = Lr[h];
     if (L >= nk) continue;
     double u = x1[h], vv = x2[h];
@@ -4016,8 +4046,12 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   };
   switch (which) {
     case 20:
-      hipLaunchKernelGGL(dt_wave_kernel, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
-                         (const double *)D(2));
+      if (G.nk <= 64)
+        hipLaunchKernelGGL(dt_wave_kernel<1>, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
+                           (const double *)D(2));
+      else
+        hipLaunchKernelGGL(dt_wave_kernel<2>, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
+                           (const double *)D(2));
       break;
     case 25:
     case 26: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), minor source tiles,
@@ -4139,8 +4173,11 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
           else
             hipLaunchKernelGGL(oplus_tile_kernel<false>, gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                                (const double *)D(19), (const double *)D(20), opout, optm1out);
+        else if (G.nk <= 64)
+          hipLaunchKernelGGL(oplus_wave_stage3<1>, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+                             OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
         else
-          hipLaunchKernelGGL(oplus_wave_stage3, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+          hipLaunchKernelGGL(oplus_wave_stage3<2>, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                              OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
       }
       break;
@@ -4150,7 +4187,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       break;
     case 22: {  // duv stage 2 (PCR): bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
       UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
-      hipLaunchKernelGGL(duv_wave_stage2, gridw, block, 0, s, g, G, in, out, UT);
+      if (G.nk <= 64)
+        hipLaunchKernelGGL(duv_wave_stage2<1>, gridw, block, 0, s, g, G, in, out, UT);
+      else
+        hipLaunchKernelGGL(duv_wave_stage2<2>, gridw, block, 0, s, g, G, in, out, UT);
       break;
     }
     case 8:
