@@ -3525,6 +3525,32 @@ __global__ void __launch_bounds__(256) duv_stage1(tgx_ng_grid g, Geo G, tgx_ng_f
   }
 }
 
+// The same with one wavefront per column, for columns of at most 64 levels
+// on small grids: lane k forms level k's increment, and the running sum is
+// carried through the lanes bottom to top in the order of the loop above, so
+// every partial sum is bitwise the same.
+__global__ void __launch_bounds__(256) duv_stage1_wave(tgx_ng_grid g, Geo G, tgx_ng_fields in,
+                                                       const double *__restrict__ tn_upd,
+                                                       double *__restrict__ ztmp) {
+  int i, j;
+  if (!wave_column_of(G, i, j)) return;
+  const int lane = threadIdx.x & 63, nk = G.nk;
+  const double dz = g.dz, wt = 0.225;
+  double dzt = 0.;
+  if (lane < nk - 1) {
+    const size_t p = G.a3(lane, i, j);
+    double tbar = (in.tn[p] * (-2.) + in.tn_nm[p] + tn_upd[p]) * wt + in.tn[p];
+    dzt = dz / DZGRAV * tbar / in.mbar[p];
+  }
+  double zt = in.z[G.a2(i, j)];  // level 0
+  double mine = zt;
+  for (int k = 0; k < nk - 1; ++k) {
+    zt = __shfl(dzt, k, 64) + zt;
+    mine = lane == k + 1 ? zt : mine;
+  }
+  if (lane < nk) ztmp[G.a3(lane, i, j)] = mine;
+}
+
 // stage 2: assembly of the 2x2 (UN,VN) system, block Thomas, outputs (:83-321)
 struct UvTerms {  // prelude planes
   const double *divx_ztmp, *divy_z, *unms, *vnms, *advu, *advv;
@@ -4183,7 +4209,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       break;
     }
     case 6:  // duv stage 1: bufs = {ztmp}
-      hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
+      if (G.nk <= 64 && ncol < 65536)
+        hipLaunchKernelGGL(duv_stage1_wave, gridw, block, 0, s, g, G, in, (const double *)out.tn, D(0));
+      else
+        hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
       break;
     case 22: {  // duv stage 2 (PCR): bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
       UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
