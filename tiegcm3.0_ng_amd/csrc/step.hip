@@ -1409,13 +1409,14 @@ __device__ __forceinline__ void chem_level(const tgx_ng_fields &in, const tgx_ng
 
 // The pointwise chemistry of the whole step (comp_o2o_ng fs, the N2D / N4S /
 // NO sources, O+ loss and production and xiop2p/xiop2d) in one pass.  A
-// thread takes a run of levels of one column, bottom to top, so the fields
-// read at two levels are loaded once each.
+// thread takes one of nrun runs of levels of one column, bottom to top, so the
+// fields read at two levels are loaded once each (runs of about CHEM_RUN
+// levels; shorter on small grids, which would not fill the chip).
 __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                   tgx_ng_out out, ChemOut C) {
+                                                   tgx_ng_out out, ChemOut C, int nrun) {
   int nb = gridDim.x;
   long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
-  const int nk = G.nk, nrun = chem_runs(nk);
+  const int nk = G.nk;
   long npt = (long)G.nlond * G.nlatd * nrun;
   if (t >= npt) return;
   int i = (int)(t % G.nlond);
@@ -4092,9 +4093,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       if (which == 25) {
         if ((unsigned long long)G.nk * G.P * sizeof(double) >= (1ull << 32))
           return tgx_set_error(1, "chem_kernel: a field of %d levels x %ld points exceeds 4 GiB", G.nk, (long)G.P);
-        long nt = (long)G.nlond * G.nlatd * chem_runs(G.nk);
+        const int nrun = ncol < 65536 ? (G.nk + 1) / 2 : chem_runs(G.nk);
+        long nt = (long)G.nlond * G.nlatd * nrun;
         dim3 gridc((unsigned)((nt + 255) / 256));
-        hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C);
+        hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C, nrun);
       } else
         hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C);
       break;
