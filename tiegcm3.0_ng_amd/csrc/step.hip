@@ -3012,13 +3012,12 @@ __global__ void __launch_bounds__(256) oplus_stage2(tgx_ng_grid g, Geo G, tgx_ng
 }
 // Runs of levels per column for the O+ stages: the whole column once a grid
 // has enough columns to fill the chip (one column loop was measured faster
-// there), else runs of about CHEM_RUN levels (stage 1) and single levels
-// (stage 2).  TGX_OPLUS_RUNS=1 in the environment forces the whole-column
+// there), else runs of two levels (stage 1) and single levels (stage 2).  TGX_OPLUS_RUNS=1 in the environment forces the whole-column
 // loop on every grid (both forms give bitwise the same values).
 __host__ inline int oplus_runs(long ncol, int nk, int stage) {
   const char *force = getenv("TGX_OPLUS_RUNS");
   if (ncol >= 65536 || (force && force[0] == '1' && force[1] == 0)) return 1;
-  return stage == 1 ? chem_runs(nk) : nk;
+  return stage == 1 ? (nk + 1) / 2 : nk;
 }
 
 // stage 3: assembly, solve and outputs of one sub-step (:172-340)
