@@ -2933,6 +2933,7 @@ struct OpTerms {  // prelude planes of one sub-step
   const double *bd_diff, *bd_diffj, *bd_bvel, *vdotn, *bd_djbz, *opts2;
 };
 
+template <bool RUNS>  // false: nrun == 1, the whole column (a fixed loop)
 __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     const double *__restrict__ op, OpWork W,
                                                     double *__restrict__ Fe, double *__restrict__ Fn,
@@ -2940,12 +2941,13 @@ __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng
   const int nb = gridDim.x;
   const long tid = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
   const int nk = G.nk;
+  if (!RUNS) nrun = 1;
   if (tid >= (long)G.nlond * G.nlatd * nrun) return;
   const int i = (int)(tid % G.nlond);
   const long r = tid / G.nlond;
   const int j = (int)(r % G.nlatd);
-  const int run = (int)(r / G.nlatd);
-  const int k0 = (int)((long)run * nk / nrun), k1 = (int)((long)(run + 1) * nk / nrun);
+  const int run = RUNS ? (int)(r / G.nlatd) : 0;
+  const int k0 = RUNS ? (int)((long)run * nk / nrun) : 0, k1 = RUNS ? (int)((long)(run + 1) * nk / nrun) : nk;
   const double dz = g.dz;
   const size_t c2 = G.a2(i, j);
   const double bx = in.bx[c2], by = in.by[c2], bz = in.bz[c2], bm = in.bmod2[c2];
@@ -2993,16 +2995,18 @@ __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng
 
 // stage 2 (every column, nrun runs of levels as stage 1): bdotdh_opj =
 // bdotdh(tp_op) * dj (:175-176)
+template <bool RUNS>  // false: nrun == 1, the whole column (a fixed loop)
 __global__ void __launch_bounds__(256) oplus_stage2(tgx_ng_grid g, Geo G, tgx_ng_fields in, OpWork W,
                                                     int nrun) {
   const int nb = gridDim.x;
   const long tid = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
+  if (!RUNS) nrun = 1;
   if (tid >= (long)G.nlond * G.nlatd * nrun) return;
   const int i = (int)(tid % G.nlond);
   const long r = tid / G.nlond;
   const int j = (int)(r % G.nlatd);
-  const int run = (int)(r / G.nlatd);
-  const int k0 = (int)((long)run * G.nk / nrun), k1 = (int)((long)(run + 1) * G.nk / nrun);
+  const int run = RUNS ? (int)(r / G.nlatd) : 0;
+  const int k0 = RUNS ? (int)((long)run * G.nk / nrun) : 0, k1 = RUNS ? (int)((long)(run + 1) * G.nk / nrun) : G.nk;
   const size_t c2 = G.a2(i, j);
   const double bx = in.bx[c2], by = in.by[c2], cs = g.cs[j];
   for (int k = k0; k < k1; ++k) {
@@ -4174,11 +4178,17 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       } else if (which == 3) {
         const int nrun = oplus_runs(ncol, G.nk, 1);
         dim3 grid1((unsigned)((ncol * nrun + 255) / 256));
-        hipLaunchKernelGGL(oplus_stage1, grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, nrun);
+        if (nrun > 1)
+          hipLaunchKernelGGL(oplus_stage1<true>, grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, nrun);
+        else
+          hipLaunchKernelGGL(oplus_stage1<false>, grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, 1);
       } else if (which == 4) {
         const int nrun = oplus_runs(ncol, G.nk, 2);
         dim3 grid2((unsigned)((ncol * nrun + 255) / 256));
-        hipLaunchKernelGGL(oplus_stage2, grid2, block, 0, s, g, G, in, W, nrun);
+        if (nrun > 1)
+          hipLaunchKernelGGL(oplus_stage2<true>, grid2, block, 0, s, g, G, in, W, nrun);
+        else
+          hipLaunchKernelGGL(oplus_stage2<false>, grid2, block, 0, s, g, G, in, W, 1);
       } else if (which == 9) {
         StencilJobs J{};
         J.n = 6;
