@@ -244,23 +244,25 @@ def test_step_from_two_host_threads(pkg):
             assert torch.equal(a.out[n], b.out[n]), n
 
 
-def test_oplus_stage_runs_equal_column_loop(pkg, oracle, monkeypatch):
-    """O+ stages 1 and 2 walk short runs of levels on small grids and the
-    whole column on large ones; forcing the column loop (TGX_OPLUS_RUNS=1) on
-    a small grid gives bitwise the same step, within the FP64 bound of the
-    oracle."""
+@pytest.mark.parametrize("scheme", ["thomas", "pcr"])
+def test_small_grid_forms_equal_large_grid_forms(pkg, oracle, monkeypatch, scheme):
+    """On grids under 65,536 columns the chemistry pass and O+ stage 1 walk
+    runs of two levels, O+ stage 2 single levels and duv's stage 1 one
+    wavefront per column; the large-grid forms (TGX_LARGE_GRID_FORMS=1: runs
+    of about eight levels, whole columns, one thread per column) give bitwise
+    the same step, within the FP64 bound of the oracle."""
     import torch
     g = pkg.synth.make_grid("small", nstep_sub=2)
     F = pkg.synth.make_state(g)
     ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
-    a = pkg.step.NestState(g, F, scheme="thomas")
+    a = pkg.step.NestState(g, F, scheme=scheme)
     a.step()
     torch.cuda.synchronize()
-    monkeypatch.setenv("TGX_OPLUS_RUNS", "1")
-    b = pkg.step.NestState(g, F, scheme="thomas")
+    monkeypatch.setenv("TGX_LARGE_GRID_FORMS", "1")
+    b = pkg.step.NestState(g, F, scheme=scheme)
     b.step()
     torch.cuda.synchronize()
     for n in a.out:
         assert torch.equal(a.out[n], b.out[n]), n
-    for n in ("op", "op_nm", "Fe", "Fn", "un", "vn"):
+    for n in ALL_FIELDS:
         assert field_err(b.fortran(n), ref[n]) <= TOL, n

@@ -3014,13 +3014,20 @@ __global__ void __launch_bounds__(256) oplus_stage2(tgx_ng_grid g, Geo G, tgx_ng
     W.bdopj[p] = bdotdh(W.tpop + off, G, i, j, bx, by, cs, g.dlamda, g.dphi) * W.dj[p];
   }
 }
-// Runs of levels per column for the O+ stages: the whole column once a grid
-// has enough columns to fill the chip (one column loop was measured faster
-// there), else runs of two levels (stage 1) and single levels (stage 2).  TGX_OPLUS_RUNS=1 in the environment forces the whole-column
-// loop on every grid (both forms give bitwise the same values).
+// Grids of fewer than 65,536 columns do not fill the chip with one thread
+// per column: there the chemistry pass and O+ stage 1 take runs of two
+// levels, O+ stage 2 single levels, and duv's stage 1 one wavefront per
+// column (nk <= 64).  Every form gives bitwise the same values;
+// TGX_LARGE_GRID_FORMS=1 in the environment selects the large-grid forms on
+// every grid (used by the tests to check the two against each other).
+__host__ inline bool small_grid(long ncol) {
+  const char *force = getenv("TGX_LARGE_GRID_FORMS");
+  return ncol < 65536 && !(force && force[0] == '1' && force[1] == 0);
+}
+// Runs of levels per column for the O+ stages: the whole column on large
+// grids (one column loop was measured faster there), else as above.
 __host__ inline int oplus_runs(long ncol, int nk, int stage) {
-  const char *force = getenv("TGX_OPLUS_RUNS");
-  if (ncol >= 65536 || (force && force[0] == '1' && force[1] == 0)) return 1;
+  if (!small_grid(ncol)) return 1;
   return stage == 1 ? (nk + 1) / 2 : nk;
 }
 
@@ -4096,7 +4103,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       if (which == 25) {
         if ((unsigned long long)G.nk * G.P * sizeof(double) >= (1ull << 32))
           return tgx_set_error(1, "chem_kernel: a field of %d levels x %ld points exceeds 4 GiB", G.nk, (long)G.P);
-        const int nrun = ncol < 65536 ? (G.nk + 1) / 2 : chem_runs(G.nk);
+        const int nrun = small_grid(ncol) ? (G.nk + 1) / 2 : chem_runs(G.nk);
         long nt = (long)G.nlond * G.nlatd * nrun;
         dim3 gridc((unsigned)((nt + 255) / 256));
         hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C, nrun);
@@ -4220,7 +4227,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       break;
     }
     case 6:  // duv stage 1: bufs = {ztmp}
-      if (G.nk <= 64 && ncol < 65536)
+      if (G.nk <= 64 && small_grid(ncol))
         hipLaunchKernelGGL(duv_stage1_wave, gridw, block, 0, s, g, G, in, (const double *)out.tn, D(0));
       else
         hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
