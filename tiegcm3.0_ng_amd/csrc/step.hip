@@ -1444,25 +1444,6 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
   }
 }
 
-// The NO and N4S sources of this step from the linear coefficients of
-// chem_kernel and this step's N2D (after the N2D solve).
-__global__ void __launch_bounds__(256) minor_lin_kernel(Geo G, tgx_ng_out out, ChemOut C) {
-  int nb = gridDim.x;
-  long t = (long)tgx_xcd_block(nb) * blockDim.x + threadIdx.x;
-  long npt = (long)G.nlond * G.nlatd * G.nk;
-  if (t >= npt) return;
-  int i = (int)(t % G.nlond);
-  long r = t / G.nlond;
-  int j = (int)(r % G.nlatd);
-  int k = (int)(r / G.nlatd);
-  const size_t p = G.a3(k, i, j);
-  const double n2d = out.n2d[p];
-  const size_t ts = G.tile(k, i, j, 0, MS_NV), tl = G.tile(k, i, j, 0, ML_NV);
-  C.st[ts + ms_prod(1) * 64] = C.lt[tl] + n2d * C.lt[tl + 64];
-  C.st[ts + ms_loss(0) * 64] = C.lt[tl + 128] + n2d * C.lt[tl + 192];
-  C.st[ts + ms_prod(0) * 64] = C.lt[tl + 256] + n2d * C.lt[tl + 320];
-}
-
 // =============================================================================
 // One wavefront per column.  Lane l holds level l ("lo") and level l+64
 // ("hi"), so a wave covers up to 128 levels; neighbouring levels come from
@@ -1831,7 +1812,9 @@ __device__ __forceinline__ bool wave_column_of(const Geo &G, int &i, int &j) {
 
 // minor_ng.F90:60-237 for species SA and SB, one wavefront per column, the
 // solve by parallel cyclic reduction over the nk rows.
-template <int SA, int SB, int NH>
+// LIN: the NO and N4S sources formed here from the chemistry pass's linear
+// coefficients and this step's N2D (a + N2D * b, as the Thomas NO/N4S kernel does).
+template <int SA, int SB, int NH, bool LIN = false>
 __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                          tgx_ng_out out, MinorTerms MT) {
   int i, j;
@@ -2005,6 +1988,16 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
       rhs[h][q] = MT.rt[G.tile(Lc, i, j, SP[q], MR_NV)];
       const size_t ts = G.tile(Lc, i, j, 0, MS_NV);
       if (SP[q] == 3) { slo[h][q] = 0.; s0p[h][q] = 0.; }
+      else if (LIN && SP[q] == 0) {
+        const size_t tl = G.tile(Lc, i, j, 0, ML_NV);
+        const double n2d = MT.n2d[p];
+        slo[h][q] = MT.lt[tl + 128] + n2d * MT.lt[tl + 192];
+        s0p[h][q] = MT.lt[tl + 256] + n2d * MT.lt[tl + 320];
+      } else if (LIN && SP[q] == 1) {
+        const size_t tl = G.tile(Lc, i, j, 0, ML_NV);
+        slo[h][q] = MT.st[ts + ms_loss(1) * 64];
+        s0p[h][q] = MT.lt[tl] + MT.n2d[p] * MT.lt[tl + 64];
+      }
       else { slo[h][q] = MT.st[ts + ms_loss(SP[q]) * 64]; s0p[h][q] = MT.st[ts + ms_prod(SP[q]) * 64]; }
     }
   }
@@ -4090,8 +4083,7 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL(dt_wave_kernel<2>, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
                            (const double *)D(2));
       break;
-    case 25:
-    case 26: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), minor source tiles,
+    case 25: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), minor source tiles,
                 //         (unused x2), linear-coefficient tiles, (unused x5), oploss, opprod}
       ChemOut C;
       C.ci = D(0);
@@ -4100,15 +4092,12 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       C.st = D(6);
       C.lt = D(9);
       C.oploss = D(15); C.opprod = D(16);
-      if (which == 25) {
-        if ((unsigned long long)G.nk * G.P * sizeof(double) >= (1ull << 32))
-          return tgx_set_error(1, "chem_kernel: a field of %d levels x %ld points exceeds 4 GiB", G.nk, (long)G.P);
-        const int nrun = small_grid(ncol) ? (G.nk + 1) / 2 : chem_runs(G.nk);
-        long nt = (long)G.nlond * G.nlatd * nrun;
-        dim3 gridc((unsigned)((nt + 255) / 256));
-        hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C, nrun);
-      } else
-        hipLaunchKernelGGL(minor_lin_kernel, gridp, block, 0, s, G, out, C);
+      if ((unsigned long long)G.nk * G.P * sizeof(double) >= (1ull << 32))
+        return tgx_set_error(1, "chem_kernel: a field of %d levels x %ld points exceeds 4 GiB", G.nk, (long)G.P);
+      const int nrun = small_grid(ncol) ? (G.nk + 1) / 2 : chem_runs(G.nk);
+      long nt = (long)G.nlond * G.nlatd * nrun;
+      dim3 gridc((unsigned)((nt + 255) / 256));
+      hipLaunchKernelGGL(chem_kernel, gridc, block, 0, s, g, G, in, out, C, nrun);
       break;
     }
     case 28: {  // duv with stage 1 and the stencils fused: bufs = {ztmp (unused), Thomas tiles (UW_NV)}
@@ -4133,7 +4122,8 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 16:
     case 18:
     case 19:
-    case 27: {  // minors: bufs = {ratio tiles (MW_NV), right-hand-side tiles (MR_NV), (unused x8),
+    case 27:
+    case 30: {  // minors: bufs = {ratio tiles (MW_NV), right-hand-side tiles (MR_NV), (unused x8),
                 // source tiles (MS_NV), (unused x3)}; N2D's source values are reused by NO after
                 // N2D's solve
       MinorTerms MT{};
@@ -4158,7 +4148,14 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         hipLaunchKernelGGL((minor_wave_kernel<2, 3, 1>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 18)
         hipLaunchKernelGGL((minor_wave_kernel<2, 3, 2>), gridw, block, 0, s, g, G, in, out, MT);
-      else if (which == 19 && G.nk <= 64)
+      else if (which == 30) {  // buf 14: chem_kernel's linear-coefficient tiles
+        MT.lt = D(14);
+        MT.n2d = out.n2d;
+        if (G.nk <= 64)
+          hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1, true>), gridw, block, 0, s, g, G, in, out, MT);
+        else
+          hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2, true>), gridw, block, 0, s, g, G, in, out, MT);
+      } else if (which == 19 && G.nk <= 64)
         hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1>), gridw, block, 0, s, g, G, in, out, MT);
       else if (which == 19)
         hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2>), gridw, block, 0, s, g, G, in, out, MT);

@@ -145,11 +145,13 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 }
 
 // Kernel ids of tgx_ng_launch (0, 5 and 7 are retired: the Thomas dt, O+ and
-// duv column kernels are the tile kernels K_DT_TILE, K_OP_TILE, K_UV_TILE).
+// duv column kernels are the tile kernels K_DT_TILE, K_OP_TILE, K_UV_TILE; 26
+// too: the NO / N4S column kernels form their sources, K_MIN_B_LIN and
+// K_MIN_WB_LIN).
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
-       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE };
+       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE, K_MIN_WB_LIN };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -194,7 +196,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
 }
 
 namespace {
-// Buffers of chem_kernel / minor_lin_kernel (the step-wide chemistry pass).
+// Buffers of chem_kernel (the step-wide chemistry pass).
 void chem_bufs(void *work, const tgx_ng_grid *g, void **b) {
   b[0] = tiles(work, g, T_CI);                                       // comp fs(:,1:2,1:2)
   b[1] = plane(work, g, W_CO + 0);                                   // comp fs(:,1,0)
@@ -256,21 +258,15 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
     ProfScope kern(R_N + R_MINORS, s);  // the N2D + Ar column kernel
     if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
   }
-  if (chem_done && !pcr) {  // NO and N4S sources formed in the column kernel from the
-                            // step's chemistry pass and this N2D
+  if (chem_done) {  // NO and N4S sources formed in the column kernel from the
+                    // step's chemistry pass and this N2D
     void *b2[20];
     for (int q = 0; q < 14; ++q) b2[q] = b[q];
     b2[14] = tiles(work, g, T_ML);
     for (int q = 15; q < 20; ++q) b2[q] = nullptr;
-    return tgx_ng_launch(K_MIN_B_LIN, g, in, out, b2, 0, s);
+    return tgx_ng_launch(pcr ? K_MIN_WB_LIN : K_MIN_B_LIN, g, in, out, b2, 0, s);
   }
-  if (chem_done) {  // NO and N4S sources from the step's chemistry pass and this N2D
-    void *c[20];
-    chem_bufs(work, g, c);
-    if (int rc = tgx_ng_launch(K_MIN_LIN, g, in, out, c, 0, s)) return rc;
-  } else {
-    if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
-  }
+  if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
   return tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s);
 }
 
