@@ -324,8 +324,9 @@ enum { STC_NONE = 0, STC_MINOR, STC_COMP };
 // fs(:,1,1), fs(:,1,2), fs(:,2,1), fs(:,2,2) of comp_o2o_ng (3-6).
 constexpr int CI_NV = 7;
 // Tiled minor-species right-hand-side parts, one value per species id
-// (NO, N4S, N2D, AR), and the Thomas ratios of the two species of a pass.
-constexpr int MR_NV = 4, MW_NV = 2;
+// (NO, N4S, N2D, AR), and the Thomas ratios (0, 1) and forward solutions
+// (2, 3) of the two species of a pass.
+constexpr int MR_NV = 4, MW_NV = 4;
 // Tiled minor-species sources: loss and scaled production of NO or N2D
 // (values 0, 1; N2D's until its solve, then NO's) and of N4S (2, 3), and the
 // six coefficients of the NO / N4S sources linear in N2D.
@@ -1204,7 +1205,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
         w1[q] = qc - pc * w2;
         w3[q] = (fr - pc * w3[q]) / w1[q];
       }
-      outs[q][G.a3(k, i, j)] = w3[q];
+      w2t[G.tile(k, i, j, 2 + q, MW_NV)] = w3[q];  // forward solution
       rprev[q] = rc;
     }
     Sc = Sn;
@@ -1221,7 +1222,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
     struct BL { double x, w2, f, fm; };
     auto load_bl = [&](int k, BL &B) {
       size_t p = G.a3(k, i, j);
-      B.x = outs[q][p];
+      B.x = w2t[G.tile(k, i, j, 2 + q, MW_NV)];
       B.w2 = k < nk - 1 ? w2t[G.tile(k, i, j, q, MW_NV)] : 0.;
       B.f = fc[q][p];
       B.fm = fctm1[q][p];

@@ -32,8 +32,8 @@ enum {
   T_CI = 0,        // comp inputs: 3 right-hand-side parts, 4 fs values
   T_CE = T_CI + 7, // comp elimination: 9 gamma, 3 forward solution
   T_MR = T_CE + 12, // minor species: right-hand-side parts of NO, N4S, N2D, AR
-  T_MW = T_MR + 4,  // minor species: Thomas ratios of the two species of a pass
-  T_UW = T_MW + 2,  // winds: 2x2 block gamma and forward solution
+  T_MW = T_MR + 4,  // minor species: Thomas ratios and forward solutions of a pass's two species
+  T_UW = T_MW + 4,  // winds: 2x2 block gamma and forward solution
   T_DW = T_UW + 6,  // TN: Thomas ratio and forward solution
   T_OW = T_DW + 2,  // O+: Thomas ratio and forward solution
   T_MS = T_OW + 2,  // minor species: loss and scaled production of NO or N2D, and of N4S
