@@ -235,8 +235,9 @@ def main():
                     help="also run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (the column step needs no "
-              "communication; see DESIGN.md for how it shards). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: measures one GPU only (the column step of one domain needs no "
+              "communication between GPUs; a run over several would give each its own latitude "
+              "bands or nests). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
