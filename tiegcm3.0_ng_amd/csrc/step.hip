@@ -1942,7 +1942,9 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
     }
     Q.barm = R[h].barm;
     double df[3], al[9];
-    for (int n = 0; n < 3; ++n) df[n] = pow(Q.tni / T00, 1.75 - ssn(n));
+    const TPow tp = tpows(Q.tni);  // the temperature powers from one logarithm
+    for (int n = 0; n < 3; ++n) df[n] = tp.df[n];
+    Q.q = tp.q;
     al[0] = -(PHI(1, 4) + (PHI(1, 2) - PHI(1, 4)) * Q.pso1 + (df[0] * PHI(1, 3) - PHI(1, 4)) * Q.pshe);
     al[4] = -(PHI(2, 4) + (PHI(2, 1) - PHI(2, 4)) * Q.pso2 + (df[1] * PHI(2, 3) - PHI(2, 4)) * Q.pshe);
     al[8] = -(df[2] * PHI(3, 4) + (df[0] * PHI(3, 1) - df[2] * PHI(3, 4)) * Q.pso2 +
@@ -1983,7 +1985,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
                   sai[0] * (Q.do2dz - (1. - (RMO2 + Q.dmdz) / Q.barm) * Q.pso2) +
                   sai[1] * (Q.do1dz - (1. - (RMO1 + Q.dmdz) / Q.barm) * Q.pso1) +
                   sai[2] * (Q.dhedz - (1. - (RMHE + Q.dmdz) / Q.barm) * Q.pshe);
-      ax[h][q] = -(Q.barm / (TAU * RMN2) * pow(T00 / Q.tni, 0.25) /
+      ax[h][q] = -(Q.barm / (TAU * RMN2) * Q.q /
                    (cq.phix[2] + cq.salfax[0] * Q.pso2 + cq.salfax[1] * Q.pso1 + cq.salfax[2] * Q.pshe));
       thd[h][q] = ex - cq.alfax * dtnidz / (dzp * Q.tni);
       rhs[h][q] = MT.rt[G.tile(Lc, i, j, SP[q], MR_NV)];
@@ -2721,7 +2723,8 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       }
       dembardz = dembardz / dz;
       dtndz = dtndz / dz;
-      for (int n = 0; n < 3; ++n) df[n] = pow(tni / T00, 1.75 - ssn(n));
+      const TPow tp = tpows(tni);  // the temperature powers from one logarithm
+      for (int n = 0; n < 3; ++n) df[n] = tp.df[n];
       for (int l = 0; l < 3; ++l) ep[h][l] = 1. - 1. / embari * (rmass[l] + dembardz);
       if (L <= nk - 2) ep[h][2] = ep[h][2] - thdiffalpha * dtndz / tni;
       double al[9];
@@ -2737,7 +2740,7 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       al[2] = (d_[0] * PHI(3, 1) - d_[2] * PHI(3, 4)) * hei;
       al[5] = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * hei;
       matinv3_ng(al, ak[h]);
-      double wks1 = embari * RIN2 * pow(T00 / tni, 0.25) / TAU;
+      double wks1 = embari * RIN2 * tp.q / TAU;
       for (int n = 0; n < 9; ++n) ak[h][n] = ak[h][n] * wks1;
       wks[h] = dembardz / (embari * 2.);
       // upper boundary terms of the top interface (:601-652)
