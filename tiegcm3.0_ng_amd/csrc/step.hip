@@ -1822,7 +1822,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
   if (!wave_column_of(G, i, j)) return;
   const int lane = threadIdx.x & 63, nk = G.nk;
   const int Lr[2] = {lane, lane + 64};
-  const double dzp = g.dz, dtx2inv = g.dtx2inv, small = 1.e-12;
+  const double dzp = g.dz, dtx2inv = g.dtx2inv, small = 1.e-12, rdzp = 1. / g.dz;
   const size_t c2 = G.a2(i, j);
   const int SP[2] = {SA, SB};
   double *outs[2], *outs_nm[2];
@@ -1922,22 +1922,22 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
     const int L = Lr[h];
     MinorShared &Q = S[h];
     if (L == 0) {
-      Q.dmdz = (R[h].mbar - xmbari) / dzp;
+      Q.dmdz = (R[h].mbar - xmbari) * rdzp;
       Q.pso1 = .5 * (bo1 + o1_0);
       Q.pso2 = .5 * (bo2 + o2_0);
       Q.pshe = .5 * (bhe + he_0);
-      Q.do1dz = (o1_0 - bo1) / dzp;
-      Q.do2dz = (o2_0 - bo2) / dzp;
-      Q.dhedz = (he_0 - bhe) / dzp;
+      Q.do1dz = (o1_0 - bo1) * rdzp;
+      Q.do2dz = (o2_0 - bo2) * rdzp;
+      Q.dhedz = (he_0 - bhe) * rdzp;
       Q.tni = in.tlbc[c2];
     } else {
-      Q.dmdz = (R[h].mbar - A[h].mbar) / dzp;
+      Q.dmdz = (R[h].mbar - A[h].mbar) * rdzp;
       Q.pso1 = .5 * (R[h].o1 + A[h].o1);
       Q.pso2 = .5 * (R[h].o2 + A[h].o2);
       Q.pshe = .5 * (R[h].he + A[h].he);
-      Q.do1dz = (R[h].o1 - A[h].o1) / dzp;
-      Q.do2dz = (R[h].o2 - A[h].o2) / dzp;
-      Q.dhedz = (R[h].he - A[h].he) / dzp;
+      Q.do1dz = (R[h].o1 - A[h].o1) * rdzp;
+      Q.do2dz = (R[h].o2 - A[h].o2) * rdzp;
+      Q.dhedz = (R[h].he - A[h].he) * rdzp;
       Q.tni = L == nk - 1 ? A[h].tn : .5 * (R[h].tn + A[h].tn);  // :89-93
     }
     Q.barm = R[h].barm;
@@ -1955,7 +1955,7 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
     al[7] = (df[1] * PHI(2, 3) - PHI(2, 4)) * Q.pso1;
     al[2] = (df[0] * PHI(3, 1) - df[2] * PHI(3, 4)) * Q.pshe;
     al[5] = (df[1] * PHI(3, 2) - df[2] * PHI(3, 4)) * Q.pshe;
-    matinv3_ng(al, Q.inv);
+    matinv3_r(al, Q.inv);
   }
   double tnim[2], tnip[2];
   wave_dn_h<NH>(S[0].tni, S[1].tni, lane, tnim[0], tnim[1]);
@@ -2032,16 +2032,16 @@ __global__ void __launch_bounds__(256) minor_wave_kernel(tgx_ng_grid g, Geo G, t
     for (int q = 0; q < 2; ++q) {
       double pc, qc, rc, fr;
       if (L <= nk - 2) {
-        double molp = ax[h][q] * (1. / dzp + .5 * thd[h][q]);
-        double molr = axn[h][q] * (1. / dzp - .5 * thdn[h][q]);
-        double molq = ax[h][q] * (1. / dzp - .5 * thd[h][q]) + axn[h][q] * (1. / dzp + .5 * thdn[h][q]);
-        double eddyp = g.expzmid_inv * difk0 * (1. / dzp - .5 * dmdzc);
-        double eddyr = g.expzmid * difk1 * (1. / dzp + .5 * dmn[h]);
-        double eddyq = g.expzmid_inv * difk0 * (1. / dzp + .5 * dmdzc) +
-                       g.expzmid * difk1 * (1. / dzp - .5 * dmn[h]);
-        pc = molp / dzp - expzk * (eddyp * dfac + .5 * wic) / dzp;
-        rc = molr / dzp - expzk * (eddyr * dfac - .5 * wic) / dzp;
-        qc = -molq / dzp + expzk * (eddyq * dfac / dzp + dtx2inv - slo[h][q]);
+        double molp = ax[h][q] * (rdzp + .5 * thd[h][q]);
+        double molr = axn[h][q] * (rdzp - .5 * thdn[h][q]);
+        double molq = ax[h][q] * (rdzp - .5 * thd[h][q]) + axn[h][q] * (rdzp + .5 * thdn[h][q]);
+        double eddyp = g.expzmid_inv * difk0 * (rdzp - .5 * dmdzc);
+        double eddyr = g.expzmid * difk1 * (rdzp + .5 * dmn[h]);
+        double eddyq = g.expzmid_inv * difk0 * (rdzp + .5 * dmdzc) +
+                       g.expzmid * difk1 * (rdzp - .5 * dmn[h]);
+        pc = molp * rdzp - expzk * (eddyp * dfac + .5 * wic) * rdzp;
+        rc = molr * rdzp - expzk * (eddyr * dfac - .5 * wic) * rdzp;
+        qc = -molq * rdzp + expzk * (eddyq * dfac * rdzp + dtx2inv - slo[h][q]);
         fr = rhs[h][q] + s0p[h][q];
         fr = fr * expzk;
         if (L == 0) {
@@ -2659,7 +2659,7 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
   if (!wave_column_of(G, i, j)) return;
   const int lane = threadIdx.x & 63, nk = G.nk;
   const int Lr[2] = {lane, lane + 64};
-  const double dz = g.dz, dtx2inv = g.dtx2inv, thdiffalpha = -0.38, small = 1.e-9;
+  const double dz = g.dz, dtx2inv = g.dtx2inv, thdiffalpha = -0.38, small = 1.e-9, rdz = 1. / g.dz;
   const size_t c2 = G.a2(i, j);
   const double fhe = g.calc_helium == 0 ? 0. : in.flx_he[c2];
   if (lane == 0) out.flx_he[c2] = fhe;
@@ -2721,8 +2721,8 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
         dembardz = R[h].mbar - A[h].mbar;
         dtndz = R[h].tn - A[h].tn;
       }
-      dembardz = dembardz / dz;
-      dtndz = dtndz / dz;
+      dembardz = dembardz * rdz;
+      dtndz = dtndz * rdz;
       const TPow tp = tpows(tni);  // the temperature powers from one logarithm
       for (int n = 0; n < 3; ++n) df[n] = tp.df[n];
       for (int l = 0; l < 3; ++l) ep[h][l] = 1. - 1. / embari * (rmass[l] + dembardz);
@@ -2739,7 +2739,7 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       al[7] = (d_[1] * PHI(2, 3) - PHI(2, 4)) * o1i;
       al[2] = (d_[0] * PHI(3, 1) - d_[2] * PHI(3, 4)) * hei;
       al[5] = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * hei;
-      matinv3_ng(al, ak[h]);
+      matinv3_r(al, ak[h]);
       double wks1 = embari * RIN2 * tp.q / TAU;
       for (int n = 0; n < 9; ++n) ak[h][n] = ak[h][n] * wks1;
       wks[h] = dembardz / (embari * 2.);
@@ -2780,21 +2780,21 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       const double *akm = ak[h], *epm = ep[h], *akpp = akp[h], *eppp = epp[h];
       const double wks3 = wks[h], wks4 = wksp[h];
       double pk[9], rk[9], qk[9], fk[3];
-      double eddyp = g.expzmid_inv * g.difk[k] * (1. / dz - wks3);
-      double eddyr = g.expzmid * g.difk[k + 1] * (1. / dz + wks4);
-      double eddyq = g.expzmid_inv * g.difk[k] * (1. / dz + wks3) + g.expzmid * g.difk[k + 1] * (1. / dz - wks4);
+      double eddyp = g.expzmid_inv * g.difk[k] * (rdz - wks3);
+      double eddyr = g.expzmid * g.difk[k + 1] * (rdz + wks4);
+      double eddyq = g.expzmid_inv * g.difk[k] * (rdz + wks3) + g.expzmid * g.difk[k + 1] * (rdz - wks4);
       const double wik = .5 * (R[h].w + wp[h]);
       for (int m = 0; m < 3; ++m)
         for (int isp = 0; isp < 3; ++isp) {
           int e = isp + 3 * m;
-          double molp = akm[e] * (1. / dz + epm[m] / 2.);
-          double molr = akpp[e] * (1. / dz - eppp[m] / 2.);
-          double molq = akm[e] * (1. / dz - epm[m] / 2.) + akpp[e] * (1. / dz + eppp[m] / 2.);
+          double molp = akm[e] * (rdz + epm[m] / 2.);
+          double molr = akpp[e] * (rdz - eppp[m] / 2.);
+          double molq = akm[e] * (rdz - epm[m] / 2.) + akpp[e] * (rdz + eppp[m] / 2.);
           double delta = isp == m ? 1. : 0.;
           double fsv = (isp < 2 && m < 2) ? fs[isp][m + 1] : 0.;
-          pk[e] = molp / dz - expzk * delta / dz * (eddyp * dfac + .5 * wik);
-          rk[e] = molr / dz - expzk * delta / dz * (eddyr * dfac - .5 * wik);
-          qk[e] = -molq / dz + expzk * delta * (eddyq * dfac / dz + dtx2inv) - expzk * fsv;
+          pk[e] = molp * rdz - expzk * delta * rdz * (eddyp * dfac + .5 * wik);
+          rk[e] = molr * rdz - expzk * delta * rdz * (eddyr * dfac - .5 * wik);
+          qk[e] = -molq * rdz + expzk * delta * (eddyq * dfac * rdz + dtx2inv) - expzk * fsv;
         }
       fk[0] = expzk * (CT.ci[t] + in.hdo2[p]);
       fk[1] = expzk * (CT.ci[t + 64] + in.hdo1[p]);
@@ -2814,8 +2814,8 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
         const double flx00 = flx[h];
         const double alpha22 = a22p[h], alpha33 = a33p[h], alpha23 = a23p[h], alpha32 = a32p[h];
         for (int isp = 0; isp < 3; ++isp)
-          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * eppp[1])) -
-                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * eppp[2]));
+          fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * eppp[1])) -
+                    rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * eppp[2]));
         for (int n = 0; n < 9; ++n) rk[n] = 0.;
       }
       // to row-major blocks, normalised by the diagonal block
@@ -2849,8 +2849,8 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
         double top[3];
         for (int isp = 0; isp < 3; ++isp) top[isp] = (1. + .5 * e_[isp] * dz) / (1. - .5 * e_[isp] * dz) * xm[h][isp];
         const double flx00 = flxm[h];
-        top[1] = top[1] + (a23[h] - a22[h]) * fhe / (flx00 * (1. / dz - 0.5 * e_[1]));
-        top[2] = top[2] + (a33[h] - a32[h]) * fhe / (flx00 * (1. / dz - 0.5 * e_[2]));
+        top[1] = top[1] + (a23[h] - a22[h]) * fhe / (flx00 * (rdz - 0.5 * e_[1]));
+        top[2] = top[2] + (a33[h] - a32[h]) * fhe / (flx00 * (rdz - 0.5 * e_[2]));
         for (int n = 0; n < 3; ++n) x[h][n] = top[n];
       }
       if (g.calc_helium == 0) x[h][2] = g.pshelb;
@@ -2884,7 +2884,7 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
       zmtop = g.zpmid[kf];
       n2top = log(n2a * xna);
       double n2top1 = log(n2b * xnb);
-      dn2dz = (n2top - n2top1) / dz;
+      dn2dz = (n2top - n2top1) * rdz;
     }
   }
 #pragma unroll
