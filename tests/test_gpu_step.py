@@ -27,9 +27,10 @@ def field_err(got, ref):
 @pytest.fixture(scope="module", params=[("tiny", 2, "thomas"), ("small", 1, "thomas"), ("small", 3, "thomas"),
                                         ("tall", 1, "thomas"), ("tiny", 2, "pcr"), ("small", 1, "pcr"),
                                         ("tall", 1, "pcr"),
-                                        # the 5 and 2.5 degree global grids and a nest at full size
+                                        # the 5, 2.5 and 1.25 degree global grids and both nests at
+                                        # full size
                                         ("g5", 1, "thomas"), ("g2p5", 1, "thomas"), ("g2p5", 1, "pcr"),
-                                        ("nest1", 2, "thomas")],
+                                        ("g1p25", 1, "thomas"), ("nest1", 2, "thomas"), ("nest2", 1, "pcr")],
                 ids=lambda p: "%s-nsub%d-%s" % p)
 def case(request, pkg, oracle):
     import torch
