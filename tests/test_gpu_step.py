@@ -267,3 +267,37 @@ def test_small_grid_forms_equal_large_grid_forms(pkg, oracle, monkeypatch, schem
         assert torch.equal(a.out[n], b.out[n]), n
     for n in ALL_FIELDS:
         assert field_err(b.fortran(n), ref[n]) <= TOL, n
+
+
+def test_full_size_step_properties(pkg):
+    """At the benchmark's size (0.625 deg: 576x288 columns x 113 levels), where
+    the large-grid forms run: the forked step equals the serial step bit for
+    bit, and the solution keeps the reference's floors and normalisation
+    (comp_ng.F90:415-457, minor_ng.F90:234-237, dt_ng.F90:208-213,
+    oplus_ng.F90:337-340): O2, O, He positive with O2 + O + He < 1, the minor
+    species >= 1e-12, TN >= 100, O+ >= 0, every value finite."""
+    import torch
+    g = pkg.synth.make_grid("g0p625", nstep_sub=1)
+    F = pkg.synth.make_state(g, device="cuda")
+    a = pkg.step.NestState(g, F, scheme="thomas")
+    a.step_serial()
+    torch.cuda.synchronize()
+    ref = {n: t.clone() for n, t in a.out.items()}
+    del a
+    b = pkg.step.NestState(g, F, scheme="thomas")
+    b.step()
+    torch.cuda.synchronize()
+    for n in ref:
+        assert torch.equal(ref[n], b.out[n]), n
+    nl = g["nlon"] + 4
+    o = {n: t[..., :nl] for n, t in b.out.items()}
+    for n, t in o.items():
+        assert torch.isfinite(t).all(), n
+    for sfx in ("", "_nm"):
+        s3 = o["o2" + sfx] + o["o1" + sfx] + o["he" + sfx]
+        assert (o["o2" + sfx] > 0).all() and (o["o1" + sfx] > 0).all() and (o["he" + sfx] > 0).all()
+        assert (s3 < 1).all(), sfx
+        for n in ("no", "n4s", "n2d", "ar"):
+            assert (o[n + sfx] >= 1e-12).all(), n + sfx
+        assert (o["op" + sfx] >= 0).all()
+    assert (o["tn"] >= 100).all()
