@@ -16,8 +16,8 @@ Prints one JSON line:
             duration (in a few serial steps after the timed region, where
             it runs alone), against the 8 TB/s HBM peak; traffic from the
             committed rocprofv3 --pmc summary of that kernel when present,
-  cpu_baseline: the C oracle (a restatement of the reference) on one host
-            core over a bounded sub-grid of the same workload.
+  cpu_baseline: the C oracle (a restatement of the reference) on the host
+            cores, the workload split into latitude bands, one process each.
 """
 import argparse
 import importlib
@@ -49,28 +49,89 @@ def load_pkg():
 ROUTINES = ("oplus", "minors", "dt", "duv", "comp")  # the library's routine order
 
 
-def cpu_baseline(pkg, config, nstep_sub, budget_s=20.0):
-    """The oracle's column step, one host core, on a sub-grid of the workload
-    (same levels, timestep and physics; fewer columns)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cc_version():
+    import subprocess
+    try:
+        return subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except Exception:
+        return "gcc"
+
+
+def _cpu_worker(conn, grid, Fo, reps):
+    """Child process: times `reps` oracle steps of one latitude band."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        pkg = load_pkg()
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            pkg.step.run_oracle_step(oracle, grid, Fo)
+            times.append(time.perf_counter() - t0)
+        conn.send(times)
+    except Exception as e:  # reported by the parent
+        conn.send(repr(e))
+    conn.close()
+
+
+def cpu_baseline(pkg, config, nstep_sub, budget_s=30.0):
+    """The oracle's column step (a C restatement of the reference routines,
+    gcc -O3, reference arithmetic order) on the host cores: the workload's
+    latitude rows split into P bands, one process per band, as the reference
+    splits a grid over MPI tasks (the column step needs no communication
+    inside a step).  P = the host cores this process may use, at most 16 (the
+    CPU share that goes with one GPU).  Every band has the workload's
+    longitudes, levels, timestep and physics; the value counts the interior
+    columns of all P bands over the slowest band's median step time."""
+    import multiprocessing as mp
     nlon, nlat, nk, dlev, step, dstep, region = pkg.synth.CONFIGS[config]
-    sub = "cpu_sample"
-    pkg.synth.CONFIGS[sub] = (32, 16, nk, dlev, step, dstep, region)
+    P = max(1, min(16, len(os.sched_getaffinity(0))))
+    rows = -(-nlat // P)
+    sub = "cpu_band"
+    pkg.synth.CONFIGS[sub] = (nlon, rows, nk, dlev, step, dstep, region)
     g = pkg.synth.make_grid(sub, nstep_sub=nstep_sub)
     Fo = pkg.synth.to_fortran_layout(pkg.synth.make_state(g))
-    cols = g["nlon"] * g["nlat"]
-    reps, t_total = 0, 0.0
-    while t_total < budget_s and reps < 2000:
-        t0 = time.perf_counter()
-        pkg.step.run_oracle_step(oracle, g, Fo)
-        t_total += time.perf_counter() - t0
-        reps += 1
-    per_step = t_total / reps
-    value = cols * nk * (10 + nstep_sub) / per_step / 1e6
-    return {"value": round(value, 4), "unit": "Mcolumn-level solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle (C restatement, gcc -O2, no FMA) full column step on a 32x16-column "
-                      "x %d-level sub-grid of the same workload, %d reps, %.2f s/step" % (nk, reps, per_step)}
+    # one step of one band first, in this process, to size the repetitions
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.perf_counter()
+    pkg.step.run_oracle_step(oracle, g, Fo)
+    t_one = time.perf_counter() - t0
+    reps = max(1, min(5, int(budget_s / max(t_one, 1e-3))))
+    ctx = mp.get_context("fork")
+    procs = []
+    for _ in range(P):
+        a, b = ctx.Pipe(duplex=False)
+        pr = ctx.Process(target=_cpu_worker, args=(b, g, Fo, reps))
+        pr.start()
+        procs.append((pr, a))
+    results = []
+    for pr, a in procs:
+        results.append(a.recv())
+        pr.join()
+    bad = [r for r in results if not isinstance(r, list)]
+    if bad:
+        raise RuntimeError("cpu baseline worker failed: %s" % bad[0])
+    per_band = sorted(sorted(r)[len(r) // 2] for r in results)
+    t_step = per_band[-1]
+    cols = nlon * rows * P
+    value = cols * nk * (10 + nstep_sub) / t_step / 1e6
+    return {"value": round(value, 3), "unit": "Mcolumn-level solves/s", "cores": P, "kind": "port",
+            "cpu": _cpu_model(), "compiler": _cc_version() + " -O3 -ffp-contract=off",
+            "sample": "the whole workload: %d latitude bands of %dx%d columns x %d levels, one process per "
+                      "band, oracle column step (C restatement of the reference routines), median of %d "
+                      "steps per band, slowest band %.3f s/step" % (P, nlon, rows, nk, reps, t_step)}
 
 
 def traffic_from_profiles(kernel, workload, scheme):
@@ -214,12 +275,13 @@ def bench_precision(pkg, args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="g0p625")
     ap.add_argument("--nstep-sub", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of oracle time per process for the CPU baseline")
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
     ap.add_argument("--scheme", default="auto", choices=("thomas", "pcr", "auto"),
                     help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
@@ -242,6 +304,12 @@ def main():
 
     import torch
     pkg = load_pkg()
+    cpu = None
+    if (not args.no_cpu_baseline and not args.only and not args.precision and args.config != "nests"):
+        # before anything touches the GPU: the baseline's worker processes are
+        # forked from this one and must not inherit a HIP context
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(pkg, args.config, args.nstep_sub, args.cpu_budget)
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     if args.config == "nests":
@@ -280,14 +348,20 @@ def main():
     # per-routine and per-column-kernel hipEvent pairs, recorded by the
     # library on the stream each runs on, inside the timed region
     pkg.step.profile_begin(args.steps * max(1, args.nstep_sub))
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    evs[0].record()
     for s in range(args.steps):
         step_fn()
+        evs[s + 1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof, kern = pkg.step.profile_end()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    # per-step times between events on the caller's stream, which every
+    # forked stream joins before the step returns
+    step_ms = sorted(evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps))
     rt = {r: prof[r][0] / max(prof[r][1], 1) for r in routines}
     kt = {r: kern[r][0] / max(kern[r][1], 1) for r in routines}
     serial_ms = ms_per_step if serial else None
@@ -349,12 +423,13 @@ def main():
                          "%d serial steps after the timed region" % max(1, args.roofline_steps))},
         "routine_ms": {r: round(rt[r], 4) for r in routines},
         "column_kernel_ms": {r: round(kt[r], 4) for r in routines},
+        "step_ms": {"median": round(step_ms[len(step_ms) // 2], 4), "min": round(step_ms[0], 4),
+                    "max": round(step_ms[-1], 4), "timing": "hipEvents between consecutive steps"},
         "step_alg_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "serial_ms_per_step": round(serial_ms, 4),
     }
-    if not args.no_cpu_baseline and not args.only:
-        print("# cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(pkg, args.config, nsub, args.cpu_budget)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

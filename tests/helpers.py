@@ -49,6 +49,42 @@ def relerr(got, ref):
     return float(np.abs(got - ref).max() / (den if den > 0 else 1.0))
 
 
+# Pointwise error floors: |got - ref| / max(|ref|, floor).  The floors are the
+# reference's own clamps where a field has one: minor species >= 1e-12
+# (minor_ng.F90:236-237), O2/O/He >= 1e-9 before normalisation (comp_ng.F90:
+# 434-456), TN >= 100 K (dt_ng.F90:208-213).  O+ is clamped at 0
+# (oplus_ng.F90:337-340): its floor is 1e-6 cm^-3, far below any value a
+# thermosphere holds.  Fields that change sign (winds, the O+ flux terms Fe and
+# Fn) take a floor of 1e-9 of the field's largest magnitude: where they pass
+# through zero a relative error is meaningless.  xiop2p/xiop2d are positive
+# pointwise chemistry with no floor needed.
+POINT_FLOORS = {
+    "no": 1e-12, "n4s": 1e-12, "n2d": 1e-12, "ar": 1e-12,
+    "o2": 1e-9, "o1": 1e-9, "he": 1e-9,
+    "tn": 100., "op": 1e-6, "xiop2p": 1e-30, "xiop2d": 1e-30,
+}
+SIGN_CHANGING = ("un", "vn", "Fe", "Fn")
+
+
+def point_floor(name, ref):
+    base = name[:-3] if name.endswith("_nm") else name
+    if base in SIGN_CHANGING:
+        return 1e-9 * float(np.abs(ref).max())
+    return POINT_FLOORS[base]
+
+
+def pointwise_err(got, ref, floor):
+    """max over points of |got-ref| / max(|ref|, floor)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return float((np.abs(got - ref) / np.maximum(np.abs(ref), floor)).max())
+
+
+def field_errs(name, got, ref):
+    """(field-normalised error, pointwise relative error with the field's floor)."""
+    return relerr(got, ref), pointwise_err(got, ref, point_floor(name, ref))
+
+
 def diag_dominant_system(rng, ncol, nk, margin=(0.1, 1.0)):
     """a,c = -U(0.5,2), b = -(a+c) + U(margin) (strictly diagonally dominant)."""
     a = -rng.uniform(0.5, 2.0, size=(ncol, nk))

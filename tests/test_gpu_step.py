@@ -2,23 +2,30 @@
 
 Each reference routine (oplus_ng with its sub-cycles, the four minor species,
 dt_ng, duv_ng, comp_o2o_ng + comp_ng) runs on device-resident synthetic state
-and is compared field by field with the oracle.  Tolerance: 1e-12 of the
-field's maximum magnitude (the FP64 drop-in bar); the kernels follow the
-reference's operation order, so only pow/log/exp/sqrt last-ulp differences
-between the device and host math libraries remain.
+and is compared field by field with the oracle, pointwise:
+|got - ref| / max(|ref|, floor) <= 1e-12 at every point, with the floors of
+helpers.POINT_FLOORS (the reference's own clamps).  The dt and duv Thomas
+kernels follow the reference's operation order; the minor-species and comp
+kernels take the temperature powers of their diffusion coefficients from one
+logarithm and divide by dz and by the 3x3 determinant through reciprocals,
+and the PCR scheme reorders the elimination, so a few ulp per term remain.
 """
 import numpy as np
 import pytest
 
-from helpers import gpu_available
+from helpers import field_errs, gpu_available
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
 
 TOL = 1e-12
 
 
-def field_err(got, ref):
-    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+def field_err(got, ref, name=None):
+    """The larger of the field-normalised and the pointwise relative error
+    (the pointwise one needs the field's name for its floor)."""
+    if name is None:
+        return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+    return max(field_errs(name, got, ref))
 
 
 # (grid, O+ sub-cycles, column-solve scheme): "thomas" follows the reference's
@@ -56,9 +63,9 @@ def case(request, pkg, oracle):
 def test_step_fields_match_oracle(case, fields):
     name, ns, ref = case
     for n in fields:
-        err = field_err(ns.fortran(n), ref[n])
-        print(name, n, "%.3e" % err)
-        assert err <= TOL, (name, n, err)
+        fe, pe = field_errs(n, ns.fortran(n), ref[n])
+        print("%s %-7s field %.3e pointwise %.3e" % (name, n, fe, pe))
+        assert fe <= TOL and pe <= TOL, (name, n, fe, pe)
 
 
 def test_step_2d_bookkeeping(case):
@@ -97,7 +104,7 @@ def test_routines_one_at_a_time_match_oracle(pkg, oracle, name, nsub):
         getattr(ns, r)()
     torch.cuda.synchronize()
     for n in ("tn", "un", "vn", "op", "xiop2p", "xiop2d", "no", "n4s", "n2d", "ar", "o2", "o1", "he"):
-        err = field_err(ns.fortran(n), ref[n])
+        err = field_err(ns.fortran(n), ref[n], n)
         assert err <= TOL, (n, err)
 
 
@@ -116,7 +123,7 @@ def test_comp_n2_extrapolation_matches_oracle(pkg, oracle, scheme, calc_helium):
         getattr(ns, fn)()
         torch.cuda.synchronize()
         for n in ("o2", "o1", "he", "o2_nm", "o1_nm", "he_nm"):
-            err = field_err(ns.fortran(n), ref[n])
+            err = field_err(ns.fortran(n), ref[n], n)
             assert err <= TOL, (fn, n, err)
 
 
@@ -140,7 +147,7 @@ def test_step_run_switches_match_oracle(pkg, oracle, scheme):
         getattr(ns, fn)()
         torch.cuda.synchronize()
         for n in ALL_FIELDS:
-            err = field_err(ns.fortran(n), ref[n])
+            err = field_err(ns.fortran(n), ref[n], n)
             assert err <= TOL, (fn, n, err)
 
 
@@ -157,7 +164,7 @@ def test_auto_scheme_tunes_and_matches_oracle(pkg, oracle):
     torch.cuda.synchronize()
     assert set(ns.schemes().values()) <= {"thomas", "pcr"}
     for n in ("tn", "un", "vn", "op", "no", "n4s", "n2d", "ar", "o2", "o1", "he"):
-        assert field_err(ns.fortran(n), ref[n]) <= TOL, n
+        assert field_err(ns.fortran(n), ref[n], n) <= TOL, n
 
 
 def test_profile_hooks_count_calls(pkg):
@@ -266,7 +273,7 @@ def test_small_grid_forms_equal_large_grid_forms(pkg, oracle, monkeypatch, schem
     for n in a.out:
         assert torch.equal(a.out[n], b.out[n]), n
     for n in ALL_FIELDS:
-        assert field_err(b.fortran(n), ref[n]) <= TOL, n
+        assert field_err(b.fortran(n), ref[n], n) <= TOL, n
 
 
 def test_full_size_step_properties(pkg):

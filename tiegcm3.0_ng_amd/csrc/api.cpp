@@ -3,6 +3,9 @@
 // autotuner.  See include/tgx.h for the contract of every symbol.
 #include <hip/hip_runtime.h>
 
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -35,6 +38,8 @@ extern "C" void tgx_clear_error(void) {
 }
 extern "C" int tgx_version(void) { return TGX_VERSION; }
 
+bool tgx_on_main_thread() { return (pid_t)syscall(SYS_gettid) == getpid(); }
+
 // ---- launchers (solvers.hip) -------------------------------------------------
 extern "C" int tgx_launch_thomas_f64(const double *, const double *, const double *, const double *,
                                      double *, double *, int, int, int, int, int, hipStream_t);
@@ -52,8 +57,10 @@ extern "C" int tgx_launch_blktri3_f64(const double *, const double *, const doub
 // read by a kernel still in flight, and calls from other threads or on other
 // streams get buffers of their own.  Staging buffers and the copy stream of
 // the host drop-ins (which return only when their copies are done) are kept
-// per host thread and device.  Buffers are grown on demand and held until the
-// process ends.
+// per host thread and device.  Buffers are grown on demand.  A worker
+// thread's buffers and stream are released when the thread exits, after the
+// device has finished the work queued on them; the main thread's are held
+// until the process ends.
 namespace {
 struct Buf {
   void *p = nullptr;
@@ -64,8 +71,41 @@ struct Stage {
   hipStream_t stream = nullptr;
 };
 std::mutex g_mu;  // guards the autotuner's table
-thread_local std::map<std::pair<int, hipStream_t>, Buf> t_scratch;
-thread_local std::map<int, Stage> t_stage;
+
+// The scratch buffers are keyed by the caller's stream, which the caller may
+// already have destroyed when the thread exits: wait for the whole device.
+struct ScratchSet {
+  std::map<std::pair<int, hipStream_t>, Buf> m;
+  ~ScratchSet() {
+    if (tgx_on_main_thread()) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    for (auto &kv : m) {
+      if (!kv.second.p) continue;
+      if (hipSetDevice(kv.first.first) != hipSuccess) continue;
+      (void)hipDeviceSynchronize();
+      (void)hipFree(kv.second.p);
+    }
+    (void)hipSetDevice(cur);
+  }
+};
+struct StageSet {
+  std::map<int, Stage> m;
+  ~StageSet() {
+    if (tgx_on_main_thread()) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    for (auto &kv : m) {
+      if (hipSetDevice(kv.first) != hipSuccess) continue;
+      if (kv.second.stream) (void)hipStreamSynchronize(kv.second.stream);
+      if (kv.second.buf.p) (void)hipFree(kv.second.buf.p);
+      if (kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
+    }
+    (void)hipSetDevice(cur);
+  }
+};
+thread_local ScratchSet t_scratch;
+thread_local StageSet t_stage;
 
 int current_device(int *dev) {
   hipError_t e = hipGetDevice(dev);
@@ -95,7 +135,7 @@ int grow(Buf *b, size_t need, hipStream_t s) {
 int get_scratch(size_t bytes, hipStream_t s, void **out) {
   int dev;
   if (int rc = current_device(&dev)) return rc;
-  Buf &b = t_scratch[{dev, s}];
+  Buf &b = t_scratch.m[{dev, s}];
   if (int rc = grow(&b, bytes, s)) return rc;
   *out = b.p;
   return 0;
@@ -104,7 +144,7 @@ int get_scratch(size_t bytes, hipStream_t s, void **out) {
 int get_stage(size_t bytes, void **out, hipStream_t *s) {
   int dev;
   if (int rc = current_device(&dev)) return rc;
-  Stage &d = t_stage[dev];
+  Stage &d = t_stage.m[dev];
   if (!d.stream) {
     hipError_t e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
     if (e != hipSuccess) return tgx_set_error(2, "hipStreamCreate: %s", hipGetErrorString(e));
@@ -136,20 +176,27 @@ int run_scalar(int scheme, const double *a, const double *b, const double *c, co
   return tgx_launch_thomas_f64(a, b, c, f, x, (double *)w2, ldk, k1, k2, col0, ncol, s);
 }
 
+// hipEvent pair destroyed on every exit path.
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
 int time_scheme(int scheme, const double *a, const double *b, const double *c, const double *f,
                 double *x, int ldk, int k1, int k2, int col0, int ncol, hipStream_t s, float *ms) {
-  hipEvent_t e0, e1;
-  TGX_HIP(hipEventCreate(&e0));
-  TGX_HIP(hipEventCreate(&e1));
+  EventPair ev;
+  TGX_HIP(hipEventCreate(&ev.e0));
+  TGX_HIP(hipEventCreate(&ev.e1));
   if (int rc = run_scalar(scheme, a, b, c, f, x, ldk, k1, k2, col0, ncol, s)) return rc;  // warm
-  TGX_HIP(hipEventRecord(e0, s));
+  TGX_HIP(hipEventRecord(ev.e0, s));
   for (int i = 0; i < 3; ++i)
     if (int rc = run_scalar(scheme, a, b, c, f, x, ldk, k1, k2, col0, ncol, s)) return rc;
-  TGX_HIP(hipEventRecord(e1, s));
-  TGX_HIP(hipEventSynchronize(e1));
-  TGX_HIP(hipEventElapsedTime(ms, e0, e1));
-  TGX_HIP(hipEventDestroy(e0));
-  TGX_HIP(hipEventDestroy(e1));
+  TGX_HIP(hipEventRecord(ev.e1, s));
+  TGX_HIP(hipEventSynchronize(ev.e1));
+  TGX_HIP(hipEventElapsedTime(ms, ev.e0, ev.e1));
   return 0;
 }
 }  // namespace

@@ -356,19 +356,38 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
 namespace {
 // The fork streams and events belong to the calling host thread (one set per
 // thread and device): calls from several threads never share an event, and
-// calls from one thread are ordered through its streams.  They live until the
-// process ends.
+// calls from one thread are ordered through its streams.  A worker thread's
+// set is released when the thread exits, once its streams have drained; the
+// main thread's lives until the process ends.
 struct Fork {
   hipStream_t s[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev[8] = {};
 };
-thread_local std::map<int, Fork> t_fork;
+struct ForkSet {
+  std::map<int, Fork> m;
+  ~ForkSet() {
+    if (tgx_on_main_thread()) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    for (auto &kv : m) {
+      if (hipSetDevice(kv.first) != hipSuccess) continue;
+      for (hipStream_t st : kv.second.s)
+        if (st) (void)hipStreamSynchronize(st);
+      for (hipEvent_t e : kv.second.ev)
+        if (e) (void)hipEventDestroy(e);
+      for (hipStream_t st : kv.second.s)
+        if (st) (void)hipStreamDestroy(st);
+    }
+    (void)hipSetDevice(cur);
+  }
+};
+thread_local ForkSet t_fork;
 
 int get_fork(Fork **out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return tgx_set_error(3, "no HIP device: %s", hipGetErrorString(e));
-  Fork &f = t_fork[dev];
+  Fork &f = t_fork.m[dev];
   if (!f.s[0]) {
     for (int i = 0; i < 3; ++i) TGX_HIP(hipStreamCreateWithFlags(&f.s[i], hipStreamNonBlocking));
     for (int i = 0; i < 8; ++i) TGX_HIP(hipEventCreateWithFlags(&f.ev[i], hipEventDisableTiming));
@@ -452,18 +471,30 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   TGX_HIP(hipStreamWaitEvent(sa, F->ev[0], 0));
   TGX_HIP(hipStreamWaitEvent(sb, F->ev[0], 0));
   TGX_HIP(hipStreamWaitEvent(sc, F->ev[0], 0));
+  // From here on every return joins the three fork streams back into s0, so
+  // that a failed call still leaves a graph capture joined and no forked
+  // kernel running past the caller's stream.
+  struct Join {
+    Fork *F;
+    hipStream_t s0;
+    bool sa_joined = false;
+    ~Join() {
+      if (!sa_joined) (void)join(0, 1);
+      (void)join(1, 2);
+      (void)join(2, 3);
+    }
+    hipError_t join(int stream, int ev) {
+      hipError_t e = hipEventRecord(F->ev[ev], F->s[stream]);
+      return e == hipSuccess ? hipStreamWaitEvent(s0, F->ev[ev], 0) : e;
+    }
+  } join{F, s0};
   if (int rc = run_oplus(g, in, out, work, sa, true)) return rc;    // stream a
   if (int rc = run_minors(g, in, out, work, sb, true)) return rc;   // stream b
   if (int rc = run_comp(g, in, out, work, sc, true)) return rc;     // stream c
   if (int rc = tgx_ng_dt(g, in, out, work, s0)) return rc;       // caller's stream
-  TGX_HIP(hipEventRecord(F->ev[1], sa));
-  TGX_HIP(hipStreamWaitEvent(s0, F->ev[1], 0));
-  if (int rc = tgx_ng_duv(g, in, out, work, s0)) return rc;
-  TGX_HIP(hipEventRecord(F->ev[2], sb));
-  TGX_HIP(hipEventRecord(F->ev[3], sc));
-  TGX_HIP(hipStreamWaitEvent(s0, F->ev[2], 0));
-  TGX_HIP(hipStreamWaitEvent(s0, F->ev[3], 0));
-  return 0;
+  join.sa_joined = true;  // duv needs oplus' Fe/Fn
+  TGX_HIP(join.join(0, 1));
+  return tgx_ng_duv(g, in, out, work, s0);  // streams b and c joined on return
 }
 
 // The same step strictly in the reference's order on one stream.

@@ -12,6 +12,11 @@
 
 // Record a HIP failure in the library's error slot and return the status code.
 int tgx_set_error(int code, const char *fmt, ...);
+// True on the process's main thread.  Per-thread device resources are
+// released when a worker thread exits; the main thread's are kept until the
+// process ends (its thread-local destructors run during exit, when the HIP
+// runtime may already be shutting down).
+bool tgx_on_main_thread();
 #define TGX_HIP(call)                                                              \
   do {                                                                             \
     hipError_t e_ = (call);                                                        \
