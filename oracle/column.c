@@ -49,6 +49,23 @@ static const double PHI[3][4] = {{0., 1.35, 2.16, 1.11}, {0.673, 0., 1.616, 0.76
 
 static double pi_(void) { return 4.0 * atan(1.0); }
 
+/* ---- test hooks --------------------------------------------------------
+ * Row capture: when a buffer is set, each routine writes the rows it hands to
+ * its column solve, after the boundary rows are applied, in the layout of the
+ * library's capture mode: cap[(plane*ncol + col)*nk + k], col = j*(nlon+4)+i,
+ * planes 0-3 dt a,b,c,f; 4-7 O+ (last sub-step); 8+4s..11+4s minor species s
+ * (0 NO, 1 N4S, 2 N2D, 3 Ar); 24-35 duv A, B, C (2x2 row-major), 36-37 F;
+ * 38-64 comp A, B, C (3x3 row-major), 65-67 F.
+ * Planted mistakes: tgx_o_set_mutation(id) with id != 0 makes one site of one
+ * routine wrong on purpose (each site below is marked MUT(id)), so that the
+ * tests can show that their checks fail on a wrong oracle. */
+static double *g_cap = NULL;
+static int g_mut = 0;
+void tgx_o_set_capture(double *cap) { g_cap = cap; }
+void tgx_o_set_mutation(int id) { g_mut = id; }
+#define MUT(id) (g_mut == (id))
+#define CAPW(pl, col, k, v) (g_cap[((size_t)(pl) * (size_t)nlond * (size_t)nlatd + (size_t)(col)) * (size_t)nk + (size_t)(k)] = (v))
+
 /* Dirichlet lateral-boundary hook used by every _ng routine (e.g.
  * dt_ng.F90:201-206): ghost column lon=-1 (i=0) and lon=nlon+2 (i=nlond-1)
  * from lon_b(nk,2,nlatd), then ghost rows lat=-1, nlat+2 from lat_b(nk,nlond,2). */
@@ -84,7 +101,8 @@ void tgx_o_dt_ng(const tgx_ogrid *g, const tgx_ofields *F, double *tlbc, double 
   const double dz = g->dz, dtx2inv = g->dtx2inv;
   ALLOC3(horadv);
   ALLOC3(tnsmooth);
-  tgx_o_advec(g, F->tn, F->un, F->vn, horadv);
+  if (MUT(107)) tgx_o_advec(g, F->tn, F->vn, F->un, horadv);
+  else tgx_o_advec(g, F->tn, F->un, F->vn, horadv);
   tgx_o_smooth(g, F->tn_nm, tnsmooth, g->shapiro);
   COLBUF(veradv); COLBUF(cpi); COLBUF(kmi); COLBUF(wi); COLBUF(cptn); COLBUF(dudz); COLBUF(dvdz);
   COLBUF(qsolar); COLBUF(tni); COLBUF(gg); COLBUF(ff); COLBUF(p); COLBUF(q); COLBUF(r); COLBUF(rhs);
@@ -140,24 +158,24 @@ void tgx_o_dt_ng(const tgx_ogrid *g, const tgx_ofields *F, double *tlbc, double 
         cptn[k] = cptn[k] - total_heat_a;
       }
       /* :115-119 */
-      for (int k = 1; k < nk - 1; ++k) tni[k] = .5 * (tn[k - 1] + tn[k]);
+      for (int k = 1; k < nk - 1; ++k) tni[k] = MUT(106) ? .5 * (tn[k] + tn[k + 1]) : .5 * (tn[k - 1] + tn[k]);
       tni[0] = tlbc[o2];
       tni[nk - 1] = tn[nk - 2];
       /* :121-139 */
       for (int k = 0; k < nk; ++k) {
         double h = F->schti[o + k];
-        double rho = P0 * g->expzmid_inv / (h * GRAV);
+        double rho = P0 * (MUT(104) ? g->expzmid : g->expzmid_inv) / (h * GRAV);
         rho = rho * g->expz[k];
         double gpart = h * h * rho * cp[k];
         double gpart_a = gpart * g->dift[k];
         gg[k] = GRAV * (F->kt[o + k] + gpart_a) / (P0 * h * (dz * dz));
         ff[k] = GRAV * GRAV * (h * h) * rho / (tni[k] * P0 * 2. * dz);
-        ff[k] = ff[k] * g->dift[k];
+        if (!MUT(101)) ff[k] = ff[k] * g->dift[k];
       }
       /* :141-150 */
       for (int k = 0; k < nk - 1; ++k) {
         p[k] = gg[k] - ff[k];
-        q[k] = -(gg[k + 1] - ff[k + 1] + gg[k] + ff[k]);
+        q[k] = -(gg[k + 1] + (MUT(102) ? ff[k + 1] : -ff[k + 1]) + gg[k] + ff[k]);
         r[k] = gg[k + 1] + ff[k + 1];
       }
       p[nk - 1] = gg[nk - 1] - ff[nk - 1];
@@ -168,14 +186,19 @@ void tgx_o_dt_ng(const tgx_ogrid *g, const tgx_ofields *F, double *tlbc, double 
       /* :152-163 */
       for (int k = 0; k < nk; ++k) {
         rhs[k] = cptn[k];
-        double qpart = cpi[k] * (dtx2inv + F->cool_imp[o + k]) + wi[k] * GASK / F->mbar[o + k];
-        rhs[k] = rhs[k] + F->cool_exp[o + k];
+        double qpart = cpi[k] * (dtx2inv + F->cool_imp[o + k]) + (MUT(103) ? 0. : wi[k] * GASK / F->mbar[o + k]);
+        rhs[k] = MUT(105) ? rhs[k] - F->cool_exp[o + k] : rhs[k] + F->cool_exp[o + k];
         double qpart_a = qpart * g->expz[k];
         q[k] = q[k] - qpart_a;
       }
       q[0] = q[0] - p[0];
-      rhs[0] = rhs[0] - 2. * p[0] * t_lbc;
+      rhs[0] = rhs[0] - (MUT(108) ? 1. : 2.) * p[0] * t_lbc;
       p[0] = 0.;
+      if (g_cap)
+        for (int k = 0; k < nk; ++k) {
+          CAPW(0, I2(i, j), k, p[k]); CAPW(1, I2(i, j), k, q[k]);
+          CAPW(2, I2(i, j), k, r[k]); CAPW(3, I2(i, j), k, rhs[k]);
+        }
       /* :165-167 */
       tgx_o_trsolv_ng(p, q, r, rhs, tn_upd + o, nk, 1, nk - 1);
       tn_upd[o + nk - 1] = tn_upd[o + nk - 2] * tn_upd[o + nk - 2] / tn_upd[o + nk - 3];
@@ -226,13 +249,13 @@ void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd
         zp[o + k] = (zp[o + k] + zp[o + k + 1]) * .5;
       }
       for (int k = 0; k < nk; ++k) {
-        zl[o + k] = zl[o + k] * g->racs[j];
+        if (!MUT(205)) zl[o + k] = zl[o + k] * g->racs[j];
         zl[o + k] = zl[o + k] * GRAV / g->dlamda;
         zp[o + k] = zp[o + k] * GRAV * RE_INV / g->dphi;
       }
     }
   /* :97-98 */
-  tgx_o_smooth(g, F->un_nm, unms, g->shapiro);
+  tgx_o_smooth(g, MUT(207) ? F->vn_nm : F->un_nm, unms, g->shapiro);
   tgx_o_smooth(g, F->vn_nm, vnms, g->shapiro);
   int nk4 = 4 * nk, nk2 = 2 * nk;
   double *pp = calloc(nk4, 8), *qq = calloc(nk4, 8), *rr = calloc(nk4, 8), *ss = calloc(nk2, 8),
@@ -274,11 +297,11 @@ void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd
         dwdz[k] = wi[k] / (2. * dz);
         dwdz[k] = dwdz[k] * g->expz[k];
       }
-      for (int k = 0; k < nk - 1; ++k) g_1[k] = gg[k + 1];
+      for (int k = 0; k < nk - 1; ++k) g_1[k] = MUT(204) ? gg[k] : gg[k + 1];
       g_1[nk - 1] = 2. * gg[nk - 1] - gg[nk - 2];
       /* :166-205 */
       for (int k = 0; k < nk; ++k) {
-        double p11 = -(gg[k] + dwdz[k]);
+        double p11 = MUT(203) ? -gg[k] : -(gg[k] + dwdz[k]);
         double r11 = -(g_1[k] - dwdz[k]);
         MAT(pp, 1, 1, k) = p11; MAT(pp, 2, 2, k) = p11; MAT(pp, 1, 2, k) = 0.; MAT(pp, 2, 1, k) = 0.;
         MAT(rr, 1, 1, k) = r11; MAT(rr, 2, 2, k) = r11; MAT(rr, 1, 2, k) = 0.; MAT(rr, 2, 1, k) = 0.;
@@ -289,25 +312,37 @@ void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd
         qa = dtx2inv + lyyi[k];
         qa = qa * g->expz[k];
         MAT(qq, 2, 2, k) = q11 + qa;
-        double q12 = g->cor[j] + un[k] / RE * g->tanphi[j];
+        double q12 = MUT(201) ? g->cor[j] - un[k] / RE * g->tanphi[j] : g->cor[j] + un[k] / RE * g->tanphi[j];
         double q21 = q12;
-        qa = q12 - lxyi[k];
+        qa = q12 - (MUT(202) ? lyxi[k] : lxyi[k]);
         qa = qa * g->expz[k];
         MAT(qq, 1, 2, k) = -qa;
-        qa = q21 - lyxi[k];
+        qa = q21 - (MUT(202) ? lxyi[k] : lyxi[k]);
         qa = qa * g->expz[k];
         MAT(qq, 2, 1, k) = qa;
       }
       /* :207-215 */
       for (int r = 1; r <= 2; ++r)
         for (int c = 1; c <= 2; ++c) MAT(qq, r, c, 0) = MAT(qq, r, c, 0) - MAT(pp, r, c, 0);
-      VEC(ss, 1, 0) = VEC(ss, 1, 0) - 2. * (MAT(pp, 1, 1, 0) * F->u_lbc[o2] + MAT(pp, 1, 2, 0) * F->v_lbc[o2]);
+      VEC(ss, 1, 0) = VEC(ss, 1, 0) - (MUT(206) ? 1. : 2.) * (MAT(pp, 1, 1, 0) * F->u_lbc[o2] + MAT(pp, 1, 2, 0) * F->v_lbc[o2]);
       VEC(ss, 2, 0) = VEC(ss, 2, 0) - 2. * (MAT(pp, 2, 1, 0) * F->u_lbc[o2] + MAT(pp, 2, 2, 0) * F->v_lbc[o2]);
       for (int n = 0; n < 4; ++n) pp[n] = 0.;
       for (int n = 0; n < 4; ++n) {
         qq[4 * (nk - 2) + n] = qq[4 * (nk - 2) + n] + rr[4 * (nk - 2) + n];
         rr[4 * (nk - 2) + n] = 0.;
       }
+      if (g_cap)
+        for (int k = 0; k < nk; ++k) {
+          for (int r = 1; r <= 2; ++r)
+            for (int c = 1; c <= 2; ++c) {
+              int e = 2 * (r - 1) + (c - 1);
+              CAPW(24 + e, I2(i, j), k, MAT(pp, r, c, k));
+              CAPW(28 + e, I2(i, j), k, MAT(qq, r, c, k));
+              CAPW(32 + e, I2(i, j), k, MAT(rr, r, c, k));
+            }
+          CAPW(36, I2(i, j), k, VEC(ss, 1, k));
+          CAPW(37, I2(i, j), k, VEC(ss, 2, k));
+        }
       /* :217-253 */
       tgx_o_blktri_ng(pp, qq, rr, ss, xx, nk, 1);
       VEC(xx, 1, nk - 1) = 2 * VEC(xx, 1, nk - 2) - VEC(xx, 1, nk - 3);
@@ -345,7 +380,7 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
                        const double *fcomp_tm1, const double *sloss, const double *sprod,
                        const double *flbc /* (nlond,nlatd,3) */, const double *fubc,
                        double rmx, const double phix[4], double alfax, const double *tlbc,
-                       const double *lonb, const double *latb, double *out, double *out_nm) {
+                       const double *lonb, const double *latb, double *out, double *out_nm, int cap_plane) {
   DIMS;
   const double dzp = g->dz, dtx2inv = g->dtx2inv, small = 1.e-12;
   const size_t N2 = (size_t)nlond * nlatd;
@@ -423,7 +458,7 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
                 sai[3 * k + 2] * (dhedz[k] - (1. - (RMHE + dmdz[k]) / barm[k]) * pshe[k]);
         dmdz[k] = dmdz[k] / barm[k];
         ax[k] = -(barm[k] / (TAU * RMN2) * pow(T00 / tni[k], 0.25) /
-                  (phix[2] + salfax[0] * pso2[k] + salfax[1] * pso1[k] + salfax[2] * pshe[k]));
+                  ((MUT(301) ? phix[3] : phix[2]) + salfax[0] * pso2[k] + salfax[1] * pso1[k] + salfax[2] * pshe[k]));
       }
       for (int k = 1; k < nk - 1; ++k) dtnidz[k] = (tni[k + 1] - tni[k - 1]) / 2.;
       dtnidz[0] = tni[1] - tni[0];
@@ -434,9 +469,9 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
       wi[nk - 1] = 1.5 * F->w_upd[o + nk - 1] - .5 * F->w_upd[o + nk - 2];
       for (int k = 0; k < nk - 1; ++k) {
         molp[k] = ax[k] * (1. / dzp + .5 * thdiff[k]);
-        molr[k] = ax[k + 1] * (1. / dzp - .5 * thdiff[k + 1]);
+        molr[k] = ax[MUT(304) ? k : k + 1] * (1. / dzp - .5 * thdiff[k + 1]);
         molq[k] = ax[k] * (1. / dzp - .5 * thdiff[k]) + ax[k + 1] * (1. / dzp + .5 * thdiff[k + 1]);
-        eddyp[k] = g->expzmid_inv * g->difk[k] * (1. / dzp - .5 * dmdz[k]);
+        eddyp[k] = g->expzmid_inv * g->difk[k] * (1. / dzp + (MUT(302) ? .5 : -.5) * dmdz[k]);
         eddyr[k] = g->expzmid * g->difk[k + 1] * (1. / dzp + .5 * dmdz[k + 1]);
         eddyq[k] = g->expzmid_inv * g->difk[k] * (1. / dzp + .5 * dmdz[k]) +
                    g->expzmid * g->difk[k + 1] * (1. / dzp - .5 * dmdz[k + 1]);
@@ -448,18 +483,23 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
       }
       /* :186-198 */
       for (int k = 0; k < nk; ++k) {
-        double s0prod = sprod[o + k] * rmx / F->xnmbar[o + k];
+        double s0prod = MUT(303) ? sprod[o + k] * rmx * F->xnmbar[o + k] : sprod[o + k] * rmx / F->xnmbar[o + k];
         fr[k] = ftm1s[o + k] * dtx2inv - hadvec[o + k] + s0prod;
         fr[k] = fr[k] * g->expz[k];
       }
       double l1 = flbc[o2], l2 = flbc[o2 + N2], l3 = flbc[o2 + 2 * N2];
       qc[0] = qc[0] + pc[0] * (l1 + .5 * l2 * dzp) / (l1 - .5 * l2 * dzp);
-      fr[0] = fr[0] - pc[0] * l3 * dzp / (l1 - .5 * l2 * dzp);
+      fr[0] = fr[0] - (MUT(306) ? .5 : 1.) * pc[0] * l3 * dzp / (l1 - .5 * l2 * dzp);
       pc[0] = 0.;
-      pc[nk - 1] = 1. + .5 * dzp * thdiff[nk - 1];
+      pc[nk - 1] = 1. + (MUT(305) ? -.5 : .5) * dzp * thdiff[nk - 1];
       qc[nk - 1] = pc[nk - 1] - 2.;
       rc[nk - 1] = 0.;
       fr[nk - 1] = -GRAV * rmx * fubc[o2] * dzp / (P0 * ax[nk - 1] * AVO);
+      if (g_cap)
+        for (int k = 0; k < nk; ++k) {
+          CAPW(cap_plane + 0, I2(i, j), k, pc[k]); CAPW(cap_plane + 1, I2(i, j), k, qc[k]);
+          CAPW(cap_plane + 2, I2(i, j), k, rc[k]); CAPW(cap_plane + 3, I2(i, j), k, fr[k]);
+        }
       /* :200 */
       tgx_o_trsolv_ng(pc, qc, rc, fr, out + o, nk, 1, nk);
     }
@@ -617,7 +657,8 @@ void tgx_o_minor_species(const tgx_ogrid *g, const tgx_ofields *F, int species,
     }
     for (size_t n = 0; n < N3; ++n) { sprod[n] = 0.; sloss[n] = 0.; }
   }
-  minor_core(g, F, fc, fctm1, sloss, sprod, flbc, fubc, rmx, phix, alfax, tlbc, lonb, latb, out, out_nm);
+  minor_core(g, F, fc, fctm1, sloss, sprod, flbc, fubc, rmx, phix, alfax, tlbc, lonb, latb, out, out_nm,
+             8 + 4 * species);
   free(sloss); free(sprod); free(flbc); free(fubc);
 }
 
@@ -710,11 +751,11 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
       dtp[0] = tp_op[o + 1] - tp_op[o + 0];
       for (int k = 0; k < nk; ++k) {
         size_t p = o + k;
-        diffj[p] = 1. / (F->scht[p] * dz) * dtp[k] + RMOP * GRAV / GASK * op[p];
+        diffj[p] = 1. / (F->scht[p] * dz) * dtp[k] + (MUT(405) ? -RMOP : RMOP) * GRAV / GASK * op[p];
       }
     }
   /* :159 */
-  tgx_o_smooth(g, optm1, opts2, g->shapiro / nsub);
+  tgx_o_smooth(g, optm1, opts2, MUT(406) ? g->shapiro : g->shapiro / nsub);
   /* :161-170 */
   for (int j = 0; j < nlatd; ++j)
     for (int i = 0; i < nlond; ++i) {
@@ -807,7 +848,7 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
         size_t p = o + k;
         double wii = k < nk - 1 ? .5 * (F->wi[p] + F->wi[p + 1]) : 1.5 * F->wi[p] - .5 * F->wi[p - 1];
         double driftp = wii * 0.5 * hdz[k];
-        double driftq = -wii * 6. / RE;
+        double driftq = (MUT(401) ? wii : -wii) * 6. / RE;
         double driftr = -wii * 0.5 * hdz[k];
         double windp, windq, windr;
         double bu = bdotu[p];
@@ -815,7 +856,7 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
         else if (k <= nk - 2) windp = bz * bdotu[p - 1] * 0.5 * hdz[k];
         else windp = 0.; /* windp(nk) is never set (oplus_ng.F90:265-274) nor used */
         windq = -bu * dvbz * bz;
-        if (k < nk - 2) windr = -bz * bdotu[p + 1] * 0.5 * hdz[k];
+        if (k < nk - 2) windr = -bz * bdotu[MUT(402) ? p : p + 1] * 0.5 * hdz[k];
         else if (k == nk - 2) windr = bz * (2. * bdotu[o + nk - 2] - bdotu[o + nk - 3]) * 0.5 * hdz[nk - 2];
         else windr = bz * (2. * bdotu[o + nk - 1] - bdotu[o + nk - 2]) * 0.5 * hdz[nk - 1];
         pcf[k] = diffp[k] + driftp + windp;
@@ -825,7 +866,7 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
       /* :280-286 */
       double ubcb = -(bz * bz) * djint[nk - 1] * tphdz0[nk - 1];
       double ubca = -(bz * bz) * djint[nk - 1] * tphdz1[nk - 1];
-      qcf[nk - 2] = qcf[nk - 2] + ubcb / ubca * rcf[nk - 2];
+      qcf[nk - 2] = qcf[nk - 2] + (MUT(404) ? ubca / ubcb : ubcb / ubca) * rcf[nk - 2];
       explicit_[nk - 2] = explicit_[nk - 2] - opflux[n2] * rcf[nk - 2] / ubca;
       rcf[nk - 2] = 0.;
       /* :288-303 */
@@ -843,6 +884,7 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
         xiop2p[p] = x2p;
         xiop2d[p] = x2d;
         oploss[k] = xn * (F->rk1[p] * F->o2[p] * RIO2 + F->rk2[p] * F->n2[p] * RIN2 + RK10 * F->n2d[p] * RIN2D);
+        if (MUT(403)) oploss[k] = 2. * oploss[k];
         qcf[k] = qcf[k] - oploss[k];
         double op_prod = qopi + (F->rk19[p] * ne + RK21) * x2p + (F->rk25[p] * ne + RK27) * x2d +
                          (RK18 * x2p + RK24 * x2d) * F->o1[p] * RIO1 * xn;
@@ -851,6 +893,11 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
       qcf[0] = qcf[0] - pcf[0];
       explicit_[0] = explicit_[0] - 2. * pcf[0] * F->qop[o] / (1.5 * oploss[0] - 0.5 * oploss[1]);
       pcf[0] = 0.;
+      if (g_cap)
+        for (int k = 0; k < nk; ++k) {
+          CAPW(4, I2(i, j), k, pcf[k]); CAPW(5, I2(i, j), k, qcf[k]);
+          CAPW(6, I2(i, j), k, rcf[k]); CAPW(7, I2(i, j), k, explicit_[k]);
+        }
       /* :305-307 */
       tgx_o_trsolv_ng(pcf, qcf, rcf, explicit_, opout + o, nk, 1, nk - 1);
       opout[o + nk - 1] = opout[o + nk - 2] * opout[o + nk - 2] / opout[o + nk - 3];
@@ -897,8 +944,9 @@ void tgx_o_comp_o2o_ng(const tgx_ogrid *g, const tgx_ofields *F, double *fs) {
         double pox1 = xn * xn * (F->beta3[p] * n4s / RMN4S * no / RMNO + BETA6 * n2d / RMN2D * no / RMNO) +
                       beta8i * no / RMNO * xn + xn * (RK4 * o2p * n4s / RMN4S + RK10 * op * n2d / RMN2D) +
                       (F->ra1[p] * F->nop[p] + 2. * F->ra2[p] * o2p) * nei;
-        double pox2 = xn * (F->beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + F->rk1[p] * op + RK7 * nplus + 2. * rji;
-        double lox1 = 2. * F->rkm12[p] * xn / F->mbar[p];
+        double pox2 = xn * (F->beta1[p] * n4s / RMN4S + BETA2 * n2d / RMN2D) + F->rk1[p] * op + RK7 * nplus +
+                      (MUT(502) ? 1. : 2.) * rji;
+        double lox1 = (MUT(501) ? 1. : 2.) * F->rkm12[p] * xn / F->mbar[p];
         double lox2 = F->rk3[p] * F->n2p[p] + RK8 * nplus;
         double lox3 = qopi;
         double po21 = F->rkm12[p] * xn / F->mbar[p];
@@ -908,8 +956,8 @@ void tgx_o_comp_o2o_ng(const tgx_ogrid *g, const tgx_ofields *F, double *fs) {
         double lo22 = qo2pi;
 #define FS(isp, m) fs[p + N3 * ((size_t)(isp) + 3 * (size_t)(m))]
         FS(0, 1) = -lo21;
-        FS(0, 2) = xn * po21 * F->o1[p] * (RIO1 * RIO1) * RMO2;
-        FS(1, 1) = pox2 * RMO1 * RIO2;
+        FS(0, 2) = MUT(504) ? xn * po21 * F->o1[p] * RIO1 * RMO2 : xn * po21 * F->o1[p] * (RIO1 * RIO1) * RMO2;
+        FS(1, 1) = MUT(503) ? pox2 : pox2 * RMO1 * RIO2;
         FS(1, 2) = -lox2 - lox1 * F->o1[p] * RIO1 * xn;
         FS(0, 0) = (po23 - lo22) * RMO2 / xn;
         FS(1, 0) = (pox1 - lox3) * RMO1 / xn;
@@ -1013,10 +1061,10 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
       for (int k = 0; k < nk - 1; ++k) {
         int t = km; km = kp; kp = t;
         for (int l = 0; l < 3; ++l) ep[kp][l] = 1. - 1. / embari[k + 1] * (rmass[l] + dembardz[k + 1]);
-        if (k != nk - 2) ep[kp][2] = ep[kp][2] - thdiffalpha * dtndz[k + 1] / tni[k + 1];
+        if (k != nk - 2) ep[kp][2] = ep[kp][2] - (MUT(604) ? -thdiffalpha : thdiffalpha) * dtndz[k + 1] / tni[k + 1];
         ALPHA(alpha, k + 1);
         tgx_o_matinv3(alpha, ak[kp]);
-        wks1 = embari[k + 1] * RIN2 * pow(T00 / tni[k + 1], 0.25) / TAU;
+        wks1 = embari[k + 1] * RIN2 * pow(T00 / tni[k + 1], MUT(602) ? 0.5 : 0.25) / TAU;
         wks3 = wks4;
         wks4 = dembardz[k + 1] / (embari[k + 1] * 2.);
         for (int n = 0; n < 9; ++n) ak[kp][n] = ak[kp][n] * wks1;
@@ -1025,12 +1073,13 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
           for (int m = 0; m < 3; ++m) {
             int e = isp + 3 * m;
             molp[e] = ak[km][e] * (1. / dz + ep[km][m] / 2.);
-            molr[e] = ak[kp][e] * (1. / dz - ep[kp][m] / 2.);
+            molr[e] = ak[kp][e] * (1. / dz - ep[MUT(606) ? km : kp][m] / 2.);
             molq[e] = ak[km][e] * (1. / dz - ep[km][m] / 2.) + ak[kp][e] * (1. / dz + ep[kp][m] / 2.);
           }
         double eddyp = g->expzmid_inv * g->difk[k] * (1. / dz - wks3);
         double eddyr = g->expzmid * g->difk[k + 1] * (1. / dz + wks4);
-        double eddyq = g->expzmid_inv * g->difk[k] * (1. / dz + wks3) + g->expzmid * g->difk[k + 1] * (1. / dz - wks4);
+        double eddyq = g->expzmid_inv * g->difk[k] * (1. / dz + (MUT(601) ? -wks3 : wks3)) +
+                       g->expzmid * g->difk[k + 1] * (1. / dz - wks4);
         size_t p = o + k;
         for (int m = 0; m < 3; ++m)
           for (int isp = 0; isp < 3; ++isp) {
@@ -1041,13 +1090,14 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
             qk[e] = -molq[e] / dz + g->expz[k] * delta * (eddyq * dfac / dz + dtx2inv) -
                     g->expz[k] * fs[p + N3 * ((size_t)isp + 3 * (size_t)(m + 1))];
           }
-        fk[0] = g->expz[k] * (o2s[p] * dtx2inv - (o2adv[p] - fs[p + N3 * 0]) + F->hdo2[p]);
+        fk[0] = g->expz[k] * (o2s[p] * dtx2inv - (o2adv[p] - fs[p + N3 * 0]) + (MUT(605) ? 0. : F->hdo2[p]));
         fk[1] = g->expz[k] * (o1s[p] * dtx2inv - (o1adv[p] - fs[p + N3 * 1]) + F->hdo1[p]);
         fk[2] = g->expz[k] * (hes[p] * dtx2inv - (headv[p] - fs[p + N3 * 2]) + F->hdhe[p]);
         if (k == 0) {
           for (int m = 0; m < 3; ++m)
             for (int n = 0; n < 3; ++n)
-              for (int isp = 0; isp < 3; ++isp) qk[isp + 3 * m] = qk[isp + 3 * m] + pk[isp + 3 * n] * g->b[n][m];
+              for (int isp = 0; isp < 3; ++isp)
+                qk[isp + 3 * m] = qk[isp + 3 * m] + pk[isp + 3 * n] * (MUT(603) ? g->b[m][n] : g->b[n][m]);
           for (int m = 0; m < 3; ++m)
             for (int isp = 0; isp < 3; ++isp) fk[isp] = fk[isp] - pk[isp + 3 * m] * g->fb[m];
           for (int n = 0; n < 9; ++n) pk[n] = 0.;
@@ -1067,6 +1117,15 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
                       rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][2]));
           for (int n = 0; n < 9; ++n) rk[n] = 0.;
         }
+        if (g_cap)
+          for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+              CAPW(38 + 3 * r + c, n2d_, k, pk[r + 3 * c]);
+              CAPW(47 + 3 * r + c, n2d_, k, qk[r + 3 * c]);
+              CAPW(56 + 3 * r + c, n2d_, k, rk[r + 3 * c]);
+            }
+            CAPW(65 + r, n2d_, k, fk[r]);
+          }
         for (int m = 0; m < 3; ++m)
           for (int n = 0; n < 3; ++n)
             for (int isp = 0; isp < 3; ++isp) qk[isp + 3 * m] = qk[isp + 3 * m] - pk[isp + 3 * n] * gama[9 * k + n + 3 * m];

@@ -166,6 +166,26 @@ class OracleState:
         return d
 
 
+# ---- test hooks --------------------------------------------------------------
+CAPTURE_PLANES = 68
+
+
+def set_capture(buf):
+    """Rows handed to each column solve are written into buf, a float64 array
+    of CAPTURE_PLANES*ncol*nk (layout of the library's capture mode), or not
+    at all with None.  The caller keeps buf alive while it is set."""
+    if buf is None:
+        lib().tgx_o_set_capture(None)
+    else:
+        lib().tgx_o_set_capture(_p(buf))
+
+
+def set_mutation(mid):
+    """Plants the deliberate mistake `mid` (see the MUT sites of column.c);
+    0 restores the oracle."""
+    lib().tgx_o_set_mutation(C.c_int(int(mid)))
+
+
 # ---- plain solvers ----------------------------------------------------------
 def trsolv(a, b, c, f, x, lev0, lev1, k1, k2, lon0, lon1, lonmax):
     lib().tgx_o_trsolv(_p(a), _p(b), _p(c), _p(f), _p(x), lev0, lev1, k1, k2, lon0, lon1, lonmax)

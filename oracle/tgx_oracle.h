@@ -135,6 +135,15 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
                    double *o2_upd, double *o2nm_upd, double *o1_upd, double *o1nm_upd,
                    double *he_upd, double *henm_upd);
 
+/* ---- test hooks (column.c) ---------------------------------------------
+ * tgx_o_set_capture: rows handed to each column solve are written into cap
+ * (TGX_O_CAPTURE_PLANES planes of ncol*nk doubles, the library's capture
+ * layout), or not at all with NULL.  tgx_o_set_mutation: a nonzero id plants
+ * one deliberate mistake (the MUT sites of column.c); 0 restores the oracle. */
+#define TGX_O_CAPTURE_PLANES 68
+void tgx_o_set_capture(double *cap);
+void tgx_o_set_mutation(int id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,9 +55,11 @@ def relerr(got, ref):
 # 434-456), TN >= 100 K (dt_ng.F90:208-213).  O+ is clamped at 0
 # (oplus_ng.F90:337-340): its floor is 1e-6 cm^-3, far below any value a
 # thermosphere holds.  Fields that change sign (winds, the O+ flux terms Fe and
-# Fn) take a floor of 1e-9 of the field's largest magnitude: where they pass
-# through zero a relative error is meaningless.  xiop2p/xiop2d are positive
-# pointwise chemistry with no floor needed.
+# Fn) are measured against the largest magnitude of their own column: the
+# error a column solve leaves is a fraction of the column's solution vector,
+# so where a wind passes through zero its error relative to the local value
+# is meaningless.  xiop2p/xiop2d are positive pointwise chemistry with no
+# floor needed.
 POINT_FLOORS = {
     "no": 1e-12, "n4s": 1e-12, "n2d": 1e-12, "ar": 1e-12,
     "o2": 1e-9, "o1": 1e-9, "he": 1e-9,
@@ -68,8 +70,8 @@ SIGN_CHANGING = ("un", "vn", "Fe", "Fn")
 
 def point_floor(name, ref):
     base = name[:-3] if name.endswith("_nm") else name
-    if base in SIGN_CHANGING:
-        return 1e-9 * float(np.abs(ref).max())
+    if base in SIGN_CHANGING:  # per column (level is the last axis)
+        return np.abs(np.asarray(ref, dtype=np.float64)).max(axis=-1, keepdims=True)
     return POINT_FLOORS[base]
 
 

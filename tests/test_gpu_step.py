@@ -308,3 +308,39 @@ def test_full_size_step_properties(pkg):
             assert (o[n + sfx] >= 1e-12).all(), n + sfx
         assert (o["op" + sfx] >= 0).all()
     assert (o["tn"] >= 100).all()
+
+
+@pytest.mark.parametrize("name,nsub", [("small", 2), ("tall", 1)])
+def test_assembled_rows_match_the_pinned_oracle_rows(pkg, oracle, name, nsub):
+    """The rows every Thomas column kernel hands its elimination (capture
+    mode: dt, O+ of the last sub-step, the four minor species, the 2x2 wind
+    blocks, the 3x3 O2/O/He blocks and their right-hand sides) against the
+    rows of the oracle, whose assembly tests/test_oracle_pins.py holds to the
+    model description, conservation and symmetry.  Each plane is compared per
+    column, relative to the column's largest magnitude in that plane."""
+    import torch
+    g = pkg.synth.make_grid(name, nstep_sub=nsub)
+    F = pkg.synth.make_state(g)
+    Fo = pkg.synth.to_fortran_layout(F)
+    nk, nlatd, nlond = g["nk"], g["nlat"] + 4, g["nlon"] + 4
+    buf = np.zeros(oracle.CAPTURE_PLANES * nlatd * nlond * nk)
+    oracle.set_capture(buf)
+    try:
+        pkg.step.run_oracle_step(oracle, g, Fo)
+    finally:
+        oracle.set_capture(None)
+    ref = buf.reshape(oracle.CAPTURE_PLANES, nlatd, nlond, nk)[:, 2:-2, 2:-2]
+    ns = pkg.step.NestState(g, F, scheme="thomas", capture=True)
+    ns.step()
+    torch.cuda.synchronize()
+    got = ns.capture.cpu().numpy().reshape(oracle.CAPTURE_PLANES, nlatd, nlond, nk)[:, 2:-2, 2:-2]
+    worst = {}
+    for pl in range(oracle.CAPTURE_PLANES):
+        rows = nk if 8 <= pl < 24 else nk - 1          # minor species solve every level
+        a, b = got[pl, ..., :rows], ref[pl, ..., :rows]
+        scale = np.abs(b).max(axis=-1, keepdims=True)
+        scale[scale == 0] = 1.
+        worst[pl] = float((np.abs(a - b) / scale).max())
+    print({pl: "%.1e" % e for pl, e in worst.items() if e > 1e-15})
+    bad = {pl: e for pl, e in worst.items() if e > TOL}
+    assert not bad, bad

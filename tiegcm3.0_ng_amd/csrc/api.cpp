@@ -171,7 +171,7 @@ int run_scalar(int scheme, const double *a, const double *b, const double *c, co
                double *x, int ldk, int k1, int k2, int col0, int ncol, hipStream_t s) {
   if (scheme == TGX_SCHEME_PCR) return tgx_launch_pcr(a, b, c, f, x, ldk, k1, k2, col0, ncol, 0, 0, s);
   void *w2 = nullptr;
-  size_t need = sizeof(double) * (size_t)ldk * (size_t)(col0 + ncol);
+  size_t need = sizeof(double) * (size_t)ldk * (size_t)ncol;
   if (int rc = get_scratch(need, s, &w2)) return rc;
   return tgx_launch_thomas_f64(a, b, c, f, x, (double *)w2, ldk, k1, k2, col0, ncol, s);
 }

@@ -26,12 +26,14 @@ __global__ void __launch_bounds__(256) thomas_kfast_f64(
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncol) return;
   size_t o = (size_t)(col0 + t) * (size_t)ldk;
+  // the ratios of batch column t: w2s holds ncol * ldk doubles
+  double *__restrict__ w2c = w2s + (size_t)t * (size_t)ldk;
   double w1 = b[o + k1];
   double w3 = f[o + k1] / w1;
   x[o + k1] = w3;
   for (int k = k1 + 1; k <= k2; ++k) {
     double w2 = c[o + k - 1] / w1;
-    w2s[o + k - 1] = w2;
+    w2c[k - 1] = w2;
     double ak = a[o + k];
     w1 = b[o + k] - ak * w2;
     w3 = (f[o + k] - ak * w3) / w1;
@@ -39,7 +41,7 @@ __global__ void __launch_bounds__(256) thomas_kfast_f64(
   }
   double xn = w3;
   for (int k = k2 - 1; k >= k1; --k) {
-    double xk = x[o + k] - w2s[o + k] * xn;
+    double xk = x[o + k] - w2c[k] * xn;
     x[o + k] = xk;
     xn = xk;
   }
