@@ -160,7 +160,10 @@ def bench_nests(pkg, args):
     names = ("g2p5", "nest1", "nest2")
     doms = []
     for nm in names:
-        g = pkg.synth.make_grid(nm, nstep_sub=args.nstep_sub)
+        # the nests cap the O+ diffusion (opdiffcap, oplus_ng.F90:145-148): each
+        # sub-step advances the state, and at 0.1 deg the explicit field-aligned
+        # O+ diffusion is unstable without the cap
+        g = pkg.synth.make_grid(nm, nstep_sub=args.nstep_sub, opdiffcap=1e8 if nm.startswith("nest") else 0.)
         doms.append((g, pkg.synth.make_state(g, device="cuda")))
     ns = pkg.step.NestedStep(doms, [1, 2, 5], scheme=args.scheme, forked=not args.serial)
     ns.capture()

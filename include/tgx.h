@@ -80,8 +80,30 @@ void tgx_blktri3_f64(const double *p, const double *q, const double *r, const do
  * returned row-major in b[3*(l-1)+(m-1)], fb[3]. */
 void tgx_bndcmp(double dz, double *b, double *fb);
 
+/* Replaces ringfilter (ringfilter.F:1-111):
+ *   subroutine ringfilter(f,lev0,lev1,order,name,lat)
+ * f is real(nlonp4, lev0:lev1), longitude fastest (one latitude, the two
+ * periodic points on each side included), filtered in place; name 'OPLUS'
+ * selects the O+ reconstruction.  The module state the reference reads
+ * (params_module nlon, nlat; cons_module dlamda, nlat_filter, chunk_array)
+ * is passed explicitly.  Latitudes the chunk table does not filter are left
+ * untouched, as in the reference. */
+void tgx_ringfilter_f64(double *f, int lev0, int lev1, int order, const char *name, int lat, int nlon,
+                        int nlat, double dlamda, int nlat_filter, const int *chunk);
+
 /* ------------------------------------------------------------------------ */
 /* 2. Device entry points (level-fastest batches)                            */
+
+/* The ring filter on a device field in the level-plane layout of
+ * tgx_step.h (element (k, lat row j, lon i) at k*ld*(nlat+4) + j*ld + i,
+ * j = latitude + 1, i = longitude + 1 with the periodic points at i = 0, 1,
+ * nlon+2, nlon+3): every latitude the chunk table chunk[nlat_filter]
+ * (cons.F:331-355) filters, all nk levels, in one launch.  order <= 8,
+ * nlon <= TGX_RING_MAX_NLON, nlat_filter <= TGX_RING_MAX_ROWS/2. */
+#define TGX_RING_MAX_ROWS 128
+#define TGX_RING_MAX_NLON 2048
+int tgx_ringfilter_dev(double *f, int nk, int nlon, int nlat, int ld, int order, int oplus, double dlamda,
+                       int nlat_filter, const int *chunk, void *stream);
 /* Column j of the batch starts at ptr + (col0+j)*ldk (scalars) and rows
  * k1..k2 (0-based, inclusive) are solved. */
 

@@ -63,6 +63,7 @@ static double *g_cap = NULL;
 static int g_mut = 0;
 void tgx_o_set_capture(double *cap) { g_cap = cap; }
 void tgx_o_set_mutation(int id) { g_mut = id; }
+int tgx_o_mutation_id(void) { return g_mut; }
 #define MUT(id) (g_mut == (id))
 #define CAPW(pl, col, k, v) (g_cap[((size_t)(pl) * (size_t)nlond * (size_t)nlatd + (size_t)(col)) * (size_t)nk + (size_t)(k)] = (v))
 

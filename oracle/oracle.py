@@ -180,6 +180,32 @@ def set_capture(buf):
         lib().tgx_o_set_capture(_p(buf))
 
 
+def ringfilter_row(f, nlon, order, oplus, nchnk, dlamda):
+    """ringfilter.F on one row, in place: f is (nlev, nlon+4) float64, C order
+    (the reference's f(nlonp4, lev0:lev1))."""
+    assert f.shape[1] == nlon + 4
+    rc = lib().tgx_o_ringfilter_row(_p(f), int(nlon), int(f.shape[0]), int(order), int(bool(oplus)),
+                                    int(nchnk), C.c_double(dlamda))
+    if rc != 0:
+        raise ValueError("ringfilter: order %d >= nlon %d" % (order, nlon))
+    return f
+
+
+def ringfilter_chunks(lat, nlat, chunk):
+    arr = (C.c_int * len(chunk))(*[int(c) for c in chunk])
+    return lib().tgx_o_ringfilter_chunks(int(lat), int(nlat), len(chunk), arr)
+
+
+# cons.F:331-355: latitudes filtered next to each pole and their chunk counts
+RING_CHUNKS = {
+    (72, 36): [3, 9, 18, 36, 36],
+    (144, 72): [3, 3, 9, 9, 18, 18, 36, 36, 72, 72],
+    (288, 144): [3, 3, 3, 9, 9, 9, 18, 18, 18, 36, 36, 36, 72, 72, 72, 72, 144, 144, 144, 144, 144],
+    (576, 288): [3, 3, 3, 3, 3, 9, 9, 9, 9, 9, 18, 18, 18, 18, 18, 18, 36, 36, 36, 36, 36, 36,
+                 72, 72, 72, 72, 72, 72, 144, 144, 144, 144, 144, 144, 288, 288, 288, 288, 288, 288],
+}
+
+
 def set_mutation(mid):
     """Plants the deliberate mistake `mid` (see the MUT sites of column.c);
     0 restores the oracle."""

@@ -135,6 +135,13 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
                    double *o2_upd, double *o2nm_upd, double *o1_upd, double *o1nm_upd,
                    double *he_upd, double *henm_upd);
 
+/* ---- ring filter (ringfilter.c) -------------------------------------------
+ * ringfilter.F:1-111 on one latitude row f(nlon+4, nlev), longitude fastest;
+ * returns -1 if order >= nlon.  tgx_o_ringfilter_chunks: the chunk count of
+ * 1-based latitude lat from the table chunk[nlat_filter] (0: not filtered). */
+int tgx_o_ringfilter_row(double *f, int nlon, int nlev, int order, int oplus, int nchnk, double dlamda);
+int tgx_o_ringfilter_chunks(int lat, int nlat, int nlat_filter, const int *chunk);
+
 /* ---- test hooks (column.c) ---------------------------------------------
  * tgx_o_set_capture: rows handed to each column solve are written into cap
  * (TGX_O_CAPTURE_PLANES planes of ncol*nk doubles, the library's capture

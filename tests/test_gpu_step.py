@@ -37,7 +37,10 @@ def field_err(got, ref, name=None):
                                         # the 5, 2.5 and 1.25 degree global grids and both nests at
                                         # full size
                                         ("g5", 1, "thomas"), ("g2p5", 1, "thomas"), ("g2p5", 1, "pcr"),
-                                        ("g1p25", 1, "thomas"), ("nest1", 2, "thomas"), ("nest2", 1, "pcr")],
+                                        ("g1p25", 1, "thomas"), ("nest1", 2, "thomas"), ("nest2", 1, "pcr"),
+                                        # the reference's default of ten O+ sub-cycles (the Thomas
+                                        # scheme factors the O+ matrix once for all ten)
+                                        ("small", 10, "thomas"), ("tall", 10, "thomas"), ("g2p5", 10, "pcr")],
                 ids=lambda p: "%s-nsub%d-%s" % p)
 def case(request, pkg, oracle):
     import torch
@@ -183,35 +186,92 @@ def test_profile_hooks_count_calls(pkg):
         assert kn == 2 and 0.0 < km <= ms, (r, kern[r], prof[r])
 
 
+def _latest(ns):
+    """Every domain's latest prognostic fields (clones)."""
+    import sys
+    names = sys.modules[type(ns).__module__].PROGNOSTIC
+    return [{n: st.current(n).clone() for n in names} for st in ns.states]
+
+
 @pytest.mark.parametrize("forked", [False, True], ids=["serial", "forked"])
 def test_nested_outer_step_graph_replay_matches_eager(pkg, forked):
     """The global grid and two nests stepped in the reference's recursion
-    (level i runs prod(nstep_ng[:i+1]) sub-steps) captured into one HIP graph:
-    every replay reproduces the eager outer step bit for bit, with eager work
-    and synchronisation between replays."""
+    (level i runs prod(nstep_ng[:i+1]) sub-steps), every sub-step advancing
+    its domain's state, captured into HIP graphs: each replay reproduces the
+    next eager outer step bit for bit (including the domains that take an
+    odd number of sub-steps, which alternate buffer sets), with unrelated
+    eager work and synchronisation between the replays."""
     import torch
     doms = []
     for name, nsub in (("small", 1), ("tiny", 2), ("tall", 1)):
         g = pkg.synth.make_grid(name, nstep_sub=nsub)
         doms.append((g, pkg.synth.make_state(g)))
+    eager = pkg.step.NestedStep(doms, [1, 2, 3], forked=forked)
+    assert eager.substeps() == {0: 1, 1: 2, 2: 6}
+    ref = []
+    for _ in range(4):
+        eager.run()
+        torch.cuda.synchronize()
+        ref.append(_latest(eager))
+    # consecutive outer steps differ: the state advances
+    assert not torch.equal(ref[0][0]["tn"], ref[1][0]["tn"])
     ns = pkg.step.NestedStep(doms, [1, 2, 3], forked=forked)
-    assert ns.substeps() == {0: 1, 1: 2, 2: 6}
+    ns.capture()           # its warm-up is outer step 1
+    other = pkg.step.NestState(doms[1][0], doms[1][1])
+    for rep in range(3):
+        ns.replay()        # outer steps 2, 3, 4
+        torch.cuda.synchronize()
+        got = _latest(ns)
+        for d in range(len(doms)):
+            for n, t in got[d].items():
+                assert torch.equal(t, ref[rep + 1][d][n]), (rep, d, n)
+        other.step()       # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_nested_batch_at_its_sizes_matches_chained_oracle(pkg, oracle):
+    """The nested configuration at its real sizes: the 2.5 deg global grid
+    (144x72), nest 1 (120x60, NSTEP_NG=2) and nest 2 (100x50, NSTEP_NG=5 per
+    nest-1 step), 57 levels, one outer step in the reference's recursion
+    (1, 2 and 10 sub-steps), each sub-step advancing its domain's state.
+    Every domain's state after its last sub-step matches the oracle stepped
+    the same number of times (pointwise, the floors of helpers.POINT_FLOORS),
+    and a graph replay of the next outer step equals the eager one.  The nests
+    cap the O+ diffusion coefficient (opdiffcap, oplus_ng.F90:145-148): at
+    0.1 deg the explicit field-aligned O+ diffusion of the synthetic state is
+    unstable without it and the chained state runs away within ten steps."""
+    import torch
+    doms = []
+    for name in ("g2p5", "nest1", "nest2"):
+        g = pkg.synth.make_grid(name, nstep_sub=2, opdiffcap=1e8 if name.startswith("nest") else 0.)
+        doms.append((g, pkg.synth.make_state(g)))
+    ns = pkg.step.NestedStep(doms, [1, 2, 5], forked=True)
+    sub = ns.substeps()
+    assert sub == {0: 1, 1: 2, 2: 10}
     ns.run()
     torch.cuda.synchronize()
-    eager = [{n: t.clone() for n, t in st.out.items()} for st in ns.states]
-    ns.capture()
-    for rep in range(3):
-        for st in ns.states:
-            for t in st.out.values():
-                t.zero_()
-        torch.cuda.synchronize()
-        ns.replay()
-        torch.cuda.synchronize()
-        for d, st in enumerate(ns.states):
-            for n, t in st.out.items():
-                assert torch.equal(t, eager[d][n]), (rep, d, n)
-        ns.run()  # eager work between replays
-        torch.cuda.synchronize()
+    fields = ("tn", "un", "vn", "op", "n2d", "n4s", "no", "ar", "o2", "o1", "he")
+    for d, (g, F) in enumerate(doms):
+        print("domain %s: %d sub-steps" % (g["name"], sub[d]), flush=True)
+        ref = pkg.step.run_oracle_steps(oracle, g, pkg.synth.to_fortran_layout(F), sub[d])
+        st = ns.states[d]
+        for n in fields + tuple(f + "_nm" for f in fields):
+            fe, pe = field_errs(n, st.fortran(n), ref[n])
+            assert fe <= TOL and pe <= 1e-11, (g["name"], n, fe, pe)
+    # outer step 2: eager on one copy, graph replay on another
+    eager = _latest(ns)
+    ns.run()
+    torch.cuda.synchronize()
+    step2 = _latest(ns)
+    ns2 = pkg.step.NestedStep(doms, [1, 2, 5], forked=True)
+    ns2.capture()
+    ns2.replay()
+    torch.cuda.synchronize()
+    got = _latest(ns2)
+    for d in range(3):
+        for n in got[d]:
+            assert torch.equal(got[d][n], step2[d][n]), (d, n)
+    del eager
 
 
 def test_step_from_two_host_threads(pkg):
@@ -276,26 +336,37 @@ def test_small_grid_forms_equal_large_grid_forms(pkg, oracle, monkeypatch, schem
         assert field_err(b.fortran(n), ref[n], n) <= TOL, n
 
 
-def test_full_size_step_properties(pkg):
+def test_full_size_step_properties(pkg, oracle):
     """At the benchmark's size (0.625 deg: 576x288 columns x 113 levels), where
-    the large-grid forms run: the forked step equals the serial step bit for
-    bit, and the solution keeps the reference's floors and normalisation
-    (comp_ng.F90:415-457, minor_ng.F90:234-237, dt_ng.F90:208-213,
-    oplus_ng.F90:337-340): O2, O, He positive with O2 + O + He < 1, the minor
-    species >= 1e-12, TN >= 100, O+ >= 0, every value finite."""
+    the large-grid forms run: every field of the step matches the oracle run
+    over the whole grid (pointwise, the floors of helpers.POINT_FLOORS), the
+    forked step equals the serial step bit for bit, and the solution keeps the
+    reference's floors and normalisation (comp_ng.F90:415-457,
+    minor_ng.F90:234-237, dt_ng.F90:208-213, oplus_ng.F90:337-340): O2, O, He
+    positive with O2 + O + He < 1, the minor species >= 1e-12, TN >= 100,
+    O+ >= 0, every value finite."""
     import torch
     g = pkg.synth.make_grid("g0p625", nstep_sub=1)
     F = pkg.synth.make_state(g, device="cuda")
+    print("oracle step over 576x288x113", flush=True)
+    ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
     a = pkg.step.NestState(g, F, scheme="thomas")
     a.step_serial()
     torch.cuda.synchronize()
-    ref = {n: t.clone() for n, t in a.out.items()}
+    serial = {n: t.clone() for n, t in a.out.items()}
     del a
     b = pkg.step.NestState(g, F, scheme="thomas")
     b.step()
     torch.cuda.synchronize()
-    for n in ref:
-        assert torch.equal(ref[n], b.out[n]), n
+    for n in serial:
+        assert torch.equal(serial[n], b.out[n]), n
+    worst = {}
+    for n in ALL_FIELDS:
+        fe, pe = field_errs(n, b.fortran(n), ref[n])
+        worst[n] = max(fe, pe)
+    print({n: "%.1e" % e for n, e in worst.items()})
+    assert max(worst.values()) <= TOL, worst
+    del ref
     nl = g["nlon"] + 4
     o = {n: t[..., :nl] for n, t in b.out.items()}
     for n, t in o.items():
@@ -344,3 +415,21 @@ def test_assembled_rows_match_the_pinned_oracle_rows(pkg, oracle, name, nsub):
     print({pl: "%.1e" % e for pl, e in worst.items() if e > 1e-15})
     bad = {pl: e for pl, e in worst.items() if e > TOL}
     assert not bad, bad
+
+
+@pytest.mark.parametrize("name,nsub", [("small", 10), ("tall", 3), ("small", 2)])
+def test_oplus_kept_factorisation_equals_full_assembly(pkg, name, nsub):
+    """With several O+ sub-cycles the Thomas scheme factors the O+ matrix in
+    the first sub-step and later sub-steps rebuild only the right-hand side;
+    capture mode assembles every sub-step in full.  Both give bitwise the
+    same step."""
+    import torch
+    g = pkg.synth.make_grid(name, nstep_sub=nsub)
+    F = pkg.synth.make_state(g)
+    a = pkg.step.NestState(g, F, scheme="thomas")
+    b = pkg.step.NestState(g, F, scheme="thomas", capture=True)
+    a.step()
+    b.step()
+    torch.cuda.synchronize()
+    for n in a.out:
+        assert torch.equal(a.out[n], b.out[n]), n

@@ -47,6 +47,8 @@ extern "C" int tgx_launch_pcr(const double *, const double *, const double *, co
                               double *, int, int, int, int, int, int, int, hipStream_t);
 extern "C" int tgx_launch_blktri2_f64(const double *, const double *, const double *, const double *,
                                       double *, double *, int, int, int, int, hipStream_t);
+extern "C" int tgx_launch_ringfilter(double *, int, long, int, const long *, const int *, int, int, int,
+                                     double, hipStream_t);
 extern "C" int tgx_launch_blktri3_f64(const double *, const double *, const double *, const double *,
                                       double *, double *, int, int, int, int, hipStream_t);
 
@@ -379,6 +381,33 @@ extern "C" void tgx_blktri3_f64(const double *p, const double *q, const double *
   CK(hipMemcpyAsync(dx, x, sizeof(double) * nv, hipMemcpyHostToDevice, s));
   if (tgx_blktri3_dev_f64(dp, dq, dr, df, dx, nk, k2, 0, ncol, s)) return;
   CK(hipMemcpyAsync(x, dx, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+#undef CK
+}
+
+extern "C" void tgx_ringfilter_f64(double *f, int lev0, int lev1, int order, const char *name, int lat,
+                                   int nlon, int nlat, double dlamda, int nlat_filter, const int *chunk) {
+  tgx_clear_error();
+  const int nlev = lev1 - lev0 + 1;
+  if (!f || nlev <= 0 || nlon < 4 || lat < 1 || lat > nlat || nlat_filter < 0 || (nlat_filter > 0 && !chunk)) {
+    tgx_set_error(1, "tgx_ringfilter_f64: bad arguments (lev0 %d lev1 %d nlon %d lat %d nlat %d)", lev0, lev1,
+                  nlon, lat, nlat);
+    return;
+  }
+  int nchnk = 0;  // ringfilter.F:23-34
+  if (lat <= nlat_filter) nchnk = chunk[lat - 1];
+  else if (lat >= nlat - nlat_filter + 1) nchnk = chunk[nlat - lat];
+  if (nchnk == 0) return;
+  const int oplus = name && strcmp(name, "OPLUS") == 0;
+  const size_t bytes = sizeof(double) * (size_t)(nlon + 4) * nlev;
+  void *st = nullptr;
+  hipStream_t s;
+  if (get_stage(bytes, &st, &s)) return;
+#define CK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { tgx_set_error(2, "%s: %s", #call, hipGetErrorString(e_)); return; } } while (0)
+  CK(hipMemcpyAsync(st, f, bytes, hipMemcpyHostToDevice, s));
+  const long off = 0;
+  if (tgx_launch_ringfilter((double *)st, nlev, nlon + 4, 1, &off, &nchnk, nlon, order, oplus, dlamda, s)) return;
+  CK(hipMemcpyAsync(f, st, bytes, hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
 #undef CK
 }

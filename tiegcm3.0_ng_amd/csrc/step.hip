@@ -340,6 +340,10 @@ constexpr int UW_NV = 6;
 constexpr int DW_NV = 2;
 // Tiled Thomas values of the O+ solve: the ratio and the forward solution.
 constexpr int OW_NV = 2;
+// O+ factorisation kept for the later sub-steps of a step: per level the
+// sub-diagonal after the boundary rows, the pivot, and at levels 0 and nk-2
+// the boundary terms of the right-hand side.
+constexpr int OF_NV = 3;
 
 // Horizontal stencils: a block takes a TILE_X x TILE_Y tile of the level
 // planes and a run of STENCIL_RUN levels.  Each level's inputs are loaded
@@ -2925,12 +2929,16 @@ __global__ void __launch_bounds__(256) comp_wave_kernel(tgx_ng_grid g, Geo G, tg
 // domains so that they still fill the chip (oplus_runs).
 struct OpWork {
   double *dj, *bdotu, *bvel, *tpop, *diffj, *djbz, *opbm2, *bdopj, *w2;
+  double *fac;  // the factorisation of the sub-cycles' common matrix (OF_NV values per level)
 };
 struct OpTerms {  // prelude planes of one sub-step
   const double *bd_diff, *bd_diffj, *bd_bvel, *vdotn, *bd_djbz, *opts2;
 };
 
-template <bool RUNS>  // false: nrun == 1, the whole column (a fixed loop)
+// FULL = false (the middle sub-steps of a sub-cycled step): only the values
+// that depend on O+ (bvel, tp_op, diffj, op/bmod2**2); dj, bdotu and dj*bz
+// stay those of the first sub-step, and Fe, Fn are left to the last one.
+template <bool RUNS, bool FULL>  // RUNS false: nrun == 1, the whole column (a fixed loop)
 __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                     const double *__restrict__ op, OpWork W,
                                                     double *__restrict__ Fe, double *__restrict__ Fn,
@@ -2955,6 +2963,22 @@ __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng
   double tpm = k0 > 0 ? tp_at(k0 - 1) : 0., tpc = tp_at(k0), tpn = k0 + 1 < nk ? tp_at(k0 + 1) : tpc;
   for (int k = k0; k < k1; ++k) {
     size_t p = G.a3(k, i, j);
+    double hj = in.scht[p];
+    double dtp;
+    if (k == 0) dtp = tpn - tpc;
+    else if (k <= nk - 3) dtp = (tpn - tpm) / 2.;
+    else dtp = tpc - tpm;
+    double diffj = 1. / (hj * dz) * dtp + RMOP * GRAV / GASK * op[p];
+    if (!FULL) {
+      W.bvel[p] = W.bdotu[p] * op[p];
+      W.tpop[p] = tpc;
+      W.diffj[p] = diffj;
+      W.opbm2[p] = op[p] / (bm * bm);
+      tpm = tpc;
+      tpc = tpn;
+      if (k + 2 < nk) tpn = tp_at(k + 2);
+      continue;
+    }
     double wni = k < nk - 1 ? .5 * (in.w[p] + in.w[p + G.P]) : 1.5 * in.w[p] - .5 * in.w[p - G.P];
     double tr = 0.5 * (in.tn[p] + in.ti[p]);
     double t = 1. - 0.064 * log10(tr);
@@ -2966,13 +2990,7 @@ __global__ void __launch_bounds__(256) oplus_stage1(tgx_ng_grid g, Geo G, tgx_ng
       double cap = g.opdiffcap / (1 + pow(2., g.zpint[k] - 6));
       dj = dj < cap ? dj : cap;
     }
-    double hj = in.scht[p];
     double bdotu = bx * in.un[p] + by * in.vn[p] + hj * bz * wni;
-    double dtp;
-    if (k == 0) dtp = tpn - tpc;
-    else if (k <= nk - 3) dtp = (tpn - tpm) / 2.;
-    else dtp = tpc - tpm;
-    double diffj = 1. / (hj * dz) * dtp + RMOP * GRAV / GASK * op[p];
     double wd = vni * diffj * dj;
     Fe[p] = wd * sncsdip * sndec;
     Fn[p] = wd * sncsdip * csdec;
@@ -3073,7 +3091,13 @@ __global__ void __launch_bounds__(256) oplus_chem_kernel(tgx_ng_grid g, Geo G, t
 // inputs are staged through double-buffered LDS.  Every thread of the block
 // takes part in the staging and the barriers; lateral-boundary and
 // out-of-domain columns do not solve.
-template <bool CAP>
+// MODE 0: one sub-step on its own.  MODE 1: the first sub-step of a
+// sub-cycled step, which also keeps the factorisation of the matrix (it does
+// not depend on O+, oplus_ng.F90:252-303) in W.fac.  MODE 2: a later
+// sub-step, which rebuilds only the right-hand side and runs the two sweeps
+// with the kept factorisation: the same arithmetic, so bitwise the values a
+// full sub-step gives.
+template <bool CAP, int MODE>
 __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                          const double *__restrict__ op,
                                                          const double *__restrict__ optm1,
@@ -3142,7 +3166,7 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
   double opflux = 0., dvb = 1.;
   if (solve) {
     const double pi = 4.0 * atan(1.0), rtd = 180. / pi;
-    {  // :77-91
+    if (MODE != 2) {  // :77-91
       const double phid = 2.0e8, phin = -2.0e8, ppolar = 0.;
       double rl = in.rlatm[c2], a = 0.;
       if (fabs(rl) >= pi / 4.5) a = 1.;
@@ -3173,17 +3197,21 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
   auto load_ol = [&](int L, OL &Q) {
     size_t p = G.a3(L, ic, jc);
     Q.scht = in.scht[p];
+    Q.bdopj = W.bdopj[p];
+    if (MODE == 2) return;  // the right-hand side needs no more
     Q.hdz = 1. / (Q.scht * dz);
     Q.tp1 = 0.5 * (in.ti[p] + in.te[p]);
-    Q.dj = W.dj[p]; Q.bdotu = W.bdotu[p]; Q.bdopj = W.bdopj[p]; Q.wi = in.wi[p];
+    Q.dj = W.dj[p]; Q.bdotu = W.bdotu[p]; Q.wi = in.wi[p];
   };
   OL qa = {0, 0, 0, 0, 0, 0, 0}, qb = {}, qc = {};
   double ol1 = 0., qop0 = 0.;
   if (solve) {
     load_ol(0, qb);
     load_ol(1, qc);
-    ol1 = F3(oploss_p, 1);
-    qop0 = F3(in.qop, 0);  // lower boundary (:301-303)
+    if (MODE != 2) {
+      ol1 = F3(oploss_p, 1);
+      qop0 = F3(in.qop, 0);  // lower boundary (:301-303)
+    }
   }
   double w1 = 0., w3 = 0., rprev = 0.;
   tile_load(0);
@@ -3194,7 +3222,7 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
     if (k + 1 <= nk - 2) tile_load(k + 1);
     if (solve) {
       if (k > 0) load_ol(k + 1, qc);
-      oploss = oploss_p[p];
+      if (MODE != 2) oploss = oploss_p[p];
       opprod = opprod_p[p];
       ui0 = in.ui[p]; ui1 = in.ui[p + G.P]; vi0 = in.vi[p]; vi1 = in.vi[p + G.P];
     }
@@ -3216,7 +3244,7 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
       const double bd_diff = bdotdh(Tbdopj, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
       const double bd_diffj = bdotdh(Tdiffj, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
       const double bd_bvel = bdotdh(Tbvel, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
-      const double bd_djbz = bdotdh(Tdjbz, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
+      const double bd_djbz = MODE == 2 ? 0. : bdotdh(Tdjbz, G, i, j, bxc, byc, cs, g.dlamda, g.dphi);
       const double djbz = pv(Tdjbz, G, i, j);
       double vdotn;
       {  // oplus_ng.F90:196-209
@@ -3253,6 +3281,23 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
       double expl = -1. * diffexp;
       expl = expl + vdotn + bd_bvel;
       expl = expl - opts2 * dtx2inv * nsub;
+      if (MODE == 2) {  // the kept factorisation and boundary terms
+        const double *F = W.fac;
+        if (top) expl = expl - F[G.wtile(k, 2, OF_NV)];
+        expl = expl - opprod;
+        if (k == 0) expl = expl - F[G.wtile(0, 2, OF_NV)];
+        if (k == 0) {
+          w1 = F[G.wtile(0, 1, OF_NV)];
+          w3 = expl / w1;
+        } else {
+          const double pcf = F[G.wtile(k, 0, OF_NV)];
+          w1 = F[G.wtile(k, 1, OF_NV)];
+          w3 = (expl - pcf * w3) / w1;
+        }
+        W.w2[G.wtile(k, 1, OW_NV)] = w3;  // forward solution
+        qa = qb;
+        qb = qc;
+      } else {
       // diffusion, drift and wind coefficients (:239-278)
       double divbz = dvbz + bd_djbz / (qb.dj * (bz * bz));
       double hm = qb.hdz - 0.5 * divbz, hp = qb.hdz + 0.5 * divbz;
@@ -3285,14 +3330,18 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
         double ubcb = -(bz * bz) * djn * t01;
         double ubca = -(bz * bz) * djn * t11;
         qcf = qcf + ubcb / ubca * rcf;
-        expl = expl - opflux * rcf / ubca;
+        const double ub = opflux * rcf / ubca;
+        if (MODE == 1) W.fac[G.wtile(k, 2, OF_NV)] = ub;
+        expl = expl - ub;
         rcf = 0.;
       }
       qcf = qcf - oploss;
       expl = expl - opprod;
       if (k == 0) {  // lower boundary (:301-303)
         qcf = qcf - pcf;
-        expl = expl - 2. * pcf * qop0 / (1.5 * oploss - 0.5 * ol1);
+        const double lb = 2. * pcf * qop0 / (1.5 * oploss - 0.5 * ol1);
+        if (MODE == 1) W.fac[G.wtile(0, 2, OF_NV)] = lb;
+        expl = expl - lb;
         pcf = 0.;
       }
       if (CAP) {
@@ -3309,10 +3358,15 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
         w1 = qcf - pcf * w2;
         w3 = (expl - pcf * w3) / w1;
       }
+      if (MODE == 1) {
+        W.fac[G.wtile(k, 0, OF_NV)] = pcf;
+        W.fac[G.wtile(k, 1, OF_NV)] = w1;
+      }
       W.w2[G.wtile(k, 1, OW_NV)] = w3;  // forward solution
       rprev = rcf;
       qa = qb;
       qb = qc;
+      }
     }
     if (k + 1 <= nk - 2) tile_store((k + 1) & 1);
   }
@@ -4174,22 +4228,30 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     case 9:
     case 17:
     case 21:
-    case 29: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
+    case 29:
+    case 31:
+    case 32:
+    case 33: {  // O+: bufs = {dj bdotu bvel tpop diffj djbz opbm2 bdopj w2, op, optm1, opout,
                 //           optm1out, bd_diff bd_diffj bd_bvel vdotn bd_djbz opts2, oploss, opprod}
-      OpWork W{D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8)};
+      OpWork W{D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), which >= 31 ? D(21) : nullptr};
       const double *op = D(9), *optm1 = D(10);
       double *opout = D(11), *optm1out = D(12);
       OpTerms OT{D(13), D(14), D(15), D(16), D(17), D(18)};
       if (which == 17) {
         hipLaunchKernelGGL(oplus_chem_kernel, gridp, block, 0, s, g, G, in, D(19), D(20), out.xiop2p,
                            out.xiop2d);
-      } else if (which == 3) {
+      } else if (which == 3 || which == 31) {  // 31: the O+-dependent values only
         const int nrun = oplus_runs(ncol, G.nk, 1);
         dim3 grid1((unsigned)((ncol * nrun + 255) / 256));
-        if (nrun > 1)
-          hipLaunchKernelGGL(oplus_stage1<true>, grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, nrun);
+        if (which == 31) {
+          if (nrun > 1)
+            hipLaunchKernelGGL((oplus_stage1<true, false>), grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, nrun);
+          else
+            hipLaunchKernelGGL((oplus_stage1<false, false>), grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, 1);
+        } else if (nrun > 1)
+          hipLaunchKernelGGL((oplus_stage1<true, true>), grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, nrun);
         else
-          hipLaunchKernelGGL(oplus_stage1<false>, grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, 1);
+          hipLaunchKernelGGL((oplus_stage1<false, true>), grid1, block, 0, s, g, G, in, op, W, out.Fe, out.Fn, 1);
       } else if (which == 4) {
         const int nrun = oplus_runs(ncol, G.nk, 2);
         dim3 grid2((unsigned)((ncol * nrun + 255) / 256));
@@ -4211,12 +4273,18 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         const double *lonb = in.lonb_op + (size_t)sub * 2 * g.nk * (size_t)G.nlatd;
         const double *latb = in.latb_op + (size_t)sub * 2 * g.nk * (size_t)G.ld;
         dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
-        if (which == 29)  // stencils fused: one block per column tile
+        if (which == 32)  // first of several sub-steps: keeps the factorisation
+          hipLaunchKernelGGL((oplus_tile_kernel<false, 1>), gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+                             (const double *)D(19), (const double *)D(20), opout, optm1out);
+        else if (which == 33)  // a later sub-step: right-hand side and sweeps
+          hipLaunchKernelGGL((oplus_tile_kernel<false, 2>), gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+                             (const double *)D(19), (const double *)D(20), opout, optm1out);
+        else if (which == 29)  // stencils fused: one block per column tile
           if (g.capture)
-            hipLaunchKernelGGL(oplus_tile_kernel<true>, gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+            hipLaunchKernelGGL((oplus_tile_kernel<true, 0>), gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                                (const double *)D(19), (const double *)D(20), opout, optm1out);
           else
-            hipLaunchKernelGGL(oplus_tile_kernel<false>, gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+            hipLaunchKernelGGL((oplus_tile_kernel<false, 0>), gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                                (const double *)D(19), (const double *)D(20), opout, optm1out);
         else if (G.nk <= 64)
           hipLaunchKernelGGL(oplus_wave_stage3<1>, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,

@@ -38,7 +38,8 @@ enum {
   T_OW = T_DW + 2,  // O+: Thomas ratio and forward solution
   T_MS = T_OW + 2,  // minor species: loss and scaled production of NO or N2D, and of N4S
   T_ML = T_MS + 4,  // minor species: NO / N4S source coefficients linear in N2D
-  T_N = T_ML + 6
+  T_OF = T_ML + 6,  // O+: factorisation kept across the sub-cycles of a step
+  T_N = T_OF + 3
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -151,7 +152,7 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
-       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE, K_MIN_WB_LIN };
+       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE, K_MIN_WB_LIN, K_OP1_LITE, K_OP_TILE_KEEP, K_OP_TILE_REUSE };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -279,12 +280,21 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
   double *tmp_op = plane(work, g, W_OP_X), *tmp_nm = plane(work, g, W_OP_XNM);
   const double *op_in = in->op, *nm_in = in->op_nm;
   double *oploss = plane(work, g, W_OP_C), *opprod = plane(work, g, W_OP_C + 1);
+  // The O+ matrix does not depend on O+ (oplus_ng.F90:252-303, inputs fixed
+  // at itp, dynamics_ng.F90:103-119): with the Thomas scheme and several
+  // sub-cycles, the first sub-step keeps its factorisation and the later
+  // ones rebuild only the right-hand side (bitwise the same values).  The
+  // middle sub-steps also skip the O+-independent part of stage 1 and the
+  // Fe/Fn outputs, which the last sub-step writes.  Capture mode (analysis)
+  // assembles every sub-step in full.
+  const bool pcr = use_pcr(g, R_OPLUS);
+  const bool reuse = !pcr && nsub > 1 && !g->capture;
   for (int sub = 0; sub < nsub; ++sub) {
     // the last sub-step writes the outputs; earlier ones alternate so that
     // a sub-step never reads the buffer it writes
     bool to_out = ((nsub - 1 - sub) % 2) == 0;
     double *op_o = to_out ? out->op : tmp_op, *nm_o = to_out ? out->op_nm : tmp_nm;
-    void *b[21];
+    void *b[22];
     for (int q = 0; q < 8; ++q) b[q] = plane(work, g, W_OP + q);
     b[8] = tiles(work, g, T_OW);  // Thomas ratio and forward solution (K_OP_TILE)
     b[9] = (void *)op_in;
@@ -294,16 +304,18 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
     for (int q = 0; q < 6; ++q) b[13 + q] = plane(work, g, W_OP_T + q);
     b[19] = oploss;
     b[20] = opprod;
+    b[21] = tiles(work, g, T_OF);
     // the O+ chemistry does not depend on O+: once for all sub-cycles
     if (sub == 0 && !chem_done)
       if (int rc = tgx_ng_launch(K_OP_CHEM, g, in, out, b, sub, s)) return rc;
-    if (int rc = tgx_ng_launch(K_OP1, g, in, out, b, sub, s)) return rc;
+    const bool lite = reuse && sub > 0 && sub < nsub - 1;
+    if (int rc = tgx_ng_launch(lite ? K_OP1_LITE : K_OP1, g, in, out, b, sub, s)) return rc;
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
-    const bool pcr = use_pcr(g, R_OPLUS);
     if (pcr)  // Thomas: the stencils are fused into the column kernel
       if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
     ProfScope kern(R_N + R_OPLUS, s);
-    if (int rc = tgx_ng_launch(pcr ? K_OP3_W : K_OP_TILE, g, in, out, b, sub, s)) return rc;
+    const int col = pcr ? K_OP3_W : !reuse ? K_OP_TILE : sub == 0 ? K_OP_TILE_KEEP : K_OP_TILE_REUSE;
+    if (int rc = tgx_ng_launch(col, g, in, out, b, sub, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
   }

@@ -39,6 +39,11 @@ LATERAL = "tn un vn o2 o1 he n2d n4s no ar op".split()
 OUT_3D = ("tn tn_nm un un_nm vn vn_nm n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm "
           "op op_nm xiop2p xiop2d Fe Fn o2 o2_nm o1 o1_nm he he_nm").split()
 OUT_2D = "tlbc tlbc_nm ulbc vlbc ulbc_nm vlbc_nm flx_he".split()
+# The model state a step advances: its outputs (itc) are the next step's
+# inputs (itp) under the same names (the time levels swap between steps,
+# dynamics_ng.F90 / advance_ng.F90).
+PROGNOSTIC = ("tn tn_nm un un_nm vn vn_nm o2 o2_nm o1 o1_nm he he_nm op op_nm "
+              "n2d n2d_nm n4s n4s_nm no no_nm ar ar_nm tlbc ulbc vlbc flx_he").split()
 
 
 class NgGrid(C.Structure):
@@ -173,6 +178,33 @@ class NestState:
             self.capture = torch.zeros((CAPTURE_PLANES, ncol, nk), **kw)
             self.g.capture = self.capture.data_ptr()
 
+    # ---- time stepping ------------------------------------------------------------
+    def enable_feedback(self):
+        """Makes successive steps advance the state: step n reads the
+        prognostic fields step n-1 wrote.  Two buffer sets alternate (the
+        inputs given at construction and the outputs), so no copies are made:
+        phase 0 reads set A and writes set B, phase 1 reads B and writes A.
+        current(name) returns the latest value of a prognostic field."""
+        if getattr(self, "_phases", None):
+            return
+        f1 = NgFields()
+        o1 = NgOut()
+        for n, _ in NgFields._fields_:
+            setattr(f1, n, getattr(self.f, n))
+        for n, _ in NgOut._fields_:
+            setattr(o1, n, getattr(self.o, n))
+        for n in PROGNOSTIC:
+            setattr(f1, n, self.out[n].data_ptr())
+            setattr(o1, n, self.inp[n].data_ptr())
+        self._phases = [(self.f, self.o), (f1, o1)]
+        self.phase = 0
+
+    def current(self, name):
+        """Latest value of a field in the padded device layout."""
+        if getattr(self, "_phases", None) and name in PROGNOSTIC and self.phase == 0:
+            return self.inp[name]
+        return self.out[name]
+
     # ---- layout helpers ------------------------------------------------------
     def _padlast(self, t):
         shp = t.shape[:-1] + (self.ld,)
@@ -187,8 +219,9 @@ class NestState:
         return self._padlast(t.to(dtype=torch.float64, device=self.dev))
 
     def fortran(self, name):
-        """An output in the reference's Fortran layout, numpy (nlatd, nlond, nk) or (nlatd, nlond)."""
-        t = self.out[name][..., :self.nlond]
+        """An output in the reference's Fortran layout, numpy (nlatd, nlond, nk)
+        or (nlatd, nlond); with feedback, the latest value of a prognostic field."""
+        t = self.current(name)[..., :self.nlond]
         if t.dim() == 3:
             t = t.permute(1, 2, 0)
         return t.contiguous().cpu().numpy()
@@ -196,10 +229,16 @@ class NestState:
     # ---- entry points -----------------------------------------------------------
     def _call(self, name, stream=None):
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        rc = getattr(_lib(), name)(C.byref(self.g), C.byref(self.f), C.byref(self.o),
+        f, o = (self.f, self.o)
+        phases = getattr(self, "_phases", None)
+        if phases and name in ("tgx_ng_step", "tgx_ng_step_serial"):
+            f, o = phases[self.phase]
+        rc = getattr(_lib(), name)(C.byref(self.g), C.byref(f), C.byref(o),
                                    C.c_void_p(self.work.data_ptr()), C.c_void_p(s))
         if rc != 0:
             raise _tgx.TgxError(_tgx.last_error())
+        if phases and name in ("tgx_ng_step", "tgx_ng_step_serial"):
+            self.phase ^= 1
 
     def oplus(self, stream=None):
         self._call("tgx_ng_oplus", stream)
@@ -237,18 +276,26 @@ class NestedStep:
     recursion (advance_ng.F90:57-199): nest level i runs nstep_ng[i] sub-steps
     per step of level i-1, so level i runs prod(nstep_ng[:i+1]) sub-steps per
     global step.  Each domain's column step is the fused step on its own
-    device state (synthetic, independent inputs: the lateral coupling through
-    map_in/set_bndry is outside the column step).
+    device state, and each sub-step advances that state (feedback: the
+    prognostic fields one sub-step writes are the next one's inputs; the
+    other inputs and the lateral data stay those given, since the explicit
+    physics and the map_in/set_bndry coupling are outside the column step).
 
     run() launches the sequence eagerly on the current stream; capture() records
-    it once into a HIP graph and replay() launches the whole outer step as one
-    graph.  AUTO schemes are tuned by a first eager run before capture.
+    it into a HIP graph and replay() launches the whole outer step as one
+    graph.  AUTO schemes are tuned by a first eager run before capture.  With
+    feedback, a domain taking an odd number of sub-steps per outer step ends
+    it on the other buffer set, so capture() records two graphs, one for each
+    starting buffer set, and replay() alternates them.
     forked: each domain step is tgx_ng_step (routines on forked streams,
     joined before the next sub-step) rather than tgx_ng_step_serial.
     """
 
-    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda", forked=True):
+    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda", forked=True, feedback=True):
         self.states = [NestState(g, F, device=device, scheme=scheme) for g, F in domains]
+        if feedback:
+            for st in self.states:
+                st.enable_feedback()
         self.nstep_ng = list(nstep_ng)
         self.forked = forked
         self.graph = None
@@ -271,17 +318,37 @@ class NestedStep:
             else:
                 self.states[i].step_serial()
 
+    def _phases(self):
+        return [getattr(st, "phase", 0) for st in self.states]
+
+    def _set_phases(self, ph):
+        for st, p in zip(self.states, ph):
+            if getattr(st, "_phases", None):
+                st.phase = p
+
     def capture(self):
         torch.cuda.synchronize()
         self.run()  # tunes AUTO schemes and settles allocations before capture
         torch.cuda.synchronize()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.run()
+        start = self._phases()
+        odd = any(getattr(st, "_phases", None) and self.order.count(i) % 2
+                  for i, st in enumerate(self.states))
+        self.graphs = []
+        for _ in range(2 if odd else 1):   # nothing runs while capturing
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                self.run()
+            self.graphs.append((graph, self._phases()))
+        self._set_phases(start)             # the data is still where the warm-up left it
+        self._next = 0
+        self.graph = self.graphs[0][0]
         torch.cuda.synchronize()
 
     def replay(self):
-        self.graph.replay()
+        graph, end = self.graphs[self._next]
+        graph.replay()
+        self._set_phases(end)
+        self._next = (self._next + 1) % len(self.graphs)
 
 
 # Algorithmic HBM traffic of each routine: the distinct 3-D fields its fused
@@ -363,6 +430,19 @@ def column_kernel_bytes_per_column_level(routine, scheme="thomas"):
     if scheme == "pcr":
         ins = PCR_INS.get(routine, ins)
     return 8 * (len(set(ins)) + len(set(outs)))
+
+
+def run_oracle_steps(oracle, grid, Fo, nsteps):
+    """The oracle's column step taken nsteps times, each step's prognostic
+    outputs (PROGNOSTIC) the next step's inputs; returns the last step's
+    outputs."""
+    Fo = dict(Fo)
+    res = None
+    for _ in range(nsteps):
+        res = run_oracle_step(oracle, grid, Fo)
+        for n in PROGNOSTIC:
+            Fo[n] = res[n]
+    return res
 
 
 def run_oracle_step(oracle, grid, Fo):

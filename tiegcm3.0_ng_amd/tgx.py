@@ -38,6 +38,8 @@ _SIGS = {
     "tgx_trsolv_dev_mixed": ([_D] * 5 + [_I] * 6 + [_D], _I),
     "tgx_blktri2_dev_f64": ([_D] * 5 + [_I] * 4 + [_D], _I),
     "tgx_blktri3_dev_f64": ([_D] * 5 + [_I] * 4 + [_D], _I),
+    "tgx_ringfilter_f64": ([_D, _I, _I, _I, C.c_char_p, _I, _I, _I, C.c_double, _I, _D], None),
+    "tgx_ringfilter_dev": ([_D] + [_I] * 6 + [C.c_double, _I, _D, _D], _I),
 }
 
 
@@ -131,6 +133,22 @@ def blktri3(p, q, r, f, x, k2):
     return x
 
 
+def _chunks(chunk):
+    arr = (C.c_int * max(1, len(chunk)))(*[int(c) for c in chunk])
+    return arr, len(chunk)
+
+
+def ringfilter(f, order, name, lat, nlon, nlat, dlamda, chunk):
+    """ringfilter.F:1-111 on one latitude, in place.  f is the reference's
+    f(nlonp4, lev0:lev1): a numpy array (nlev, nlon+4); chunk = the chunk
+    table cons_module chunk_array(1:nlat_filter)."""
+    arr, nf = _chunks(chunk)
+    lib().tgx_ringfilter_f64(_hp(f), 1, f.shape[0], int(order), name.encode(), int(lat), int(nlon), int(nlat),
+                             float(dlamda), nf, C.cast(arr, C.c_void_p))
+    _check_void()
+    return f
+
+
 def bndcmp(dz):
     b = np.zeros(9)
     fb = np.zeros(3)
@@ -165,6 +183,16 @@ def blktri2_dev(a, b, c, f, x, nrow, stream=None):
     _check(lib().tgx_blktri2_dev_f64(_dp(a), _dp(b), _dp(c), _dp(f), _dp(x), ldk, nrow, 0, ncol,
                                      _stream(stream)))
     return x
+
+
+def ringfilter_dev(f, nlon, nlat, order, oplus, dlamda, chunk, stream=None):
+    """The ring filter on a device tensor (nk, nlat+4, ld) in the level-plane
+    layout: every latitude the chunk table filters, all levels, in place."""
+    nk, ld = f.shape[0], f.shape[-1]
+    arr, nf = _chunks(chunk)
+    _check(lib().tgx_ringfilter_dev(_dp(f), nk, int(nlon), int(nlat), ld, int(order), int(bool(oplus)),
+                                    float(dlamda), nf, C.cast(arr, C.c_void_p), _stream(stream)))
+    return f
 
 
 def blktri3_dev(p, q, r, f, x, nrow, stream=None):
