@@ -116,6 +116,23 @@ int tgx_trsolv_dev_f64(const double *a, const double *b, const double *c, const 
 /* The choice the autotuner made for this shape, or TGX_SCHEME_AUTO if none yet. */
 int tgx_autotune_choice(int ldk, int k1, int k2, int ncol);
 
+/* Allocation-free tgx_trsolv_dev_f64.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  Same arguments
+ * and results as tgx_trsolv_dev_f64; it calls no hipMalloc, hipFree or other
+ * allocating routine.  The Thomas scheme keeps its ratios in a fixed device
+ * array of TGX_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) that the library holds
+ * on every GPU it runs on until the process ends, so ldk*ncol must not exceed
+ * TGX_NOALLOC_SCRATCH_DOUBLES (TGX_ERR_ARG otherwise; PCR needs no scratch).
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (other streams, other host threads) race and must be ordered by
+ * the caller.  TGX_SCHEME_AUTO takes the autotuner's cached choice for the
+ * shape if tgx_trsolv_dev_f64 made one, else Thomas; it never times. */
+#define TGX_NOALLOC_SCRATCH_DOUBLES (8u * 1024u * 1024u)
+int tgx_trsolv_dev_f64_noalloc(const double *a, const double *b, const double *c, const double *f,
+                               double *x, int ldk, int k1, int k2, int col0, int ncol, int scheme,
+                               void *stream);
+
 /* Mixed precision: FP32 cyclic reduction of the FP64 system, FP64 residual
  * r = f - A x and nsweep (0..8) refinement sweeps reusing the FP32
  * reduction.  At most 256 rows. */

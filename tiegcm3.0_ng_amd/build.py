@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtgx.so")
-SOURCES = ["solvers.hip", "step.hip", "api.cpp", "step_api.cpp", "filter.hip"]
+SOURCES = ["solvers.hip", "step.hip", "api.cpp", "step_api.cpp", "noalloc.hip", "filter.hip"]
 
 # FP64 parity kernels follow the reference's arithmetic order: no contraction.
 # Packed FP32 instructions are disabled for the whole library.
