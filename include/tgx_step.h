@@ -142,6 +142,50 @@ int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_
 int tgx_ng_profile_begin(int max_calls);
 int tgx_ng_profile_end(double *ms_total /* [10] */, int *calls /* [10] */);
 
+/* ------------------------------------------------------------------------ */
+/* The model's own layout                                                    */
+
+/* Level-fastest model arrays <-> level planes, on the device.  The model
+ * holds a 3-D nest field as f(nk, lond0:lond1, latd0:latd1), level fastest
+ * (fields_ng.F90:226-261); src/dst of that shape are nk*nlond*nlatd doubles,
+ * the planes nk*ld*nlatd (pitch ld >= nlond).  One launch, LDS-tiled, on
+ * `stream`; nk <= 240. */
+int tgx_ng_pack_levels(const double *src, double *dst, int nk, int nlond, int nlatd, int ld, void *stream);
+int tgx_ng_unpack_levels(const double *src, double *dst, int nk, int nlond, int nlatd, int ld, void *stream);
+
+/* A host-layout context for one nest domain: device copies of every field of
+ * tgx_ng_fields and tgx_ng_out in level planes, a workspace and a stream,
+ * filled and read back by field name from arrays in the model's layout.  The
+ * Fortran drop-ins with the reference's routine signatures (dt_ng, duv_ng,
+ * oplus_ng, comp_ng, comp_{n2d,n4s,no,ar}_ng) are built on it; arrays are
+ * copied on every call, so it is for models whose state lives on the host.
+ *
+ * create: g as for the step, except that its level and latitude arrays
+ *   (expz .. glat) are HOST pointers, copied; g->ld and g->capture are
+ *   ignored.  Returns NULL on failure (message in tgx_last_error()).
+ * put: an input field by its tgx_ng_fields name: 3-D f(nk, nlon+4, nlat+4),
+ *   2-D f(nlon+4, nlat+4); lateral data in the model's lon_b(nk, 2, latd) and
+ *   lat_b(nk, lond, 2) slices (fields_ng.F90:304-309), lonb_op/latb_op with
+ *   nstep_sub such blocks, and "lonb_op_sub"/"latb_op_sub" for the one block
+ *   of a single O+ sub-step.
+ * put_out: a tgx_ng_out field that a later routine reads (duv reads this
+ *   step's tn, Fe, Fn).
+ * put_dev: a 3-D or 2-D input from a device array of the model's layout,
+ *   after the work already queued on `stream`.
+ * get: a tgx_ng_out field (or a 3-D/2-D input) into a host array of the
+ *   model's layout.
+ * run: "oplus" (all nstep_sub sub-steps), "oplus_sub" (one sub-step with the
+ *   nstep_sub time factor, lateral data of lonb_op_sub/latb_op_sub),
+ *   "minors", "dt", "duv", "comp" or "step".
+ * All calls return when the copies and the routine have finished. */
+void *tgx_ng_ctx_create(const tgx_ng_grid *g);
+int tgx_ng_ctx_destroy(void *ctx);
+int tgx_ng_ctx_put(void *ctx, const char *name, const double *host);
+int tgx_ng_ctx_put_out(void *ctx, const char *name, const double *host);
+int tgx_ng_ctx_put_dev(void *ctx, const char *name, const double *dev, void *stream);
+int tgx_ng_ctx_get(void *ctx, const char *name, double *host);
+int tgx_ng_ctx_run(void *ctx, const char *routine);
+
 #ifdef __cplusplus
 }
 #endif

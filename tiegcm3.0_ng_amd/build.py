@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtgx.so")
-SOURCES = ["solvers.hip", "step.hip", "api.cpp", "step_api.cpp", "noalloc.hip", "filter.hip"]
+SOURCES = ["solvers.hip", "step.hip", "api.cpp", "step_api.cpp", "noalloc.hip", "filter.hip", "pack.hip",
+           "host_ctx.cpp"]
 
 # FP64 parity kernels follow the reference's arithmetic order: no contraction.
 # Packed FP32 instructions are disabled for the whole library.
@@ -62,31 +63,51 @@ def build(verbose=True, jobs=4):
     return OUT
 
 
+def _fc(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
 def build_fortran(verbose=True):
-    """Fortran drop-in shims (trsolv, blktri) and their test driver, built
-    with amdflang against libtgx.so.  Skipped if no Fortran compiler."""
+    """The Fortran drop-ins and their test drivers, built with amdflang
+    against libtgx.so.  Skipped if no Fortran compiler.
+
+    - fortran/tgx_c.F90, tgx_ng_c.F90: iso_c_binding interfaces (product).
+    - fortran/shim_driver: trsolv / blktri drop-ins (trsolv_shim.F90) called
+      as the global model calls them.
+    - tests/fortran/ng_driver: trsolv_ng_shim.F90 and the nest routine
+      drop-ins (ng_shims.F90) compiled against the test stand-ins of the
+      model modules (tests/fortran/model_modules.F90), one nest step run
+      through them in the model's order."""
     fc = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
     if not os.path.exists(fc):
         if verbose:
             print("amdflang not found: Fortran shims not built")
         return None
     fdir = os.path.join(HERE, "fortran")
+    tdir = os.path.join(ROOT, "tests", "fortran")
     bdir = os.path.join(HERE, "_build", "fortran")
-    os.makedirs(bdir, exist_ok=True)
-    flags = ["-fdefault-real-8", "-O2", "-cpp", "-DTGX_NLONP4=76", "-J", bdir]
-    objs = []
-    for src in ("tgx_c.F90", "trsolv_shim.F90", "shim_driver.F90"):
-        obj = os.path.join(bdir, src + ".o")
-        cmd = [fc] + flags + ["-c", os.path.join(fdir, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        objs.append(obj)
+    tbdir = os.path.join(bdir, "test")
+    os.makedirs(tbdir, exist_ok=True)
+    base = ["-fdefault-real-8", "-O2", "-cpp"]
+    link = ["-L" + HERE, "-ltgx"]
+
+    def compile_(src, moddir, extra=()):
+        obj = os.path.join(moddir, os.path.basename(src) + ".o")
+        _fc([fc] + base + list(extra) + ["-J", moddir, "-I", bdir, "-c", src, "-o", obj], verbose)
+        return obj
+
+    common = [compile_(os.path.join(fdir, s), bdir) for s in ("tgx_c.F90", "tgx_ng_c.F90")]
+    objs = common + [compile_(os.path.join(fdir, s), bdir, ["-DTGX_NLONP4=76"])
+                     for s in ("trsolv_shim.F90", "shim_driver.F90")]
     exe = os.path.join(fdir, "shim_driver")
-    cmd = [fc, "-o", exe] + objs + ["-L" + HERE, "-ltgx", "-Wl,-rpath,$ORIGIN/.."]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
+    _fc([fc, "-o", exe] + objs + link + ["-Wl,-rpath,$ORIGIN/.."], verbose)
+    tobjs = [compile_(os.path.join(tdir, "model_modules.F90"), tbdir)]
+    tobjs += [compile_(os.path.join(fdir, s), tbdir) for s in ("trsolv_ng_shim.F90", "ng_shims.F90")]
+    tobjs += [compile_(os.path.join(tdir, "ng_driver.F90"), tbdir)]
+    texe = os.path.join(tdir, "ng_driver")
+    _fc([fc, "-o", texe] + tobjs + common + link + ["-Wl,-rpath,$ORIGIN/../../tiegcm3.0_ng_amd"], verbose)
     return exe
 
 

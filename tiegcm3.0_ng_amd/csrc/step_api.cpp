@@ -220,7 +220,7 @@ int run_chem(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
 int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                void *stream, bool chem_done);
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
-              void *stream, bool chem_done);
+              void *stream, bool chem_done, int nrun);
 int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
              void *stream, bool chem_done);
 }  // namespace
@@ -231,7 +231,16 @@ extern "C" int tgx_ng_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, cons
 }
 extern "C" int tgx_ng_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                             void *work, void *stream) {
-  return run_oplus(g, in, out, work, stream, false);
+  return run_oplus(g, in, out, work, stream, false, g ? g->nstep_sub : 0);
+}
+// nrun (1..nstep_sub) O+ sub-steps of a step whose matrix carries the full
+// nstep_sub time factor, with the lateral data blocks 0..nrun-1.  The host
+// context runs the reference's one-sub-step oplus_ng calls with nrun = 1.
+int tgx_ng_oplus_substeps(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                          void *stream, int nrun) {
+  if (g && (nrun < 1 || nrun > g->nstep_sub))
+    return tgx_set_error(TGX_ERR_ARG, "tgx_ng_oplus_substeps: nrun %d outside 1..nstep_sub (%d)", nrun, g->nstep_sub);
+  return run_oplus(g, in, out, work, stream, false, nrun);
 }
 extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                            void *work, void *stream) {
@@ -272,11 +281,11 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
 }
 
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
-              void *stream, bool chem_done) {
+              void *stream, bool chem_done, int nrun) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_OPLUS, s);
-  const int nsub = g->nstep_sub;
+  const int nsub = nrun;  // sub-steps run here (the time factor is g->nstep_sub)
   double *tmp_op = plane(work, g, W_OP_X), *tmp_nm = plane(work, g, W_OP_XNM);
   const double *op_in = in->op, *nm_in = in->op_nm;
   double *oploss = plane(work, g, W_OP_C), *opprod = plane(work, g, W_OP_C + 1);
@@ -500,7 +509,7 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
       return e == hipSuccess ? hipStreamWaitEvent(s0, F->ev[ev], 0) : e;
     }
   } join{F, s0};
-  if (int rc = run_oplus(g, in, out, work, sa, true)) return rc;    // stream a
+  if (int rc = run_oplus(g, in, out, work, sa, true, g->nstep_sub)) return rc;    // stream a
   if (int rc = run_minors(g, in, out, work, sb, true)) return rc;   // stream b
   if (int rc = run_comp(g, in, out, work, sc, true)) return rc;     // stream c
   if (int rc = tgx_ng_dt(g, in, out, work, s0)) return rc;       // caller's stream
@@ -516,7 +525,7 @@ extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
   if (g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
     if (int rc = tgx_ng_autotune(g, in, out, work, stream)) return rc;
   if (int rc = run_chem(g, in, out, work, (hipStream_t)stream)) return rc;
-  if (int rc = run_oplus(g, in, out, work, stream, true)) return rc;
+  if (int rc = run_oplus(g, in, out, work, stream, true, g->nstep_sub)) return rc;
   if (int rc = run_minors(g, in, out, work, stream, true)) return rc;
   if (int rc = tgx_ng_dt(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_duv(g, in, out, work, stream)) return rc;
