@@ -286,6 +286,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of oracle time per process for the CPU baseline")
     ap.add_argument("--only", default=None, help="time a single routine (profiling runs)")
+    ap.add_argument("--no-precision-variant", action="store_true",
+                    help="skip comp's FP32+refinement form")
+    ap.add_argument("--oplus-sub", type=int, default=10,
+                    help="also time the O+ routine at this many sub-cycles (1: skip)")
     ap.add_argument("--scheme", default="auto", choices=("thomas", "pcr", "auto"),
                     help="column solve: thread per column Thomas, wavefront per column PCR, or timed")
     ap.add_argument("--concurrent", action="store_true",
@@ -431,9 +435,88 @@ def main():
         "step_alg_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "serial_ms_per_step": round(serial_ms, 4),
     }
+    if args.only in (None, "comp") and not args.no_precision_variant:
+        out["precision_variant"] = bench_comp_precision(pkg, ns, cols * nk)
+    if not args.only and nsub == 1 and args.oplus_sub > 1:
+        out["oplus_nstep_sub%d" % args.oplus_sub] = bench_oplus_subcycled(pkg, args, ns)
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
+
+
+def bench_comp_precision(pkg, ns, colev, reps=10):
+    """comp, the step's dominant routine, in its precision form
+    (tgx_ng_comp_mixed: FP32 assembly and elimination, FP64 block pivots,
+    0, 1 or 2 FP64-residual refinement sweeps) beside the FP64 routine, each
+    timed alone on one stream; the error of each form is the largest
+    pointwise relative difference from the FP64 outputs over the whole grid,
+    with comp's floor 1e-9 (the reference's clamp)."""
+    import torch
+    fields = ("o2", "o1", "he", "o2_nm", "o1_nm", "he_nm")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        pkg.step.profile_begin(reps)
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        prof, kern = pkg.step.profile_end()
+        return (prof["comp"][0] / max(prof["comp"][1], 1), kern["comp"][0] / max(kern["comp"][1], 1))
+
+    ms64 = timed(ns.comp)
+    ref = {n: ns.out[n].clone() for n in fields}
+    res = {"routine": "comp (O2, O, He)", "unknowns": 3,
+           "fp64": {"routine_ms": round(ms64[0], 4), "column_kernels_ms": round(ms64[1], 4),
+                    "Mcolumn_level_solves_per_s": round(3 * colev / (ms64[0] * 1e-3) / 1e6, 1)}}
+    for nsweep in (0, 1, 2):
+        ms = timed(lambda: ns.comp_mixed(nsweep))
+        err = {}
+        for n in fields:
+            d = (ns.out[n] - ref[n]).abs() / torch.clamp(ref[n].abs(), min=1e-9)
+            err[n] = float(d.max())
+        res["fp32_refine%d" % nsweep] = {
+            "routine_ms": round(ms[0], 4), "column_kernels_ms": round(ms[1], 4),
+            "Mcolumn_level_solves_per_s": round(3 * colev / (ms[0] * 1e-3) / 1e6, 1),
+            "vs_fp64": round(ms64[0] / ms[0], 4), "max_rel_err": {n: float("%.3e" % e) for n, e in err.items()}}
+    ns.comp()  # leave the FP64 outputs in place
+    torch.cuda.synchronize()
+    res["timing"] = "hipEvents around each routine call, %d calls each, one stream" % reps
+    return res
+
+
+def bench_oplus_subcycled(pkg, args, ns1, reps=10):
+    """The O+ routine at the reference's default of ten sub-cycles
+    (dynamics_ng.F90:99-121) beside ten times the one-sub-cycle routine, each
+    called on its own (its chemistry included) on one stream: the sub-cycles
+    share one factorisation, so the later ones rebuild only the right-hand
+    side."""
+    import torch
+    nsub = args.oplus_sub
+
+    def timed(ns):
+        ns.autotune()  # the scheme choice a step of this shape would make
+        ns.oplus()
+        torch.cuda.synchronize()
+        pkg.step.profile_begin(reps * nsub)
+        for _ in range(reps):
+            ns.oplus()
+        torch.cuda.synchronize()
+        prof, _ = pkg.step.profile_end()
+        return prof["oplus"][0] / max(prof["oplus"][1], 1)
+
+    ms1 = timed(ns1)
+    g = pkg.synth.make_grid(args.config, nstep_sub=nsub)
+    F = pkg.synth.make_state(g, device="cuda")
+    ns = pkg.step.NestState(g, F, scheme=args.scheme)
+    del F
+    ms = timed(ns)
+    ns_scheme = ns.schemes()["oplus"]
+    del ns
+    torch.cuda.empty_cache()
+    return {"routine_ms": round(ms, 4), "nstep_sub1_routine_ms": round(ms1, 4), "scheme": ns_scheme,
+            "ratio_to_%dx_nstep_sub1" % nsub: round(ms / (nsub * ms1), 4), "reps": reps,
+            "timing": "hipEvents around each oplus routine call, one stream"}
 
 
 if __name__ == "__main__":

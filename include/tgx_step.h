@@ -112,6 +112,14 @@ int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
                void *stream);
 int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                 void *stream);
+/* The precision form of tgx_ng_comp (no counterpart in the reference): the
+ * 3x3 block rows assembled and eliminated in FP32 with the block pivots
+ * inverted in FP64, then nsweep (0..2) iterative-refinement sweeps, each
+ * forming the residual with the rows assembled again in FP64 and correcting
+ * with the FP32 elimination.  Same arguments and outputs as tgx_ng_comp; not
+ * for capture mode (g->capture must be NULL). */
+int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                      void *stream, int nsweep);
 /* The whole column step (dynamics_ng.F90:99-233).  The independent routines
  * (oplus, minors, dt, comp) run concurrently on library-owned streams forked
  * from and joined back into `stream`; duv follows dt and oplus.  Results are

@@ -69,8 +69,9 @@ def test_wavefront_tile_map_is_one_to_one_and_in_region(name, nv):
 # values per column and level the kernels keep in tiled regions: comp inputs
 # (7) and elimination (12); minor right-hand sides (4), ratios and forward
 # solutions (4), sources (4) and source coefficients (6); wind (6), TN (2) and
-# O+ (2) Thomas values; the O+ factorisation kept across sub-cycles (3)
-TILED_VALUES = 7 + 12 + 4 + 4 + 4 + 6 + 6 + 2 + 2 + 3
+# O+ (2) Thomas values; the O+ factorisation kept across sub-cycles (3);
+# comp's precision form: the solution between refinement sweeps (3)
+TILED_VALUES = 7 + 12 + 4 + 4 + 4 + 6 + 6 + 2 + 2 + 3 + 3
 
 
 @pytest.mark.parametrize("name", sorted(SHAPES))

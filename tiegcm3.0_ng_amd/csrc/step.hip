@@ -2290,6 +2290,60 @@ __global__ void __launch_bounds__(256) o2o_kernel(tgx_ng_grid g, Geo G, tgx_ng_f
   CT.fs0[1][p] = (pox1 - lox3) * RMO1 / xn;                                  // fs(k,2,0)
 }
 
+// comp_ng.F90 on a lateral-boundary column: the Dirichlet values of O2, O
+// and He (:391-413), then the post pass :415-457 on them.
+__device__ void comp_lateral_column(const tgx_ng_grid &g, const Geo &G, const tgx_ng_fields &in,
+                                    const tgx_ng_out &out, int i, int j) {
+  const int nk = G.nk;
+  const double small = 1.e-9, rdz = 1. / g.dz;
+  double v;
+  for (int k = 0; k < nk; ++k) {
+    size_t p = G.a3(k, i, j);
+    lateral_value(G, in.lonb_o2, in.latb_o2, k, i, j, v); out.o2[p] = v;
+    lateral_value(G, in.lonb_o1, in.latb_o1, k, i, j, v); out.o1[p] = v;
+    lateral_value(G, in.lonb_he, in.latb_he, k, i, j, v); out.he[p] = v;
+  }
+  bool found = false, extrap = false;
+  double n2top = 0., dn2dz = 0., zmtop = 0., n2prev = 0., xnprev = 0.;
+  for (int k = 0; k < nk; ++k) {
+    size_t p = G.a3(k, i, j);
+    const double o2u = out.o2[p], o1u = out.o1[p], heu = out.he[p], xn = in.xnmbar[p];
+    double o2nm = DTSMOOTH_DIV2 * (in.o2_nm[p] + o2u) + DTSMOOTH * in.o2[p];
+    double o1nm = DTSMOOTH_DIV2 * (in.o1_nm[p] + o1u) + DTSMOOTH * in.o1[p];
+    double henm = DTSMOOTH_DIV2 * (in.he_nm[p] + heu) + DTSMOOTH * in.he[p];
+    double n2 = 1. - o2u - o1u - heu;
+    double n2nm = 1. - o2nm - o1nm - henm;
+    if (!found) {
+      if (n2 < 0.01) {
+        found = true;
+        if (k + 1 < nk && k >= 1) {
+          extrap = true;
+          zmtop = g.zpmid[k];
+          n2top = log(n2 * xn);
+          double n2top1 = log(n2prev * xnprev);
+          dn2dz = (n2top - n2top1) * rdz;
+        }
+      }
+    } else if (extrap) {
+      n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / xn;
+    }
+    n2prev = n2;
+    xnprev = xn;
+    double a = o2u > small ? o2u : small, b = o1u > small ? o1u : small;
+    double c = heu > small ? heu : small, d = n2 > small ? n2 : small;
+    double norm = a + b + c + d;
+    out.o2[p] = a / norm;
+    out.o1[p] = b / norm;
+    out.he[p] = c / norm;
+    double an = o2nm > small ? o2nm : small, bn = o1nm > small ? o1nm : small;
+    double cn = henm > small ? henm : small, dn = n2nm > small ? n2nm : small;
+    double normn = an + bn + cn + dn;
+    out.o2_nm[p] = an / normn;
+    out.o1_nm[p] = bn / normn;
+    out.he_nm[p] = cn / normn;
+  }
+}
+
 template <bool CAP>
 __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
                                                    tgx_ng_out out, double *__restrict__ gs,
@@ -2305,13 +2359,10 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
   double v;
   const bool bnd = lateral_value(G, in.lonb_o2, in.latb_o2, 0, i, j, v);
   if (bnd) {
-    for (int k = 0; k < nk; ++k) {
-      size_t p = G.a3(k, i, j);
-      lateral_value(G, in.lonb_o2, in.latb_o2, k, i, j, v); out.o2[p] = v;
-      lateral_value(G, in.lonb_o1, in.latb_o1, k, i, j, v); out.o1[p] = v;
-      lateral_value(G, in.lonb_he, in.latb_he, k, i, j, v); out.he[p] = v;
-    }
-  } else {
+    comp_lateral_column(g, G, in, out, i, j);
+    return;
+  }
+  {
     const double dfac = dfactor_at(g, j), racs = g.racs[j];
     const double tl = in.t_lbc[c2];  // flds%tlbc after dt_ng of this step
     const double rmass[3] = {RMO2, RMO1, RMHE};
@@ -2401,12 +2452,19 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     // The elimination's gamma (9) and forward solution z (3) of level k, in
     // a tiled region of 12 values.
     auto tile = [&](int k, int n) { return G.tile(k, i, j, n, 12); };
+    // The loads of level k+1 are issued before the arithmetic of level k, so
+    // the memory round trip overlaps the elimination.
+    LR r2n;
+    LS lsn;
+    load_lr(1, r2n);
+    load_ls(0, lsn);
     for (int k = 0; k < nk - 1; ++k) {
-      // every load of this level is issued before any of its arithmetic, so
-      // the level costs one memory round trip
-      LS ls;
-      load_lr(k + 1, r2);
-      load_ls(k, ls);
+      const LS ls = lsn;
+      r2 = r2n;
+      if (k + 1 < nk - 1) {
+        load_lr(k + 2, r2n);
+        load_ls(k + 1, lsn);
+      }
       asm volatile("" ::: "memory");
       for (int n = 0; n < 9; ++n) akm[n] = akp[n];
       for (int n = 0; n < 3; ++n) epm[n] = epp[n];
@@ -2571,15 +2629,29 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       top_of(u1, ut);
       post(nk - 1, ut, bt);
       post(nk - 2, u1, bc);
-      bc = bn;
-      for (int k = nk - 3; k >= 0; --k) {
-        if (k > 0) load_bs(k - 1, bn);
+      // Levels nk-3 .. 0, each level's loads issued three levels ahead: four
+      // register slots rotate through a loop unrolled by four, so no slot
+      // is indexed at run time.
+      BS s0 = bn, s1, s2, s3;
+      int k = nk - 3;
+      if (k - 1 >= 0) load_bs(k - 1, s1);
+      if (k - 2 >= 0) load_bs(k - 2, s2);
+      auto it = [&](int kk, const BS &cur, BS &pf) {
+        if (kk - 3 >= 0) load_bs(kk - 3, pf);
         asm volatile("" ::: "memory");
-        back(bc, u1, u);
-        post(k, u, bc);
+        back(cur, u1, u);
+        post(kk, u, cur);
         for (int isp = 0; isp < 3; ++isp) u1[isp] = u[isp];
-        bc = bn;
+      };
+      for (; k >= 3; k -= 4) {
+        it(k, s0, s3);
+        it(k - 1, s1, s0);
+        it(k - 2, s2, s1);
+        it(k - 3, s3, s2);
       }
+      if (k >= 0) it(k, s0, s3);
+      if (k - 1 >= 0) it(k - 1, s1, s0);
+      if (k - 2 >= 0) it(k - 2, s2, s1);
     }
     if (kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
       const double zmtop = g.zpmid[kf];
@@ -2609,39 +2681,393 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     }
     return;
   }
-  // lateral-boundary columns: :415-457 on the Dirichlet values
-  bool found = false, extrap = false;
-  double n2top = 0., dn2dz = 0., zmtop = 0., n2prev = 0., xnprev = 0.;
-  for (int k = 0; k < nk; ++k) {
-    size_t p = G.a3(k, i, j);
-    const double o2u = out.o2[p], o1u = out.o1[p], heu = out.he[p], xn = in.xnmbar[p];
-    double o2nm = DTSMOOTH_DIV2 * (in.o2_nm[p] + o2u) + DTSMOOTH * in.o2[p];
-    double o1nm = DTSMOOTH_DIV2 * (in.o1_nm[p] + o1u) + DTSMOOTH * in.o1[p];
-    double henm = DTSMOOTH_DIV2 * (in.he_nm[p] + heu) + DTSMOOTH * in.he[p];
-    double n2 = 1. - o2u - o1u - heu;
-    double n2nm = 1. - o2nm - o1nm - henm;
-    if (!found) {
-      if (n2 < 0.01) {
-        found = true;
-        if (k + 1 < nk && k >= 1) {
-          extrap = true;
-          zmtop = g.zpmid[k];
-          n2top = log(n2 * xn);
-          double n2top1 = log(n2prev * xnprev);
-          dn2dz = (n2top - n2top1) * rdz;
-        }
-      }
-    } else if (extrap) {
-      n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / xn;
+#undef F3
+}
+
+// =============================================================================
+// The precision form of comp (comp_ng.F90:71-457): the block rows assembled
+// and eliminated in FP32, the 3x3 block pivots inverted in FP64, then
+// iterative refinement: each sweep forms the residual r = f - A u with the
+// rows assembled again in FP64 and solves A d = r with the FP32 elimination
+// of those rows, u += d.  The last sweep applies the post pass.  u (nk levels
+// of O2, O, He before the post pass, the top level from the closed form) is
+// kept in a tiled region (CM_NV values), the FP32 gamma and forward solution
+// in the elimination's tiled region (12 floats where the FP64 kernel keeps 12
+// doubles).
+
+template <class T> __device__ __forceinline__ T t_log(T x);
+template <> __device__ __forceinline__ float t_log<float>(float x) { return logf(x); }
+template <> __device__ __forceinline__ double t_log<double>(double x) { return log(x); }
+template <class T> __device__ __forceinline__ T t_exp(T x);
+template <> __device__ __forceinline__ float t_exp<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
+
+template <class T>
+__device__ __forceinline__ void matinv3_t(const T *A, T *Bm) {
+#define M3(r, c) A[((r) - 1) + 3 * ((c) - 1)]
+  T d = M3(1,1)*M3(2,2)*M3(3,3) - M3(1,1)*M3(2,3)*M3(3,2)
+      - M3(1,2)*M3(2,1)*M3(3,3) + M3(1,2)*M3(2,3)*M3(3,1)
+      + M3(1,3)*M3(2,1)*M3(3,2) - M3(1,3)*M3(2,2)*M3(3,1);
+  const T r = T(1) / d;
+  Bm[0] =  (M3(2,2)*M3(3,3) - M3(2,3)*M3(3,2)) * r;
+  Bm[1] = -(M3(2,1)*M3(3,3) - M3(2,3)*M3(3,1)) * r;
+  Bm[2] =  (M3(2,1)*M3(3,2) - M3(2,2)*M3(3,1)) * r;
+  Bm[3] = -(M3(1,2)*M3(3,3) - M3(1,3)*M3(3,2)) * r;
+  Bm[4] =  (M3(1,1)*M3(3,3) - M3(1,3)*M3(3,1)) * r;
+  Bm[5] = -(M3(1,1)*M3(3,2) - M3(1,2)*M3(3,1)) * r;
+  Bm[6] =  (M3(1,2)*M3(2,3) - M3(1,3)*M3(2,2)) * r;
+  Bm[7] = -(M3(1,1)*M3(2,3) - M3(1,3)*M3(2,1)) * r;
+  Bm[8] =  (M3(1,1)*M3(2,2) - M3(1,2)*M3(2,1)) * r;
+#undef M3
+}
+
+constexpr int CM_NV = 3;  // u: O2, O, He
+
+// The block rows of comp_ng.F90:168-326 for one column, level by level
+// (k = 0 .. nk-2 in order), in arithmetic type T: the same terms as
+// comp_kernel's forward sweep.
+template <class T>
+struct CompRows {
+  const tgx_ng_grid &g;
+  const Geo &G;
+  const tgx_ng_fields &in;
+  const CompTerms &CT;
+  int i, j;
+  T rdz, dz, dtx2inv, dfac, fhe, tl, thdiffalpha;
+  T ps0[3], embar0, o2_0, o1_0, he_0;
+  T r1[6];  // o2, o1, he, mbar, tn, w at the level being eliminated
+  T epp[3], akp[9], wks4, embari_c;
+  T alpha22 = 0, alpha33 = 0, alpha23 = 0, alpha32 = 0, flx00 = 0;
+  struct Lev { T o2i, o1i, hei, embari, tni, dembardz, dtndz, df[3], q; };
+
+  __device__ CompRows(const tgx_ng_grid &g_, const Geo &G_, const tgx_ng_fields &in_, const CompTerms &CT_,
+                      int i_, int j_)
+      : g(g_), G(G_), in(in_), CT(CT_), i(i_), j(j_) {}
+  __device__ T rmass(int l) const { return T(l == 0 ? RMO2 : (l == 1 ? RMO1 : RMHE)); }
+  __device__ void load_r(int L, T *r) const {
+    const size_t p = G.a3(L, i, j);
+    r[0] = T(in.o2[p]); r[1] = T(in.o1[p]); r[2] = T(in.he[p]);
+    r[3] = T(in.mbar[p]); r[4] = T(in.tn[p]); r[5] = T(in.w[p]);
+  }
+  __device__ void lev_df(Lev &Q) const {
+    Q.dembardz = Q.dembardz * rdz;
+    Q.dtndz = Q.dtndz * rdz;
+    const T L = t_log<T>(Q.tni / T(T00));
+    for (int n = 0; n < 3; ++n) Q.df[n] = t_exp<T>(T(1.75 - ssn(n)) * L);
+    Q.q = t_exp<T>(T(-0.25) * L);
+  }
+  __device__ void alpha_of(const Lev &Q, T *al) const {
+    const T *d_ = Q.df;
+#define PH(r, c) T(PHI(r, c))
+    al[0] = -(PH(1, 4) + (PH(1, 2) - PH(1, 4)) * Q.o1i + (d_[0] * PH(1, 3) - PH(1, 4)) * Q.hei);
+    al[4] = -(PH(2, 4) + (PH(2, 1) - PH(2, 4)) * Q.o2i + (d_[1] * PH(2, 3) - PH(2, 4)) * Q.hei);
+    al[8] = -(d_[2] * PH(3, 4) + (d_[0] * PH(3, 1) - d_[2] * PH(3, 4)) * Q.o2i +
+              (d_[1] * PH(3, 2) - d_[2] * PH(3, 4)) * Q.o1i);
+    al[3] = (PH(1, 2) - PH(1, 4)) * Q.o2i;
+    al[6] = (d_[0] * PH(1, 3) - PH(1, 4)) * Q.o2i;
+    al[1] = (PH(2, 1) - PH(2, 4)) * Q.o1i;
+    al[7] = (d_[1] * PH(2, 3) - PH(2, 4)) * Q.o1i;
+    al[2] = (d_[0] * PH(3, 1) - d_[2] * PH(3, 4)) * Q.hei;
+    al[5] = (d_[1] * PH(3, 2) - d_[2] * PH(3, 4)) * Q.hei;
+#undef PH
+  }
+  __device__ void init() {
+    rdz = T(1. / g.dz);
+    dz = T(g.dz);
+    dtx2inv = T(g.dtx2inv);
+    thdiffalpha = T(-0.38);
+    dfac = T(dfactor_at(g, j));
+    const size_t c2 = G.a2(i, j);
+    fhe = g.calc_helium == 0 ? T(0) : T(in.flx_he[c2]);
+    tl = T(in.t_lbc[c2]);  // flds%tlbc after dt_ng of this step
+    o2_0 = T(in.o2[G.a3(0, i, j)]); o1_0 = T(in.o1[G.a3(0, i, j)]); he_0 = T(in.he[G.a3(0, i, j)]);
+    for (int l = 0; l < 3; ++l)
+      ps0[l] = T(g.b[3 * l + 0]) * o2_0 + T(g.b[3 * l + 1]) * o1_0 + T(g.b[3 * l + 2]) * he_0 + T(g.fb[l]);
+    embar0 = T(1) / (ps0[0] * T(RIO2) + ps0[1] * T(RIO1) + ps0[2] * T(RIHE) +
+                     (T(1) - ps0[0] - ps0[1] - ps0[2]) * T(RIN2));
+    load_r(0, r1);
+    Lev Q;
+    Q.o2i = T(.5) * (ps0[0] + o2_0);
+    Q.o1i = T(.5) * (ps0[1] + o1_0);
+    Q.hei = T(.5) * (ps0[2] + he_0);
+    Q.embari = T(.5) * (embar0 + r1[3]);
+    Q.tni = tl;
+    Q.dembardz = r1[3] - embar0;
+    Q.dtndz = (r1[4] - tl) * T(2);
+    lev_df(Q);
+    wks4 = Q.dembardz / (Q.embari * T(2));
+    for (int l = 0; l < 3; ++l) epp[l] = T(1) - T(1) / Q.embari * (rmass(l) + Q.dembardz);
+    epp[2] = epp[2] - thdiffalpha * Q.dtndz / Q.tni;
+    T alpha[9];
+    alpha_of(Q, alpha);
+    matinv3_t<T>(alpha, akp);
+    const T wks1 = Q.embari * T(RIN2) * Q.q * T(1. / TAU);
+    for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
+    embari_c = Q.embari;
+  }
+  // rows of level k; the boundary conditions of the bottom (k = 0) and top
+  // (k = nk-2) rows are folded in as comp_kernel does
+  __device__ void rows(int k, T *pk, T *qk, T *rk, T *fk) {
+    const int nk = G.nk;
+    T r2[6];
+    load_r(k + 1, r2);
+    const size_t p = G.a3(k, i, j);
+    const size_t t = G.tile(k, i, j, 0, CI_NV);
+    const T rhs0 = T(CT.ci[t]), rhs1 = T(CT.ci[t + 64]), rhs2 = T(CT.ci[t + 128]);
+    const T fs01 = T(CT.ci[t + 192]), fs02 = T(CT.ci[t + 256]), fs11 = T(CT.ci[t + 320]), fs12 = T(CT.ci[t + 384]);
+    const T hdo2 = T(in.hdo2[p]), hdo1 = T(in.hdo1[p]), hdhe = T(in.hdhe[p]);
+    const T expzk = T(g.expz[k]), difk0 = T(g.difk[k]), difk1 = T(g.difk[k + 1]);
+    T akm[9], epm[3];
+    for (int n = 0; n < 9; ++n) akm[n] = akp[n];
+    for (int n = 0; n < 3; ++n) epm[n] = epp[n];
+    Lev Q;
+    Q.o2i = T(.5) * (r1[0] + r2[0]);
+    Q.o1i = T(.5) * (r1[1] + r2[1]);
+    Q.hei = T(.5) * (r1[2] + r2[2]);
+    Q.embari = T(.5) * (r1[3] + r2[3]);
+    Q.tni = T(.5) * (r1[4] + r2[4]);
+    Q.dembardz = r2[3] - r1[3];
+    Q.dtndz = r2[4] - r1[4];
+    lev_df(Q);
+    const T rem = T(1) / Q.embari;
+    for (int l = 0; l < 3; ++l) epp[l] = T(1) - rem * (rmass(l) + Q.dembardz);
+    if (k != nk - 2) epp[2] = epp[2] - thdiffalpha * Q.dtndz / Q.tni;
+    {
+      T alpha[9];
+      alpha_of(Q, alpha);
+      matinv3_t<T>(alpha, akp);
     }
-    n2prev = n2;
-    xnprev = xn;
+    const T wks1 = Q.embari * T(RIN2) * Q.q * T(1. / TAU);
+    const T wks3 = wks4;
+    wks4 = Q.dembardz / (Q.embari * T(2));
+    for (int n = 0; n < 9; ++n) akp[n] = akp[n] * wks1;
+    const T eddyp = T(g.expzmid_inv) * difk0 * (rdz - wks3);
+    const T eddyr = T(g.expzmid) * difk1 * (rdz + wks4);
+    const T eddyq = T(g.expzmid_inv) * difk0 * (rdz + wks3) + T(g.expzmid) * difk1 * (rdz - wks4);
+    const T wik = T(.5) * (r1[5] + r2[5]);
+    const T fs[2][3] = {{T(0), fs01, fs02}, {T(0), fs11, fs12}};
+    for (int m = 0; m < 3; ++m)
+      for (int isp = 0; isp < 3; ++isp) {
+        const int e = isp + 3 * m;
+        const T molp = akm[e] * (rdz + epm[m] / T(2));
+        const T molr = akp[e] * (rdz - epp[m] / T(2));
+        const T molq = akm[e] * (rdz - epm[m] / T(2)) + akp[e] * (rdz + epp[m] / T(2));
+        const T delta = isp == m ? T(1) : T(0);
+        const T fsv = (isp < 2 && m < 2) ? fs[isp][m + 1] : T(0);
+        pk[e] = molp * rdz - expzk * delta * rdz * (eddyp * dfac + T(.5) * wik);
+        rk[e] = molr * rdz - expzk * delta * rdz * (eddyr * dfac - T(.5) * wik);
+        qk[e] = -molq * rdz + expzk * delta * (eddyq * dfac * rdz + dtx2inv) - expzk * fsv;
+      }
+    fk[0] = expzk * (rhs0 + hdo2);
+    fk[1] = expzk * (rhs1 + hdo1);
+    fk[2] = expzk * (rhs2 + hdhe);
+    if (k == 0) {
+      for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n)
+          for (int isp = 0; isp < 3; ++isp)
+            qk[isp + 3 * m] = qk[isp + 3 * m] + pk[isp + 3 * n] * T(g.b[3 * n + m]);
+      for (int m = 0; m < 3; ++m)
+        for (int isp = 0; isp < 3; ++isp) fk[isp] = fk[isp] - pk[isp + 3 * m] * T(g.fb[m]);
+      for (int n = 0; n < 9; ++n) pk[n] = T(0);
+    } else if (k == nk - 2) {
+      for (int m = 0; m < 3; ++m)
+        for (int isp = 0; isp < 3; ++isp)
+          qk[isp + 3 * m] = qk[isp + 3 * m] + (T(1) + T(.5) * epp[m] * dz) / (T(1) - T(.5) * epp[m] * dz) *
+                                                  rk[isp + 3 * m];
+      const T *d_ = Q.df;
+#define PH(r, c) T(PHI(r, c))
+      alpha22 = -(PH(2, 4) + (PH(2, 1) - PH(2, 4)) * Q.o2i + (d_[1] * PH(2, 3) - PH(2, 4)) * Q.hei);
+      alpha33 = -(d_[2] * PH(3, 4) + (d_[0] * PH(3, 1) - d_[2] * PH(3, 4)) * Q.o2i +
+                  (d_[1] * PH(3, 2) - d_[2] * PH(3, 4)) * Q.o1i);
+      alpha23 = (d_[1] * PH(2, 3) - PH(2, 4)) * Q.o1i;
+      alpha32 = (d_[1] * PH(3, 2) - d_[2] * PH(3, 4)) * Q.hei;
+#undef PH
+      flx00 = embari_c * T(RIN2) * T(P0) * t_exp<T>(T(0.25) * t_log<T>(T(T00) / r1[4])) / T(TAU * GRAV);
+      for (int isp = 0; isp < 3; ++isp)
+        fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (rdz - T(0.5) * epp[1])) -
+                  rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (rdz - T(0.5) * epp[2]));
+      for (int n = 0; n < 9; ++n) rk[n] = T(0);
+    }
+    embari_c = Q.embari;
+    for (int n = 0; n < 6; ++n) r1[n] = r2[n];
+  }
+  // :317-332, the top level from the solution at nk-2
+  __device__ void top_of(const double *u, double *t) const {
+    for (int isp = 0; isp < 3; ++isp)
+      t[isp] = double((T(1) + T(.5) * epp[isp] * dz) / (T(1) - T(.5) * epp[isp] * dz)) * u[isp];
+    t[1] = t[1] + double((alpha23 - alpha22) * fhe / (flx00 * (rdz - T(0.5) * epp[1])));
+    t[2] = t[2] + double((alpha33 - alpha32) * fhe / (flx00 * (rdz - T(0.5) * epp[2])));
+  }
+};
+
+// One level of the FP32 block elimination with an FP64 pivot inverse:
+// gamma(k) = W r_k, z(k) = W (f_k - p_k z(k-1)), W = (q_k - p_k gamma(k-1))^-1.
+__device__ __forceinline__ void comp_elim32(const float *pk, float *qk, const float *rk, const float *fk,
+                                            float *gama, float *zz) {
+  for (int m = 0; m < 3; ++m)
+    for (int n = 0; n < 3; ++n)
+      for (int isp = 0; isp < 3; ++isp) qk[isp + 3 * m] = qk[isp + 3 * m] - pk[isp + 3 * n] * gama[n + 3 * m];
+  double q64[9], w[9];
+  for (int n = 0; n < 9; ++n) q64[n] = qk[n];
+  matinv3_t<double>(q64, w);
+  float wkv[3];
+  for (int isp = 0; isp < 3; ++isp) wkv[isp] = fk[isp];
+  for (int m = 0; m < 3; ++m)
+    for (int isp = 0; isp < 3; ++isp) wkv[isp] = wkv[isp] - pk[isp + 3 * m] * zz[m];
+  for (int m = 0; m < 3; ++m)
+    for (int isp = 0; isp < 3; ++isp) {
+      double a = 0.;
+      for (int n = 0; n < 3; ++n) a = a + w[isp + 3 * n] * (double)rk[n + 3 * m];
+      gama[isp + 3 * m] = (float)a;
+    }
+  for (int isp = 0; isp < 3; ++isp) {
+    double a = 0.;
+    for (int m = 0; m < 3; ++m) a = a + w[isp + 3 * m] * (double)wkv[m];
+    zz[isp] = (float)a;
+  }
+}
+
+// Pass 0: FP32 assembly and elimination, u to the tiled region (FP64).
+__global__ void __launch_bounds__(256) comp_mixed_solve(tgx_ng_grid g, Geo G, tgx_ng_fields in, float *gs,
+                                                        double *U, CompTerms CT) {
+  int i, j;
+  if (!column_of(G, i, j)) return;
+  double v;
+  if (lateral_value(G, in.lonb_o2, in.latb_o2, 0, i, j, v)) return;
+  const int nk = G.nk;
+  CompRows<float> R(g, G, in, CT, i, j);
+  R.init();
+  float gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
+  for (int k = 0; k < nk - 1; ++k) {
+    float pk[9], qk[9], rk[9], fk[3];
+    R.rows(k, pk, qk, rk, fk);
+    comp_elim32(pk, qk, rk, fk, gama, zz);
+    for (int n = 0; n < 9; ++n) gs[G.tile(k, i, j, n, 12)] = gama[n];
+    for (int n = 0; n < 3; ++n) gs[G.tile(k, i, j, 9 + n, 12)] = zz[n];
+  }
+  float u1[3] = {zz[0], zz[1], zz[2]};
+  double ud[3] = {u1[0], u1[1], u1[2]}, t[3];
+  R.top_of(ud, t);
+  for (int n = 0; n < 3; ++n) {
+    U[G.tile(nk - 1, i, j, n, CM_NV)] = t[n];
+    U[G.tile(nk - 2, i, j, n, CM_NV)] = ud[n];
+  }
+  for (int k = nk - 3; k >= 0; --k) {
+    float gm[9], z[3], u[3];
+    for (int n = 0; n < 9; ++n) gm[n] = gs[G.tile(k, i, j, n, 12)];
+    for (int n = 0; n < 3; ++n) z[n] = gs[G.tile(k, i, j, 9 + n, 12)];
+    for (int isp = 0; isp < 3; ++isp) {
+      float uu = z[isp];
+      for (int m = 0; m < 3; ++m) uu = uu - gm[isp + 3 * m] * u1[m];
+      u[isp] = uu;
+    }
+    for (int n = 0; n < 3; ++n) {
+      U[G.tile(k, i, j, n, CM_NV)] = (double)u[n];
+      u1[n] = u[n];
+    }
+  }
+}
+
+// A refinement sweep (REFINE) and/or the post pass :386-457 (FINAL).
+template <bool REFINE, bool FINAL>
+__global__ void __launch_bounds__(256) comp_mixed_refine(tgx_ng_grid g, Geo G, tgx_ng_fields in, tgx_ng_out out,
+                                                         float *gs, double *U, CompTerms CT) {
+  int i, j;
+  if (!column_of(G, i, j)) return;
+  const int nk = G.nk;
+  const size_t c2 = G.a2(i, j);
+  if (FINAL) out.flx_he[c2] = g.calc_helium == 0 ? 0. : in.flx_he[c2];
+  double v;
+  if (lateral_value(G, in.lonb_o2, in.latb_o2, 0, i, j, v)) {
+    if (FINAL) comp_lateral_column(g, G, in, out, i, j);
+    return;
+  }
+  auto uat = [&](int k, int n) -> double & { return U[G.tile(k, i, j, n, CM_NV)]; };
+  if (REFINE) {
+    CompRows<double> R(g, G, in, CT, i, j);
+    R.init();
+    float gama[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, zz[3] = {0, 0, 0};
+    double um[3] = {0, 0, 0}, u0[3], up[3];
+    for (int n = 0; n < 3; ++n) u0[n] = uat(0, n);
+    for (int k = 0; k < nk - 1; ++k) {
+      for (int n = 0; n < 3; ++n) up[n] = k + 1 <= nk - 2 ? uat(k + 1, n) : 0.;
+      double pk[9], qk[9], rk[9], fk[3];
+      R.rows(k, pk, qk, rk, fk);
+      // r = f - (p u(k-1) + q u(k) + r u(k+1)) in FP64
+      float p32[9], q32[9], r32[9], f32[3];
+      for (int isp = 0; isp < 3; ++isp) {
+        double a = fk[isp];
+        for (int m = 0; m < 3; ++m)
+          a = a - (pk[isp + 3 * m] * um[m] + qk[isp + 3 * m] * u0[m] + rk[isp + 3 * m] * up[m]);
+        f32[isp] = (float)a;
+      }
+      for (int n = 0; n < 9; ++n) {
+        p32[n] = (float)pk[n];
+        q32[n] = (float)qk[n];
+        r32[n] = (float)rk[n];
+      }
+      comp_elim32(p32, q32, r32, f32, gama, zz);
+      for (int n = 0; n < 9; ++n) gs[G.tile(k, i, j, n, 12)] = gama[n];
+      for (int n = 0; n < 3; ++n) gs[G.tile(k, i, j, 9 + n, 12)] = zz[n];
+      for (int n = 0; n < 3; ++n) {
+        um[n] = u0[n];
+        u0[n] = up[n];
+      }
+    }
+    // correction, top down, u += d; the top level again from the closed form
+    float d1[3] = {zz[0], zz[1], zz[2]};
+    double un[3], t[3];
+    for (int n = 0; n < 3; ++n) un[n] = uat(nk - 2, n) + (double)d1[n];
+    R.top_of(un, t);
+    for (int n = 0; n < 3; ++n) {
+      uat(nk - 2, n) = un[n];
+      uat(nk - 1, n) = t[n];
+    }
+    for (int k = nk - 3; k >= 0; --k) {
+      float gm[9], z[3], d[3];
+      for (int n = 0; n < 9; ++n) gm[n] = gs[G.tile(k, i, j, n, 12)];
+      for (int n = 0; n < 3; ++n) z[n] = gs[G.tile(k, i, j, 9 + n, 12)];
+      for (int isp = 0; isp < 3; ++isp) {
+        float dd = z[isp];
+        for (int m = 0; m < 3; ++m) dd = dd - gm[isp + 3 * m] * d1[m];
+        d[isp] = dd;
+      }
+      for (int n = 0; n < 3; ++n) {
+        uat(k, n) = uat(k, n) + (double)d[n];
+        d1[n] = d[n];
+      }
+    }
+  }
+  if (!FINAL) return;
+  // :386-457 on u: time smoothing, N2 above the O-dominated top, clamps and
+  // normalisation, as comp_kernel's post pass
+  const double small = 1.e-9, rdz = 1. / g.dz;
+  auto put_norm = [&](size_t p, double o2u, double o1u, double heu, double n2) {
     double a = o2u > small ? o2u : small, b = o1u > small ? o1u : small;
     double c = heu > small ? heu : small, d = n2 > small ? n2 : small;
     double norm = a + b + c + d;
     out.o2[p] = a / norm;
     out.o1[p] = b / norm;
     out.he[p] = c / norm;
+  };
+  int kf = -1;
+  double n2kf = 0., xnkf = 0., n2b = 0., xnb = 0.;
+  for (int k = nk - 1; k >= 0; --k) {
+    const size_t p = G.a3(k, i, j);
+    const double o2u = uat(k, 0), o1u = uat(k, 1), heu = g.calc_helium == 0 ? g.pshelb : uat(k, 2);
+    double o2nm = DTSMOOTH_DIV2 * (in.o2_nm[p] + o2u) + DTSMOOTH * in.o2[p];
+    double o1nm = DTSMOOTH_DIV2 * (in.o1_nm[p] + o1u) + DTSMOOTH * in.o1[p];
+    double henm = DTSMOOTH_DIV2 * (in.he_nm[p] + heu) + DTSMOOTH * in.he[p];
+    double n2 = 1. - o2u - o1u - heu;
+    double n2nm = 1. - o2nm - o1nm - henm;
+    const double xn = in.xnmbar[p];
+    const bool lo = n2 < 0.01, below = !lo && kf == k + 1;
+    n2kf = lo ? n2 : n2kf;
+    xnkf = lo ? xn : xnkf;
+    kf = lo ? k : kf;
+    n2b = below ? n2 : n2b;
+    xnb = below ? xn : xnb;
+    put_norm(p, o2u, o1u, heu, n2);
     double an = o2nm > small ? o2nm : small, bn = o1nm > small ? o1nm : small;
     double cn = henm > small ? henm : small, dn = n2nm > small ? n2nm : small;
     double normn = an + bn + cn + dn;
@@ -2649,7 +3075,17 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     out.o1_nm[p] = bn / normn;
     out.he_nm[p] = cn / normn;
   }
-#undef F3
+  if (kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
+    const double zmtop = g.zpmid[kf];
+    const double n2top = log(n2kf * xnkf);
+    const double n2top1 = log(n2b * xnb);
+    const double dn2dz = (n2top - n2top1) * rdz;
+    for (int k = nk - 1; k > kf; --k) {
+      const size_t p = G.a3(k, i, j);
+      const double n2 = exp(n2top + dn2dz * (g.zpmid[k] - zmtop)) / in.xnmbar[p];
+      put_norm(p, uat(k, 0), uat(k, 1), g.calc_helium == 0 ? g.pshelb : uat(k, 2), n2);
+    }
+  }
 }
 
 // comp_ng.F90:71-457 with one wavefront per column: the 3x3 block rows
@@ -4327,6 +4763,23 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
         dim3 grido((unsigned)((ncol * (G.nk - 1) + 255) / 256));
         hipLaunchKernelGGL(o2o_kernel, grido, block, 0, s, g, G, in, CT);
       }
+      break;
+    }
+    case 34:
+    case 35:
+    case 36:
+    case 37: {  // comp precision form: bufs = {elimination tiles (12 floats), comp input tiles (7),
+                // fs(:,1,0), fs(:,2,0), u tiles (CM_NV)}
+      CompTerms CT{D(1), {D(2), D(3)}};
+      float *gs = (float *)D(0);
+      if (which == 34)
+        hipLaunchKernelGGL(comp_mixed_solve, grid, block, 0, s, g, G, in, gs, D(4), CT);
+      else if (which == 35)
+        hipLaunchKernelGGL((comp_mixed_refine<true, false>), grid, block, 0, s, g, G, in, out, gs, D(4), CT);
+      else if (which == 36)
+        hipLaunchKernelGGL((comp_mixed_refine<true, true>), grid, block, 0, s, g, G, in, out, gs, D(4), CT);
+      else
+        hipLaunchKernelGGL((comp_mixed_refine<false, true>), grid, block, 0, s, g, G, in, out, gs, D(4), CT);
       break;
     }
     case 10: {  // dt prelude: bufs = {w2, hadv_tn, smooth_tn_nm}

@@ -39,7 +39,8 @@ enum {
   T_MS = T_OW + 2,  // minor species: loss and scaled production of NO or N2D, and of N4S
   T_ML = T_MS + 4,  // minor species: NO / N4S source coefficients linear in N2D
   T_OF = T_ML + 6,  // O+: factorisation kept across the sub-cycles of a step
-  T_N = T_OF + 3
+  T_CU = T_OF + 3,  // comp precision form: u (O2, O, He) between refinement sweeps
+  T_N = T_CU + 3
 };
 
 size_t plane_elems(const tgx_ng_grid *g) { return (size_t)g->nk * (size_t)g->ld * (size_t)(g->nlat + 4); }
@@ -152,7 +153,8 @@ extern "C" int tgx_ng_profile_end(double *ms_total, int *calls) {
 enum { K_DT = 0, K_MIN_A, K_MIN_B, K_OP1, K_OP2, K_OP3, K_UV1, K_UV2, K_COMP, K_OP_PRE,
        K_DT_PRE, K_MIN_PRE, K_CO_PRE, K_UV_PRE, K_O2O, K_MIN_SRC_A, K_MIN_SRC_B, K_OP_CHEM,
        K_MIN_WA, K_MIN_WB, K_DT_W, K_OP3_W, K_UV2_W, K_COMP_W, K_DT_TILE, K_CHEM, K_MIN_LIN,
-       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE, K_MIN_WB_LIN, K_OP1_LITE, K_OP_TILE_KEEP, K_OP_TILE_REUSE };
+       K_MIN_B_LIN, K_UV_TILE, K_OP_TILE, K_MIN_WB_LIN, K_OP1_LITE, K_OP_TILE_KEEP, K_OP_TILE_REUSE,
+       K_COMP_M0, K_COMP_M_REFINE, K_COMP_M_LAST, K_COMP_M_POST };
 
 namespace {
 // Solve scheme of each routine.  The wavefront-per-column cyclic reduction
@@ -329,6 +331,32 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
     nm_in = nm_o;
   }
   return 0;
+}
+
+// The precision form of comp: FP32 assembly and elimination with FP64 block
+// pivots, then nsweep (0..2) refinement sweeps with FP64 residuals; the last
+// kernel applies the post pass.  Same inputs, outputs and workspace as
+// tgx_ng_comp.
+extern "C" int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                                 void *work, void *stream, int nsweep) {
+  if (int rc = check(g, work)) return rc;
+  if (nsweep < 0 || nsweep > 2) return tgx_set_error(TGX_ERR_ARG, "tgx_ng_comp_mixed: nsweep %d outside 0..2", nsweep);
+  if (g->capture) return tgx_set_error(TGX_ERR_ARG, "tgx_ng_comp_mixed: no capture mode");
+  hipStream_t s = (hipStream_t)stream;
+  ProfScope prof(R_COMP, s);
+  void *b[5];
+  b[0] = tiles(work, g, T_CE);  // FP32 gamma and forward solution (12 floats per level)
+  b[1] = tiles(work, g, T_CI);
+  b[2] = plane(work, g, W_CO + 0);
+  b[3] = plane(work, g, W_CO + 1);
+  b[4] = tiles(work, g, T_CU);
+  if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;
+  if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
+  ProfScope kern(R_N + R_COMP, s);
+  if (int rc = tgx_ng_launch(K_COMP_M0, g, in, out, b, 0, s)) return rc;
+  for (int sw = 0; sw + 1 < nsweep; ++sw)
+    if (int rc = tgx_ng_launch(K_COMP_M_REFINE, g, in, out, b, 0, s)) return rc;
+  return tgx_ng_launch(nsweep > 0 ? K_COMP_M_LAST : K_COMP_M_POST, g, in, out, b, 0, s);
 }
 
 extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,

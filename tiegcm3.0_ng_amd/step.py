@@ -66,6 +66,7 @@ _SIGS = {
     "tgx_ng_dt": ([_P] * 5, C.c_int),
     "tgx_ng_duv": ([_P] * 5, C.c_int),
     "tgx_ng_comp": ([_P] * 5, C.c_int),
+    "tgx_ng_comp_mixed": ([_P] * 5 + [C.c_int], C.c_int),
     "tgx_ng_step": ([_P] * 5, C.c_int),
     "tgx_ng_step_serial": ([_P] * 5, C.c_int),
     "tgx_ng_profile_begin": ([C.c_int], C.c_int),
@@ -254,6 +255,15 @@ class NestState:
 
     def comp(self, stream=None):
         self._call("tgx_ng_comp", stream)
+
+    def comp_mixed(self, nsweep, stream=None):
+        """comp's precision form: FP32 assembly and elimination with FP64 block
+        pivots and nsweep (0..2) FP64-residual refinement sweeps."""
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = _lib().tgx_ng_comp_mixed(C.byref(self.g), C.byref(self.f), C.byref(self.o),
+                                      C.c_void_p(self.work.data_ptr()), C.c_void_p(s), C.c_int(int(nsweep)))
+        if rc != 0:
+            raise _tgx.TgxError(_tgx.last_error())
 
     def step(self, stream=None):
         self._call("tgx_ng_step", stream)
