@@ -435,6 +435,8 @@ def main():
         "step_alg_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "serial_ms_per_step": round(serial_ms, 4),
     }
+    if not args.only:
+        out["aux_kernels"] = bench_aux_kernels(pkg, g, ns)
     if args.only in (None, "comp") and not args.no_precision_variant:
         out["precision_variant"] = bench_comp_precision(pkg, ns, cols * nk)
     if not args.only and nsub == 1 and args.oplus_sub > 1:
@@ -442,6 +444,52 @@ def main():
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
+
+
+def bench_aux_kernels(pkg, g, ns, reps=20):
+    """The kernels beside the column step at this grid: the conversion of a
+    model-layout field (nk, lond, latd) to level planes and back, and the
+    polar ring filter (ringfilter.F, order 3 as for TN at dt.F:486) over
+    every latitude the chunk table filters.  HBM bytes are one read and one
+    write of what each launch touches."""
+    import math
+    import torch
+    nk, nlon, nlat = g["nk"], g["nlon"], g["nlat"]
+    nlond, nlatd = nlon + 4, nlat + 4
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    res = {}
+    src = torch.randn(nlatd, nlond, nk, dtype=torch.float64, device="cuda")
+    planes = torch.zeros(nk, nlatd, ns.ld, dtype=torch.float64, device="cuda")
+    back = torch.empty_like(src)
+    nbytes = 2 * 8 * nk * nlond * nlatd
+    for name, fn in (("pack_levels", lambda: pkg.tgx.pack_levels(src, planes, nk, nlond, nlatd)),
+                     ("unpack_levels", lambda: pkg.tgx.unpack_levels(planes, back, nk, nlond, nlatd))):
+        ms = timed(fn)
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    assert torch.equal(back, src)
+    del src, planes, back
+    chunk = pkg.tgx.RING_CHUNKS.get((nlon, nlat))
+    if chunk is not None:
+        f = ns.out["tn"].clone()
+        rows = 2 * len(chunk)
+        ms = timed(lambda: pkg.tgx.ringfilter_dev(f, nlon, nlat, 3, 0, 2 * math.pi / nlon, chunk))
+        rbytes = 2 * 8 * rows * nk * (nlon + 4)
+        res["ringfilter_tn"] = {"ms": round(ms, 4), "rows": rows, "levels": nk,
+                                "GBps": round(rbytes / (ms * 1e-3) / 1e9, 1)}
+        del f
+    res["timing"] = "torch.cuda events around %d launches each" % reps
+    return res
 
 
 def bench_comp_precision(pkg, ns, colev, reps=10):

@@ -40,6 +40,8 @@ _SIGS = {
     "tgx_blktri3_dev_f64": ([_D] * 5 + [_I] * 4 + [_D], _I),
     "tgx_ringfilter_f64": ([_D, _I, _I, _I, C.c_char_p, _I, _I, _I, C.c_double, _I, _D], None),
     "tgx_ringfilter_dev": ([_D] + [_I] * 6 + [C.c_double, _I, _D, _D], _I),
+    "tgx_ng_pack_levels": ([_D, _D] + [_I] * 4 + [_D], _I),
+    "tgx_ng_unpack_levels": ([_D, _D] + [_I] * 4 + [_D], _I),
 }
 
 
@@ -183,6 +185,31 @@ def blktri2_dev(a, b, c, f, x, nrow, stream=None):
     _check(lib().tgx_blktri2_dev_f64(_dp(a), _dp(b), _dp(c), _dp(f), _dp(x), ldk, nrow, 0, ncol,
                                      _stream(stream)))
     return x
+
+
+# The ring filter's chunk counts per latitude from each pole, by model
+# resolution (nlon, nlat): cons_module's chunk_array (cons.F:331-355), which
+# the model passes in; the filter takes any such table.
+RING_CHUNKS = {
+    (72, 36): [3, 9, 18, 36, 36],
+    (144, 72): [3, 3, 9, 9, 18, 18, 36, 36, 72, 72],
+    (288, 144): [3, 3, 3, 9, 9, 9, 18, 18, 18, 36, 36, 36, 72, 72, 72, 72, 144, 144, 144, 144, 144],
+    (576, 288): [3, 3, 3, 3, 3, 9, 9, 9, 9, 9, 18, 18, 18, 18, 18, 18, 36, 36, 36, 36, 36, 36,
+                 72, 72, 72, 72, 72, 72, 144, 144, 144, 144, 144, 144, 288, 288, 288, 288, 288, 288],
+}
+
+
+def pack_levels(src, dst, nk, nlond, nlatd, stream=None):
+    """Model-layout device array f(nk, nlond, nlatd) (level fastest; a torch
+    tensor (nlatd, nlond, nk)) into level planes dst (nk, nlatd, ld)."""
+    _check(lib().tgx_ng_pack_levels(_dp(src), _dp(dst), nk, nlond, nlatd, dst.shape[-1], _stream(stream)))
+    return dst
+
+
+def unpack_levels(src, dst, nk, nlond, nlatd, stream=None):
+    """Level planes src (nk, nlatd, ld) into the model layout dst (nlatd, nlond, nk)."""
+    _check(lib().tgx_ng_unpack_levels(_dp(src), _dp(dst), nk, nlond, nlatd, src.shape[-1], _stream(stream)))
+    return dst
 
 
 def ringfilter_dev(f, nlon, nlat, order, oplus, dlamda, chunk, stream=None):
