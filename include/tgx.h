@@ -98,10 +98,11 @@ void tgx_ringfilter_f64(double *f, int lev0, int lev1, int order, const char *na
  * tgx_step.h (element (k, lat row j, lon i) at k*ld*(nlat+4) + j*ld + i,
  * j = latitude + 1, i = longitude + 1 with the periodic points at i = 0, 1,
  * nlon+2, nlon+3): every latitude the chunk table chunk[nlat_filter]
- * (cons.F:331-355) filters, all nk levels, in one launch.  order <= 8,
+ * (cons.F:331-355) filters, all nk levels, in one launch.  order <= TGX_RING_MAX_ORDER,
  * nlon <= TGX_RING_MAX_NLON, nlat_filter <= TGX_RING_MAX_ROWS/2. */
 #define TGX_RING_MAX_ROWS 128
 #define TGX_RING_MAX_NLON 2048
+#define TGX_RING_MAX_ORDER 8
 int tgx_ringfilter_dev(double *f, int nk, int nlon, int nlat, int ld, int order, int oplus, double dlamda,
                        int nlat_filter, const int *chunk, void *stream);
 /* Column j of the batch starts at ptr + (col0+j)*ldk (scalars) and rows

@@ -1,12 +1,15 @@
 ! Test stand-ins for the model modules that the Fortran drop-ins read
 ! (params_module, fields_ng_module, init_module, input_module, cons_module,
-! lbc, ar_module, char_module) and the model's shutdown.  They hold only the
+! lbc, ar_module, char_module) and the model's shutdown.  The global-grid
+! members the ringfilter drop-in reads (nlon, nlat, dlamda, the chunk table)
+! are variables here, parameters or allocated in the model.  They hold only the
 ! members the drop-ins and the test driver use, under the model's names and
 ! shapes (fields_ng.F90:12-60 and :163-166, advance.F:219-222), so that
 ! tiegcm3.0_ng_amd/fortran/ng_shims.F90 and trsolv_ng_shim.F90 compile and
 ! run here exactly as they would inside the model build.
 module params_module
   implicit none
+  integer :: nlon = 0, nlonp4 = 0, nlat = 0   ! global grid (ringfilter)
   integer, parameter :: mx_ng = 5
   integer :: n_ng = 1
   integer, dimension(mx_ng) :: nlevp1_ng = 0, nlat_ng = 0, nlon_ng = 0
@@ -59,6 +62,9 @@ end module input_module
 module cons_module
   implicit none
   integer, parameter :: difhor = 1
+  real :: dlamda = 0.                        ! global grid (ringfilter)
+  integer :: nlat_filter = 0
+  integer, dimension(:), allocatable :: chunk_array
 end module cons_module
 
 module lbc

@@ -4,9 +4,10 @@
 ! (the O+ sub-cycles, the four minor species, dt_ng, duv_ng, comp_ng) and
 ! writes every result field back.  Also checks trsolv_ng's drop-in on the
 ! dt-shaped system of the same grid (mode 2 of the directory's run.txt);
-! mode 3 is mode 1 with the nest's west side not a domain edge.
+! mode 3 is mode 1 with the nest's west side not a domain edge; mode 4 runs
+! the ringfilter drop-in on every latitude of a global field.
 program ng_driver
-  use params_module
+  use params_module, only: mx_ng, n_ng, nlevp1_ng, nlon_ng, nlat_ng, zpint_ng, zpmid_ng, glat_ng
   use fields_ng_module
   use init_module, only: iday
   use input_module, only: nstep_sub, calc_helium, colfac, opdiffcap
@@ -52,6 +53,10 @@ program ng_driver
   flds(1)%is_bndry = .true.
   if (mode == 3) flds(1)%is_bndry(1) = .false.   ! a nest split over tasks: refused
 
+  if (mode == 4) then
+    call ring_rows()
+    stop
+  endif
   if (mode == 2) then
     ! trsolv_ng on a system of this grid's shape, x written back
     allocate(a(nk, -1:nlon + 2, -1:nlat + 2), bb(nk, -1:nlon + 2, -1:nlat + 2), c(nk, -1:nlon + 2, -1:nlat + 2))
@@ -175,6 +180,35 @@ program ng_driver
   call wr('flx_he', flds(1)%flx_he, size(flds(1)%flx_he))
 
 contains
+
+  ! mode 4: the ringfilter drop-in on every latitude of rows.bin, f(nlon+4,
+  ! nk, nlat) (run.txt: nstep_sub = order, calc_helium = 1 for 'OPLUS',
+  ! dlamda; chunk.txt: the chunk table)
+  subroutine ring_rows()
+    use params_module, only: g_nlon => nlon, g_nlonp4 => nlonp4, g_nlat => nlat
+    use cons_module, only: c_dlamda => dlamda, nlat_filter, chunk_array
+    real, allocatable :: rows(:, :, :)
+    integer :: lat, uu
+    g_nlon = nlon
+    g_nlonp4 = nlon + 4
+    g_nlat = nlat
+    c_dlamda = rv(2)
+    open(newunit=uu, file=trim(dir)//'/chunk.txt', status='old')
+    read(uu, *) nlat_filter
+    allocate(chunk_array(nlat_filter))
+    read(uu, *) chunk_array
+    close(uu)
+    allocate(rows(nlon + 4, nk, nlat))
+    call rd('rows', rows, size(rows))
+    do lat = 1, nlat
+      if (calc_helium == 1) then
+        call ringfilter(rows(:, :, lat), 1, nk, nstep_sub, 'OPLUS', lat)
+      else
+        call ringfilter(rows(:, :, lat), 1, nk, nstep_sub, 'TN', lat)
+      endif
+    enddo
+    call wr('rows', rows, size(rows))
+  end subroutine ring_rows
 
   subroutine alloc_fields()
     integer :: l0, l1, j0, j1

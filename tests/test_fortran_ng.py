@@ -123,3 +123,58 @@ def test_fortran_drop_in_refuses_a_nest_split_over_tasks(tmp_path, pkg):
     assert r.returncode != 0
     assert "needs the nest on one task" in r.stdout, r.stdout
     assert not os.path.exists(os.path.join(tmp_path, "out_op.bin"))
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a GPU")
+@pytest.mark.parametrize("name,order,oplus", [("TN", 3, 0), ("OPLUS", 3, 1)])
+def test_fortran_ringfilter_shim_matches_oracle(tmp_path, oracle, name, order, oplus):
+    """ringfilter (ringfilter.F:1-111) through its Fortran drop-in, called on
+    every latitude of the 5 deg grid as filter_tn does (dt.F:486), the module
+    state (nlon, nlat, dlamda, the chunk table of cons.F:331-355) from the
+    stand-ins: filtered rows within 1e-12 of the oracle's, the others
+    untouched."""
+    import math
+    nlon, nlat, nk = 72, 36, 29
+    chunk = oracle.RING_CHUNKS[(nlon, nlat)]
+    dlam = 2 * math.pi / nlon
+    rng = np.random.default_rng(41 + oplus)
+    rows = rng.normal(size=(nlat, nk, nlon + 4)) * 3. + 250.
+    rows[..., :2] = rows[..., nlon:nlon + 2]
+    rows[..., nlon + 2:] = rows[..., 2:4]
+    with open(tmp_path / "run.txt", "w") as fh:
+        fh.write("4 %d %d %d %d %d\n" % (nk, nlon, nlat, order, oplus))
+        fh.write(" ".join(repr(float(x)) for x in [0., dlam] + [0.] * 22) + "\n")
+    with open(tmp_path / "chunk.txt", "w") as fh:
+        fh.write("%d\n%s\n" % (len(chunk), " ".join(str(c) for c in chunk)))
+    rows.tofile(str(tmp_path / "rows.bin"))
+    r = run_driver(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(str(tmp_path / "out_rows.bin"), dtype=np.float64).reshape(nlat, nk, nlon + 4)
+    for lat in range(1, nlat + 1):
+        nc = oracle.ringfilter_chunks(lat, nlat, chunk)
+        want = np.ascontiguousarray(rows[lat - 1])
+        if nc == 0:
+            assert np.array_equal(got[lat - 1], want), lat
+            continue
+        oracle.ringfilter_row(want, nlon, order, oplus, nc, dlam)
+        err = float((np.abs(got[lat - 1] - want) / np.abs(want).max(axis=1, keepdims=True)).max())
+        assert err <= 1e-12, (lat, err)
+
+
+@needs_driver
+def test_fortran_ringfilter_shim_refuses_a_too_high_order(tmp_path):
+    """An order the filter cannot take (>= nlon, ringfilter.F:36) ends in the
+    model's shutdown, as the reference's does; checked on a host without a
+    GPU too, since the order is refused before any device work."""
+    nlon, nlat, nk = 8, 6, 4
+    with open(tmp_path / "run.txt", "w") as fh:
+        fh.write("4 %d %d %d %d 0\n" % (nk, nlon, nlat, nlon))
+        fh.write(" ".join(repr(float(x)) for x in [0., 0.785] + [0.] * 22) + "\n")
+    with open(tmp_path / "chunk.txt", "w") as fh:
+        fh.write("1\n2\n")
+    np.full((nlat, nk, nlon + 4), 1.0).tofile(str(tmp_path / "rows.bin"))
+    r = run_driver(tmp_path)
+    assert r.returncode != 0
+    assert "shutdown: ringfilter" in r.stdout, r.stdout

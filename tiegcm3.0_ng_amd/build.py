@@ -77,9 +77,9 @@ def build_fortran(verbose=True):
     - fortran/shim_driver: trsolv / blktri drop-ins (trsolv_shim.F90) called
       as the global model calls them.
     - tests/fortran/ng_driver: trsolv_ng_shim.F90 and the nest routine
-      drop-ins (ng_shims.F90) compiled against the test stand-ins of the
-      model modules (tests/fortran/model_modules.F90), one nest step run
-      through them in the model's order."""
+      drop-ins (ng_shims.F90) and ringfilter_shim.F90 compiled against the
+      test stand-ins of the model modules (tests/fortran/model_modules.F90),
+      one nest step run through them in the model's order."""
     fc = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
     if not os.path.exists(fc):
         if verbose:
@@ -104,7 +104,8 @@ def build_fortran(verbose=True):
     exe = os.path.join(fdir, "shim_driver")
     _fc([fc, "-o", exe] + objs + link + ["-Wl,-rpath,$ORIGIN/.."], verbose)
     tobjs = [compile_(os.path.join(tdir, "model_modules.F90"), tbdir)]
-    tobjs += [compile_(os.path.join(fdir, s), tbdir) for s in ("trsolv_ng_shim.F90", "ng_shims.F90")]
+    tobjs += [compile_(os.path.join(fdir, s), tbdir)
+              for s in ("trsolv_ng_shim.F90", "ng_shims.F90", "ringfilter_shim.F90")]
     tobjs += [compile_(os.path.join(tdir, "ng_driver.F90"), tbdir)]
     texe = os.path.join(tdir, "ng_driver")
     _fc([fc, "-o", texe] + tobjs + common + link + ["-Wl,-rpath,$ORIGIN/../../tiegcm3.0_ng_amd"], verbose)

@@ -398,6 +398,11 @@ extern "C" void tgx_ringfilter_f64(double *f, int lev0, int lev1, int order, con
   if (lat <= nlat_filter) nchnk = chunk[lat - 1];
   else if (lat >= nlat - nlat_filter + 1) nchnk = chunk[nlat - lat];
   if (nchnk == 0) return;
+  if (order < 0 || order > TGX_RING_MAX_ORDER || order >= nlon || nchnk > nlon) {  // ringfilter.F:36
+    tgx_set_error(1, "tgx_ringfilter_f64: order %d (0..%d, < nlon %d) or %d chunks", order, TGX_RING_MAX_ORDER, nlon,
+                  nchnk);
+    return;
+  }
   const int oplus = name && strcmp(name, "OPLUS") == 0;
   const size_t bytes = sizeof(double) * (size_t)(nlon + 4) * nlev;
   void *st = nullptr;

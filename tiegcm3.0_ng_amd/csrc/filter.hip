@@ -13,7 +13,7 @@
 namespace tgx {
 namespace ring {
 
-constexpr int kMaxOrder = 8;
+constexpr int kMaxOrder = TGX_RING_MAX_ORDER;
 
 // The rows of one launch, by value in the kernel arguments: row r is at
 // f + k * plane + off[r] and is filtered with n[r] chunks.

@@ -28,6 +28,16 @@ module tgx_c
       integer(c_int), value :: lev0, lev1, lon0, lon1, lonmax
     end subroutine tgx_blktri_f64
 
+    subroutine tgx_ringfilter_f64(f, lev0, lev1, order, name, lat, nlon, nlat, dlamda, nlat_filter, &
+                                  chunk) bind(C, name="tgx_ringfilter_f64")
+      import :: c_double, c_int, c_char
+      real(c_double), intent(inout) :: f(*)
+      integer(c_int), value :: lev0, lev1, order, lat, nlon, nlat, nlat_filter
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), value :: dlamda
+      integer(c_int), intent(in) :: chunk(*)
+    end subroutine tgx_ringfilter_f64
+
     integer(c_int) function tgx_last_status() bind(C, name="tgx_last_status")
       import :: c_int
     end function tgx_last_status
