@@ -161,6 +161,30 @@ int tgx_ng_profile_end(double *ms_total /* [10] */, int *calls /* [10] */);
 int tgx_ng_pack_levels(const double *src, double *dst, int nk, int nlond, int nlatd, int ld, void *stream);
 int tgx_ng_unpack_levels(const double *src, double *dst, int nk, int nlond, int nlatd, int ld, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Coupling between nest levels                                              */
+
+/* interp3d (interp.F90:6-48) on the device, as map_in, set_bndry and map_out
+ * use it (levels_inner_ng.F90:131-410): the source field in level planes
+ * (src[k*ld*nyp + j*ld + i], nzp x nyp rows of nxp values) on the axes zp,
+ * xp, yp (increasing), interpolated to the target points (z[k], x[i], y[j]),
+ * k < nz, i < nx, j < ny, written to dst[k*dk + i*di + j*dj]; log-linear in z
+ * when zlog (the reference's zlog fields).  Bilinear in (x, y) from the
+ * bracketing source columns, then linear in z, the end intervals
+ * extrapolated, in the reference's order of operations.  All arrays,
+ * coordinates included, are device arrays; one launch on `stream`. */
+int tgx_ng_interp3d_dev(const double *src, int nzp, int nxp, int nyp, int ld, const double *zp,
+                        const double *xp, const double *yp, double *dst, int nz, int nx, int ny, long dk, long di,
+                        long dj, const double *z, const double *x, const double *y, int zlog, void *stream);
+/* interp2d (interp.F90:50-70) on one plane src[j*ld + i], to dst[i*di + j*dj]. */
+int tgx_ng_interp2d_dev(const double *src, int nxp, int nyp, int ld, const double *xp, const double *yp,
+                        double *dst, int nx, int ny, long di, long dj, const double *x, const double *y,
+                        void *stream);
+/* interp_fields_ng.F90:17-63: the n values at t = ta..tb between f0 (t = 0) and
+ * f1 (t = t1), linear in t, to dst[(t - ta)*n + p]. */
+int tgx_ng_time_interp_dev(const double *f0, const double *f1, double *dst, long n, int t1, int ta, int tb,
+                           void *stream);
+
 /* A host-layout context for one nest domain: device copies of every field of
  * tgx_ng_fields and tgx_ng_out in level planes, a workspace and a stream,
  * filled and read back by field name from arrays in the model's layout.  The

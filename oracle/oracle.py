@@ -287,3 +287,51 @@ def _methods():
 
 
 _methods()
+
+
+# ---- interpolation between nest levels (interp.F90, interp_fields_ng.F90) ---
+def _v(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def interp_find(xp, x0):
+    """find (interp.F90:105-126): 1-based bracket, 0 below, len(xp) at or above."""
+    xp = _v(xp)
+    lib().tgx_o_interp_find.restype = C.c_int
+    return lib().tgx_o_interp_find(_p(xp), len(xp), C.c_double(x0))
+
+
+def interp1d(x, xp, fp):
+    x, xp, fp = _v(x), _v(xp), _v(fp)
+    f = np.empty(len(x))
+    lib().tgx_o_interp1d(_p(x), len(x), _p(xp), len(xp), _p(fp), _p(f))
+    return f
+
+
+def interp2d(x, y, xp, yp, fp):
+    """fp: C-order (nyp, nxp) (the reference's fp(nxp, nyp)); returns (ny, nx)."""
+    x, y, xp, yp, fp = _v(x), _v(y), _v(xp), _v(yp), _v(fp)
+    assert fp.shape == (len(yp), len(xp))
+    f = np.empty((len(y), len(x)))
+    lib().tgx_o_interp2d(_p(x), len(x), _p(y), len(y), _p(xp), len(xp), _p(yp), len(yp), _p(fp), _p(f))
+    return f
+
+
+def interp3d(z, x, y, zp, xp, yp, fp, zlog):
+    """fp: C-order (nyp, nxp, nzp) (the reference's fp(nzp, nxp, nyp));
+    returns (ny, nx, nz)."""
+    z, x, y, zp, xp, yp, fp = _v(z), _v(x), _v(y), _v(zp), _v(xp), _v(yp), _v(fp)
+    assert fp.shape == (len(yp), len(xp), len(zp))
+    f = np.empty((len(y), len(x), len(z)))
+    lib().tgx_o_interp3d(_p(z), len(z), _p(x), len(x), _p(y), len(y), _p(zp), len(zp), _p(xp), len(xp), _p(yp),
+                         len(yp), _p(fp), int(bool(zlog)), _p(f))
+    return f
+
+
+def time_interp(f0, f1, t1, ta, tb):
+    """interp_fields_ng.F90: values at t = ta..tb between f0 (t = 0) and f1
+    (t = t1); returns (tb-ta+1,) + f0.shape."""
+    f0, f1 = _v(f0), _v(f1)
+    f = np.empty((tb - ta + 1,) + f0.shape)
+    lib().tgx_o_time_interp(_p(f0), _p(f1), C.c_long(f0.size), int(t1), int(ta), int(tb), _p(f))
+    return f

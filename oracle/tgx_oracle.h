@@ -142,6 +142,21 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
 int tgx_o_ringfilter_row(double *f, int nlon, int nlev, int order, int oplus, int nchnk, double dlamda);
 int tgx_o_ringfilter_chunks(int lat, int nlat, int nlat_filter, const int *chunk);
 
+/* ---- interpolation between nest levels (interp.c) ----------------------
+ * interp.F90:1-128 and interp_fields_ng.F90:17-63, Fortran layouts:
+ * tgx_o_interp_find is find (1-based, 0 below, nx at or above the range);
+ * interp1d f(nx) from fp(nxp); interp2d f(nx, ny) from fp(nxp, nyp);
+ * interp3d f(nz, nx, ny) from fp(nzp, nxp, nyp), log-linear in z when zlog;
+ * time_interp: f(n, tb-ta+1) at t = ta..tb between f0 (t = 0) and f1 (t = t1). */
+int tgx_o_interp_find(const double *xp, int nx, double x0);
+void tgx_o_interp1d(const double *x, int nx, const double *xp, int nxp, const double *fp, double *f);
+void tgx_o_interp2d(const double *x, int nx, const double *y, int ny, const double *xp, int nxp, const double *yp,
+                    int nyp, const double *fp, double *f);
+void tgx_o_interp3d(const double *z, int nz, const double *x, int nx, const double *y, int ny, const double *zp,
+                    int nzp, const double *xp, int nxp, const double *yp, int nyp, const double *fp, int zlog,
+                    double *f);
+void tgx_o_time_interp(const double *f0, const double *f1, long n, int t1, int ta, int tb, double *f);
+
 /* ---- test hooks (column.c) ---------------------------------------------
  * tgx_o_set_capture: rows handed to each column solve are written into cap
  * (TGX_O_CAPTURE_PLANES planes of ncol*nk doubles, the library's capture

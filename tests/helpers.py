@@ -20,7 +20,7 @@ def load_pkg():
     mod = importlib.util.module_from_spec(spec)
     sys.modules[PKG_NAME] = mod
     spec.loader.exec_module(mod)
-    for sub in ("synth", "tgx", "step"):
+    for sub in ("synth", "tgx", "step", "nest"):
         setattr(mod, sub, importlib.import_module(PKG_NAME + "." + sub))
     return mod
 

@@ -39,6 +39,10 @@ CONFIGS = {
     "g0p625": (576, 288, 113, 0.125, 5., 5., None),     # 0.625 deg, 113 levels
     "nest1": (120, 60, 57, 0.25, 15., 30., (20.0, 60.0, 0.5, 0.5)),
     "nest2": (100, 50, 57, 0.25, 3., 30., (30.0, 80.0, 0.1, 0.1)),
+    # a small nested pair inside "small" for the coupling tests: a 2.5 deg nest
+    # on the same levels and a 0.5 deg nest inside it on finer levels
+    "nest_s1": (20, 12, 29, 0.5, 30., 60., (-10.0, 40.0, 2.5, 2.5)),
+    "nest_s2": (10, 8, 57, 0.25, 10., 60., (-4.0, 45.0, 0.5, 0.5)),
 }
 SEED_BASE = 0x7E6C0000
 
@@ -77,9 +81,11 @@ def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor
         dlat = 180. / (nlatd + 1)         # keep ghost rows off the poles
         glat = [-90. + (j + 1) * dlat for j in range(nlatd)]
         dlon = 360. / nlon
+        glon = [(i - 2) * dlon for i in range(nlond)]
     else:
-        glat0, _, dlat, dlon = region
+        glat0, glon0, dlat, dlon = region
         glat = [glat0 + (j - 2) * dlat for j in range(nlatd)]
+        glon = [glon0 + (i - 2) * dlon for i in range(nlond)]
     expdz = math.exp(-dlev)
     expz, difk, dift = [math.exp(-ZIBOT - .5 * dlev)], [5.0e-6], [5.0e-6]
     for k in range(1, nk):                # set_const_fields_ng.F90:57-66
@@ -97,7 +103,7 @@ def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor
         expz=expz, difk=difk, dift=dift, xmue=xmue,
         zpint=[ZIBOT + k * dlev for k in range(nk)],
         zpmid=[ZIBOT + .5 * dlev + k * dlev for k in range(nk)],
-        glat=glat, cs=cs, tanphi=[math.tan(x * dtr) for x in glat],
+        glat=glat, glon=glon, cs=cs, tanphi=[math.tan(x * dtr) for x in glat],
         cor=[2. * OMEGA * math.sin(x * dtr) for x in glat], racs=[1. / (RE * c) for c in cs],
         nstep_sub=nstep_sub, calc_helium=calc_helium, colfac=colfac, opdiffcap=opdiffcap,
         difhor=difhor, b=b, fb=fb, pshelb=0.1154E-5, ar_glbm=[1.2e-3, 1.1e-3],
@@ -283,4 +289,49 @@ def to_fortran_layout(F):
             out[nm] = t.permute(1, 2, 0).contiguous().cpu().numpy()
         else:
             out[nm] = t.contiguous().cpu().numpy()
+    return out
+
+
+def _lin_axis(a, xp, x, axis):
+    """Linear interpolation of array a along `axis` from points xp to x
+    (inside the range of xp)."""
+    import numpy as np
+    i1 = np.clip(np.searchsorted(xp, x, side="right"), 1, len(xp) - 1)
+    i0 = i1 - 1
+    w = (x - xp[i0]) / (xp[i1] - xp[i0])
+    shape = [1] * a.ndim
+    shape[axis] = len(x)
+    w = w.reshape(shape)
+    return np.take(a, i0, axis=axis) * (1. - w) + np.take(a, i1, axis=axis) * w
+
+
+def nested_states(grids, device="cpu"):
+    """Synthetic states for a nest chain that agree with each other: the
+    outermost domain's state as make_state gives it, and every nest's 3-D and
+    2-D fields interpolated (linearly in z, latitude and longitude) from the
+    outermost state, as a model initialises its nests from the global state
+    (init_fields_ng.F90).  The nests' lateral data are those of make_state
+    (a coupled run replaces them).  Returns one dict per grid."""
+    import numpy as np
+    outer = make_state(grids[0], device="cpu")
+    out = [outer]
+    g0 = grids[0]
+    z0, y0, x0 = (np.asarray(g0[k], dtype=np.float64) for k in ("zpint", "glat", "glon"))
+    for g in grids[1:]:
+        F = make_state(g, device="cpu")
+        z, y, x = (np.asarray(g[k], dtype=np.float64) for k in ("zpint", "glat", "glon"))
+        for n, t in outer.items():
+            if n.startswith(("lonb_", "latb_")) or n not in F:
+                continue
+            a = t.numpy()
+            if a.ndim == 3:
+                a = _lin_axis(_lin_axis(_lin_axis(a, z0, z, 0), y0, y, 1), x0, x, 2)
+            elif a.ndim == 2:
+                a = _lin_axis(_lin_axis(a, y0, y, 0), x0, x, 1)
+            else:
+                continue
+            F[n] = torch.from_numpy(np.ascontiguousarray(a))
+        out.append(F)
+    if device != "cpu":
+        out = [{k: v.to(device) for k, v in F.items()} for F in out]
     return out

@@ -41,6 +41,9 @@ _SIGS = {
     "tgx_ringfilter_f64": ([_D, _I, _I, _I, C.c_char_p, _I, _I, _I, C.c_double, _I, _D], None),
     "tgx_ringfilter_dev": ([_D] + [_I] * 6 + [C.c_double, _I, _D, _D], _I),
     "tgx_ng_pack_levels": ([_D, _D] + [_I] * 4 + [_D], _I),
+    "tgx_ng_interp3d_dev": ([_D] + [_I] * 4 + [_D] * 4 + [_I] * 3 + [C.c_long] * 3 + [_D] * 3 + [_I, _D], _I),
+    "tgx_ng_interp2d_dev": ([_D] + [_I] * 3 + [_D] * 3 + [_I] * 2 + [C.c_long] * 2 + [_D] * 3, _I),
+    "tgx_ng_time_interp_dev": ([_D] * 3 + [C.c_long] + [_I] * 3 + [_D], _I),
     "tgx_ng_unpack_levels": ([_D, _D] + [_I] * 4 + [_D], _I),
 }
 
@@ -210,6 +213,31 @@ def unpack_levels(src, dst, nk, nlond, nlatd, stream=None):
     """Level planes src (nk, nlatd, ld) into the model layout dst (nlatd, nlond, nk)."""
     _check(lib().tgx_ng_unpack_levels(_dp(src), _dp(dst), nk, nlond, nlatd, src.shape[-1], _stream(stream)))
     return dst
+
+
+def interp3d_dev(src, zp, xp, yp, dst, z, x, y, dk, di, dj, zlog, offset=0, stream=None):
+    """interp3d (interp.F90) of the level-plane tensor src (nzp, nyp, ld) on
+    the device axes zp, xp, yp to the target axes z, x, y, written to the
+    flat view of dst at offset + k*dk + i*di + j*dj."""
+    nzp, nyp, ld = src.shape
+    dptr = C.c_void_p(dst.data_ptr() + 8 * int(offset))
+    _check(lib().tgx_ng_interp3d_dev(_dp(src), nzp, len(xp), nyp, ld, _dp(zp), _dp(xp), _dp(yp), dptr, len(z),
+                                     len(x), len(y), int(dk), int(di), int(dj), _dp(z), _dp(x), _dp(y),
+                                     int(bool(zlog)), _stream(stream)))
+
+
+def interp2d_dev(src, xp, yp, dst, x, y, di, dj, offset=0, stream=None):
+    """interp2d of the plane src (nyp, ld) to dst[offset + i*di + j*dj]."""
+    nyp, ld = src.shape
+    dptr = C.c_void_p(dst.data_ptr() + 8 * int(offset))
+    _check(lib().tgx_ng_interp2d_dev(_dp(src), len(xp), nyp, ld, _dp(xp), _dp(yp), dptr, len(x), len(y), int(di),
+                                     int(dj), _dp(x), _dp(y), _stream(stream)))
+
+
+def time_interp_dev(f0, f1, dst, t1, ta, tb, stream=None):
+    """dst[(t - ta)*n + p] for t = ta..tb between f0 (t = 0) and f1 (t = t1)."""
+    _check(lib().tgx_ng_time_interp_dev(_dp(f0), _dp(f1), _dp(dst), f0.numel(), int(t1), int(ta), int(tb),
+                                        _stream(stream)))
 
 
 def ringfilter_dev(f, nlon, nlat, order, oplus, dlamda, chunk, stream=None):
