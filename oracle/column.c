@@ -85,6 +85,45 @@ static void lateral(double *x, const double *lonb, const double *latb, int nk, i
       }
 }
 
+/* ---- global grid ---------------------------------------------------------
+ * params.F:38 */
+#define SPVAL 1.e36
+
+/* mp_periodic_f3d (mpi.F:2932-3056) on the interior latitudes: the periodic
+ * points from the interior longitudes they repeat (lon 1, 2 = nlon+1, nlon+2;
+ * nlon+3, nlon+4 = 3, 4, i.e. i = lon - 1). */
+static void glb_periodic(const tgx_ogrid *g, double *f) {
+  DIMS;
+  const int src0 = MUT(803) ? nx - 1 : nx;
+  for (int j = 2; j < nlatd - 2; ++j)
+    for (int k = 0; k < nk; ++k) {
+      f[IX(k, 0, j)] = f[IX(k, src0, j)];
+      f[IX(k, 1, j)] = f[IX(k, nx + 1, j)];
+      f[IX(k, nx + 2, j)] = f[IX(k, 2, j)];
+      f[IX(k, nx + 3, j)] = f[IX(k, 3, j)];
+    }
+}
+
+/* filter_tn (dt.F:431-511) and its siblings filter_uv (duv.F:910-994),
+ * filter_minor (minor.F:621-702), filter_op (oplus.F:1210-1284) and
+ * filter_o2o (comp.F:1065-1145): ringfilter.F on every interior latitude the
+ * chunk table filters, levels 0..nlev-1 of the field. */
+static void glb_filter(const tgx_ogrid *g, double *f, int nlev, int order, int oplus) {
+  DIMS;
+  double *row = malloc(sizeof(double) * (size_t)nlond * (size_t)nlev);
+  for (int lat = 1; lat <= ny; ++lat) {
+    int nch = tgx_o_ringfilter_chunks(lat, ny, g->nlat_filter, g->chunk);
+    if (nch == 0) continue;
+    const int j = lat + 1;
+    for (int l = 0; l < nlev; ++l)
+      for (int i = 0; i < nlond; ++i) row[(size_t)l * nlond + i] = f[IX(l, i, j)];
+    tgx_o_ringfilter_row(row, nx, nlev, order, oplus, nch, g->dlamda);
+    for (int l = 0; l < nlev; ++l)
+      for (int i = 0; i < nlond; ++i) f[IX(l, i, j)] = row[(size_t)l * nlond + i];
+  }
+  free(row);
+}
+
 /* minor_ng.F90:145-154 / comp_ng.F90:113-122 */
 static double dfactor_of(const tgx_ogrid *g, int j) {
   const double hor = .25, pi = pi_(), dtr = pi / 180.;
@@ -202,8 +241,31 @@ void tgx_o_dt_ng(const tgx_ogrid *g, const tgx_ofields *F, double *tlbc, double 
         }
       /* :165-167 */
       tgx_o_trsolv_ng(p, q, r, rhs, tn_upd + o, nk, 1, nk - 1);
-      tn_upd[o + nk - 1] = tn_upd[o + nk - 2] * tn_upd[o + nk - 2] / tn_upd[o + nk - 3];
+      if (!g->glb) tn_upd[o + nk - 1] = tn_upd[o + nk - 2] * tn_upd[o + nk - 2] / tn_upd[o + nk - 3];
     }
+  if (g->glb) {
+    /* dt.F:389-424: filter_tn, the time smoothing below the top level, spval
+     * at the top (:407-408), the lower-boundary bookkeeping, the 100 K floor */
+    if (!MUT(802)) glb_filter(g, tn_upd, nk - 1, 3, 0);
+    for (int j = 0; j < nlatd; ++j)
+      for (int i = 0; i < nlond; ++i) {
+        size_t o = IX(0, i, j);
+        for (int k = 0; k < nk - 1; ++k)
+          tn_nm_upd[o + k] = DTSMOOTH_DIV2 * (F->tn_nm[o + k] + tn_upd[o + k]) + DTSMOOTH * F->tn[o + k];
+        tn_upd[o + nk - 1] = MUT(806) ? tn_upd[o + nk - 2] : SPVAL;
+        tn_nm_upd[o + nk - 1] = SPVAL;
+      }
+    for (size_t n = 0; n < (size_t)nlond * nlatd; ++n) {
+      tlbc_nm[n] = tlbc[n];
+      tlbc[n] = F->t_lbc[n];
+    }
+    for (size_t n = 0; n < N3; ++n) tn_upd[n] = tn_upd[n] > 100. ? tn_upd[n] : 100.;
+    glb_periodic(g, tn_upd);
+    glb_periodic(g, tn_nm_upd);
+    free(horadv); free(tnsmooth); free(veradv); free(cpi); free(kmi); free(wi); free(cptn); free(dudz);
+    free(dvdz); free(qsolar); free(tni); free(gg); free(ff); free(p); free(q); free(r); free(rhs);
+    return;
+  }
   /* :201-213 */
   lateral(tn_upd, F->lonb_tn, F->latb_tn, nk, nlond, nlatd);
   for (size_t n = 0; n < N3; ++n) tn_nm_upd[n] = DTSMOOTH_DIV2 * (F->tn_nm[n] + tn_upd[n]) + DTSMOOTH * F->tn[n];
@@ -346,8 +408,13 @@ void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd
         }
       /* :217-253 */
       tgx_o_blktri_ng(pp, qq, rr, ss, xx, nk, 1);
-      VEC(xx, 1, nk - 1) = 2 * VEC(xx, 1, nk - 2) - VEC(xx, 1, nk - 3);
-      VEC(xx, 2, nk - 1) = 2 * VEC(xx, 2, nk - 2) - VEC(xx, 2, nk - 3);
+      if (g->glb) { /* duv.F:385-386 */
+        VEC(xx, 1, nk - 1) = SPVAL;
+        VEC(xx, 2, nk - 1) = SPVAL;
+      } else {
+        VEC(xx, 1, nk - 1) = 2 * VEC(xx, 1, nk - 2) - VEC(xx, 1, nk - 3);
+        VEC(xx, 2, nk - 1) = 2 * VEC(xx, 2, nk - 2) - VEC(xx, 2, nk - 3);
+      }
       for (int k = 0; k < nk; ++k) {
         un_upd[o + k] = VEC(xx, 1, k);
         vn_upd[o + k] = VEC(xx, 2, k);
@@ -356,8 +423,14 @@ void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd
 #undef MAT
 #undef VEC
   /* :296-321 */
-  lateral(un_upd, F->lonb_un, F->latb_un, nk, nlond, nlatd);
-  lateral(vn_upd, F->lonb_vn, F->latb_vn, nk, nlond, nlatd);
+  if (g->glb) {
+    /* duv.F:490-493 filter_uv of UN and VN below the top level */
+    glb_filter(g, un_upd, nk - 1, 3, 0);
+    glb_filter(g, vn_upd, nk - 1, 3, 0);
+  } else {
+    lateral(un_upd, F->lonb_un, F->latb_un, nk, nlond, nlatd);
+    lateral(vn_upd, F->lonb_vn, F->latb_vn, nk, nlond, nlatd);
+  }
   for (size_t n = 0; n < (size_t)nlond * nlatd; ++n) {
     ulbc_nm[n] = ulbc[n];
     vlbc_nm[n] = vlbc[n];
@@ -367,6 +440,14 @@ void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd
   for (size_t n = 0; n < N3; ++n) {
     unm_upd[n] = DTSMOOTH_DIV2 * (F->un_nm[n] + un_upd[n]) + DTSMOOTH * F->un[n];
     vnm_upd[n] = DTSMOOTH_DIV2 * (F->vn_nm[n] + vn_upd[n]) + DTSMOOTH * F->vn[n];
+  }
+  if (g->glb) {
+    /* duv.F:501-514: the time smoothing below the top level, spval at the top */
+    for (size_t n = (size_t)nk - 1; n < N3; n += (size_t)nk) unm_upd[n] = vnm_upd[n] = SPVAL;
+    glb_periodic(g, un_upd);
+    glb_periodic(g, vn_upd);
+    glb_periodic(g, unm_upd);
+    glb_periodic(g, vnm_upd);
   }
   free(adv_u); free(adv_v); free(ztmp); free(zl); free(zp); free(unms); free(vnms);
   free(pp); free(qq); free(rr); free(ss); free(xx);
@@ -386,6 +467,9 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
   const double dzp = g->dz, dtx2inv = g->dtx2inv, small = 1.e-12;
   const size_t N2 = (size_t)nlond * nlatd;
   double salfax[3] = {phix[0] - phix[3], phix[1] - phix[3], phix[2] - phix[3]};
+  /* the denominator of ax: phix(3) in minor_ng.F90:135-136, phix(4) in the
+   * global minor.F:359-362 */
+  const double phix_ax = g->glb ? (MUT(801) ? phix[2] : phix[3]) : (MUT(301) ? phix[3] : phix[2]);
   ALLOC3(ftm1s); ALLOC3(hadvec);
   tgx_o_smooth(g, fcomp_tm1, ftm1s, g->shapiro);
   tgx_o_advec(g, fcomp, F->un, F->vn, hadvec);
@@ -459,7 +543,7 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
                 sai[3 * k + 2] * (dhedz[k] - (1. - (RMHE + dmdz[k]) / barm[k]) * pshe[k]);
         dmdz[k] = dmdz[k] / barm[k];
         ax[k] = -(barm[k] / (TAU * RMN2) * pow(T00 / tni[k], 0.25) /
-                  ((MUT(301) ? phix[3] : phix[2]) + salfax[0] * pso2[k] + salfax[1] * pso1[k] + salfax[2] * pshe[k]));
+                  (phix_ax + salfax[0] * pso2[k] + salfax[1] * pso1[k] + salfax[2] * pshe[k]));
       }
       for (int k = 1; k < nk - 1; ++k) dtnidz[k] = (tni[k + 1] - tni[k - 1]) / 2.;
       dtnidz[0] = tni[1] - tni[0];
@@ -505,12 +589,28 @@ static void minor_core(const tgx_ogrid *g, const tgx_ofields *F, const double *f
       tgx_o_trsolv_ng(pc, qc, rc, fr, out + o, nk, 1, nk);
     }
   }
-  /* :228-237 */
-  lateral(out, lonb, latb, nk, nlond, nlatd);
-  for (size_t n = 0; n < N3; ++n) {
-    out_nm[n] = DTSMOOTH * fcomp[n] + DTSMOOTH_DIV2 * (fcomp_tm1[n] + out[n]);
-    out[n] = out[n] > small ? out[n] : small;
-    out_nm[n] = out_nm[n] > small ? out_nm[n] : small;
+  if (g->glb) {
+    /* minor.F:546-597: filter_minor, the time smoothing, the periodic points,
+     * the floor */
+    if (MUT(805))
+      for (size_t n = 0; n < N3; ++n) out_nm[n] = DTSMOOTH * fcomp[n] + DTSMOOTH_DIV2 * (fcomp_tm1[n] + out[n]);
+    if (!MUT(804)) glb_filter(g, out, nk, 3, 0);
+    if (!MUT(805))
+      for (size_t n = 0; n < N3; ++n) out_nm[n] = DTSMOOTH * fcomp[n] + DTSMOOTH_DIV2 * (fcomp_tm1[n] + out[n]);
+    glb_periodic(g, out);
+    glb_periodic(g, out_nm);
+    for (size_t n = 0; n < N3; ++n) {
+      out[n] = out[n] > small ? out[n] : small;
+      out_nm[n] = out_nm[n] > small ? out_nm[n] : small;
+    }
+  } else {
+    /* :228-237 */
+    lateral(out, lonb, latb, nk, nlond, nlatd);
+    for (size_t n = 0; n < N3; ++n) {
+      out_nm[n] = DTSMOOTH * fcomp[n] + DTSMOOTH_DIV2 * (fcomp_tm1[n] + out[n]);
+      out[n] = out[n] > small ? out[n] : small;
+      out_nm[n] = out_nm[n] > small ? out_nm[n] : small;
+    }
   }
   free(ftm1s); free(hadvec); free(sai);
   free(dmdz); free(pso1); free(pso2); free(pshe); free(do1dz); free(do2dz); free(dhedz);
@@ -901,14 +1001,33 @@ void tgx_o_oplus_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *op,
         }
       /* :305-307 */
       tgx_o_trsolv_ng(pcf, qcf, rcf, explicit_, opout + o, nk, 1, nk - 1);
-      opout[o + nk - 1] = opout[o + nk - 2] * opout[o + nk - 2] / opout[o + nk - 3];
+      if (!g->glb) opout[o + nk - 1] = opout[o + nk - 2] * opout[o + nk - 2] / opout[o + nk - 3];
     }
-  /* :332-340 */
-  lateral(opout, lonb, latb, nk, nlond, nlatd);
-  for (size_t n = 0; n < N3; ++n) {
-    optm1out[n] = DTSMOOTH * op[n] + DTSMOOTH_DIV2 * (optm1[n] + opout[n]);
-    opout[n] = opout[n] > 0. ? opout[n] : 0.;
-    optm1out[n] = optm1out[n] > 0. ? optm1out[n] : 0.;
+  if (g->glb) {
+    /* oplus.F:896-941: filter_op ('OPLUS', the chunk-mean-preserving
+     * reconstruction) below the top level, the time smoothing, zero at the
+     * top level (:922-923), the floor, the periodic points */
+    glb_filter(g, opout, nk - 1, 3, 1);
+    for (size_t n = 0; n < N3; ++n) {
+      if (n % (size_t)nk == (size_t)nk - 1) {
+        opout[n] = MUT(807) ? opout[n - 1] : 0.;
+        optm1out[n] = 0.;
+        continue;
+      }
+      optm1out[n] = DTSMOOTH * op[n] + DTSMOOTH_DIV2 * (optm1[n] + opout[n]);
+      opout[n] = opout[n] > 0. ? opout[n] : 0.;
+      optm1out[n] = optm1out[n] > 0. ? optm1out[n] : 0.;
+    }
+    glb_periodic(g, opout);
+    glb_periodic(g, optm1out);
+  } else {
+    /* :332-340 */
+    lateral(opout, lonb, latb, nk, nlond, nlatd);
+    for (size_t n = 0; n < N3; ++n) {
+      optm1out[n] = DTSMOOTH * op[n] + DTSMOOTH_DIV2 * (optm1[n] + opout[n]);
+      opout[n] = opout[n] > 0. ? opout[n] : 0.;
+      optm1out[n] = optm1out[n] > 0. ? optm1out[n] : 0.;
+    }
   }
   free(opflux); free(dvb); free(csby); free(dbxdx); free(dcsbydy);
   free(dj); free(vni); free(bdotu); free(bvel); free(diffj); free(tp_op); free(opts2); free(djbz);
@@ -1025,6 +1144,10 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
         dembardz[k] = dembardz[k] / dz;
         dtndz[k] = dtndz[k] / dz;
       }
+      /* the global comp.F takes the top interface's temperature from the top
+       * midpoint level below it, tn(lev1-1), in diff_fac (:174-175) and wks1
+       * (:446-449) */
+      if (g->glb && !MUT(808)) tni[nk - 1] = tn[nk - 2];
       for (int k = 0; k < nk - 1; ++k) wi[k] = .5 * (F->w[o + k] + F->w[o + k + 1]);
       wi[nk - 1] = 1.5 * F->w[o + nk - 1] - .5 * F->w[o + nk - 2];
       /* :124-126 */
@@ -1112,7 +1235,10 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
                       (d_[1] * PH(3, 2) - d_[2] * PH(3, 4)) * o1i[k + 1]);
           alpha23 = (d_[1] * PH(2, 3) - PH(2, 4)) * o1i[k + 1];
           alpha32 = (d_[1] * PH(3, 2) - d_[2] * PH(3, 4)) * hei[k + 1];
-          flx00 = embari[k] * RIN2 * P0 * pow(T00 / tn[k], 0.25) / (TAU * GRAV);
+          /* the global comp.F:632-633 averages the mean mass over levels k and
+           * k+1 (the top interface), comp_ng.F90 over k-1 and k */
+          flx00 = (g->glb && !MUT(809) ? embari[k + 1] : embari[k]) * RIN2 * P0 * pow(T00 / tn[k], 0.25) /
+                  (TAU * GRAV);
           for (int isp = 0; isp < 3; ++isp)
             fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][1])) -
                       rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (1. / dz - 0.5 * ep[kp][2]));
@@ -1168,9 +1294,16 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
   /* :386-417 */
   if (g->calc_helium == 0)
     for (size_t n = 0; n < N3; ++n) he_upd[n] = g->pshelb;
-  lateral(o2_upd, F->lonb_o2, F->latb_o2, nk, nlond, nlatd);
-  lateral(o1_upd, F->lonb_o1, F->latb_o1, nk, nlond, nlatd);
-  lateral(he_upd, F->lonb_he, F->latb_he, nk, nlond, nlatd);
+  if (g->glb) {
+    /* comp.F:932-934 filter_o2o (order 1) of O2, O and He */
+    glb_filter(g, o2_upd, nk, 1, 0);
+    glb_filter(g, o1_upd, nk, 1, 0);
+    glb_filter(g, he_upd, nk, 1, 0);
+  } else {
+    lateral(o2_upd, F->lonb_o2, F->latb_o2, nk, nlond, nlatd);
+    lateral(o1_upd, F->lonb_o1, F->latb_o1, nk, nlond, nlatd);
+    lateral(he_upd, F->lonb_he, F->latb_he, nk, nlond, nlatd);
+  }
   for (size_t n = 0; n < N3; ++n) {
     o2nm_upd[n] = DTSMOOTH_DIV2 * (F->o2_nm[n] + o2_upd[n]) + DTSMOOTH * F->o2[n];
     o1nm_upd[n] = DTSMOOTH_DIV2 * (F->o1_nm[n] + o1_upd[n]) + DTSMOOTH * F->o1[n];
@@ -1215,6 +1348,10 @@ void tgx_o_comp_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *fs,
         henm_upd[p] = cn / normn;
       }
     }
+  if (g->glb) {
+    glb_periodic(g, o2_upd); glb_periodic(g, o1_upd); glb_periodic(g, he_upd);
+    glb_periodic(g, o2nm_upd); glb_periodic(g, o1nm_upd); glb_periodic(g, henm_upd);
+  }
   free(o2adv); free(o1adv); free(headv); free(o2s); free(o1s); free(hes);
   free(o2i); free(o1i); free(hei); free(embari); free(tni); free(dembardz); free(dtndz); free(wi);
   free(df); free(gama); free(zz); free(upd); free(n2);

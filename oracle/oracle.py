@@ -24,6 +24,7 @@ GRID_FIELDS = [
     ("nstep_sub", C.c_int), ("calc_helium", C.c_int),
     ("colfac", C.c_double), ("opdiffcap", C.c_double), ("difhor", C.c_double),
     ("b", C.c_double * 9), ("fb", C.c_double * 3), ("pshelb", C.c_double),
+    ("glb", C.c_int), ("nlat_filter", C.c_int), ("chunk", C.c_void_p),
 ]
 
 # Order must match tgx_ofields in tgx_oracle.h.
@@ -87,6 +88,11 @@ class OracleState:
             g.b[i] = float(bb.reshape(-1)[i])
         for i in range(3):
             g.fb[i] = float(grid["fb"][i])
+        g.glb = int(bool(grid.get("glb", False)))
+        chunk = np.ascontiguousarray(grid.get("chunk", []) if g.glb else [], dtype=np.int32)
+        self._keep.append(chunk)
+        g.nlat_filter = len(chunk)
+        g.chunk = chunk.ctypes.data if len(chunk) else None
         self.g = g
         f = OFields()
         for n in FIELD_NAMES_3D + FIELD_NAMES_2D:
@@ -143,12 +149,22 @@ class OracleState:
         return dict(zip(("op", "op_nm", "xiop2p", "xiop2d", "Fe", "Fn"), outs))
 
     def oplus_subcycled(self, lonb_sub, latb_sub):
-        """dynamics_ng.F90:99-121: nstep_sub sub-cycles, itp/itc swapped between them."""
+        """dynamics_ng.F90:99-121 (dynamics.F:319-359 on a global grid):
+        nstep_sub sub-cycles, itp/itc swapped between them.  On a global grid
+        a sub-step's O+ keeps the rows across the poles of the step's input
+        (the reference exchanges no latitude halo there, oplus.F:942, so they
+        hold what the model's buffer held before the sub-cycles)."""
         op, optm1 = self.fields_np["op"], self.fields_np["op_nm"]
+        op0, optm10 = op, optm1
         out = None
         for s in range(int(self.grid_np["nstep_sub"])):
             out = self.oplus_ng(op, optm1, lonb_sub[s], latb_sub[s])
             op, optm1 = out["op"], out["op_nm"]
+            if self.g.glb:
+                op, optm1 = op.copy(), optm1.copy()
+                for a, a0 in ((op, op0), (optm1, optm10)):
+                    a[:2] = a0[:2]
+                    a[-2:] = a0[-2:]
         return out
 
     def comp_o2o_ng(self):

@@ -54,6 +54,19 @@ typedef struct {
   double b[3][3];       /* lbc b(l,m) stored b[l-1][m-1] (lbc.F:527-594) */
   double fb[3];
   double pshelb;
+  /* Global grid (dt.F, duv.F, minor.F, oplus.F, comp_o2o.F, comp.F) instead of
+   * a nest: glb != 0.  The ghost columns then hold the periodic points (i = 0,
+   * 1 copies of i = nlon, nlon+1; i = nlon+2, nlon+3 copies of i = 2, 3) and
+   * the ghost rows the latitudes across the poles, both filled by the caller
+   * as the model fills them before the step (mk_polelat.F, mp_periodic_f4d);
+   * no lateral Dirichlet data are applied, the solutions are ring-filtered
+   * (ringfilter.F with the chunk table chunk[nlat_filter], cons.F:331-355)
+   * before their time smoothing, and every output's periodic points are set
+   * from the interior (mp_periodic_f3d).  Outputs on the ghost rows are not
+   * defined (the model recomputes them from the interior). */
+  int glb;
+  int nlat_filter;
+  const int *chunk;
 } tgx_ogrid;
 
 /* Model state read by the column step.  All 3-D fields are (nk, nlond, nlatd),
@@ -113,7 +126,11 @@ void tgx_o_advec(const tgx_ogrid *g, const double *f, const double *un, const do
 void tgx_o_smooth(const tgx_ogrid *g, const double *fin, double *fout, double shapiro);
 void tgx_o_div(const tgx_ogrid *g, const double *f, double *df, int dir);
 
-/* ---- column-step routines ------------------------------------------- */
+/* ---- column-step routines -------------------------------------------
+ * Each routine follows its nest form (the _ng file) when g->glb == 0 and
+ * its global form (dt.F, duv.F, minor.F + comp_{no,n4s,n2d,ar}.F, oplus.F,
+ * comp_o2o.F, comp.F) when g->glb != 0; the global forms' differences are
+ * cited where they are made. */
 void tgx_o_dt_ng(const tgx_ogrid *g, const tgx_ofields *F, double *tlbc, double *tlbc_nm,
                  double *tn_upd, double *tn_nm_upd);
 void tgx_o_duv_ng(const tgx_ogrid *g, const tgx_ofields *F, const double *tn_upd,

@@ -517,7 +517,7 @@ def _minor_state(w_zero=True, difk_zero=True):
     return g, Fo, j, i, dmdz
 
 
-def _minor_AE(g, Fo, j, i, dmdz, sp):
+def _minor_AE(g, Fo, j, i, dmdz, sp, nest_quirk=True):
     """A_k and E_k of the description at interfaces k = 0..nk-1 (k >= 1 valid)."""
     _, mx, phi, alfx = MINOR[sp]
     nk, dz = g["nk"], g["dz"]
@@ -534,7 +534,8 @@ def _minor_AE(g, Fo, j, i, dmdz, sp):
     blanc = phi[0] * psi["o2"] + phi[1] * psi["o1"] + phi[2] * psi["he"] + phi[3] * n2
     # minor_ng.F90:136 puts phi_x(He) where the description's phi_x(N2) belongs
     # (the global minor.F:359-362 has phi_x(N2)); the nest form is reproduced
-    blanc = blanc + (phi[2] - phi[3])
+    if nest_quirk:
+        blanc = blanc + (phi[2] - phi[3])
     barm = Fo["barm"][j, i]
     A = -barm / (TAU * 28.) * (T00 / T) ** 0.25 / blanc
     dTdz = np.empty(nk)

@@ -70,14 +70,31 @@ def bndcmp(dz):
     return b, fb
 
 
-def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor=1, n2_top=None):
+# Ring-filter chunk tables of the small test grids run as global grids (the
+# model resolutions' tables are tgx.RING_CHUNKS, cons.F:331-355): chunk counts
+# dividing nlon, from the pole equatorward.
+TEST_RING_CHUNKS = {(8, 6): [2, 4], (16, 12): [4, 8, 16], (12, 8): [3, 6]}
+
+
+def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor=1, n2_top=None, glb=False):
     """Grid constants of configuration `name`.  n2_top: if set, make_state
     shifts O in the upper levels so that N2 = 1-O-O2-He falls to n2_top there
-    (the comp post pass then extrapolates N2 above its 0.01 level)."""
+    (the comp post pass then extrapolates N2 above its 0.01 level).  glb: the
+    configuration as a global grid (dt.F ... comp.F): nlat latitude cells
+    from pole to pole (cell centres -90+(j+0.5)dphi, cons.F:304, dphi =
+    pi/nlat), periodic longitudes, the ghost rows on the latitudes across
+    the poles (where cos(lat) changes sign, cons.F:314-317), and the ring
+    filter's chunk table."""
     nlon, nlat, nk, dlev, step, default_step, region = CONFIGS[name]
     nlond, nlatd = nlon + 4, nlat + 4
     dtr = math.pi / 180.
-    if region is None:
+    if glb:
+        assert region is None, "a nest domain is not a global grid"
+        dlat = 180. / nlat
+        glat = [-90. + (j - 2 + .5) * dlat for j in range(nlatd)]
+        dlon = 360. / nlon
+        glon = [-180. + (i - 2) * dlon for i in range(nlond)]
+    elif region is None:
         dlat = 180. / (nlatd + 1)         # keep ghost rows off the poles
         glat = [-90. + (j + 1) * dlat for j in range(nlatd)]
         dlon = 360. / nlon
@@ -108,7 +125,53 @@ def make_grid(name, nstep_sub=1, calc_helium=1, colfac=1.5, opdiffcap=0., difhor
         nstep_sub=nstep_sub, calc_helium=calc_helium, colfac=colfac, opdiffcap=opdiffcap,
         difhor=difhor, b=b, fb=fb, pshelb=0.1154E-5, ar_glbm=[1.2e-3, 1.1e-3],
         step=step, seed=SEED_BASE + sorted(CONFIGS).index(name), n2_top=n2_top,
+        glb=bool(glb), chunk=_ring_chunks(nlon, nlat) if glb else [],
     )
+
+
+def _ring_chunks(nlon, nlat):
+    from . import tgx
+    t = tgx.RING_CHUNKS.get((nlon, nlat)) or TEST_RING_CHUNKS.get((nlon, nlat))
+    if t is None:
+        raise ValueError("no ring-filter chunk table for a %dx%d global grid" % (nlon, nlat))
+    return list(t)
+
+
+# Fields whose values across the poles change sign (fields.F:548-553: the
+# horizontal wind components and their n-1 levels; the other components of
+# vectors are treated alike in the synthetic state).
+POLE_ODD = ("un", "vn", "un_nm", "vn_nm", "ui", "vi", "u_lbc", "v_lbc", "ulbc", "vlbc", "bx", "by")
+
+
+def fill_global_halos(F, grid):
+    """The halos of a global state, as the model fills them before a step:
+    the periodic points from the interior longitudes (mp_periodic_f4d:
+    i = 0, 1 <- nlon, nlon+1 and nlon+2, nlon+3 <- 2, 3), then the two rows
+    across each pole from the interior rows next to it, shifted by 180
+    degrees in longitude, with the sign of POLE_ODD (mk_polelat.F:2-57:
+    latitude 0 <- 1, -1 <- 2, nlat+1 <- nlat, nlat+2 <- nlat-1).  F: dict of
+    logical-layout tensors (3-D (nk, nlat+4, nlon+4), 2-D (nlat+4, nlon+4)),
+    changed in place and returned."""
+    nlon, nlat = grid["nlon"], grid["nlat"]
+    nlond, nlatd = nlon + 4, nlat + 4
+    src = [((i - 2 + nlon // 2) % nlon) + 2 for i in range(nlond)]
+    for n, t in F.items():
+        if n.startswith(("lonb_", "latb_")) or t.shape[-2:] != (nlatd, nlond):
+            continue
+        t[..., 0] = t[..., nlon]
+        t[..., 1] = t[..., nlon + 1]
+        t[..., nlon + 2] = t[..., 2]
+        t[..., nlon + 3] = t[..., 3]
+        sign = -1. if n in POLE_ODD else 1.
+        for jnew, jold in ((1, 2), (0, 3), (nlat + 2, nlat + 1), (nlat + 3, nlat)):
+            t[..., jnew, :] = sign * t[..., jold, src]
+    return F
+
+
+def make_global_state(grid, device="cpu", seed=None):
+    """make_state for a global grid, its halos filled by fill_global_halos."""
+    assert grid.get("glb"), "make_global_state needs a grid made with glb=True"
+    return fill_global_halos(make_state(grid, device=device, seed=seed), grid)
 
 
 def make_state(grid, device="cpu", seed=None):
@@ -126,7 +189,10 @@ def make_state(grid, device="cpu", seed=None):
     zmid = torch.tensor(grid["zpmid"], **kw).view(nk, 1, 1)
     zint = torch.tensor(grid["zpint"], **kw).view(nk, 1, 1)
     s = zmid - ZIBOT
-    X = (2 * math.pi * torch.arange(nlond, **kw) / nlond).view(1, 1, nlond)
+    if grid.get("glb"):   # periodic in longitude over the nlon interior points
+        X = (2 * math.pi * (torch.arange(nlond, **kw) - 2) / nlon).view(1, 1, nlond)
+    else:
+        X = (2 * math.pi * torch.arange(nlond, **kw) / nlond).view(1, 1, nlond)
     Y = (math.pi * (torch.arange(nlatd, **kw) + .5) / nlatd).view(1, nlatd, 1)
     h1 = torch.sin(2 * X + .3) * torch.cos(Y)
     h2 = torch.cos(3 * X - .2) * torch.sin(2 * Y + .1)
