@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtgx.so")
 SOURCES = ["solvers.hip", "step.hip", "api.cpp", "step_api.cpp", "noalloc.hip", "filter.hip", "pack.hip",
-           "host_ctx.cpp", "nest.hip"]
+           "host_ctx.cpp", "nest.hip", "global.hip"]
 
 # FP64 parity kernels follow the reference's arithmetic order: no contraction.
 # Packed FP32 instructions are disabled for the whole library.

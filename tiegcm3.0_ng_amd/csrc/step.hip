@@ -46,6 +46,10 @@ __device__ __forceinline__ double PHI(int r, int c) {
 
 struct Geo {
   int nk, nlon, nlat, nlond, nlatd, ld;
+  // Global grid (dt.F ... comp.F): no lateral Dirichlet data, and the column
+  // kernels write their raw solutions (the host then ring-filters them and
+  // runs the global post passes); 0 for a nest domain.
+  int glb;
   size_t P;
   long ngw;  // groups of 64 columns (j * nlond + i) in a tiled workspace region
   long nwt;  // wavefronts of a tile-kernel launch (4 per column tile)
@@ -80,8 +84,14 @@ __device__ __forceinline__ void cap_put(const tgx_ng_grid &g, const Geo &G, int 
   g.capture[((size_t)pl * ncol + col) * G.nk + k] = v;
 }
 
+// Set by the global-grid entry points around their launches on the calling
+// host thread (tgx_glb_* in step_api.cpp).
+thread_local int t_glb_mode = 0;
+extern "C" void tgx_ng_set_global_mode(int on) { t_glb_mode = on; }
+
 static Geo make_geo(const tgx_ng_grid &g) {
   Geo G;
+  G.glb = t_glb_mode;
   G.nk = g.nk; G.nlon = g.nlon; G.nlat = g.nlat;
   G.nlond = g.nlon + 4; G.nlatd = g.nlat + 4; G.ld = g.ld;
   G.P = (size_t)g.ld * (size_t)(g.nlat + 4);
@@ -283,7 +293,10 @@ __device__ __forceinline__ bool column_of(const Geo &G, int &i, int &j) {
 __device__ __forceinline__ bool lateral_value(const Geo &G, const double *lonb, const double *latb,
                                               int k, int i, int j, double &v) {
   // dt_ng.F90:201-206 and the same hook in every _ng routine: lon ghosts
-  // first, then lat ghosts (corners take the lat data).
+  // first, then lat ghosts (corners take the lat data).  A global grid has
+  // none (its ghost columns are periodic points, its ghost rows lie across
+  // the poles).
+  if (G.glb) return false;
   if (j == 0 || j == G.nlatd - 1) {
     v = latb[(size_t)(k * 2 + (j == 0 ? 0 : 1)) * G.ld + i];
     return true;
@@ -535,7 +548,7 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
   const int i0 = (bb % ntx) * TILE_X, j0 = (bb / ntx) * TILE_Y;
   const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
   const bool active = i < G.nlond && j < G.nlatd;
-  const bool lat = active && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
+  const bool lat = active && !G.glb && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
   const bool solve = active && !lat;
   const int nk = G.nk;
   const double dz = g.dz, dtx2inv = g.dtx2inv;
@@ -762,6 +775,12 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
     if (k > 0) load_db(k - 1, bn);
     asm volatile("" ::: "memory");
     double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
+    if (G.glb) {  // the raw solution; filter_tn and the post pass follow (dt.F:389-424)
+      out.tn[G.a3(k, ic, jc)] = x;
+      xn = x;
+      bc = bn;
+      continue;
+    }
     if (k == nk - 3) {
       double xt = xn * xn / x;
       size_t pt = G.a3(nk - 1, ic, jc);
@@ -1097,8 +1116,10 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
                 sai[0] * (S.do2dz - (1. - (RMO2 + S.dmdz) * rb) * S.pso2) +
                 sai[1] * (S.do1dz - (1. - (RMO1 + S.dmdz) * rb) * S.pso1) +
                 sai[2] * (S.dhedz - (1. - (RMHE + S.dmdz) * rb) * S.pshe);
+    // phi_x(He) in the nest form's sum (minor_ng.F90:135-136), phi_x(N2) in
+    // the global minor.F:359-362
     T.ax = -(S.barm * (1. / (TAU * RMN2)) * S.q /
-             (c.phix[2] + c.salfax[0] * S.pso2 + c.salfax[1] * S.pso1 + c.salfax[2] * S.pshe));
+             (c.phix[G.glb ? 3 : 2] + c.salfax[0] * S.pso2 + c.salfax[1] * S.pso1 + c.salfax[2] * S.pshe));
     T.thdiff = ex - c.alfax * dtnidz / (dzp * S.tni);
     T.sloss = R.sloss;
     T.s0prod = R.s0prod;
@@ -1240,6 +1261,11 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
       size_t p = G.a3(k, i, j);
       double x = k == nk - 1 ? xn : bc.x - bc.w2 * xn;
       xn = x;
+      if (G.glb) {  // the raw solution; filter_minor and the post pass follow (minor.F:546-597)
+        outs[q][p] = x;
+        bc = bn;
+        continue;
+      }
       double nm = DTSMOOTH * bc.f + DTSMOOTH_DIV2 * (bc.fm + x);
       outs[q][p] = x > small ? x : small;
       outs_nm[q][p] = nm > small ? nm : small;
@@ -2412,12 +2438,14 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       Q.dtndz = (R.tn - tl) * 2;
       lev_df(Q);
     };
-    auto levk = [&](const LR &A, const LR &B, Lev &Q) {  // interface between A (L-1) and B (L)
+    // interface between A (L-1) and B (L); the global comp.F takes the top
+    // interface's temperature from the level below it (:174-175, :446-449)
+    auto levk = [&](const LR &A, const LR &B, Lev &Q, bool top) {
       Q.o2i = .5 * (A.o2 + B.o2);
       Q.o1i = .5 * (A.o1 + B.o1);
       Q.hei = .5 * (A.he + B.he);
       Q.embari = .5 * (A.mbar + B.mbar);
-      Q.tni = .5 * (A.tn + B.tn);
+      Q.tni = (G.glb && top) ? A.tn : .5 * (A.tn + B.tn);
       Q.dembardz = B.mbar - A.mbar;
       Q.dtndz = B.tn - A.tn;
       lev_df(Q);
@@ -2468,7 +2496,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       asm volatile("" ::: "memory");
       for (int n = 0; n < 9; ++n) akm[n] = akp[n];
       for (int n = 0; n < 3; ++n) epm[n] = epp[n];
-      levk(r1, r2, Qn);
+      levk(r1, r2, Qn, k == nk - 2);
       const double rem = 1. / Qn.embari;
       for (int l = 0; l < 3; ++l) epp[l] = 1. - rem * (rmass[l] + Qn.dembardz);
       if (k != nk - 2) epp[2] = epp[2] - thdiffalpha * Qn.dtndz / Qn.tni;
@@ -2521,7 +2549,9 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
                     (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Qn.o1i);
         alpha23 = (d_[1] * PHI(2, 3) - PHI(2, 4)) * Qn.o1i;
         alpha32 = (d_[1] * PHI(3, 2) - d_[2] * PHI(3, 4)) * Qn.hei;
-        flx00 = Qc.embari * RIN2 * P0 * pow(T00 / r1.tn, 0.25) / (TAU * GRAV);
+        // the mean mass of the interface below (comp_ng.F90:272) or of the top
+        // interface (the global comp.F:632-633)
+        flx00 = (G.glb ? Qn.embari : Qc.embari) * RIN2 * P0 * pow(T00 / r1.tn, 0.25) / (TAU * GRAV);
         for (int isp = 0; isp < 3; ++isp)
           fk[isp] = fk[isp] - rk[isp + 3 * 1] * (alpha23 - alpha22) * fhe / (flx00 * (rdz - 0.5 * epp[1])) -
                     rk[isp + 3 * 2] * (alpha33 - alpha32) * fhe / (flx00 * (rdz - 0.5 * epp[2]));
@@ -2597,6 +2627,12 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     auto post = [&](int k, const double *u, const BS &B) {
       size_t p = G.a3(k, i, j);
       const double o2u = u[0], o1u = u[1], heu = g.calc_helium == 0 ? g.pshelb : u[2];
+      if (G.glb) {  // the raw solution; filter_o2o and the post pass follow (comp.F:925-1022)
+        out.o2[p] = o2u;
+        out.o1[p] = o1u;
+        out.he[p] = heu;
+        return;
+      }
       double o2nm = DTSMOOTH_DIV2 * (B.o2nm + o2u) + DTSMOOTH * B.o2;
       double o1nm = DTSMOOTH_DIV2 * (B.o1nm + o1u) + DTSMOOTH * B.o1;
       double henm = DTSMOOTH_DIV2 * (B.henm + heu) + DTSMOOTH * B.he;
@@ -2653,7 +2689,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       if (k - 1 >= 0) it(k - 1, s1, s0);
       if (k - 2 >= 0) it(k - 2, s2, s1);
     }
-    if (kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
+    if (!G.glb && kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
       const double zmtop = g.zpmid[kf];
       const double n2top = log(n2kf * xnkf);
       const double n2top1 = log(n2b * xnb);
@@ -3552,7 +3588,7 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
   const int i0 = (bb % ntx) * TILE_X, j0 = (bb / ntx) * TILE_Y;
   const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
   const bool active = i < G.nlond && j < G.nlatd;
-  const bool lat = active && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
+  const bool lat = active && !G.glb && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
   const bool solve = active && !lat;
   const int nk = G.nk, nsub = g.nstep_sub;
   const double dz = g.dz, dtx2inv = g.dtx2inv;
@@ -3823,6 +3859,12 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
     if (k > 0) load_ob(k - 1, bn);
     asm volatile("" ::: "memory");
     double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
+    if (G.glb) {  // the raw solution; filter_op and the post pass follow (oplus.F:896-941)
+      opout[G.a3(k, ic, jc)] = x;
+      xn = x;
+      bc = bn;
+      continue;
+    }
     if (k == nk - 3) {
       double xt = xn * xn / x;
       size_t pt = G.a3(nk - 1, ic, jc);
@@ -4077,7 +4119,7 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
   const int i0 = (bb % ntx) * TILE_X, j0 = (bb / ntx) * TILE_Y;
   const int i = i0 + (int)threadIdx.x % TILE_X, j = j0 + (int)threadIdx.x / TILE_X;
   const bool active = i < G.nlond && j < G.nlatd;
-  const bool lat = active && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
+  const bool lat = active && !G.glb && (j == 0 || j == G.nlatd - 1 || i == 0 || i == G.nlond - 1);
   const bool solve = active && !lat;
   const int nk = G.nk;
   const double dz = g.dz, dtx2inv = g.dtx2inv, wt = 0.225;
@@ -4353,6 +4395,15 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
     else {
       x1 = bc.x1 - bc.g11 * xn1 - bc.g12 * xn2;
       x2 = bc.x2 - bc.g21 * xn1 - bc.g22 * xn2;
+    }
+    if (G.glb) {  // the raw solution; filter_uv and the post pass follow (duv.F:490-514)
+      const size_t p = G.a3(k, i, j);
+      out.un[p] = x1;
+      out.vn[p] = x2;
+      xn1 = x1;
+      xn2 = x2;
+      bc = bn;
+      continue;
     }
     if (k == nk - 3) {
       size_t pt = G.a3(nk - 1, i, j);

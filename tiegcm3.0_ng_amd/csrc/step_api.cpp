@@ -8,12 +8,32 @@
 #include <vector>
 
 #include "../../include/tgx.h"
+#include "../../include/tgx_global.h"
 #include "../../include/tgx_step.h"
 #include "tgx_common.h"
 
-// launchers defined in step.hip
+// launchers defined in step.hip and global.hip
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_fields *in,
                              const tgx_ng_out *out, void *const *bufs, int sub, hipStream_t s);
+extern "C" void tgx_ng_set_global_mode(int on);
+extern "C" int tgx_glb_post(int which, const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                            const double *op_in, const double *nm_in, double *op_o, double *nm_o,
+                            hipStream_t s);
+
+namespace {
+// The global grid of the tgx_glb_* call running on this host thread, or null
+// for a nest domain.  The routines below then run the Thomas scheme with the
+// global grid's column kernels and, after each column kernel, the ring filter
+// and the global post pass of that routine.
+thread_local const tgx_glb_grid *t_glb = nullptr;
+
+// filter_tn / filter_uv / filter_minor / filter_op / filter_o2o: the levels
+// 0..nlev-1 of f on the chunk table's latitudes.
+int glb_filter(const tgx_ng_grid *g, double *f, int nlev, int order, int oplus, hipStream_t s) {
+  return tgx_ringfilter_dev(f, nlev, g->nlon, g->nlat, g->ld, order, oplus, g->dlamda, t_glb->nlat_filter,
+                            t_glb->chunk, s);
+}
+}  // namespace
 
 namespace {
 // Workspace planes, each nk * ld * (nlat+4) doubles.
@@ -176,7 +196,7 @@ int tuned_choice(const tgx_ng_grid *g, int routine) {
   return it == g_tune.end() ? TGX_SCHEME_AUTO : it->second;
 }
 bool use_pcr(const tgx_ng_grid *g, int routine) {
-  if (g->nk > 128) return false;
+  if (g->nk > 128 || t_glb) return false;
   if (t_forced >= 0) return t_forced == TGX_SCHEME_PCR;
   if (g->scheme != TGX_SCHEME_AUTO) return g->scheme == TGX_SCHEME_PCR;
   return tuned_choice(g, routine) != TGX_SCHEME_THOMAS;
@@ -190,8 +210,14 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
   ProfScope prof(R_DT, s);
   void *b[3] = {tiles(work, g, T_DW), plane(work, g, W_DT_T), plane(work, g, W_DT_T + 1)};
   if (!use_pcr(g, R_DT)) {  // Thomas: the stencils are fused into the column kernel
-    ProfScope kern(R_N + R_DT, s);
-    return tgx_ng_launch(K_DT_TILE, g, in, out, b, 0, s);
+    {
+      ProfScope kern(R_N + R_DT, s);
+      if (int rc = tgx_ng_launch(K_DT_TILE, g, in, out, b, 0, s)) return rc;
+    }
+    if (!t_glb) return 0;
+    // dt.F:389-424: filter_tn below the top level, then the post pass
+    if (int rc = glb_filter(g, out->tn, g->nk - 1, 3, 0, s)) return rc;
+    return tgx_glb_post(0, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
   }
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_DT, s);
@@ -270,16 +296,28 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
     ProfScope kern(R_N + R_MINORS, s);  // the N2D + Ar column kernel
     if (int rc = tgx_ng_launch(pcr ? K_MIN_WA : K_MIN_A, g, in, out, b, 0, s)) return rc;
   }
+  // minor.F:546-597 for N2D and Ar before NO and N4S read this step's N2D
+  // (dynamics.F:485-600): filter_minor, then the post pass
+  if (t_glb) {
+    if (int rc = glb_filter(g, out->n2d, g->nk, 3, 0, s)) return rc;
+    if (int rc = glb_filter(g, out->ar, g->nk, 3, 0, s)) return rc;
+    if (int rc = tgx_glb_post(2, g, in, out, nullptr, nullptr, nullptr, nullptr, s)) return rc;
+  }
   if (chem_done) {  // NO and N4S sources formed in the column kernel from the
                     // step's chemistry pass and this N2D
     void *b2[20];
     for (int q = 0; q < 14; ++q) b2[q] = b[q];
     b2[14] = tiles(work, g, T_ML);
     for (int q = 15; q < 20; ++q) b2[q] = nullptr;
-    return tgx_ng_launch(pcr ? K_MIN_WB_LIN : K_MIN_B_LIN, g, in, out, b2, 0, s);
+    if (int rc = tgx_ng_launch(pcr ? K_MIN_WB_LIN : K_MIN_B_LIN, g, in, out, b2, 0, s)) return rc;
+  } else {
+    if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
+    if (int rc = tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s)) return rc;
   }
-  if (int rc = tgx_ng_launch(K_MIN_SRC_B, g, in, out, b, 0, s)) return rc;
-  return tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s);
+  if (!t_glb) return 0;
+  if (int rc = glb_filter(g, out->no, g->nk, 3, 0, s)) return rc;
+  if (int rc = glb_filter(g, out->n4s, g->nk, 3, 0, s)) return rc;
+  return tgx_glb_post(3, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
@@ -324,9 +362,15 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
     if (pcr)  // Thomas: the stencils are fused into the column kernel
       if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
-    ProfScope kern(R_N + R_OPLUS, s);
-    const int col = pcr ? K_OP3_W : !reuse ? K_OP_TILE : sub == 0 ? K_OP_TILE_KEEP : K_OP_TILE_REUSE;
-    if (int rc = tgx_ng_launch(col, g, in, out, b, sub, s)) return rc;
+    {
+      ProfScope kern(R_N + R_OPLUS, s);
+      const int col = pcr ? K_OP3_W : !reuse ? K_OP_TILE : sub == 0 ? K_OP_TILE_KEEP : K_OP_TILE_REUSE;
+      if (int rc = tgx_ng_launch(col, g, in, out, b, sub, s)) return rc;
+    }
+    if (t_glb) {  // oplus.F:896-941: filter_op below the top level, then the post pass
+      if (int rc = glb_filter(g, op_o, g->nk - 1, 3, 1, s)) return rc;
+      if (int rc = tgx_glb_post(4, g, in, out, op_in, nm_in, op_o, nm_o, s)) return rc;
+    }
     op_in = op_o;
     nm_in = nm_o;
   }
@@ -369,8 +413,15 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
   b[1] = tiles(work, g, T_UW);  // Thomas gamma and forward solution (K_UV_TILE)
   for (int q = 0; q < 6; ++q) b[2 + q] = plane(work, g, W_UV + 1 + q);
   if (!use_pcr(g, R_DUV)) {  // Thomas: stage 1 and the stencils are fused into the column kernel
-    ProfScope kern(R_N + R_DUV, s);
-    return tgx_ng_launch(K_UV_TILE, g, in, out, b, 0, s);
+    {
+      ProfScope kern(R_N + R_DUV, s);
+      if (int rc = tgx_ng_launch(K_UV_TILE, g, in, out, b, 0, s)) return rc;
+    }
+    if (!t_glb) return 0;
+    // duv.F:490-514: filter_uv of UN and VN below the top level, then the post pass
+    if (int rc = glb_filter(g, out->un, g->nk - 1, 3, 0, s)) return rc;
+    if (int rc = glb_filter(g, out->vn, g->nk - 1, 3, 0, s)) return rc;
+    return tgx_glb_post(1, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
   }
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
@@ -393,8 +444,16 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   if (!chem_done)
     if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
-  ProfScope kern(R_N + R_COMP, s);
-  return tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s);
+  {
+    ProfScope kern(R_N + R_COMP, s);
+    if (int rc = tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s)) return rc;
+  }
+  if (!t_glb) return 0;
+  // comp.F:925-1022: filter_o2o of O2, O and He, then the post pass
+  if (int rc = glb_filter(g, out->o2, g->nk, 1, 0, s)) return rc;
+  if (int rc = glb_filter(g, out->o1, g->nk, 1, 0, s)) return rc;
+  if (int rc = glb_filter(g, out->he, g->nk, 1, 0, s)) return rc;
+  return tgx_glb_post(5, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 }  // namespace
 
@@ -509,7 +568,7 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   // dynamics_ng.F90 order: oplus (:99-121), minors (:147-165), dt (:203),
   // duv (:210), comp_o2o + comp (:221-233)
   if (int rc = check(g, work)) return rc;
-  if (g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
+  if (!t_glb && g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
     if (int rc = tgx_ng_autotune(g, in, out, work, stream)) return rc;
   Fork *F = nullptr;
   if (int rc = get_fork(&F)) return rc;
@@ -550,7 +609,7 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
 extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
                                   const tgx_ng_out *out, void *work, void *stream) {
   if (int rc = check(g, work)) return rc;
-  if (g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
+  if (!t_glb && g->scheme == TGX_SCHEME_AUTO && g->nk <= 128 && tuned_choice(g, R_OPLUS) == TGX_SCHEME_AUTO)
     if (int rc = tgx_ng_autotune(g, in, out, work, stream)) return rc;
   if (int rc = run_chem(g, in, out, work, (hipStream_t)stream)) return rc;
   if (int rc = run_oplus(g, in, out, work, stream, true, g->nstep_sub)) return rc;
@@ -558,4 +617,74 @@ extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
   if (int rc = tgx_ng_dt(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_duv(g, in, out, work, stream)) return rc;
   return run_comp(g, in, out, work, stream, true);
+}
+
+// ---------------------------------------------------------------------------
+// The global grid (include/tgx_global.h): the routines above on a grid whose
+// ghost columns are periodic points and whose ghost rows lie across the poles.
+namespace {
+int check_glb(const tgx_glb_grid *gg) {
+  if (!gg) return tgx_set_error(TGX_ERR_ARG, "tgx_glb: null grid");
+  const tgx_ng_grid &g = gg->g;
+  if (g.nlon < 4 || (g.nlon & 1) || g.nlon > TGX_RING_MAX_NLON)
+    return tgx_set_error(TGX_ERR_ARG, "tgx_glb: nlon %d must be even, 4..%d", g.nlon, TGX_RING_MAX_NLON);
+  if (gg->nlat_filter < 0 || 2 * gg->nlat_filter > g.nlat || gg->nlat_filter > TGX_RING_MAX_ROWS / 2 ||
+      (gg->nlat_filter > 0 && !gg->chunk))
+    return tgx_set_error(TGX_ERR_ARG, "tgx_glb: nlat_filter %d (0..%d, at most half of nlat %d) or chunk table",
+                         gg->nlat_filter, TGX_RING_MAX_ROWS / 2, g.nlat);
+  for (int l = 0; l < gg->nlat_filter; ++l)
+    if (gg->chunk[l] < 1 || gg->chunk[l] > g.nlon || g.nlon % gg->chunk[l] != 0)
+      return tgx_set_error(TGX_ERR_ARG, "tgx_glb: chunk count %d does not divide nlon %d", gg->chunk[l], g.nlon);
+  return 0;
+}
+// Sets this thread's global grid for the duration of one entry point.
+struct GlbScope {
+  const tgx_glb_grid *prev;
+  explicit GlbScope(const tgx_glb_grid *gg) : prev(t_glb) {
+    t_glb = gg;
+    tgx_ng_set_global_mode(1);
+  }
+  ~GlbScope() {
+    t_glb = prev;
+    tgx_ng_set_global_mode(prev != nullptr);
+  }
+};
+int glb_call(RoutineFn fn, const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+             void *stream) {
+  if (int rc = check_glb(gg)) return rc;
+  GlbScope scope(gg);
+  return fn(&gg->g, in, out, work, stream);
+}
+}  // namespace
+
+extern "C" unsigned long long tgx_glb_workspace_bytes(const tgx_glb_grid *gg) {
+  return gg ? tgx_ng_workspace_bytes(&gg->g) : 0;
+}
+extern "C" int tgx_glb_oplus(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                             void *stream) {
+  return glb_call(tgx_ng_oplus, gg, in, out, work, stream);
+}
+extern "C" int tgx_glb_minors(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out,
+                              void *work, void *stream) {
+  return glb_call(tgx_ng_minors, gg, in, out, work, stream);
+}
+extern "C" int tgx_glb_dt(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                          void *stream) {
+  return glb_call(tgx_ng_dt, gg, in, out, work, stream);
+}
+extern "C" int tgx_glb_duv(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                           void *stream) {
+  return glb_call(tgx_ng_duv, gg, in, out, work, stream);
+}
+extern "C" int tgx_glb_comp(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                            void *stream) {
+  return glb_call(tgx_ng_comp, gg, in, out, work, stream);
+}
+extern "C" int tgx_glb_step(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                            void *stream) {
+  return glb_call(tgx_ng_step, gg, in, out, work, stream);
+}
+extern "C" int tgx_glb_step_serial(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out,
+                                   void *work, void *stream) {
+  return glb_call(tgx_ng_step_serial, gg, in, out, work, stream);
 }

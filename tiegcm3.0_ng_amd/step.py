@@ -11,6 +11,7 @@ import ctypes as C
 
 import torch
 
+from . import synth as _synth
 from . import tgx as _tgx
 
 _P = C.c_void_p
@@ -73,6 +74,15 @@ _SIGS = {
     "tgx_ng_autotune": ([_P] * 5, C.c_int),
     "tgx_ng_scheme_choice": ([_P, C.c_int], C.c_int),
     "tgx_ng_profile_end": ([_P, _P], C.c_int),
+    "tgx_glb_workspace_bytes": ([_P], C.c_ulonglong),
+    "tgx_glb_oplus": ([_P] * 5, C.c_int),
+    "tgx_glb_minors": ([_P] * 5, C.c_int),
+    "tgx_glb_dt": ([_P] * 5, C.c_int),
+    "tgx_glb_duv": ([_P] * 5, C.c_int),
+    "tgx_glb_comp": ([_P] * 5, C.c_int),
+    "tgx_glb_step": ([_P] * 5, C.c_int),
+    "tgx_glb_step_serial": ([_P] * 5, C.c_int),
+    "tgx_glb_halos": ([_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _P], C.c_int),
 }
 
 ROUTINES = ("oplus", "minors", "dt", "duv", "comp")
@@ -491,3 +501,66 @@ def smoke_check(oracle, name="tiny", tol=1e-12):
         assert err <= tol, (n, err)
     print("smoke ok: fused column step on %s, worst field-relative error %.3e" % (name, worst))
     return worst
+
+
+class GlbGrid(C.Structure):
+    _fields_ = [("g", NgGrid), ("nlat_filter", C.c_int), ("chunk", _P)]
+
+
+
+
+class GlobalState(NestState):
+    """The global grid on the GPU (include/tgx_global.h): the same state,
+    outputs and workspace as a nest domain, run through the global forms of
+    the routines (dt.F, duv.F, minor.F, oplus.F, comp_o2o.F + comp.F) with
+    their ring filters.  grid: synth.make_grid(..., glb=True); fields: a
+    state whose halos are filled (synth.make_global_state).  The global forms
+    run the Thomas scheme."""
+
+    _NAMES = {"tgx_ng_oplus": "tgx_glb_oplus", "tgx_ng_minors": "tgx_glb_minors", "tgx_ng_dt": "tgx_glb_dt",
+              "tgx_ng_duv": "tgx_glb_duv", "tgx_ng_comp": "tgx_glb_comp", "tgx_ng_step": "tgx_glb_step",
+              "tgx_ng_step_serial": "tgx_glb_step_serial"}
+
+    def __init__(self, grid, fields, device="cuda", capture=False):
+        assert grid.get("glb"), "GlobalState needs a grid made with glb=True"
+        super().__init__(grid, fields, device=device, scheme="thomas", capture=capture)
+        self.chunk = (C.c_int * max(1, len(grid["chunk"])))(*[int(c) for c in grid["chunk"]])
+        self.gg = GlbGrid()
+        self.gg.g = self.g
+        self.gg.nlat_filter = len(grid["chunk"])
+        self.gg.chunk = C.cast(self.chunk, _P)
+
+    def _call(self, name, stream=None):
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        f, o = (self.f, self.o)
+        phases = getattr(self, "_phases", None)
+        stepping = name in ("tgx_ng_step", "tgx_ng_step_serial")
+        if phases and stepping:
+            f, o = phases[self.phase]
+        self.gg.g = self.g
+        rc = getattr(_lib(), self._NAMES[name])(C.byref(self.gg), C.byref(f), C.byref(o),
+                                                C.c_void_p(self.work.data_ptr()), C.c_void_p(s))
+        if rc != 0:
+            raise _tgx.TgxError(_tgx.last_error())
+        if phases and stepping:
+            self.phase ^= 1
+
+    def comp_mixed(self, nsweep, stream=None):
+        raise NotImplementedError("the precision form of comp is built for nest domains only")
+
+    def autotune(self, stream=None):
+        pass  # the global forms run the Thomas scheme
+
+    def fill_halos(self, names=PROGNOSTIC, stream=None):
+        """tgx_glb_halos on the current value of each field: its periodic
+        points and the rows across the poles (as the model does before a
+        step, mk_polelat.F), sign-flipped for synth.POLE_ODD."""
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        for n in names:
+            t = self.current(n)
+            nk = t.shape[0] if t.dim() == 3 else 1
+            sign = -1. if n in _synth.POLE_ODD else 1.
+            rc = _lib().tgx_glb_halos(C.c_void_p(t.data_ptr()), nk, self.grid["nlon"], self.grid["nlat"],
+                                      self.ld, C.c_double(sign), C.c_void_p(s))
+            if rc != 0:
+                raise _tgx.TgxError(_tgx.last_error())
