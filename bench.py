@@ -4,10 +4,14 @@
 Workload (default): the 0.625 deg global grid, 576 x 288 columns x 113 levels,
 all 11 unknowns per column per step (TN; UN+VN as the 2x2 block; O+; NO,
 N4S, N2D, Ar; O2+O+He as the 3x3 block), one O+ sub-cycle, FP64, seeded
-synthetic fields resident in HBM.  A step is one call of tgx_ng_step, the
-whole column step over that state: the routines of dynamics_ng.F90 with the
-independent ones on forked streams, bitwise equal to the reference's order
-(--serial times that order on one stream instead).
+synthetic fields resident in HBM.  A step is one call of tgx_glb_step, the
+whole column step of the global model over that state (dt.F, duv.F, minor.F,
+oplus.F, comp.F with their ring filters, periodic longitudes and the rows
+across the poles), the independent routines on forked streams, bitwise equal
+to the reference's order (--serial times that order on one stream instead).
+--domain nest runs the same grid as one nest domain (tgx_ng_step, lateral
+Dirichlet data at its edges); the global run also reports that form's step
+time beside it.
 
 Prints one JSON line:
   value   = Mcolumn-level solves/s over the interior columns (whole job),
@@ -320,6 +324,8 @@ def main():
                     help="FP32 + FP64-refinement solves against the FP64 solvers on the assembled systems")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
+    ap.add_argument("--domain", default="global", choices=("global", "nest"),
+                    help="the grid as the global model (tgx_glb_step) or as one nest domain (tgx_ng_step)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: measures one GPU only (the column step of one domain needs no "
@@ -341,10 +347,16 @@ def main():
         return bench_nests(pkg, args)
     if args.precision:
         return bench_precision(pkg, args)
-    g = pkg.synth.make_grid(args.config, nstep_sub=args.nstep_sub)
-    print("# generating %s state on the GPU" % args.config, file=sys.stderr, flush=True)
-    F = pkg.synth.make_state(g, device="cuda")
-    ns = pkg.step.NestState(g, F, scheme=args.scheme)
+    glb = args.domain == "global" and pkg.synth.CONFIGS[args.config][6] is None
+    g = pkg.synth.make_grid(args.config, nstep_sub=args.nstep_sub, glb=glb)
+    print("# generating %s state on the GPU (%s)" % (args.config, "global" if glb else "nest"), file=sys.stderr,
+          flush=True)
+    if glb:
+        F = pkg.synth.make_global_state(g, device="cuda")
+        ns = pkg.step.GlobalState(g, F)
+    else:
+        F = pkg.synth.make_state(g, device="cuda")
+        ns = pkg.step.NestState(g, F, scheme=args.scheme)
     del F
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
@@ -434,9 +446,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded profiles, device-generated)",
-        "config": {"workload": "%s global %dx%d columns x %d levels, %d unknowns/column/level, "
-                               "nstep_sub=%d, fused assembly+solve FP64" % (
-                                   args.config, g["nlon"], g["nlat"], nk, unknowns, nsub),
+        "config": {"workload": "%s %s %dx%d columns x %d levels, %d unknowns/column/level, "
+                               "nstep_sub=%d, fused assembly+solve FP64%s" % (
+                                   args.config, "global grid (periodic, poles, ring filters)" if glb else
+                                   "as one nest domain (lateral Dirichlet edges)", g["nlon"], g["nlat"], nk,
+                                   unknowns, nsub, "" if not glb else ", tgx_glb_step"),
+                   "domain": "global" if glb else "nest",
                    "columns": cols, "levels": nk, "nstep_sub": nsub, "precision": "fp64",
                    "schedule": "serial" if serial else "concurrent", "scheme": args.scheme,
                    "schemes": schemes},
@@ -455,13 +470,43 @@ def main():
     }
     if not args.only:
         out["aux_kernels"] = bench_aux_kernels(pkg, g, ns)
-    if args.only in (None, "comp") and not args.no_precision_variant:
-        out["precision_variant"] = bench_comp_precision(pkg, ns, cols * nk)
     if not args.only and nsub == 1 and args.oplus_sub > 1:
         out["oplus_nstep_sub%d" % args.oplus_sub] = bench_oplus_subcycled(pkg, args, ns)
+    nest = ns
+    if glb and (not args.only or args.only == "comp"):
+        # the same grid as one nest domain: the precision form of comp is built
+        # there, and its step time shows what the global passes cost
+        nest = None
+        del ns
+        torch.cuda.empty_cache()
+        gn = pkg.synth.make_grid(args.config, nstep_sub=args.nstep_sub)
+        F = pkg.synth.make_state(gn, device="cuda")
+        nest = pkg.step.NestState(gn, F, scheme=args.scheme)
+        del F
+        torch.cuda.empty_cache()
+        if not args.only:
+            out["nest_form"] = bench_nest_form(pkg, args, nest, cols * nk * unknowns)
+    if args.only in (None, "comp") and not args.no_precision_variant:
+        out["precision_variant"] = bench_comp_precision(pkg, nest, cols * nk)
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
+
+
+def bench_nest_form(pkg, args, nest, colev):
+    """tgx_ng_step on the same grid run as one nest domain (lateral Dirichlet
+    data, no ring filters): the global form's cost over it."""
+    import torch
+    for _ in range(args.warmup):
+        nest.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nest.step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    return {"ms_per_step": round(ms, 4), "Mcolumn_level_solves_per_s": round(colev / (ms * 1e-3) / 1e6, 2),
+            "schemes": nest.schemes(), "steps": args.steps}
 
 
 def bench_aux_kernels(pkg, g, ns, reps=20):
@@ -572,9 +617,14 @@ def bench_oplus_subcycled(pkg, args, ns1, reps=10):
         return prof["oplus"][0] / max(prof["oplus"][1], 1)
 
     ms1 = timed(ns1)
-    g = pkg.synth.make_grid(args.config, nstep_sub=nsub)
-    F = pkg.synth.make_state(g, device="cuda")
-    ns = pkg.step.NestState(g, F, scheme=args.scheme)
+    glb = bool(ns1.grid.get("glb"))
+    g = pkg.synth.make_grid(args.config, nstep_sub=nsub, glb=glb)
+    if glb:
+        F = pkg.synth.make_global_state(g, device="cuda")
+        ns = pkg.step.GlobalState(g, F)
+    else:
+        F = pkg.synth.make_state(g, device="cuda")
+        ns = pkg.step.NestState(g, F, scheme=args.scheme)
     del F
     ms = timed(ns)
     ns_scheme = ns.schemes()["oplus"]

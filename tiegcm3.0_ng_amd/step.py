@@ -551,6 +551,9 @@ class GlobalState(NestState):
     def autotune(self, stream=None):
         pass  # the global forms run the Thomas scheme
 
+    def schemes(self):
+        return {r: "thomas" for r in ROUTINES}
+
     def fill_halos(self, names=PROGNOSTIC, stream=None):
         """tgx_glb_halos on the current value of each field: its periodic
         points and the rows across the poles (as the model does before a
