@@ -324,6 +324,8 @@ def main():
                     help="FP32 + FP64-refinement solves against the FP64 solvers on the assembled systems")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
+    ap.add_argument("--no-nest-form", action="store_true",
+                    help="global runs: skip timing the same grid as a nest domain")
     ap.add_argument("--domain", default="global", choices=("global", "nest"),
                     help="the grid as the global model (tgx_glb_step) or as one nest domain (tgx_ng_step)")
     args = ap.parse_args()
@@ -473,7 +475,8 @@ def main():
     if not args.only and nsub == 1 and args.oplus_sub > 1:
         out["oplus_nstep_sub%d" % args.oplus_sub] = bench_oplus_subcycled(pkg, args, ns)
     nest = ns
-    if glb and (not args.only or args.only == "comp"):
+    if glb and ((not args.only and not args.no_nest_form) or args.only == "comp") and \
+            not (args.no_nest_form and args.no_precision_variant):
         # the same grid as one nest domain: the precision form of comp is built
         # there, and its step time shows what the global passes cost
         nest = None
@@ -484,7 +487,7 @@ def main():
         nest = pkg.step.NestState(gn, F, scheme=args.scheme)
         del F
         torch.cuda.empty_cache()
-        if not args.only:
+        if not args.only and not args.no_nest_form:
             out["nest_form"] = bench_nest_form(pkg, args, nest, cols * nk * unknowns)
     if args.only in (None, "comp") and not args.no_precision_variant:
         out["precision_variant"] = bench_comp_precision(pkg, nest, cols * nk)

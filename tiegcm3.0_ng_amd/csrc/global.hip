@@ -1,11 +1,12 @@
 // The global grid's passes around the column kernels: what each global
 // routine does after its column solve and ring filter (the time smoothing,
-// the top level, the floors, comp's N2 and normalisation), the periodic
-// points of the outputs (mp_periodic_f3d) and the halos of an input field
-// (mk_polelat.F, mp_periodic_f4d).  Level planes as in tgx_step.h; the
-// elementwise passes take one thread per (interior longitude, interior
-// latitude, level), the longitudes fastest, so a wavefront reads 64
-// consecutive doubles of a plane.
+// the top level, the floors, comp's N2 and normalisation) on the latitudes
+// the ring filter reaches (the column kernels apply it themselves on the
+// others), the periodic points of the outputs (mp_periodic_f3d) and the
+// halos of an input field (mk_polelat.F, mp_periodic_f4d).  Level planes as
+// in tgx_step.h; the elementwise passes take one thread per (interior
+// longitude, latitude, level), the longitudes fastest, so a wavefront reads
+// 64 consecutive doubles of a plane.
 #include <hip/hip_runtime.h>
 
 #include "../../include/tgx.h"
@@ -22,8 +23,15 @@ enum Kind { P_DT = 0, P_DUV, P_MINOR, P_OPLUS, P_COMP };
 
 struct Dims {
   int nk, nlon, nlat, ld;
+  int nfilt;     // latitudes the ring filter reaches next to each pole
   size_t plane;  // ld * (nlat + 4)
 };
+
+// Row j of the r-th filtered latitude (r < 2 nfilt): the nfilt next to the
+// south pole, then the nfilt next to the north pole.
+__device__ __forceinline__ int filtered_row(const Dims &D, int r) {
+  return r < D.nfilt ? 2 + r : D.nlat + 2 - 2 * D.nfilt + r;
+}
 
 // Up to four solved fields of one pass: the solution (raw on entry), its
 // n-1 output, and the step's inputs at n and n-1.
@@ -33,15 +41,17 @@ struct Fields {
   int n;
 };
 
-// Interior point (i, j) of the calling thread and its level k, or false.
-__device__ __forceinline__ bool interior(const Dims &D, int &i, int &j, int &k) {
+// Interior longitude i of a filtered latitude j and level k of the calling
+// thread, or false.
+__device__ __forceinline__ bool filtered_point(const Dims &D, int &i, int &j, int &k) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nrow = (long)D.nlon * D.nlat;
+  const long nrow = (long)D.nlon * 2 * D.nfilt;
   if (t >= nrow * D.nk) return false;
   k = (int)(t / nrow);
   const long r = t - (long)k * nrow;
-  j = 2 + (int)(r / D.nlon);
-  i = 2 + (int)(r - (long)(j - 2) * D.nlon);
+  const int row = (int)(r / D.nlon);
+  j = filtered_row(D, row);
+  i = 2 + (int)(r - (long)row * D.nlon);
   return true;
 }
 
@@ -51,7 +61,7 @@ __device__ __forceinline__ bool interior(const Dims &D, int &i, int &j, int &k) 
 template <int KIND>
 __global__ void __launch_bounds__(256) post_kernel(Dims D, Fields F) {
   int i, j, k;
-  if (!interior(D, i, j, k)) return;
+  if (!filtered_point(D, i, j, k)) return;
   const size_t p = (size_t)k * D.plane + (size_t)j * D.ld + i;
   const bool top = k == D.nk - 1;
   for (int q = 0; q < F.n; ++q) {
@@ -101,8 +111,9 @@ struct CompIO {
 
 __global__ void __launch_bounds__(256) comp_post_kernel(Dims D, CompIO C) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)D.nlon * D.nlat) return;
-  const int j = 2 + (int)(t / D.nlon), i = 2 + (int)(t - (long)(j - 2) * D.nlon);
+  if (t >= (long)D.nlon * 2 * D.nfilt) return;
+  const int row = (int)(t / D.nlon);
+  const int j = filtered_row(D, row), i = 2 + (int)(t - (long)row * D.nlon);
   const size_t c = (size_t)j * D.ld + i;
   const int nk = D.nk;
   const double small = 1.e-9;
@@ -216,8 +227,9 @@ __global__ void __launch_bounds__(256) halos_kernel(Dims D, double *__restrict__
 
 inline unsigned nblocks(long n) { return (unsigned)((n + 255) / 256); }
 
-Dims dims_of(const tgx_ng_grid *g) {
+Dims dims_of(const tgx_ng_grid *g, int nfilt) {
   Dims D;
+  D.nfilt = nfilt;
   D.nk = g->nk;
   D.nlon = g->nlon;
   D.nlat = g->nlat;
@@ -241,16 +253,19 @@ int periodic(const Dims &D, double *const *f, int n, int nk, hipStream_t s) {
 
 using namespace tgx::glb;
 
-// The post passes of the global routines, launched by step_api.cpp after each
-// routine's column kernel and ring filter.  which: 0 dt, 1 duv, 2 minor
-// species N2D and Ar, 3 minor species NO and N4S, 4 O+ sub-step (op_in,
-// nm_in: the sub-step's inputs, op_o, nm_o: its outputs, whose rows across
-// the poles take those of in->op / in->op_nm), 5 comp.
-extern "C" int tgx_glb_post(int which, const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
-                            const double *op_in, const double *nm_in, double *op_o, double *nm_o,
-                            hipStream_t s) {
-  const Dims D = dims_of(g);
-  const long nel = (long)D.nlon * D.nlat * D.nk;
+// The post passes of the global routines on the nfilt filtered latitudes
+// next to each pole, launched by step_api.cpp after each routine's column
+// kernel and ring filter.  which: 0 dt, 1 duv, 2 minor species N2D and Ar,
+// 3 minor species NO and N4S, 4 O+ sub-step (op_in, nm_in: the sub-step's
+// inputs, op_o, nm_o: its outputs, whose rows across the poles take those of
+// in->op / in->op_nm), 5 comp.  Then the periodic points of the outputs on
+// every interior latitude.
+extern "C" int tgx_glb_post(int which, int nfilt, const tgx_ng_grid *g, const tgx_ng_fields *in,
+                            const tgx_ng_out *out, const double *op_in, const double *nm_in, double *op_o,
+                            double *nm_o, hipStream_t s) {
+  const Dims D = dims_of(g, nfilt);
+  const long nel = (long)D.nlon * 2 * nfilt * D.nk;
+  if (nel == 0) which += 100;  // nothing to filter: the periodic points only
   Fields F = {};
   switch (which) {
     case 0:
@@ -290,11 +305,12 @@ extern "C" int tgx_glb_post(int which, const tgx_ng_grid *g, const tgx_ng_fields
       C.xnmbar = in->xnmbar;
       C.zpmid = g->zpmid;
       C.dz = g->dz;
-      hipLaunchKernelGGL(comp_post_kernel, dim3(nblocks((long)D.nlon * D.nlat)), dim3(256), 0, s, D, C);
+      hipLaunchKernelGGL(comp_post_kernel, dim3(nblocks((long)D.nlon * 2 * nfilt)), dim3(256), 0, s, D, C);
       break;
     }
     default:
-      return tgx_set_error(TGX_ERR_ARG, "tgx_glb_post: unknown pass %d", which);
+      if (which < 100 || which > 105) return tgx_set_error(TGX_ERR_ARG, "tgx_glb_post: unknown pass %d", which);
+      which -= 100;
   }
   TGX_HIP(hipGetLastError());
   // mp_periodic_f3d of every output the pass wrote
@@ -329,6 +345,7 @@ extern "C" int tgx_glb_halos(double *f, int nk, int nlon, int nlat, int ld, doub
     return tgx_set_error(TGX_ERR_ARG, "tgx_glb_halos: bad arguments (nk %d nlon %d (even) nlat %d ld %d)", nk, nlon,
                          nlat, ld);
   Dims D;
+  D.nfilt = 0;
   D.nk = nk;
   D.nlon = nlon;
   D.nlat = nlat;

@@ -24,6 +24,7 @@ constexpr double RE = 6.37122e8, AVO = 6.023e23, P0 = 5.0e-4, GASK = 8.314e7;
 constexpr double BRN2D = 0.6, DTSMOOTH = 0.95, GRAV = 870.0;
 constexpr double TSURPLUS = 5.11 * 1.602e-12;
 constexpr double DTSMOOTH_DIV2 = 0.5 * (1. - DTSMOOTH);
+constexpr double SPVAL = 1.e36;  // params.F:38
 constexpr double DZGRAV = GRAV / GASK;
 constexpr double RE_INV = 1. / RE;
 constexpr double RMO2 = 32., RMO1 = 16., RMHE = 4., RMN2 = 28., RMN4S = 14., RMNO = 30.;
@@ -46,10 +47,16 @@ __device__ __forceinline__ double PHI(int r, int c) {
 
 struct Geo {
   int nk, nlon, nlat, nlond, nlatd, ld;
-  // Global grid (dt.F ... comp.F): no lateral Dirichlet data, and the column
-  // kernels write their raw solutions (the host then ring-filters them and
-  // runs the global post passes); 0 for a nest domain.
-  int glb;
+  // Global grid (dt.F ... comp.F): no lateral Dirichlet data; on the
+  // latitudes the ring filter reaches (nfilt next to each pole, and the rows
+  // across the poles) the column kernels write their raw solutions, which
+  // the host then filters and passes through the global post pass; on the
+  // other latitudes they apply the post pass themselves.  glb = 0 for a nest
+  // domain.
+  int glb, nfilt;
+  __device__ __forceinline__ bool raw_row(int j) const {
+    return glb && (j <= nfilt + 1 || j >= nlat - nfilt + 2);
+  }
   size_t P;
   long ngw;  // groups of 64 columns (j * nlond + i) in a tiled workspace region
   long nwt;  // wavefronts of a tile-kernel launch (4 per column tile)
@@ -86,12 +93,16 @@ __device__ __forceinline__ void cap_put(const tgx_ng_grid &g, const Geo &G, int 
 
 // Set by the global-grid entry points around their launches on the calling
 // host thread (tgx_glb_* in step_api.cpp).
-thread_local int t_glb_mode = 0;
-extern "C" void tgx_ng_set_global_mode(int on) { t_glb_mode = on; }
+thread_local int t_glb_mode = 0, t_glb_nfilt = 0;
+extern "C" void tgx_ng_set_global_mode(int on, int nfilt) {
+  t_glb_mode = on;
+  t_glb_nfilt = on ? nfilt : 0;
+}
 
 static Geo make_geo(const tgx_ng_grid &g) {
   Geo G;
   G.glb = t_glb_mode;
+  G.nfilt = t_glb_nfilt;
   G.nk = g.nk; G.nlon = g.nlon; G.nlat = g.nlat;
   G.nlond = g.nlon + 4; G.nlatd = g.nlat + 4; G.ld = g.ld;
   G.P = (size_t)g.ld * (size_t)(g.nlat + 4);
@@ -775,13 +786,18 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
     if (k > 0) load_db(k - 1, bn);
     asm volatile("" ::: "memory");
     double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
-    if (G.glb) {  // the raw solution; filter_tn and the post pass follow (dt.F:389-424)
+    if (G.raw_row(jc)) {  // the raw solution; filter_tn and the post pass follow (dt.F:389-424)
       out.tn[G.a3(k, ic, jc)] = x;
       xn = x;
       bc = bn;
       continue;
     }
-    if (k == nk - 3) {
+    if (G.glb && k == nk - 2) {  // spval at the top level (dt.F:407-408)
+      size_t pt = G.a3(nk - 1, ic, jc);
+      out.tn[pt] = SPVAL;
+      out.tn_nm[pt] = SPVAL;
+    }
+    if (!G.glb && k == nk - 3) {
       double xt = xn * xn / x;
       size_t pt = G.a3(nk - 1, ic, jc);
       out.tn_nm[pt] = DTSMOOTH_DIV2 * (tnm_top + xt) + DTSMOOTH * tn_top;
@@ -1261,7 +1277,7 @@ __global__ void __launch_bounds__(256) minor_kernel(tgx_ng_grid g, Geo G, tgx_ng
       size_t p = G.a3(k, i, j);
       double x = k == nk - 1 ? xn : bc.x - bc.w2 * xn;
       xn = x;
-      if (G.glb) {  // the raw solution; filter_minor and the post pass follow (minor.F:546-597)
+      if (G.raw_row(j)) {  // the raw solution; filter_minor and the post pass follow (minor.F:546-597)
         outs[q][p] = x;
         bc = bn;
         continue;
@@ -2627,7 +2643,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
     auto post = [&](int k, const double *u, const BS &B) {
       size_t p = G.a3(k, i, j);
       const double o2u = u[0], o1u = u[1], heu = g.calc_helium == 0 ? g.pshelb : u[2];
-      if (G.glb) {  // the raw solution; filter_o2o and the post pass follow (comp.F:925-1022)
+      if (G.raw_row(j)) {  // the raw solution; filter_o2o and the post pass follow (comp.F:925-1022)
         out.o2[p] = o2u;
         out.o1[p] = o1u;
         out.he[p] = heu;
@@ -2689,7 +2705,7 @@ __global__ void __launch_bounds__(256) comp_kernel(tgx_ng_grid g, Geo G, tgx_ng_
       if (k - 1 >= 0) it(k - 1, s1, s0);
       if (k - 2 >= 0) it(k - 2, s2, s1);
     }
-    if (!G.glb && kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
+    if (!G.raw_row(j) && kf >= 1 && kf + 1 < nk) {  // :419-431 extrapolated N2 above kf
       const double zmtop = g.zpmid[kf];
       const double n2top = log(n2kf * xnkf);
       const double n2top1 = log(n2b * xnb);
@@ -3859,13 +3875,18 @@ __global__ void __launch_bounds__(256) oplus_tile_kernel(tgx_ng_grid g, Geo G, t
     if (k > 0) load_ob(k - 1, bn);
     asm volatile("" ::: "memory");
     double x = k == nk - 2 ? xn : bc.x - bc.w2 * xn;
-    if (G.glb) {  // the raw solution; filter_op and the post pass follow (oplus.F:896-941)
+    if (G.raw_row(jc)) {  // the raw solution; filter_op and the post pass follow (oplus.F:896-941)
       opout[G.a3(k, ic, jc)] = x;
       xn = x;
       bc = bn;
       continue;
     }
-    if (k == nk - 3) {
+    if (G.glb && k == nk - 2) {  // zero at the top level (oplus.F:922-923)
+      size_t pt = G.a3(nk - 1, ic, jc);
+      opout[pt] = 0.;
+      optm1out[pt] = 0.;
+    }
+    if (!G.glb && k == nk - 3) {
       double xt = xn * xn / x;
       size_t pt = G.a3(nk - 1, ic, jc);
       double nm = DTSMOOTH * op_top + DTSMOOTH_DIV2 * (optm1_top + xt);
@@ -4396,7 +4417,7 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
       x1 = bc.x1 - bc.g11 * xn1 - bc.g12 * xn2;
       x2 = bc.x2 - bc.g21 * xn1 - bc.g22 * xn2;
     }
-    if (G.glb) {  // the raw solution; filter_uv and the post pass follow (duv.F:490-514)
+    if (G.raw_row(j)) {  // the raw solution; filter_uv and the post pass follow (duv.F:490-514)
       const size_t p = G.a3(k, i, j);
       out.un[p] = x1;
       out.vn[p] = x2;
@@ -4405,7 +4426,14 @@ __global__ void __launch_bounds__(256) duv_tile_kernel(tgx_ng_grid g, Geo G, tgx
       bc = bn;
       continue;
     }
-    if (k == nk - 3) {
+    if (G.glb && k == nk - 2) {  // spval at the top level (duv.F:385-386, :512-513)
+      size_t pt = G.a3(nk - 1, i, j);
+      out.un[pt] = SPVAL;
+      out.vn[pt] = SPVAL;
+      out.un_nm[pt] = SPVAL;
+      out.vn_nm[pt] = SPVAL;
+    }
+    if (!G.glb && k == nk - 3) {
       size_t pt = G.a3(nk - 1, i, j);
       double u = 2 * xn1 - x1, vv = 2 * xn2 - x2;
       out.un[pt] = u;

@@ -15,10 +15,10 @@
 // launchers defined in step.hip and global.hip
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_fields *in,
                              const tgx_ng_out *out, void *const *bufs, int sub, hipStream_t s);
-extern "C" void tgx_ng_set_global_mode(int on);
-extern "C" int tgx_glb_post(int which, const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
-                            const double *op_in, const double *nm_in, double *op_o, double *nm_o,
-                            hipStream_t s);
+extern "C" void tgx_ng_set_global_mode(int on, int nfilt);
+extern "C" int tgx_glb_post(int which, int nfilt, const tgx_ng_grid *g, const tgx_ng_fields *in,
+                            const tgx_ng_out *out, const double *op_in, const double *nm_in, double *op_o,
+                            double *nm_o, hipStream_t s);
 
 namespace {
 // The global grid of the tgx_glb_* call running on this host thread, or null
@@ -217,7 +217,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
     if (!t_glb) return 0;
     // dt.F:389-424: filter_tn below the top level, then the post pass
     if (int rc = glb_filter(g, out->tn, g->nk - 1, 3, 0, s)) return rc;
-    return tgx_glb_post(0, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+    return tgx_glb_post(0, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
   }
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_DT, s);
@@ -301,7 +301,7 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   if (t_glb) {
     if (int rc = glb_filter(g, out->n2d, g->nk, 3, 0, s)) return rc;
     if (int rc = glb_filter(g, out->ar, g->nk, 3, 0, s)) return rc;
-    if (int rc = tgx_glb_post(2, g, in, out, nullptr, nullptr, nullptr, nullptr, s)) return rc;
+    if (int rc = tgx_glb_post(2, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s)) return rc;
   }
   if (chem_done) {  // NO and N4S sources formed in the column kernel from the
                     // step's chemistry pass and this N2D
@@ -317,7 +317,7 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   if (!t_glb) return 0;
   if (int rc = glb_filter(g, out->no, g->nk, 3, 0, s)) return rc;
   if (int rc = glb_filter(g, out->n4s, g->nk, 3, 0, s)) return rc;
-  return tgx_glb_post(3, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+  return tgx_glb_post(3, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
@@ -369,7 +369,7 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
     }
     if (t_glb) {  // oplus.F:896-941: filter_op below the top level, then the post pass
       if (int rc = glb_filter(g, op_o, g->nk - 1, 3, 1, s)) return rc;
-      if (int rc = tgx_glb_post(4, g, in, out, op_in, nm_in, op_o, nm_o, s)) return rc;
+      if (int rc = tgx_glb_post(4, t_glb->nlat_filter, g, in, out, op_in, nm_in, op_o, nm_o, s)) return rc;
     }
     op_in = op_o;
     nm_in = nm_o;
@@ -421,7 +421,7 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
     // duv.F:490-514: filter_uv of UN and VN below the top level, then the post pass
     if (int rc = glb_filter(g, out->un, g->nk - 1, 3, 0, s)) return rc;
     if (int rc = glb_filter(g, out->vn, g->nk - 1, 3, 0, s)) return rc;
-    return tgx_glb_post(1, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+    return tgx_glb_post(1, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
   }
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
@@ -453,7 +453,7 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   if (int rc = glb_filter(g, out->o2, g->nk, 1, 0, s)) return rc;
   if (int rc = glb_filter(g, out->o1, g->nk, 1, 0, s)) return rc;
   if (int rc = glb_filter(g, out->he, g->nk, 1, 0, s)) return rc;
-  return tgx_glb_post(5, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+  return tgx_glb_post(5, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 }  // namespace
 
@@ -642,11 +642,11 @@ struct GlbScope {
   const tgx_glb_grid *prev;
   explicit GlbScope(const tgx_glb_grid *gg) : prev(t_glb) {
     t_glb = gg;
-    tgx_ng_set_global_mode(1);
+    tgx_ng_set_global_mode(1, gg->nlat_filter);
   }
   ~GlbScope() {
     t_glb = prev;
-    tgx_ng_set_global_mode(prev != nullptr);
+    tgx_ng_set_global_mode(prev != nullptr, prev ? prev->nlat_filter : 0);
   }
 };
 int glb_call(RoutineFn fn, const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
