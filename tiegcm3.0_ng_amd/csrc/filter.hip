@@ -1,10 +1,12 @@
 // The high-latitude longitude ring filter of the global model
 // (ringfilter.F:1-111), applied after the column solves of dt, duv, minor,
 // comp and oplus (filter_tn dt.F:431-511 and its siblings) to the latitudes
-// next to each pole.  One wavefront filters one (latitude row, level): the
-// row's longitudes are staged in LDS, the Fourier coefficients of the kept
-// harmonics are wavefront reductions, and the chunk averages and the
-// reconstruction are read back from LDS.
+// next to each pole.  A block takes one latitude row and a run of levels:
+// its threads first tabulate cos(m ang) and sin(m ang) of the kept harmonics
+// in LDS (the only transcendental work, shared by every level of the run),
+// then each wavefront filters one level at a time with the row's longitudes
+// in registers, the Fourier coefficients as wavefront reductions and the
+// residual and chunk averages in its own LDS region.
 #include <hip/hip_runtime.h>
 
 #include "../../include/tgx.h"
@@ -14,6 +16,8 @@ namespace tgx {
 namespace ring {
 
 constexpr int kMaxOrder = TGX_RING_MAX_ORDER;
+constexpr int kLdsMax = 65536;  // dynamic LDS of one block
+static_assert(64 * 32 >= TGX_RING_MAX_NLON, "ringfilter_kernel<32> covers the longest row");
 
 // The rows of one launch, by value in the kernel arguments: row r is at
 // f + k * plane + off[r] and is filtered with n[r] chunks.
@@ -28,9 +32,135 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Each row holds nlon + 4 longitudes (two periodic points on each side).
+// NPL: longitudes per lane (nlon <= 64 NPL); lane l holds i = l + 64 t.
+// LDS: the table cs[(m-1) nlon + i], sn[...] for m = 1..order, then per
+// wavefront fx[nlon] (the residual) and avg[nlon].  Wavefront w of
+// block (r, y) filters levels (y lpw + kk) W + w, kk < lpw.
+template <int NPL>
+__global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f, long plane, Rows R, int nlon,
+                                                         int nk, int order, int oplus, double dlamda, int lpw) {
+  extern __shared__ double sh[];
+  const int W = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = blockIdx.x;
+  const int nchnk = R.n[r];
+  const int ntab = order * nlon;
+  double *cs = sh, *sn = sh + ntab;
+  double *fx = sh + 2 * ntab + (size_t)wave * 2 * nlon, *avg = fx + nlon;
+  for (int t = threadIdx.x; t < ntab; t += blockDim.x) {
+    const int m = t / nlon + 1, i = t - (m - 1) * nlon;
+    const double ang = (i + 1) * dlamda;
+    double s, c;
+    sincos(m * ang, &s, &c);
+    cs[t] = c;
+    sn[t] = s;
+  }
+  auto trig = [&](int m, int i, double &c, double &s) {
+    c = cs[(m - 1) * nlon + i];
+    s = sn[(m - 1) * nlon + i];
+  };
+  const int n = nlon / nchnk;
+  for (int kk = 0; kk < lpw; ++kk) {
+    const int k = ((int)blockIdx.y * lpw + kk) * W + wave;
+    const bool act = k < nk;  // every wavefront reaches every barrier
+    double *row = f + (size_t)(act ? k : 0) * plane + R.off[r];
+    double w[NPL], v[NPL];
+    __syncthreads();  // the table is written; this wavefront's fx, avg are free
+    if (act) {
+      // :40-42 the interior longitudes and their mean
+      double s0 = 0.;
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        const int i = lane + 64 * t;
+        w[t] = i < nlon ? row[2 + i] : 0.;
+        if (i < nlon) s0 += w[t];
+      }
+      const double a0 = wave_sum(s0) / nlon;
+      // :43-61 the Fourier coefficients of harmonic m (ang = i dlamda,
+      // i = 1..nlon), then its term of the harmonic part, harmonics in
+      // ascending order as the reference adds them
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) v[t] = a0;
+#pragma unroll 1
+      for (int m = 1; m <= order; ++m) {
+        double sa = 0., sb = 0.;
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) {
+          const int i = lane + 64 * t;
+          if (i < nlon) {
+            double cn, sg;
+            trig(m, i, cn, sg);
+            sa += w[t] * cn;
+            sb += w[t] * sg;
+          }
+        }
+        const double am = 2. * wave_sum(sa) / nlon, bm = 2. * wave_sum(sb) / nlon;
+#pragma unroll
+        for (int t = 0; t < NPL; ++t) {
+          const int i = lane + 64 * t;
+          if (i < nlon) {
+            double cn, sg;
+            trig(m, i, cn, sg);
+            v[t] = v[t] + am * cn + bm * sg;
+          }
+        }
+      }
+      // :63 the residual
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        const int i = lane + 64 * t;
+        if (i >= nlon) continue;
+        w[t] = w[t] - v[t];
+        fx[i] = w[t];
+      }
+    }
+    __syncthreads();
+    // :65-69 chunk averages, each summed in longitude order
+    if (act) {
+      for (int c = lane; c < nchnk; c += 64) {
+        double s = 0.;
+        for (int m = 0; m < n; ++m) s = s + fx[c * n + m];
+        avg[c] = s / n;
+      }
+    }
+    __syncthreads();
+    if (!act) continue;
+    // :71-101 reconstruction from the four neighbouring chunk averages, the
+    // harmonic part added back, the periodic points
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) {
+      const int i = lane + 64 * t;
+      if (i >= nlon) continue;
+      const int c = i / n;
+      double x = w[t];
+      if (c < nchnk) {  // points past nchnk * n keep their residual (:84-93 stop at nchnk * n)
+        const int m = i - c * n + 1;
+        const int im2 = (c - 2 + 2 * nchnk) % nchnk, im1 = (c - 1 + nchnk) % nchnk;
+        const int ip1 = (c + 1) % nchnk, ip2 = (c + 2) % nchnk;
+        const double A0 = avg[c];
+        const double left = (-avg[im2] + 7. * avg[im1] + 7. * A0 - avg[ip1]) / 12.;
+        const double right = (-avg[im1] + 7. * A0 + 7. * avg[ip1] - avg[ip2]) / 12.;
+        if (oplus) {
+          const double fa = (left + right - 2. * A0) / (double)(n * n);
+          const double fb = (3. * A0 - 2. * left - right) / n;
+          x = fa * (3. * (m * m) - 3. * m + 1.) + fb * (2. * m - 1.) + left;
+        } else {
+          x = (left * (n - m) + right * m) / n;
+        }
+      }
+      const double y = x + v[t];
+      row[2 + i] = y;
+      if (i == nlon - 2) row[0] = y;
+      if (i == nlon - 1) row[1] = y;
+      if (i == 0) row[nlon + 2] = y;
+      if (i == 1) row[nlon + 3] = y;
+    }
+  }
+}
+
+// The same filter where the table does not fit in LDS (long rows, many
+// harmonics): one wavefront per (row, level), sincos evaluated where needed.
 // LDS: wm[nlon] (then the rebuilt row), fx[nlon], avg[nlon].
-__global__ void __launch_bounds__(64) ringfilter_kernel(double *__restrict__ f, long plane, Rows R, int nlon,
+__global__ void __launch_bounds__(64) ringfilter_direct_kernel(double *__restrict__ f, long plane, Rows R, int nlon,
                                                         int order, int oplus, double dlamda) {
   extern __shared__ double sh[];
   double *wm = sh, *fx = sh + nlon, *avg = sh + 2 * nlon;
@@ -111,6 +241,27 @@ __global__ void __launch_bounds__(64) ringfilter_kernel(double *__restrict__ f, 
   if (lane == 3) row[nlon + 3] = wm[1];
 }
 
+template <int NPL>
+int launch(double *f, int nk, long plane, int nrows, const Rows &R, int nlon, int order, int oplus,
+           double dlamda, hipStream_t s) {
+  // wavefronts per block and levels per wavefront: the table in LDS beside
+  // each wavefront's fx and avg, at most kLdsMax bytes
+  const size_t tab = sizeof(double) * 2 * (size_t)order * nlon, per = sizeof(double) * 2 * (size_t)nlon;
+  int W = 4;
+  while (W > 1 && tab + W * per > kLdsMax) W >>= 1;
+  if (tab + W * per > kLdsMax) {
+    hipLaunchKernelGGL(ringfilter_direct_kernel, dim3((unsigned)nrows, (unsigned)nk), dim3(64),
+                       sizeof(double) * 3 * (size_t)nlon, s, f, plane, R, nlon, order, oplus, dlamda);
+  } else {
+    const int lpw = 2;
+    const unsigned ny = (unsigned)((nk + W * lpw - 1) / (W * lpw));
+    hipLaunchKernelGGL(ringfilter_kernel<NPL>, dim3((unsigned)nrows, ny), dim3(64 * W), tab + W * per, s, f, plane,
+                       R, nlon, nk, order, oplus, dlamda, lpw);
+  }
+  TGX_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace ring
 }  // namespace tgx
 
@@ -131,11 +282,13 @@ extern "C" int tgx_launch_ringfilter(double *f, int nk, long plane, int nrows, c
     R.off[r] = rowoff[r];
     R.n[r] = nchnk[r];
   }
-  const size_t lds = sizeof(double) * 3 * (size_t)nlon;
-  hipLaunchKernelGGL(tgx::ring::ringfilter_kernel, dim3((unsigned)nrows, (unsigned)nk), dim3(64), lds, s, f,
-                     plane, R, nlon, order, oplus, dlamda);
-  TGX_HIP(hipGetLastError());
-  return 0;
+  using namespace tgx::ring;
+  if (nlon <= 64 * 2) return launch<2>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 3) return launch<3>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 5) return launch<5>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 9) return launch<9>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 18) return launch<18>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  return launch<32>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
 }
 
 // The latitudes next to each pole that the chunk table filters
