@@ -74,14 +74,15 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f,
         w[t] = i < nlon ? row[2 + i] : 0.;
         if (i < nlon) s0 += w[t];
       }
-      const double a0 = wave_sum(s0) / nlon;
-      // :43-61 the Fourier coefficients of harmonic m (ang = i dlamda,
-      // i = 1..nlon), then its term of the harmonic part, harmonics in
-      // ascending order as the reference adds them
+      // :43-53 the Fourier sums of the kept harmonics (ang = i dlamda,
+      // i = 1..nlon), then every sum's wavefront reduction, their steps
+      // interleaved
+      double sum[2 * kMaxOrder + 1];
+      sum[0] = s0;
 #pragma unroll
-      for (int t = 0; t < NPL; ++t) v[t] = a0;
-#pragma unroll 1
-      for (int m = 1; m <= order; ++m) {
+      for (int m = 1; m <= kMaxOrder; ++m) {
+        sum[2 * m - 1] = sum[2 * m] = 0.;
+        if (m > order) continue;
         double sa = 0., sb = 0.;
 #pragma unroll
         for (int t = 0; t < NPL; ++t) {
@@ -93,7 +94,24 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f,
             sb += w[t] * sg;
           }
         }
-        const double am = 2. * wave_sum(sa) / nlon, bm = 2. * wave_sum(sb) / nlon;
+        sum[2 * m - 1] = sa;
+        sum[2 * m] = sb;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int q = 0; q <= 2 * kMaxOrder; ++q)
+          if (q <= 2 * order) sum[q] += __shfl_xor(sum[q], o, 64);
+      }
+      // :42, :53-61 the mean and the harmonic part, harmonics in ascending
+      // order as the reference adds them
+      const double a0 = sum[0] / nlon;
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) v[t] = a0;
+#pragma unroll
+      for (int m = 1; m <= kMaxOrder; ++m) {
+        if (m > order) continue;
+        const double am = 2. * sum[2 * m - 1] / nlon, bm = 2. * sum[2 * m] / nlon;
 #pragma unroll
         for (int t = 0; t < NPL; ++t) {
           const int i = lane + 64 * t;
@@ -114,28 +132,52 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f,
       }
     }
     __syncthreads();
-    // :65-69 chunk averages, each summed in longitude order
+    // :65-69 chunk averages, each summed in longitude order; the values are
+    // read from LDS sixteen at a time so that the reads overlap the additions
     if (act) {
       for (int c = lane; c < nchnk; c += 64) {
+        const double *q = fx + c * n;
         double s = 0.;
-        for (int m = 0; m < n; ++m) s = s + fx[c * n + m];
+        int m = 0;
+        for (; m + 16 <= n; m += 16) {
+          double x[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) x[u] = q[m + u];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) s = s + x[u];
+        }
+        for (; m < n; ++m) s = s + q[m];
         avg[c] = s / n;
       }
     }
     __syncthreads();
     if (!act) continue;
     // :71-101 reconstruction from the four neighbouring chunk averages, the
-    // harmonic part added back, the periodic points
+    // harmonic part added back, the periodic points.  Longitude i = lane + 64 t
+    // is point m (1-based) of chunk c; both advance by 64 = q64 n + r64 per t.
+    const int q64 = 64 / n, r64 = 64 - q64 * n;
+    int c = lane / n, m = lane - c * n + 1;
 #pragma unroll
     for (int t = 0; t < NPL; ++t) {
       const int i = lane + 64 * t;
+      if (t > 0) {
+        c += q64;
+        m += r64;
+        if (m > n) {
+          m -= n;
+          ++c;
+        }
+      }
       if (i >= nlon) continue;
-      const int c = i / n;
       double x = w[t];
       if (c < nchnk) {  // points past nchnk * n keep their residual (:84-93 stop at nchnk * n)
-        const int m = i - c * n + 1;
-        const int im2 = (c - 2 + 2 * nchnk) % nchnk, im1 = (c - 1 + nchnk) % nchnk;
-        const int ip1 = (c + 1) % nchnk, ip2 = (c + 2) % nchnk;
+        int im2 = c - 2, im1 = c - 1, ip1 = c + 1, ip2 = c + 2;  // :73-81, periodic in the chunks
+        if (im2 < 0) im2 += nchnk;
+        if (im2 < 0) im2 += nchnk;
+        if (im1 < 0) im1 += nchnk;
+        if (ip1 >= nchnk) ip1 -= nchnk;
+        if (ip2 >= nchnk) ip2 -= nchnk;
+        if (ip2 >= nchnk) ip2 -= nchnk;
         const double A0 = avg[c];
         const double left = (-avg[im2] + 7. * avg[im1] + 7. * A0 - avg[ip1]) / 12.;
         const double right = (-avg[im1] + 7. * A0 + 7. * avg[ip1] - avg[ip2]) / 12.;
@@ -253,7 +295,7 @@ int launch(double *f, int nk, long plane, int nrows, const Rows &R, int nlon, in
     hipLaunchKernelGGL(ringfilter_direct_kernel, dim3((unsigned)nrows, (unsigned)nk), dim3(64),
                        sizeof(double) * 3 * (size_t)nlon, s, f, plane, R, nlon, order, oplus, dlamda);
   } else {
-    const int lpw = 2;
+    const int lpw = 3;
     const unsigned ny = (unsigned)((nk + W * lpw - 1) / (W * lpw));
     hipLaunchKernelGGL(ringfilter_kernel<NPL>, dim3((unsigned)nrows, ny), dim3(64 * W), tab + W * per, s, f, plane,
                        R, nlon, nk, order, oplus, dlamda, lpw);
