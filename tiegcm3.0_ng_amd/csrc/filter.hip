@@ -6,7 +6,10 @@
 // in LDS (the only transcendental work, shared by every level of the run),
 // then each wavefront filters one level at a time with the row's longitudes
 // in registers, the Fourier coefficients as wavefront reductions and the
-// residual and chunk averages in its own LDS region.
+// residual and chunk averages in its own LDS region.  The global routines
+// launch it once per routine for all the fields they filter, with their post
+// pass (time smoothing, top level, floors) applied to each filtered value as
+// it is written.
 #include <hip/hip_runtime.h>
 
 #include "../../include/tgx.h"
@@ -26,6 +29,43 @@ struct Rows {
   int n[TGX_RING_MAX_ROWS];
 };
 
+// The post pass that follows the filter in each global routine, on the
+// filtered latitudes: dt.F:396-424 (POST_DT), duv.F:501-514 (POST_DUV),
+// minor.F:556-597 (POST_MINOR), oplus.F:911-934 (POST_OPLUS).
+enum PostKind { POST_NONE = 0, POST_DT, POST_DUV, POST_MINOR, POST_OPLUS };
+constexpr double DTSMOOTH = 0.95, DTSMOOTH_DIV2 = 0.5 * (1. - DTSMOOTH), SPVAL = 1.e36;
+
+// The fields of one launch: f[q] is filtered in place on levels 0..nlev-1;
+// with a post pass, xnm[q] receives the n-1 field from the step inputs fin[q]
+// (n) and fnmi[q] (n-1), and a level nlev < nk (the top) is set to spval or
+// zero without filtering.
+struct Fields {
+  double *f[4], *xnm[4];
+  const double *fin[4], *fnmi[4];
+  int nf, nlev, kind;
+};
+
+// The post pass of one filtered value y with the step inputs fi (n) and fm
+// (n-1): the solution xo and the n-1 field xn, in the reference's order.
+__device__ __forceinline__ void post_point(int kind, double y, double fi, double fm, double &xo, double &xn) {
+  if (kind == POST_DT) {
+    xn = DTSMOOTH_DIV2 * (fm + y) + DTSMOOTH * fi;
+    xo = y < 100. ? 100. : y;
+  } else if (kind == POST_DUV) {
+    xn = DTSMOOTH_DIV2 * (fm + y) + DTSMOOTH * fi;
+    xo = y;
+  } else if (kind == POST_MINOR) {
+    const double small = 1.e-12;
+    const double nm = DTSMOOTH * fi + DTSMOOTH_DIV2 * (fm + y);
+    xo = y < small ? small : y;
+    xn = nm < small ? small : nm;
+  } else {  // POST_OPLUS
+    const double nm = DTSMOOTH * fi + DTSMOOTH_DIV2 * (fm + y);
+    xo = y < 0. ? 0. : y;
+    xn = nm < 0. ? 0. : nm;
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -34,11 +74,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // NPL: longitudes per lane (nlon <= 64 NPL); lane l holds i = l + 64 t.
 // LDS: the table cs[(m-1) nlon + i], sn[...] for m = 1..order, then per
-// wavefront fx[nlon] (the residual) and avg[nlon].  Wavefront w of
-// block (r, y) filters levels (y lpw + kk) W + w, kk < lpw.
+// wavefront fx[nlon] (the residual) and avg[nlon].  The launch covers nkp
+// levels of each of the P.nf fields, field-major; wavefront w of block
+// (r, y) takes the levels (y lpw + kk) W + w, kk < lpw, of that order.
 template <int NPL>
-__global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f, long plane, Rows R, int nlon,
-                                                         int nk, int order, int oplus, double dlamda, int lpw) {
+__global__ void __launch_bounds__(256) ringfilter_kernel(Fields P, long plane, Rows R, int nlon, int nkp, int order,
+                                                         int oplus, double dlamda, int lpw) {
   extern __shared__ double sh[];
   const int W = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = blockIdx.x;
@@ -60,9 +101,12 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f,
   };
   const int n = nlon / nchnk;
   for (int kk = 0; kk < lpw; ++kk) {
-    const int k = ((int)blockIdx.y * lpw + kk) * W + wave;
-    const bool act = k < nk;  // every wavefront reaches every barrier
-    double *row = f + (size_t)(act ? k : 0) * plane + R.off[r];
+    const int kq = ((int)blockIdx.y * lpw + kk) * W + wave;
+    const bool live = kq < P.nf * nkp;  // every wavefront reaches every barrier
+    const int q = live ? kq / nkp : 0, k = live ? kq - q * nkp : 0;
+    const bool act = live && k < P.nlev;  // a level to filter (else the top, or nothing)
+    const size_t off = (size_t)k * plane + R.off[r];
+    double *row = P.f[q] + off;
     double w[NPL], v[NPL];
     __syncthreads();  // the table is written; this wavefront's fx, avg are free
     if (act) {
@@ -151,7 +195,20 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f,
       }
     }
     __syncthreads();
-    if (!act) continue;
+    if (!act) {
+      if (live && P.kind != POST_NONE) {  // the top level: spval (dt.F:407-408, duv.F:385-386, :512-513), zero (oplus.F:922-923)
+        const double top = P.kind == POST_OPLUS ? 0. : SPVAL;
+        double *xr = P.xnm[q] + off;
+        for (int i = lane; i < nlon + 4; i += 64) {
+          row[i] = top;
+          xr[i] = top;
+        }
+      }
+      continue;
+    }
+    const bool post = P.kind != POST_NONE;
+    const double *fin = post ? P.fin[q] + off : row, *fnmi = post ? P.fnmi[q] + off : row;  // the step inputs
+    double *xr = post ? P.xnm[q] + off : row;
     // :71-101 reconstruction from the four neighbouring chunk averages, the
     // harmonic part added back, the periodic points.  Longitude i = lane + 64 t
     // is point m (1-based) of chunk c; both advance by 64 = q64 n + r64 per t.
@@ -190,11 +247,20 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(double *__restrict__ f,
         }
       }
       const double y = x + v[t];
-      row[2 + i] = y;
-      if (i == nlon - 2) row[0] = y;
-      if (i == nlon - 1) row[1] = y;
-      if (i == 0) row[nlon + 2] = y;
-      if (i == 1) row[nlon + 3] = y;
+      double xo = y, xn = 0.;
+      if (post) {
+        post_point(P.kind, y, fin[2 + i], fnmi[2 + i], xo, xn);
+        xr[2 + i] = xn;
+        if (i == nlon - 2) xr[0] = xn;
+        if (i == nlon - 1) xr[1] = xn;
+        if (i == 0) xr[nlon + 2] = xn;
+        if (i == 1) xr[nlon + 3] = xn;
+      }
+      row[2 + i] = xo;
+      if (i == nlon - 2) row[0] = xo;
+      if (i == nlon - 1) row[1] = xo;
+      if (i == 0) row[nlon + 2] = xo;
+      if (i == 1) row[nlon + 3] = xo;
     }
   }
 }
@@ -283,36 +349,59 @@ __global__ void __launch_bounds__(64) ringfilter_direct_kernel(double *__restric
   if (lane == 3) row[nlon + 3] = wm[1];
 }
 
-template <int NPL>
-int launch(double *f, int nk, long plane, int nrows, const Rows &R, int nlon, int order, int oplus,
-           double dlamda, hipStream_t s) {
-  // wavefronts per block and levels per wavefront: the table in LDS beside
-  // each wavefront's fx and avg, at most kLdsMax bytes
+// Whether the table and four wavefronts' rows fit in a block's LDS; if not,
+// the launcher filters with ringfilter_direct_kernel (one field, no post pass).
+inline bool table_fits(int nlon, int order, int *waves) {
   const size_t tab = sizeof(double) * 2 * (size_t)order * nlon, per = sizeof(double) * 2 * (size_t)nlon;
   int W = 4;
   while (W > 1 && tab + W * per > kLdsMax) W >>= 1;
-  if (tab + W * per > kLdsMax) {
-    hipLaunchKernelGGL(ringfilter_direct_kernel, dim3((unsigned)nrows, (unsigned)nk), dim3(64),
-                       sizeof(double) * 3 * (size_t)nlon, s, f, plane, R, nlon, order, oplus, dlamda);
+  if (waves) *waves = W;
+  return tab + W * per <= kLdsMax;
+}
+
+template <int NPL>
+int launch(const Fields &P, int nkp, long plane, int nrows, const Rows &R, int nlon, int order, int oplus,
+           double dlamda, hipStream_t s) {
+  int W;
+  if (!table_fits(nlon, order, &W)) {
+    if (P.nf != 1 || P.kind != POST_NONE || nkp != P.nlev)
+      return tgx_set_error(TGX_ERR_ARG, "ring filter: fused post pass needs the LDS table (nlon %d, order %d)",
+                           nlon, order);
+    hipLaunchKernelGGL(ringfilter_direct_kernel, dim3((unsigned)nrows, (unsigned)nkp), dim3(64),
+                       sizeof(double) * 3 * (size_t)nlon, s, P.f[0], plane, R, nlon, order, oplus, dlamda);
   } else {
+    // wavefronts per block and levels per wavefront: the table in LDS beside
+    // each wavefront's fx and avg, at most kLdsMax bytes
+    const size_t lds = sizeof(double) * (2 * (size_t)order * nlon + 2 * (size_t)W * nlon);
     const int lpw = 3;
-    const unsigned ny = (unsigned)((nk + W * lpw - 1) / (W * lpw));
-    hipLaunchKernelGGL(ringfilter_kernel<NPL>, dim3((unsigned)nrows, ny), dim3(64 * W), tab + W * per, s, f, plane,
-                       R, nlon, nk, order, oplus, dlamda, lpw);
+    const long nlev = (long)P.nf * nkp;
+    const unsigned ny = (unsigned)((nlev + W * lpw - 1) / (W * lpw));
+    hipLaunchKernelGGL(ringfilter_kernel<NPL>, dim3((unsigned)nrows, ny), dim3(64 * W), lds, s, P, plane, R, nlon,
+                       nkp, order, oplus, dlamda, lpw);
   }
   TGX_HIP(hipGetLastError());
   return 0;
 }
 
+inline int launch_any(const Fields &P, int nkp, long plane, int nrows, const Rows &R, int nlon, int order,
+                      int oplus, double dlamda, hipStream_t s) {
+  if (nlon <= 64 * 2) return launch<2>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 3) return launch<3>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 5) return launch<5>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 9) return launch<9>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (nlon <= 64 * 18) return launch<18>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  return launch<32>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+}
+
 }  // namespace ring
 }  // namespace tgx
 
-// Launcher shared by the device entry point and the host drop-in: nrows rows
-// (host arrays rowoff, nchnk, at most TGX_RING_MAX_ROWS) of nk levels.
-extern "C" int tgx_launch_ringfilter(double *f, int nk, long plane, int nrows, const long *rowoff,
-                                     const int *nchnk, int nlon, int order, int oplus, double dlamda,
-                                     hipStream_t s) {
-  if (nrows <= 0 || nk <= 0) return 0;
+namespace {
+// Validates a launch of nrows rows (host arrays rowoff, nchnk, at most
+// TGX_RING_MAX_ROWS) and launches it.
+int launch_rows(const tgx::ring::Fields &P, int nkp, long plane, int nrows, const long *rowoff, const int *nchnk,
+                int nlon, int order, int oplus, double dlamda, hipStream_t s) {
+  if (nrows <= 0 || nkp <= 0 || P.nf <= 0) return 0;
   if (nrows > TGX_RING_MAX_ROWS) return tgx_set_error(TGX_ERR_ARG, "ring filter: %d rows > %d", nrows, TGX_RING_MAX_ROWS);
   if (nlon < 4 || nlon > TGX_RING_MAX_NLON || order < 0 || order > tgx::ring::kMaxOrder || order >= nlon)
     return tgx_set_error(TGX_ERR_ARG, "ring filter: nlon %d (4..%d) or order %d (0..%d, < nlon)", nlon,
@@ -324,17 +413,43 @@ extern "C" int tgx_launch_ringfilter(double *f, int nk, long plane, int nrows, c
     R.off[r] = rowoff[r];
     R.n[r] = nchnk[r];
   }
-  using namespace tgx::ring;
-  if (nlon <= 64 * 2) return launch<2>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 3) return launch<3>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 5) return launch<5>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 9) return launch<9>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 18) return launch<18>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  return launch<32>(f, nk, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  return tgx::ring::launch_any(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
 }
 
 // The latitudes next to each pole that the chunk table filters
-// (ringfilter.F:23-34), in the level-plane layout of tgx_step.h.
+// (ringfilter.F:23-34), as row offsets in the level-plane layout of
+// tgx_step.h (latitude lat is row lat + 1 of the nlat + 4 rows).
+int polar_rows(int nlat, int ld, int nlat_filter, const int *chunk, long *off, int *nc) {
+  int nrows = 0;
+  for (int lat = 1; lat <= nlat; ++lat) {
+    int c = 0;
+    if (lat <= nlat_filter) c = chunk[lat - 1];
+    else if (lat >= nlat - nlat_filter + 1) c = chunk[nlat - lat];
+    if (c == 0) continue;
+    off[nrows] = (long)(lat + 1) * ld;
+    nc[nrows] = c;
+    ++nrows;
+  }
+  return nrows;
+}
+
+tgx::ring::Fields one_field(double *f, int nk) {
+  tgx::ring::Fields P = {};
+  P.f[0] = f;
+  P.nf = 1;
+  P.nlev = nk;
+  P.kind = tgx::ring::POST_NONE;
+  return P;
+}
+}  // namespace
+
+// Launcher of the host drop-in: nrows rows of nk levels of one field.
+extern "C" int tgx_launch_ringfilter(double *f, int nk, long plane, int nrows, const long *rowoff,
+                                     const int *nchnk, int nlon, int order, int oplus, double dlamda,
+                                     hipStream_t s) {
+  return launch_rows(one_field(f, nk), nk, plane, nrows, rowoff, nchnk, nlon, order, oplus, dlamda, s);
+}
+
 extern "C" int tgx_ringfilter_dev(double *f, int nk, int nlon, int nlat, int ld, int order, int oplus,
                                   double dlamda, int nlat_filter, const int *chunk, void *stream) {
   if (!f || nk <= 0 || nlat <= 0 || ld < nlon + 4 || nlat_filter < 0 || nlat_filter > TGX_RING_MAX_ROWS / 2 ||
@@ -343,16 +458,49 @@ extern "C" int tgx_ringfilter_dev(double *f, int nk, int nlon, int nlat, int ld,
                          nk, nlon, nlat, ld, nlat_filter);
   long off[TGX_RING_MAX_ROWS];
   int nc[TGX_RING_MAX_ROWS];
-  int nrows = 0;
-  for (int lat = 1; lat <= nlat; ++lat) {
-    int c = 0;
-    if (lat <= nlat_filter) c = chunk[lat - 1];
-    else if (lat >= nlat - nlat_filter + 1) c = chunk[nlat - lat];
-    if (c == 0) continue;
-    off[nrows] = (long)(lat + 1) * ld;  // latitude lat is row lat + 1 of the nlat + 4 rows
-    nc[nrows] = c;
-    ++nrows;
+  const int nrows = polar_rows(nlat, ld, nlat_filter, chunk, off, nc);
+  return launch_rows(one_field(f, nk), nk, (long)ld * (nlat + 4), nrows, off, nc, nlon, order, oplus, dlamda,
+                     (hipStream_t)stream);
+}
+
+// Whether tgx_ringfilter_post_dev can run at this row length and order (its
+// table of cos/sin fits in LDS).
+extern "C" int tgx_ringfilter_post_fits(int nlon, int order) {
+  return nlon >= 4 && nlon <= TGX_RING_MAX_NLON && order >= 0 && order <= tgx::ring::kMaxOrder &&
+         tgx::ring::table_fits(nlon, order, nullptr);
+}
+
+// The ring filters of one global routine in one launch: the nf (<= 4) fields
+// f[q] on their polar latitudes, levels 0..nlev-1, each filtered value then
+// passed through the routine's post pass `kind` (tgx::ring::PostKind) with
+// the step inputs fin[q], fnmi[q] into f[q] and xnm[q]; with a post pass and
+// nlev < nk, the top level nlev is set as the routine sets it.  The periodic
+// points of the filtered rows are rewritten; the other latitudes are not
+// touched.  Used by the global routines (step_api.cpp).
+extern "C" int tgx_ringfilter_post_dev(int nf, double *const *f, double *const *xnm, const double *const *fin,
+                                       const double *const *fnmi, int kind, int nlev, int nk, int nlon, int nlat,
+                                       int ld, int order, int oplus, double dlamda, int nlat_filter,
+                                       const int *chunk, hipStream_t s) {
+  if (nf < 1 || nf > 4 || kind < tgx::ring::POST_NONE || kind > tgx::ring::POST_OPLUS || nlev < 1 || nlev > nk ||
+      (kind != tgx::ring::POST_NONE && nk - nlev > 1) || nlat <= 0 || ld < nlon + 4 || nlat_filter < 0 ||
+      nlat_filter > TGX_RING_MAX_ROWS / 2 || (nlat_filter > 0 && !chunk))
+    return tgx_set_error(TGX_ERR_ARG, "ring filter + post: bad arguments (nf %d kind %d nlev %d nk %d)", nf, kind,
+                         nlev, nk);
+  tgx::ring::Fields P = {};
+  for (int q = 0; q < nf; ++q) {
+    P.f[q] = f[q];
+    if (kind != tgx::ring::POST_NONE) {
+      P.xnm[q] = xnm[q];
+      P.fin[q] = fin[q];
+      P.fnmi[q] = fnmi[q];
+    }
   }
-  return tgx_launch_ringfilter(f, nk, (long)ld * (nlat + 4), nrows, off, nc, nlon, order, oplus, dlamda,
-                               (hipStream_t)stream);
+  P.nf = nf;
+  P.nlev = nlev;
+  P.kind = kind;
+  long off[TGX_RING_MAX_ROWS];
+  int nc[TGX_RING_MAX_ROWS];
+  const int nrows = polar_rows(nlat, ld, nlat_filter, chunk, off, nc);
+  return launch_rows(P, kind != tgx::ring::POST_NONE ? nk : nlev, (long)ld * (nlat + 4), nrows, off, nc, nlon,
+                     order, oplus, dlamda, s);
 }

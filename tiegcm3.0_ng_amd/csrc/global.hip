@@ -258,14 +258,15 @@ using namespace tgx::glb;
 // kernel and ring filter.  which: 0 dt, 1 duv, 2 minor species N2D and Ar,
 // 3 minor species NO and N4S, 4 O+ sub-step (op_in, nm_in: the sub-step's
 // inputs, op_o, nm_o: its outputs, whose rows across the poles take those of
-// in->op / in->op_nm), 5 comp.  Then the periodic points of the outputs on
+// in->op / in->op_nm), 5 comp; which + 100 when the ring filter launch has
+// already applied the post pass.  Then the periodic points of the outputs on
 // every interior latitude.
 extern "C" int tgx_glb_post(int which, int nfilt, const tgx_ng_grid *g, const tgx_ng_fields *in,
                             const tgx_ng_out *out, const double *op_in, const double *nm_in, double *op_o,
                             double *nm_o, hipStream_t s) {
   const Dims D = dims_of(g, nfilt);
   const long nel = (long)D.nlon * 2 * nfilt * D.nk;
-  if (nel == 0) which += 100;  // nothing to filter: the periodic points only
+  if (nel == 0 && which < 100) which += 100;  // nothing filtered: the periodic points only
   Fields F = {};
   switch (which) {
     case 0:

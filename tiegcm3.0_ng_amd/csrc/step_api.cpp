@@ -16,6 +16,12 @@
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_fields *in,
                              const tgx_ng_out *out, void *const *bufs, int sub, hipStream_t s);
 extern "C" void tgx_ng_set_global_mode(int on, int nfilt);
+// filter.hip
+extern "C" int tgx_ringfilter_post_fits(int nlon, int order);
+extern "C" int tgx_ringfilter_post_dev(int nf, double *const *f, double *const *xnm, const double *const *fin,
+                                       const double *const *fnmi, int kind, int nlev, int nk, int nlon, int nlat,
+                                       int ld, int order, int oplus, double dlamda, int nlat_filter,
+                                       const int *chunk, hipStream_t s);
 extern "C" int tgx_glb_post(int which, int nfilt, const tgx_ng_grid *g, const tgx_ng_fields *in,
                             const tgx_ng_out *out, const double *op_in, const double *nm_in, double *op_o,
                             double *nm_o, hipStream_t s);
@@ -32,6 +38,79 @@ thread_local const tgx_glb_grid *t_glb = nullptr;
 int glb_filter(const tgx_ng_grid *g, double *f, int nlev, int order, int oplus, hipStream_t s) {
   return tgx_ringfilter_dev(f, nlev, g->nlon, g->nlat, g->ld, order, oplus, g->dlamda, t_glb->nlat_filter,
                             t_glb->chunk, s);
+}
+
+// After a global routine's column kernel, its ring filters and post pass
+// (which: as tgx_glb_post).  dt.F:389-424 (filter_tn below the top level),
+// duv.F:490-514 (filter_uv of UN and VN below the top level),
+// minor.F:546-597 (filter_minor, all levels; a pair of species per pass),
+// oplus.F:896-941 (filter_op below the top level, one sub-step), comp.F:925-1022
+// (filter_o2o of O2, O, He, all levels, order 1).  Where the filter's
+// cos/sin table fits in LDS, all the routine's fields are filtered in one
+// launch with the post pass applied to each filtered value (comp's post pass,
+// which needs the whole column, stays its own kernel), and tgx_glb_post adds
+// the periodic points; otherwise each field is filtered alone and
+// tgx_glb_post runs the post kernels on the filtered latitudes.
+int glb_finish(int which, const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+               const double *op_in, const double *nm_in, double *op_o, double *nm_o, hipStream_t s) {
+  enum { NONE = 0, DT, DUV, MINOR, OPLUS };  // the post kinds of tgx_ringfilter_post_dev
+  double *f[4] = {}, *xnm[4] = {};
+  const double *fi[4] = {}, *fm[4] = {};
+  int nf = 0, kind = NONE, nlev = g->nk - 1, order = 3, oplus = 0;
+  auto add = [&](double *x, double *xn, const double *a, const double *b) {
+    f[nf] = x;
+    xnm[nf] = xn;
+    fi[nf] = a;
+    fm[nf] = b;
+    ++nf;
+  };
+  switch (which) {
+    case 0:
+      kind = DT;
+      add(out->tn, out->tn_nm, in->tn, in->tn_nm);
+      break;
+    case 1:
+      kind = DUV;
+      add(out->un, out->un_nm, in->un, in->un_nm);
+      add(out->vn, out->vn_nm, in->vn, in->vn_nm);
+      break;
+    case 2:
+      kind = MINOR;
+      nlev = g->nk;
+      add(out->n2d, out->n2d_nm, in->n2d, in->n2d_nm);
+      add(out->ar, out->ar_nm, in->ar, in->ar_nm);
+      break;
+    case 3:
+      kind = MINOR;
+      nlev = g->nk;
+      add(out->no, out->no_nm, in->no, in->no_nm);
+      add(out->n4s, out->n4s_nm, in->n4s, in->n4s_nm);
+      break;
+    case 4:
+      kind = OPLUS;
+      oplus = 1;
+      add(op_o, nm_o, op_in, nm_in);
+      break;
+    case 5:
+      nlev = g->nk;
+      order = 1;
+      add(out->o2, nullptr, nullptr, nullptr);
+      add(out->o1, nullptr, nullptr, nullptr);
+      add(out->he, nullptr, nullptr, nullptr);
+      break;
+    default:
+      return tgx_set_error(TGX_ERR_ARG, "glb_finish: unknown routine %d", which);
+  }
+  const int nfilt = t_glb->nlat_filter;
+  if (tgx_ringfilter_post_fits(g->nlon, order)) {
+    if (int rc = tgx_ringfilter_post_dev(nf, f, xnm, fi, fm, kind, nlev, g->nk, g->nlon, g->nlat, g->ld, order,
+                                         oplus, g->dlamda, nfilt, t_glb->chunk, s))
+      return rc;
+    return tgx_glb_post(kind != NONE ? which + 100 : which, nfilt, g, in, out, op_in, nm_in, op_o, nm_o, s);
+  }
+  for (int q = 0; q < nf; ++q)
+    if (int rc = glb_filter(g, f[q], nlev, order, oplus, s)) return rc;
+  return tgx_glb_post(which, nfilt, g, in, out, op_in, nm_in, op_o, nm_o, s);
 }
 }  // namespace
 
@@ -215,9 +294,7 @@ extern "C" int tgx_ng_dt(const tgx_ng_grid *g, const tgx_ng_fields *in, const tg
       if (int rc = tgx_ng_launch(K_DT_TILE, g, in, out, b, 0, s)) return rc;
     }
     if (!t_glb) return 0;
-    // dt.F:389-424: filter_tn below the top level, then the post pass
-    if (int rc = glb_filter(g, out->tn, g->nk - 1, 3, 0, s)) return rc;
-    return tgx_glb_post(0, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+    return glb_finish(0, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
   }
   if (int rc = tgx_ng_launch(K_DT_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_DT, s);
@@ -298,11 +375,8 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   }
   // minor.F:546-597 for N2D and Ar before NO and N4S read this step's N2D
   // (dynamics.F:485-600): filter_minor, then the post pass
-  if (t_glb) {
-    if (int rc = glb_filter(g, out->n2d, g->nk, 3, 0, s)) return rc;
-    if (int rc = glb_filter(g, out->ar, g->nk, 3, 0, s)) return rc;
-    if (int rc = tgx_glb_post(2, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s)) return rc;
-  }
+  if (t_glb)
+    if (int rc = glb_finish(2, g, in, out, nullptr, nullptr, nullptr, nullptr, s)) return rc;
   if (chem_done) {  // NO and N4S sources formed in the column kernel from the
                     // step's chemistry pass and this N2D
     void *b2[20];
@@ -315,9 +389,7 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
     if (int rc = tgx_ng_launch(pcr ? K_MIN_WB : K_MIN_B, g, in, out, b, 0, s)) return rc;
   }
   if (!t_glb) return 0;
-  if (int rc = glb_filter(g, out->no, g->nk, 3, 0, s)) return rc;
-  if (int rc = glb_filter(g, out->n4s, g->nk, 3, 0, s)) return rc;
-  return tgx_glb_post(3, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+  return glb_finish(3, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
@@ -367,10 +439,8 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
       const int col = pcr ? K_OP3_W : !reuse ? K_OP_TILE : sub == 0 ? K_OP_TILE_KEEP : K_OP_TILE_REUSE;
       if (int rc = tgx_ng_launch(col, g, in, out, b, sub, s)) return rc;
     }
-    if (t_glb) {  // oplus.F:896-941: filter_op below the top level, then the post pass
-      if (int rc = glb_filter(g, op_o, g->nk - 1, 3, 1, s)) return rc;
-      if (int rc = tgx_glb_post(4, t_glb->nlat_filter, g, in, out, op_in, nm_in, op_o, nm_o, s)) return rc;
-    }
+    if (t_glb)  // oplus.F:896-941: filter_op below the top level, then the post pass
+      if (int rc = glb_finish(4, g, in, out, op_in, nm_in, op_o, nm_o, s)) return rc;
     op_in = op_o;
     nm_in = nm_o;
   }
@@ -418,10 +488,7 @@ extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const t
       if (int rc = tgx_ng_launch(K_UV_TILE, g, in, out, b, 0, s)) return rc;
     }
     if (!t_glb) return 0;
-    // duv.F:490-514: filter_uv of UN and VN below the top level, then the post pass
-    if (int rc = glb_filter(g, out->un, g->nk - 1, 3, 0, s)) return rc;
-    if (int rc = glb_filter(g, out->vn, g->nk - 1, 3, 0, s)) return rc;
-    return tgx_glb_post(1, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+    return glb_finish(1, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
   }
   if (int rc = tgx_ng_launch(K_UV1, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_UV_PRE, g, in, out, b, 0, s)) return rc;
@@ -449,11 +516,7 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
     if (int rc = tgx_ng_launch(use_pcr(g, R_COMP) ? K_COMP_W : K_COMP, g, in, out, b, 0, s)) return rc;
   }
   if (!t_glb) return 0;
-  // comp.F:925-1022: filter_o2o of O2, O and He, then the post pass
-  if (int rc = glb_filter(g, out->o2, g->nk, 1, 0, s)) return rc;
-  if (int rc = glb_filter(g, out->o1, g->nk, 1, 0, s)) return rc;
-  if (int rc = glb_filter(g, out->he, g->nk, 1, 0, s)) return rc;
-  return tgx_glb_post(5, t_glb->nlat_filter, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
+  return glb_finish(5, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 }  // namespace
 
