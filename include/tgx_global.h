@@ -47,7 +47,8 @@ typedef struct {
                          each divides nlon */
 } tgx_glb_grid;
 
-/* Workspace bytes (as tgx_ng_workspace_bytes of g). */
+/* Workspace bytes: those of tgx_ng_workspace_bytes(&gg->g), then the ring
+ * filters' table of cos/sin values (6 nlon doubles). */
 unsigned long long tgx_glb_workspace_bytes(const tgx_glb_grid *gg);
 
 /* The routines on their own, in the model's order (dynamics.F:319-1000):

@@ -77,23 +77,28 @@ __device__ __forceinline__ double wave_sum(double v) {
 // wavefront fx[nlon] (the residual) and avg[nlon].  The launch covers nkp
 // levels of each of the P.nf fields, field-major; wavefront w of block
 // (r, y) takes the levels (y lpw + kk) W + w, kk < lpw, of that order.
-template <int NPL>
-__global__ void __launch_bounds__(256) ringfilter_kernel(Fields P, long plane, Rows R, int nlon, int nkp, int order,
-                                                         int oplus, double dlamda, int lpw) {
+// GTAB: the table is given in device memory (gtab: cos of harmonics
+// 1..tord at (m-1) nlon + i, then their sin; ring_table_kernel) instead of
+// being built in LDS; the LDS then holds the wavefronts' fx and avg only.
+// The kernels below wrap this body.
+template <int NPL, bool GTAB>
+__device__ __forceinline__ void ringfilter_body(const Fields &P, long plane, const Rows &R, int nlon, int nkp,
+                                                int order, int oplus, double dlamda, int lpw,
+                                                const double *__restrict__ gtab, int tord) {
   extern __shared__ double sh[];
   const int W = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = blockIdx.x;
   const int nchnk = R.n[r];
-  const int ntab = order * nlon;
-  double *cs = sh, *sn = sh + ntab;
+  const int ntab = GTAB ? 0 : order * nlon;
+  const double *cs = GTAB ? gtab : sh, *sn = GTAB ? gtab + (size_t)tord * nlon : sh + ntab;
   double *fx = sh + 2 * ntab + (size_t)wave * 2 * nlon, *avg = fx + nlon;
   for (int t = threadIdx.x; t < ntab; t += blockDim.x) {
     const int m = t / nlon + 1, i = t - (m - 1) * nlon;
     const double ang = (i + 1) * dlamda;
     double s, c;
     sincos(m * ang, &s, &c);
-    cs[t] = c;
-    sn[t] = s;
+    sh[t] = c;
+    sh[ntab + t] = s;
   }
   auto trig = [&](int m, int i, double &c, double &s) {
     c = cs[(m - 1) * nlon + i];
@@ -265,6 +270,37 @@ __global__ void __launch_bounds__(256) ringfilter_kernel(Fields P, long plane, R
   }
 }
 
+template <int NPL>
+__global__ void __launch_bounds__(256) ringfilter_kernel(Fields P, long plane, Rows R, int nlon, int nkp, int order,
+                                                         int oplus, double dlamda, int lpw) {
+  ringfilter_body<NPL, false>(P, plane, R, nlon, nkp, order, oplus, dlamda, lpw, nullptr, 0);
+}
+
+// With the table in device memory: one wavefront per block, at most 128
+// VGPRs, so that four wavefronts of it fit on a SIMD beside the column
+// kernels of the routines running at the same time.
+template <int NPL>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+ringfilter_gtab_kernel(Fields P, long plane, Rows R, int nlon, int nkp, int order, int oplus, double dlamda,
+                       const double *__restrict__ gtab, int tord) {
+  ringfilter_body<NPL, true>(P, plane, R, nlon, nkp, order, oplus, dlamda, 1, gtab, tord);
+}
+
+// The table of ringfilter_gtab_kernel: cos(m ang) and sin(m ang) of
+// longitude i (ang = (i + 1) dlamda, i < nlon) for m = 1..tord, the values
+// the LDS form computes.
+__global__ void __launch_bounds__(256) ring_table_kernel(double *__restrict__ tab, int nlon, int tord,
+                                                         double dlamda) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tord * nlon) return;
+  const int m = t / nlon + 1, i = t - (m - 1) * nlon;
+  const double ang = (i + 1) * dlamda;
+  double s, c;
+  sincos(m * ang, &s, &c);
+  tab[t] = c;
+  tab[(size_t)tord * nlon + t] = s;
+}
+
 // The same filter where the table does not fit in LDS (long rows, many
 // harmonics): one wavefront per (row, level), sincos evaluated where needed.
 // LDS: wm[nlon] (then the rebuilt row), fx[nlon], avg[nlon].
@@ -361,9 +397,17 @@ inline bool table_fits(int nlon, int order, int *waves) {
 
 template <int NPL>
 int launch(const Fields &P, int nkp, long plane, int nrows, const Rows &R, int nlon, int order, int oplus,
-           double dlamda, hipStream_t s) {
+           double dlamda, const double *gtab, int tord, hipStream_t s) {
   int W;
-  if (!table_fits(nlon, order, &W)) {
+  if (gtab) {
+    // one wavefront per block and level: small blocks (9 KB of LDS at 576
+    // longitudes) that find room beside the column kernels of the other
+    // routines running at the same time
+    const size_t lds = sizeof(double) * 2 * (size_t)nlon;
+    const unsigned ny = (unsigned)((long)P.nf * nkp);
+    hipLaunchKernelGGL(ringfilter_gtab_kernel<NPL>, dim3((unsigned)nrows, ny), dim3(64), lds, s, P, plane, R, nlon,
+                       nkp, order, oplus, dlamda, gtab, tord);
+  } else if (!table_fits(nlon, order, &W)) {
     if (P.nf != 1 || P.kind != POST_NONE || nkp != P.nlev)
       return tgx_set_error(TGX_ERR_ARG, "ring filter: fused post pass needs the LDS table (nlon %d, order %d)",
                            nlon, order);
@@ -384,13 +428,13 @@ int launch(const Fields &P, int nkp, long plane, int nrows, const Rows &R, int n
 }
 
 inline int launch_any(const Fields &P, int nkp, long plane, int nrows, const Rows &R, int nlon, int order,
-                      int oplus, double dlamda, hipStream_t s) {
-  if (nlon <= 64 * 2) return launch<2>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 3) return launch<3>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 5) return launch<5>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 9) return launch<9>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  if (nlon <= 64 * 18) return launch<18>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
-  return launch<32>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+                      int oplus, double dlamda, const double *gtab, int tord, hipStream_t s) {
+  if (nlon <= 64 * 2) return launch<2>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
+  if (nlon <= 64 * 3) return launch<3>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
+  if (nlon <= 64 * 5) return launch<5>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
+  if (nlon <= 64 * 9) return launch<9>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
+  if (nlon <= 64 * 18) return launch<18>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
+  return launch<32>(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
 }
 
 }  // namespace ring
@@ -400,7 +444,8 @@ namespace {
 // Validates a launch of nrows rows (host arrays rowoff, nchnk, at most
 // TGX_RING_MAX_ROWS) and launches it.
 int launch_rows(const tgx::ring::Fields &P, int nkp, long plane, int nrows, const long *rowoff, const int *nchnk,
-                int nlon, int order, int oplus, double dlamda, hipStream_t s) {
+                int nlon, int order, int oplus, double dlamda, hipStream_t s, const double *gtab = nullptr,
+                int tord = 0) {
   if (nrows <= 0 || nkp <= 0 || P.nf <= 0) return 0;
   if (nrows > TGX_RING_MAX_ROWS) return tgx_set_error(TGX_ERR_ARG, "ring filter: %d rows > %d", nrows, TGX_RING_MAX_ROWS);
   if (nlon < 4 || nlon > TGX_RING_MAX_NLON || order < 0 || order > tgx::ring::kMaxOrder || order >= nlon)
@@ -413,7 +458,9 @@ int launch_rows(const tgx::ring::Fields &P, int nkp, long plane, int nrows, cons
     R.off[r] = rowoff[r];
     R.n[r] = nchnk[r];
   }
-  return tgx::ring::launch_any(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, s);
+  if (gtab && order > tord)
+    return tgx_set_error(TGX_ERR_ARG, "ring filter: order %d beyond the table's %d", order, tord);
+  return tgx::ring::launch_any(P, nkp, plane, nrows, R, nlon, order, oplus, dlamda, gtab, tord, s);
 }
 
 // The latitudes next to each pole that the chunk table filters
@@ -463,11 +510,17 @@ extern "C" int tgx_ringfilter_dev(double *f, int nk, int nlon, int nlat, int ld,
                      (hipStream_t)stream);
 }
 
-// Whether tgx_ringfilter_post_dev can run at this row length and order (its
-// table of cos/sin fits in LDS).
-extern "C" int tgx_ringfilter_post_fits(int nlon, int order) {
-  return nlon >= 4 && nlon <= TGX_RING_MAX_NLON && order >= 0 && order <= tgx::ring::kMaxOrder &&
-         tgx::ring::table_fits(nlon, order, nullptr);
+// The cos/sin table of tgx_ringfilter_post_dev (harmonics 1..tord of nlon
+// longitudes, 2 tord nlon doubles at tab), on the stream.
+extern "C" int tgx_ring_table_dev(double *tab, int nlon, int tord, double dlamda, hipStream_t s) {
+  if (!tab || nlon < 4 || nlon > TGX_RING_MAX_NLON || tord < 1 || tord > tgx::ring::kMaxOrder)
+    return tgx_set_error(TGX_ERR_ARG, "ring filter table: nlon %d (4..%d), order %d (1..%d)", nlon, TGX_RING_MAX_NLON,
+                         tord, tgx::ring::kMaxOrder);
+  const int n = tord * nlon;
+  hipLaunchKernelGGL(tgx::ring::ring_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tab, nlon,
+                     tord, dlamda);
+  TGX_HIP(hipGetLastError());
+  return 0;
 }
 
 // The ring filters of one global routine in one launch: the nf (<= 4) fields
@@ -476,14 +529,16 @@ extern "C" int tgx_ringfilter_post_fits(int nlon, int order) {
 // the step inputs fin[q], fnmi[q] into f[q] and xnm[q]; with a post pass and
 // nlev < nk, the top level nlev is set as the routine sets it.  The periodic
 // points of the filtered rows are rewritten; the other latitudes are not
-// touched.  Used by the global routines (step_api.cpp).
+// touched.  gtab: the table of tgx_ring_table_dev for harmonics 1..tord
+// (order <= tord), filled on the same stream.  Used by the global routines
+// (step_api.cpp).
 extern "C" int tgx_ringfilter_post_dev(int nf, double *const *f, double *const *xnm, const double *const *fin,
                                        const double *const *fnmi, int kind, int nlev, int nk, int nlon, int nlat,
                                        int ld, int order, int oplus, double dlamda, int nlat_filter,
-                                       const int *chunk, hipStream_t s) {
+                                       const int *chunk, const double *gtab, int tord, hipStream_t s) {
   if (nf < 1 || nf > 4 || kind < tgx::ring::POST_NONE || kind > tgx::ring::POST_OPLUS || nlev < 1 || nlev > nk ||
       (kind != tgx::ring::POST_NONE && nk - nlev > 1) || nlat <= 0 || ld < nlon + 4 || nlat_filter < 0 ||
-      nlat_filter > TGX_RING_MAX_ROWS / 2 || (nlat_filter > 0 && !chunk))
+      nlat_filter > TGX_RING_MAX_ROWS / 2 || (nlat_filter > 0 && !chunk) || !gtab)
     return tgx_set_error(TGX_ERR_ARG, "ring filter + post: bad arguments (nf %d kind %d nlev %d nk %d)", nf, kind,
                          nlev, nk);
   tgx::ring::Fields P = {};
@@ -502,5 +557,5 @@ extern "C" int tgx_ringfilter_post_dev(int nf, double *const *f, double *const *
   int nc[TGX_RING_MAX_ROWS];
   const int nrows = polar_rows(nlat, ld, nlat_filter, chunk, off, nc);
   return launch_rows(P, kind != tgx::ring::POST_NONE ? nk : nlev, (long)ld * (nlat + 4), nrows, off, nc, nlon,
-                     order, oplus, dlamda, s);
+                     order, oplus, dlamda, s, gtab, tord);
 }

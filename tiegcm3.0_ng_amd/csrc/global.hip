@@ -1,12 +1,11 @@
-// The global grid's passes around the column kernels: what each global
-// routine does after its column solve and ring filter (the time smoothing,
-// the top level, the floors, comp's N2 and normalisation) on the latitudes
-// the ring filter reaches (the column kernels apply it themselves on the
-// others), the periodic points of the outputs (mp_periodic_f3d) and the
-// halos of an input field (mk_polelat.F, mp_periodic_f4d).  Level planes as
-// in tgx_step.h; the elementwise passes take one thread per (interior
-// longitude, latitude, level), the longitudes fastest, so a wavefront reads
-// 64 consecutive doubles of a plane.
+// The global grid's passes around the column kernels: comp's post pass
+// after its ring filter (the time smoothing, N2 and the normalisation) on the
+// latitudes the filter reaches (the column kernels apply the post passes
+// themselves on the others, and the ring filter launch applies those of dt,
+// duv, the minor species and O+), the periodic points of the outputs
+// (mp_periodic_f3d) and the halos of an input field (mk_polelat.F,
+// mp_periodic_f4d).  Level planes as in tgx_step.h; the elementwise passes
+// take one thread per (point, level), the longitudes fastest.
 #include <hip/hip_runtime.h>
 
 #include "../../include/tgx.h"
@@ -16,10 +15,7 @@
 namespace tgx {
 namespace glb {
 
-constexpr double DTSMOOTH = 0.95, DTSMOOTH_DIV2 = 0.5 * (1. - DTSMOOTH), SPVAL = 1.e36;
-
-// The passes, by routine (their global source lines in the comments below).
-enum Kind { P_DT = 0, P_DUV, P_MINOR, P_OPLUS, P_COMP };
+constexpr double DTSMOOTH = 0.95, DTSMOOTH_DIV2 = 0.5 * (1. - DTSMOOTH);
 
 struct Dims {
   int nk, nlon, nlat, ld;
@@ -31,72 +27,6 @@ struct Dims {
 // south pole, then the nfilt next to the north pole.
 __device__ __forceinline__ int filtered_row(const Dims &D, int r) {
   return r < D.nfilt ? 2 + r : D.nlat + 2 - 2 * D.nfilt + r;
-}
-
-// Up to four solved fields of one pass: the solution (raw on entry), its
-// n-1 output, and the step's inputs at n and n-1.
-struct Fields {
-  double *x[4], *xnm[4];
-  const double *f[4], *fnm[4];
-  int n;
-};
-
-// Interior longitude i of a filtered latitude j and level k of the calling
-// thread, or false.
-__device__ __forceinline__ bool filtered_point(const Dims &D, int &i, int &j, int &k) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nrow = (long)D.nlon * 2 * D.nfilt;
-  if (t >= nrow * D.nk) return false;
-  k = (int)(t / nrow);
-  const long r = t - (long)k * nrow;
-  const int row = (int)(r / D.nlon);
-  j = filtered_row(D, row);
-  i = 2 + (int)(r - (long)row * D.nlon);
-  return true;
-}
-
-// dt.F:396-424 (TN), duv.F:501-514 (UN, VN), minor.F:556-597 (a minor
-// species), oplus.F:911-934 (O+): the time smoothing of the filtered
-// solution, the top level and the floors.
-template <int KIND>
-__global__ void __launch_bounds__(256) post_kernel(Dims D, Fields F) {
-  int i, j, k;
-  if (!filtered_point(D, i, j, k)) return;
-  const size_t p = (size_t)k * D.plane + (size_t)j * D.ld + i;
-  const bool top = k == D.nk - 1;
-  for (int q = 0; q < F.n; ++q) {
-    const double x = F.x[q][p];
-    if (KIND == P_DT) {
-      if (top) {
-        F.x[q][p] = SPVAL;
-        F.xnm[q][p] = SPVAL;
-      } else {
-        F.xnm[q][p] = DTSMOOTH_DIV2 * (F.fnm[q][p] + x) + DTSMOOTH * F.f[q][p];
-        F.x[q][p] = x < 100. ? 100. : x;
-      }
-    } else if (KIND == P_DUV) {
-      if (top) {
-        F.x[q][p] = SPVAL;
-        F.xnm[q][p] = SPVAL;
-      } else {
-        F.xnm[q][p] = DTSMOOTH_DIV2 * (F.fnm[q][p] + x) + DTSMOOTH * F.f[q][p];
-      }
-    } else if (KIND == P_MINOR) {
-      const double small = 1.e-12;
-      const double nm = DTSMOOTH * F.f[q][p] + DTSMOOTH_DIV2 * (F.fnm[q][p] + x);
-      F.x[q][p] = x < small ? small : x;
-      F.xnm[q][p] = nm < small ? small : nm;
-    } else {  // P_OPLUS
-      if (top) {
-        F.x[q][p] = 0.;
-        F.xnm[q][p] = 0.;
-      } else {
-        const double nm = DTSMOOTH * F.f[q][p] + DTSMOOTH_DIV2 * (F.fnm[q][p] + x);
-        F.x[q][p] = x < 0. ? 0. : x;
-        F.xnm[q][p] = nm < 0. ? 0. : nm;
-      }
-    }
-  }
 }
 
 // comp.F:936-1022 on one column per thread: the time smoothing of the
@@ -253,65 +183,29 @@ int periodic(const Dims &D, double *const *f, int n, int nk, hipStream_t s) {
 
 using namespace tgx::glb;
 
-// The post passes of the global routines on the nfilt filtered latitudes
-// next to each pole, launched by step_api.cpp after each routine's column
-// kernel and ring filter.  which: 0 dt, 1 duv, 2 minor species N2D and Ar,
-// 3 minor species NO and N4S, 4 O+ sub-step (op_in, nm_in: the sub-step's
-// inputs, op_o, nm_o: its outputs, whose rows across the poles take those of
-// in->op / in->op_nm), 5 comp; which + 100 when the ring filter launch has
-// already applied the post pass.  Then the periodic points of the outputs on
-// every interior latitude.
+// What remains of a global routine after its ring filter launch
+// (step_api.cpp), which has applied the post pass of dt, duv, the minor
+// species and O+ on the filtered latitudes: which 0 dt, 1 duv, 2 minor
+// species N2D and Ar, 3 minor species NO and N4S, 4 O+ sub-step (op_in,
+// nm_in: the sub-step's inputs, op_o, nm_o: its outputs, whose rows across
+// the poles take those of in->op / in->op_nm), 5 comp (its post pass on the
+// nfilt filtered latitudes next to each pole, which needs whole columns).
+// Then the periodic points of the outputs on every interior latitude.
 extern "C" int tgx_glb_post(int which, int nfilt, const tgx_ng_grid *g, const tgx_ng_fields *in,
                             const tgx_ng_out *out, const double *op_in, const double *nm_in, double *op_o,
                             double *nm_o, hipStream_t s) {
+  if (which < 0 || which > 5) return tgx_set_error(TGX_ERR_ARG, "tgx_glb_post: unknown pass %d", which);
   const Dims D = dims_of(g, nfilt);
-  const long nel = (long)D.nlon * 2 * nfilt * D.nk;
-  if (nel == 0 && which < 100) which += 100;  // nothing filtered: the periodic points only
-  Fields F = {};
-  switch (which) {
-    case 0:
-      F.n = 1;
-      F.x[0] = out->tn; F.xnm[0] = out->tn_nm; F.f[0] = in->tn; F.fnm[0] = in->tn_nm;
-      hipLaunchKernelGGL(post_kernel<P_DT>, dim3(nblocks(nel)), dim3(256), 0, s, D, F);
-      break;
-    case 1:
-      F.n = 2;
-      F.x[0] = out->un; F.xnm[0] = out->un_nm; F.f[0] = in->un; F.fnm[0] = in->un_nm;
-      F.x[1] = out->vn; F.xnm[1] = out->vn_nm; F.f[1] = in->vn; F.fnm[1] = in->vn_nm;
-      hipLaunchKernelGGL(post_kernel<P_DUV>, dim3(nblocks(nel)), dim3(256), 0, s, D, F);
-      break;
-    case 2:
-    case 3:
-      F.n = 2;
-      if (which == 2) {
-        F.x[0] = out->n2d; F.xnm[0] = out->n2d_nm; F.f[0] = in->n2d; F.fnm[0] = in->n2d_nm;
-        F.x[1] = out->ar; F.xnm[1] = out->ar_nm; F.f[1] = in->ar; F.fnm[1] = in->ar_nm;
-      } else {
-        F.x[0] = out->no; F.xnm[0] = out->no_nm; F.f[0] = in->no; F.fnm[0] = in->no_nm;
-        F.x[1] = out->n4s; F.xnm[1] = out->n4s_nm; F.f[1] = in->n4s; F.fnm[1] = in->n4s_nm;
-      }
-      hipLaunchKernelGGL(post_kernel<P_MINOR>, dim3(nblocks(nel)), dim3(256), 0, s, D, F);
-      break;
-    case 4:
-      F.n = 1;
-      F.x[0] = op_o; F.xnm[0] = nm_o; F.f[0] = op_in; F.fnm[0] = nm_in;
-      hipLaunchKernelGGL(post_kernel<P_OPLUS>, dim3(nblocks(nel)), dim3(256), 0, s, D, F);
-      break;
-    case 5: {
-      CompIO C;
-      C.o2 = out->o2; C.o1 = out->o1; C.he = out->he;
-      C.o2nm = out->o2_nm; C.o1nm = out->o1_nm; C.henm = out->he_nm;
-      C.o2i = in->o2; C.o1i = in->o1; C.hei = in->he;
-      C.o2nmi = in->o2_nm; C.o1nmi = in->o1_nm; C.henmi = in->he_nm;
-      C.xnmbar = in->xnmbar;
-      C.zpmid = g->zpmid;
-      C.dz = g->dz;
-      hipLaunchKernelGGL(comp_post_kernel, dim3(nblocks((long)D.nlon * 2 * nfilt)), dim3(256), 0, s, D, C);
-      break;
-    }
-    default:
-      if (which < 100 || which > 105) return tgx_set_error(TGX_ERR_ARG, "tgx_glb_post: unknown pass %d", which);
-      which -= 100;
+  if (which == 5 && nfilt > 0) {
+    CompIO C;
+    C.o2 = out->o2; C.o1 = out->o1; C.he = out->he;
+    C.o2nm = out->o2_nm; C.o1nm = out->o1_nm; C.henm = out->he_nm;
+    C.o2i = in->o2; C.o1i = in->o1; C.hei = in->he;
+    C.o2nmi = in->o2_nm; C.o1nmi = in->o1_nm; C.henmi = in->he_nm;
+    C.xnmbar = in->xnmbar;
+    C.zpmid = g->zpmid;
+    C.dz = g->dz;
+    hipLaunchKernelGGL(comp_post_kernel, dim3(nblocks((long)D.nlon * 2 * nfilt)), dim3(256), 0, s, D, C);
   }
   TGX_HIP(hipGetLastError());
   // mp_periodic_f3d of every output the pass wrote

@@ -17,40 +17,36 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_field
                              const tgx_ng_out *out, void *const *bufs, int sub, hipStream_t s);
 extern "C" void tgx_ng_set_global_mode(int on, int nfilt);
 // filter.hip
-extern "C" int tgx_ringfilter_post_fits(int nlon, int order);
+extern "C" int tgx_ring_table_dev(double *tab, int nlon, int tord, double dlamda, hipStream_t s);
 extern "C" int tgx_ringfilter_post_dev(int nf, double *const *f, double *const *xnm, const double *const *fin,
                                        const double *const *fnmi, int kind, int nlev, int nk, int nlon, int nlat,
                                        int ld, int order, int oplus, double dlamda, int nlat_filter,
-                                       const int *chunk, hipStream_t s);
+                                       const int *chunk, const double *gtab, int tord, hipStream_t s);
 extern "C" int tgx_glb_post(int which, int nfilt, const tgx_ng_grid *g, const tgx_ng_fields *in,
                             const tgx_ng_out *out, const double *op_in, const double *nm_in, double *op_o,
                             double *nm_o, hipStream_t s);
 
 namespace {
+// The ring filters' cos/sin table of the global routines: harmonics 1..3
+// (the largest order they filter with), at the end of the workspace.
+constexpr int kGlbTableOrder = 3;
 // The global grid of the tgx_glb_* call running on this host thread, or null
 // for a nest domain.  The routines below then run the Thomas scheme with the
 // global grid's column kernels and, after each column kernel, the ring filter
 // and the global post pass of that routine.
 thread_local const tgx_glb_grid *t_glb = nullptr;
-
-// filter_tn / filter_uv / filter_minor / filter_op / filter_o2o: the levels
-// 0..nlev-1 of f on the chunk table's latitudes.
-int glb_filter(const tgx_ng_grid *g, double *f, int nlev, int order, int oplus, hipStream_t s) {
-  return tgx_ringfilter_dev(f, nlev, g->nlon, g->nlat, g->ld, order, oplus, g->dlamda, t_glb->nlat_filter,
-                            t_glb->chunk, s);
-}
+// Its ring filters' cos/sin table in the workspace (filled by glb_call).
+thread_local const double *t_glb_tab = nullptr;
 
 // After a global routine's column kernel, its ring filters and post pass
 // (which: as tgx_glb_post).  dt.F:389-424 (filter_tn below the top level),
 // duv.F:490-514 (filter_uv of UN and VN below the top level),
 // minor.F:546-597 (filter_minor, all levels; a pair of species per pass),
 // oplus.F:896-941 (filter_op below the top level, one sub-step), comp.F:925-1022
-// (filter_o2o of O2, O, He, all levels, order 1).  Where the filter's
-// cos/sin table fits in LDS, all the routine's fields are filtered in one
-// launch with the post pass applied to each filtered value (comp's post pass,
-// which needs the whole column, stays its own kernel), and tgx_glb_post adds
-// the periodic points; otherwise each field is filtered alone and
-// tgx_glb_post runs the post kernels on the filtered latitudes.
+// (filter_o2o of O2, O, He, all levels, order 1).  All the routine's fields
+// are filtered in one launch, with the post pass applied to each filtered
+// value (comp's post pass, which needs the whole column, stays its own
+// kernel); tgx_glb_post then adds the periodic points.
 int glb_finish(int which, const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                const double *op_in, const double *nm_in, double *op_o, double *nm_o, hipStream_t s) {
   enum { NONE = 0, DT, DUV, MINOR, OPLUS };  // the post kinds of tgx_ringfilter_post_dev
@@ -102,14 +98,9 @@ int glb_finish(int which, const tgx_ng_grid *g, const tgx_ng_fields *in, const t
       return tgx_set_error(TGX_ERR_ARG, "glb_finish: unknown routine %d", which);
   }
   const int nfilt = t_glb->nlat_filter;
-  if (tgx_ringfilter_post_fits(g->nlon, order)) {
-    if (int rc = tgx_ringfilter_post_dev(nf, f, xnm, fi, fm, kind, nlev, g->nk, g->nlon, g->nlat, g->ld, order,
-                                         oplus, g->dlamda, nfilt, t_glb->chunk, s))
-      return rc;
-    return tgx_glb_post(kind != NONE ? which + 100 : which, nfilt, g, in, out, op_in, nm_in, op_o, nm_o, s);
-  }
-  for (int q = 0; q < nf; ++q)
-    if (int rc = glb_filter(g, f[q], nlev, order, oplus, s)) return rc;
+  if (int rc = tgx_ringfilter_post_dev(nf, f, xnm, fi, fm, kind, nlev, g->nk, g->nlon, g->nlat, g->ld, order, oplus,
+                                       g->dlamda, nfilt, t_glb->chunk, t_glb_tab, kGlbTableOrder, s))
+    return rc;
   return tgx_glb_post(which, nfilt, g, in, out, op_in, nm_in, op_o, nm_o, s);
 }
 }  // namespace
@@ -325,7 +316,7 @@ int run_chem(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
 int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                void *stream, bool chem_done);
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
-              void *stream, bool chem_done, int nrun);
+              void *stream, bool chem_done, int nrun, hipEvent_t fe_ready = nullptr);
 int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
              void *stream, bool chem_done);
 }  // namespace
@@ -392,8 +383,10 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   return glb_finish(3, g, in, out, nullptr, nullptr, nullptr, nullptr, s);
 }
 
+// fe_ready, if given, is recorded on the stream once the last sub-step's
+// stage 1 has written Fe and Fn (all that duv reads of this routine).
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
-              void *stream, bool chem_done, int nrun) {
+              void *stream, bool chem_done, int nrun, hipEvent_t fe_ready) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_OPLUS, s);
@@ -431,6 +424,7 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
       if (int rc = tgx_ng_launch(K_OP_CHEM, g, in, out, b, sub, s)) return rc;
     const bool lite = reuse && sub > 0 && sub < nsub - 1;
     if (int rc = tgx_ng_launch(lite ? K_OP1_LITE : K_OP1, g, in, out, b, sub, s)) return rc;
+    if (fe_ready && sub == nsub - 1) TGX_HIP(hipEventRecord(fe_ready, s));
     if (int rc = tgx_ng_launch(K_OP2, g, in, out, b, sub, s)) return rc;
     if (pcr)  // Thomas: the stencils are fused into the column kernel
       if (int rc = tgx_ng_launch(K_OP_PRE, g, in, out, b, sub, s)) return rc;
@@ -648,9 +642,8 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   struct Join {
     Fork *F;
     hipStream_t s0;
-    bool sa_joined = false;
     ~Join() {
-      if (!sa_joined) (void)join(0, 1);
+      (void)join(0, 1);
       (void)join(1, 2);
       (void)join(2, 3);
     }
@@ -659,13 +652,16 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
       return e == hipSuccess ? hipStreamWaitEvent(s0, F->ev[ev], 0) : e;
     }
   } join{F, s0};
-  if (int rc = run_oplus(g, in, out, work, sa, true, g->nstep_sub)) return rc;    // stream a
+  // stream a; ev[4]: its Fe/Fn are written (the last sub-step's stage 1)
+  if (int rc = run_oplus(g, in, out, work, sa, true, g->nstep_sub, F->ev[4])) return rc;
   if (int rc = run_minors(g, in, out, work, sb, true)) return rc;   // stream b
   if (int rc = run_comp(g, in, out, work, sc, true)) return rc;     // stream c
   if (int rc = tgx_ng_dt(g, in, out, work, s0)) return rc;       // caller's stream
-  join.sa_joined = true;  // duv needs oplus' Fe/Fn
-  TGX_HIP(join.join(0, 1));
-  return tgx_ng_duv(g, in, out, work, s0);  // streams b and c joined on return
+  // duv reads Fe/Fn of oplus and no other output or workspace region of it,
+  // so it starts while the rest of the O+ sub-step (its column solve, ring
+  // filter, post pass) still runs
+  TGX_HIP(hipStreamWaitEvent(s0, F->ev[4], 0));
+  return tgx_ng_duv(g, in, out, work, s0);  // the fork streams joined on return
 }
 
 // The same step strictly in the reference's order on one stream.
@@ -703,25 +699,36 @@ int check_glb(const tgx_glb_grid *gg) {
 // Sets this thread's global grid for the duration of one entry point.
 struct GlbScope {
   const tgx_glb_grid *prev;
-  explicit GlbScope(const tgx_glb_grid *gg) : prev(t_glb) {
+  const double *prev_tab;
+  GlbScope(const tgx_glb_grid *gg, const double *tab) : prev(t_glb), prev_tab(t_glb_tab) {
     t_glb = gg;
+    t_glb_tab = tab;
     tgx_ng_set_global_mode(1, gg->nlat_filter);
   }
   ~GlbScope() {
     t_glb = prev;
+    t_glb_tab = prev_tab;
     tgx_ng_set_global_mode(prev != nullptr, prev ? prev->nlat_filter : 0);
   }
 };
+// The ring filters' table follows the nest form's workspace.
+double *glb_table(const tgx_ng_grid *g, void *work) {
+  return (double *)((char *)work + tgx_ng_workspace_bytes(g));
+}
 int glb_call(RoutineFn fn, const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
              void *stream) {
   if (int rc = check_glb(gg)) return rc;
-  GlbScope scope(gg);
+  if (!work) return tgx_set_error(TGX_ERR_ARG, "tgx_glb: null workspace");
+  double *tab = glb_table(&gg->g, work);
+  if (int rc = tgx_ring_table_dev(tab, gg->g.nlon, kGlbTableOrder, gg->g.dlamda, (hipStream_t)stream)) return rc;
+  GlbScope scope(gg, tab);
   return fn(&gg->g, in, out, work, stream);
 }
 }  // namespace
 
 extern "C" unsigned long long tgx_glb_workspace_bytes(const tgx_glb_grid *gg) {
-  return gg ? tgx_ng_workspace_bytes(&gg->g) : 0;
+  if (!gg) return 0;
+  return tgx_ng_workspace_bytes(&gg->g) + sizeof(double) * 2ull * kGlbTableOrder * (unsigned long long)gg->g.nlon;
 }
 extern "C" int tgx_glb_oplus(const tgx_glb_grid *gg, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                              void *stream) {

@@ -529,6 +529,9 @@ class GlobalState(NestState):
         self.gg.g = self.g
         self.gg.nlat_filter = len(grid["chunk"])
         self.gg.chunk = C.cast(self.chunk, _P)
+        nbytes = _lib().tgx_glb_workspace_bytes(C.byref(self.gg))  # the nest form's + the ring filters' table
+        if nbytes > 8 * self.work.numel():
+            self.work = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=self.work.device)
 
     def _call(self, name, stream=None):
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
