@@ -314,7 +314,7 @@ int run_chem(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
   return tgx_ng_launch(K_CHEM, g, in, out, b, 0, s);
 }
 int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
-               void *stream, bool chem_done);
+               void *stream, bool chem_done, hipEvent_t chem_ready = nullptr);
 int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
               void *stream, bool chem_done, int nrun, hipEvent_t fe_ready = nullptr);
 int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
@@ -344,8 +344,10 @@ extern "C" int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
 }
 
 namespace {
+// chem_ready, if given, is waited for after the stencil prelude (which reads
+// no output of the chemistry pass) and before the column kernels.
 int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
-               void *stream, bool chem_done) {
+               void *stream, bool chem_done, hipEvent_t chem_ready) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   ProfScope prof(R_MINORS, s);
@@ -356,6 +358,7 @@ int run_minors(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *
   b[10] = tiles(work, g, T_MS);
   for (int q = 11; q < 14; ++q) b[q] = nullptr;
   if (int rc = tgx_ng_launch(K_MIN_PRE, g, in, out, b, 0, s)) return rc;
+  if (chem_ready) TGX_HIP(hipStreamWaitEvent(s, chem_ready, 0));
   // N2D and AR first; NO and N4S use this step's N2D (dynamics_ng.F90:147-165)
   const bool pcr = use_pcr(g, R_MINORS);
   if (!chem_done)
@@ -630,15 +633,12 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
   Fork *F = nullptr;
   if (int rc = get_fork(&F)) return rc;
   hipStream_t s0 = (hipStream_t)stream, sa = F->s[0], sb = F->s[1], sc = F->s[2];
-  // the pointwise chemistry of all routines in one pass, before the fork
-  if (int rc = run_chem(g, in, out, work, s0)) return rc;
-  TGX_HIP(hipEventRecord(F->ev[0], s0));
-  TGX_HIP(hipStreamWaitEvent(sa, F->ev[0], 0));
-  TGX_HIP(hipStreamWaitEvent(sb, F->ev[0], 0));
-  TGX_HIP(hipStreamWaitEvent(sc, F->ev[0], 0));
+  // Fork: every stream starts after what the caller queued before the step.
   // From here on every return joins the three fork streams back into s0, so
   // that a failed call still leaves a graph capture joined and no forked
   // kernel running past the caller's stream.
+  TGX_HIP(hipEventRecord(F->ev[5], s0));
+  for (hipStream_t st : {sa, sb, sc}) TGX_HIP(hipStreamWaitEvent(st, F->ev[5], 0));
   struct Join {
     Fork *F;
     hipStream_t s0;
@@ -652,9 +652,16 @@ extern "C" int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const 
       return e == hipSuccess ? hipStreamWaitEvent(s0, F->ev[ev], 0) : e;
     }
   } join{F, s0};
+  // The pointwise chemistry of all routines in one pass on s0 (ev[0]: done),
+  // beside the minor species' stencil prelude on stream b, which does not
+  // read it; the rest of every routine starts after it.
+  if (int rc = run_chem(g, in, out, work, s0)) return rc;
+  TGX_HIP(hipEventRecord(F->ev[0], s0));
+  TGX_HIP(hipStreamWaitEvent(sa, F->ev[0], 0));
+  TGX_HIP(hipStreamWaitEvent(sc, F->ev[0], 0));
   // stream a; ev[4]: its Fe/Fn are written (the last sub-step's stage 1)
   if (int rc = run_oplus(g, in, out, work, sa, true, g->nstep_sub, F->ev[4])) return rc;
-  if (int rc = run_minors(g, in, out, work, sb, true)) return rc;   // stream b
+  if (int rc = run_minors(g, in, out, work, sb, true, F->ev[0])) return rc;   // stream b
   if (int rc = run_comp(g, in, out, work, sc, true)) return rc;     // stream c
   if (int rc = tgx_ng_dt(g, in, out, work, s0)) return rc;       // caller's stream
   // duv reads Fe/Fn of oplus and no other output or workspace region of it,
