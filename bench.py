@@ -213,6 +213,7 @@ def bench_nests(pkg, args):
                    "coupling": "map_in, set_bndry, time interpolation and map_out on the device",
                    "ms_without_coupling": round(ms_plain, 4), "state_finite_after_run": finite,
                    "scheme": args.scheme, "schemes": ns.states[0].schemes(),
+                   "wave_columns_per_block": {names[i]: st.wave_columns() for i, st in enumerate(ns.states)},
                    "schedule": "serial" if args.serial else "concurrent"},
         "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,

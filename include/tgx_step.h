@@ -127,14 +127,22 @@ int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_n
  * may be captured into a hipGraph. */
 int tgx_ng_step(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                 void *stream);
-/* Autotuner of TGX_SCHEME_AUTO: times each routine under both schemes on
- * `stream` (host-synchronising; not for graph capture) and keeps the faster
- * one per device, shape and routine.  tgx_ng_step calls it on the first AUTO
- * step of a shape.  tgx_ng_scheme_choice reports the scheme a routine (0..4 =
- * oplus, minors, dt, duv, comp) runs with for this grid. */
+/* Autotuner of TGX_SCHEME_AUTO: times each routine on `stream`
+ * (host-synchronising; not for graph capture) under the Thomas scheme and
+ * under the wavefront-per-column scheme with 1, 2 and 4 columns per block, and
+ * keeps the fastest per device, shape and routine.  tgx_ng_step calls it on
+ * the first AUTO step of a shape.  tgx_ng_scheme_choice reports the scheme a
+ * routine (0..4 = oplus, minors, dt, duv, comp) runs with for this grid, and
+ * tgx_ng_wave_columns_choice its columns per block when that is
+ * TGX_SCHEME_PCR (4 unless tuned).  No reference counterpart. */
 int tgx_ng_autotune(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                     void *stream);
 int tgx_ng_scheme_choice(const tgx_ng_grid *g, int routine);
+int tgx_ng_wave_columns_choice(const tgx_ng_grid *g, int routine);
+/* Columns per block (1, 2 or 4) of the wavefront-per-column kernels launched
+ * from the calling host thread, in place of the autotuner's choice; 0 returns
+ * to the choice. */
+void tgx_ng_force_wave_columns(int cols);
 /* The same step strictly in the reference's order on `stream` alone. */
 int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                        void *work, void *stream);

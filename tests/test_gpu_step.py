@@ -166,8 +166,30 @@ def test_auto_scheme_tunes_and_matches_oracle(pkg, oracle):
     ns.step()
     torch.cuda.synchronize()
     assert set(ns.schemes().values()) <= {"thomas", "pcr"}
+    assert set(ns.wave_columns().values()) <= {1, 2, 4}
     for n in ("tn", "un", "vn", "op", "no", "n4s", "n2d", "ar", "o2", "o1", "he"):
         assert field_err(ns.fortran(n), ref[n], n) <= TOL, n
+
+
+@pytest.mark.parametrize("cols", [1, 2])
+def test_wave_kernels_with_fewer_columns_per_block_equal_four(pkg, cols):
+    """The wavefront-per-column kernels launched with 1 or 2 columns per block
+    (the autotuner's other candidates) give bitwise the step of 4 per block."""
+    import torch
+    g = pkg.synth.make_grid("small", nstep_sub=2)
+    F = pkg.synth.make_state(g)
+    a = pkg.step.NestState(g, F, scheme="pcr")
+    a.step()
+    torch.cuda.synchronize()
+    b = pkg.step.NestState(g, F, scheme="pcr")
+    pkg.step._lib().tgx_ng_force_wave_columns(cols)
+    try:
+        b.step()  # every routine's wave kernels launched with `cols` per block
+    finally:
+        pkg.step._lib().tgx_ng_force_wave_columns(0)
+    torch.cuda.synchronize()
+    for n in a.out:
+        assert torch.equal(a.out[n], b.out[n]), n
 
 
 def test_profile_hooks_count_calls(pkg):

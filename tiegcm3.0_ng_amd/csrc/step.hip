@@ -94,6 +94,10 @@ __device__ __forceinline__ void cap_put(const tgx_ng_grid &g, const Geo &G, int 
 // Set by the global-grid entry points around their launches on the calling
 // host thread (tgx_glb_* in step_api.cpp).
 thread_local int t_glb_mode = 0, t_glb_nfilt = 0;
+// Columns (wavefronts) per block of the wavefront-per-column kernels, set by
+// step_api.cpp before each routine's launches from the autotuner's choice.
+thread_local int t_wave_cols = 4;
+extern "C" void tgx_ng_set_wave_columns(int cols) { t_wave_cols = cols == 1 || cols == 2 ? cols : 4; }
 extern "C" void tgx_ng_set_global_mode(int on, int nfilt) {
   t_glb_mode = on;
   t_glb_nfilt = on ? nfilt : 0;
@@ -4639,8 +4643,9 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   int nbp = (int)((ncol * G.nk + 255) / 256);
   // thread per column (grid), per column-level (gridp), wavefront per column
   // (gridw)
-  int nbw = (int)((ncol + 3) / 4);
-  dim3 grid(nb), gridp(nbp), gridw(nbw), block(256);
+  const int wcols = t_wave_cols;
+  int nbw = (int)((ncol + wcols - 1) / wcols);
+  dim3 grid(nb), gridp(nbp), gridw(nbw), block(256), blockw(64 * wcols);
   auto D = [&](int n) { return (double *)bufs[n]; };
   auto job = [](int kind, const double *f, double *o, double sh) {
     StencilJob jb;
@@ -4650,10 +4655,10 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
   switch (which) {
     case 20:
       if (G.nk <= 64)
-        hipLaunchKernelGGL(dt_wave_kernel<1>, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
+        hipLaunchKernelGGL(dt_wave_kernel<1>, gridw, blockw, 0, s, g, G, in, out, (const double *)D(1),
                            (const double *)D(2));
       else
-        hipLaunchKernelGGL(dt_wave_kernel<2>, gridw, block, 0, s, g, G, in, out, (const double *)D(1),
+        hipLaunchKernelGGL(dt_wave_kernel<2>, gridw, blockw, 0, s, g, G, in, out, (const double *)D(1),
                            (const double *)D(2));
       break;
     case 25: {  // bufs = {comp input tiles, fs(:,1,0), fs(:,2,0), (unused x3), minor source tiles,
@@ -4718,20 +4723,20 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
       else if (which == 2)
         hipLaunchKernelGGL((minor_kernel<1, 0, false>), grid, block, 0, s, g, G, in, out, D(0), MT);
       else if (which == 18 && G.nk <= 64)
-        hipLaunchKernelGGL((minor_wave_kernel<2, 3, 1>), gridw, block, 0, s, g, G, in, out, MT);
+        hipLaunchKernelGGL((minor_wave_kernel<2, 3, 1>), gridw, blockw, 0, s, g, G, in, out, MT);
       else if (which == 18)
-        hipLaunchKernelGGL((minor_wave_kernel<2, 3, 2>), gridw, block, 0, s, g, G, in, out, MT);
+        hipLaunchKernelGGL((minor_wave_kernel<2, 3, 2>), gridw, blockw, 0, s, g, G, in, out, MT);
       else if (which == 30) {  // buf 14: chem_kernel's linear-coefficient tiles
         MT.lt = D(14);
         MT.n2d = out.n2d;
         if (G.nk <= 64)
-          hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1, true>), gridw, block, 0, s, g, G, in, out, MT);
+          hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1, true>), gridw, blockw, 0, s, g, G, in, out, MT);
         else
-          hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2, true>), gridw, block, 0, s, g, G, in, out, MT);
+          hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2, true>), gridw, blockw, 0, s, g, G, in, out, MT);
       } else if (which == 19 && G.nk <= 64)
-        hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1>), gridw, block, 0, s, g, G, in, out, MT);
+        hipLaunchKernelGGL((minor_wave_kernel<1, 0, 1>), gridw, blockw, 0, s, g, G, in, out, MT);
       else if (which == 19)
-        hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2>), gridw, block, 0, s, g, G, in, out, MT);
+        hipLaunchKernelGGL((minor_wave_kernel<1, 0, 2>), gridw, blockw, 0, s, g, G, in, out, MT);
       else if (which == 15)
         hipLaunchKernelGGL((minor_src_kernel<2, 3>), gridp, block, 0, s, g, G, in, out, MT);
       else
@@ -4802,26 +4807,26 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
             hipLaunchKernelGGL((oplus_tile_kernel<false, 0>), gridt, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
                                (const double *)D(19), (const double *)D(20), opout, optm1out);
         else if (G.nk <= 64)
-          hipLaunchKernelGGL(oplus_wave_stage3<1>, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+          hipLaunchKernelGGL(oplus_wave_stage3<1>, gridw, blockw, 0, s, g, G, in, op, optm1, lonb, latb, W,
                              OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
         else
-          hipLaunchKernelGGL(oplus_wave_stage3<2>, gridw, block, 0, s, g, G, in, op, optm1, lonb, latb, W,
+          hipLaunchKernelGGL(oplus_wave_stage3<2>, gridw, blockw, 0, s, g, G, in, op, optm1, lonb, latb, W,
                              OT, (const double *)D(19), (const double *)D(20), opout, optm1out);
       }
       break;
     }
     case 6:  // duv stage 1: bufs = {ztmp}
       if (G.nk <= 64 && small_grid(ncol))
-        hipLaunchKernelGGL(duv_stage1_wave, gridw, block, 0, s, g, G, in, (const double *)out.tn, D(0));
+        hipLaunchKernelGGL(duv_stage1_wave, gridw, blockw, 0, s, g, G, in, (const double *)out.tn, D(0));
       else
         hipLaunchKernelGGL(duv_stage1, grid, block, 0, s, g, G, in, (const double *)out.tn, D(0));
       break;
     case 22: {  // duv stage 2 (PCR): bufs = {ztmp, gamma, divx_ztmp, divy_z, unms, vnms, advu, advv}
       UvTerms UT{D(2), D(3), D(4), D(5), D(6), D(7)};
       if (G.nk <= 64)
-        hipLaunchKernelGGL(duv_wave_stage2<1>, gridw, block, 0, s, g, G, in, out, UT);
+        hipLaunchKernelGGL(duv_wave_stage2<1>, gridw, blockw, 0, s, g, G, in, out, UT);
       else
-        hipLaunchKernelGGL(duv_wave_stage2<2>, gridw, block, 0, s, g, G, in, out, UT);
+        hipLaunchKernelGGL(duv_wave_stage2<2>, gridw, blockw, 0, s, g, G, in, out, UT);
       break;
     }
     case 8:
@@ -4835,9 +4840,9 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
           hipLaunchKernelGGL(comp_kernel<false>, grid, block, 0, s, g, G, in, out, D(0), CT);
       } else if (which == 23) {
         if (G.nk <= 64)
-          hipLaunchKernelGGL(comp_wave_kernel<1>, gridw, block, 0, s, g, G, in, out, CT);
+          hipLaunchKernelGGL(comp_wave_kernel<1>, gridw, blockw, 0, s, g, G, in, out, CT);
         else
-          hipLaunchKernelGGL(comp_wave_kernel<2>, gridw, block, 0, s, g, G, in, out, CT);
+          hipLaunchKernelGGL(comp_wave_kernel<2>, gridw, blockw, 0, s, g, G, in, out, CT);
       } else {
         dim3 grido((unsigned)((ncol * (G.nk - 1) + 255) / 256));
         hipLaunchKernelGGL(o2o_kernel, grido, block, 0, s, g, G, in, CT);

@@ -16,6 +16,7 @@
 extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *g, const tgx_ng_fields *in,
                              const tgx_ng_out *out, void *const *bufs, int sub, hipStream_t s);
 extern "C" void tgx_ng_set_global_mode(int on, int nfilt);
+extern "C" void tgx_ng_set_wave_columns(int cols);
 // filter.hip
 extern "C" int tgx_ring_table_dev(double *tab, int nlon, int tord, double dlamda, hipStream_t s);
 extern "C" int tgx_ringfilter_post_dev(int nf, double *const *f, double *const *xnm, const double *const *fin,
@@ -253,7 +254,9 @@ namespace {
 typedef std::tuple<int, int, int, int, int, int, int> TuneKey;  // dev nk nlon nlat ld nsub routine
 std::mutex g_tune_mu;
 std::map<TuneKey, int> g_tune;
-thread_local int t_forced = -1;  // scheme forced while the autotuner times a routine
+std::map<TuneKey, int> g_tune_cols;  // columns per block of the wavefront-per-column kernels
+thread_local int t_forced = -1;       // scheme forced while the autotuner times a routine
+thread_local int t_forced_cols = 0;   // and its columns per block (PCR)
 
 TuneKey tune_key(const tgx_ng_grid *g, int routine) {
   int dev = 0;
@@ -265,11 +268,27 @@ int tuned_choice(const tgx_ng_grid *g, int routine) {
   auto it = g_tune.find(tune_key(g, routine));
   return it == g_tune.end() ? TGX_SCHEME_AUTO : it->second;
 }
+// Whether the routine runs the wavefront-per-column scheme; if so, its
+// columns per block (the autotuner's choice, else 4) are set for the launches
+// that follow on this host thread.
 bool use_pcr(const tgx_ng_grid *g, int routine) {
-  if (g->nk > 128 || t_glb) return false;
-  if (t_forced >= 0) return t_forced == TGX_SCHEME_PCR;
-  if (g->scheme != TGX_SCHEME_AUTO) return g->scheme == TGX_SCHEME_PCR;
-  return tuned_choice(g, routine) != TGX_SCHEME_THOMAS;
+  bool pcr;
+  if (g->nk > 128 || t_glb) pcr = false;
+  else if (t_forced >= 0) pcr = t_forced == TGX_SCHEME_PCR;
+  else if (g->scheme != TGX_SCHEME_AUTO) pcr = g->scheme == TGX_SCHEME_PCR;
+  else pcr = tuned_choice(g, routine) != TGX_SCHEME_THOMAS;
+  if (pcr) {
+    int cols = 4;
+    if (t_forced_cols > 0) {
+      cols = t_forced_cols;
+    } else {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cols.find(tune_key(g, routine));
+      if (it != g_tune_cols.end()) cols = it->second;
+    }
+    tgx_ng_set_wave_columns(cols);
+  }
+  return pcr;
 }
 }  // namespace
 
@@ -576,16 +595,21 @@ extern "C" int tgx_ng_autotune(const tgx_ng_grid *g, const tgx_ng_fields *in, co
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
   const RoutineFn fns[R_N] = {tgx_ng_oplus, tgx_ng_minors, tgx_ng_dt, tgx_ng_duv, tgx_ng_comp};
+  const int user_cols = t_forced_cols;  // tgx_ng_force_wave_columns, restored below
   hipEvent_t e0, e1;
   TGX_HIP(hipEventCreate(&e0));
   TGX_HIP(hipEventCreate(&e1));
   int rc = 0;
   for (int r = 0; r < R_N && rc == 0; ++r) {
-    float best[3] = {0.f, 0.f, 0.f};
-    const int schemes[2] = {TGX_SCHEME_THOMAS, TGX_SCHEME_PCR};
-    const int nsch = g->nk > 128 ? 1 : 2;
+    // the candidates: Thomas, and the wavefront-per-column scheme with 1, 2
+    // or 4 columns (wavefronts) per block
+    const int schemes[4] = {TGX_SCHEME_THOMAS, TGX_SCHEME_PCR, TGX_SCHEME_PCR, TGX_SCHEME_PCR};
+    const int cols[4] = {0, 1, 2, 4};
+    float best[4] = {0.f, 0.f, 0.f, 0.f};
+    const int nsch = g->nk > 128 ? 1 : 4;
     for (int q = 0; q < nsch && rc == 0; ++q) {
       t_forced = schemes[q];
+      t_forced_cols = cols[q];
       float tmin = 1e30f;
       for (int rep = 0; rep < 3 && rc == 0; ++rep) {  // the first run is a warm-up
         if (hipEventRecord(e0, s) != hipSuccess) { rc = tgx_set_error(2, "tgx_ng_autotune: event record"); break; }
@@ -599,20 +623,33 @@ extern "C" int tgx_ng_autotune(const tgx_ng_grid *g, const tgx_ng_fields *in, co
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (rep > 0 && ms < tmin) tmin = ms;
       }
-      best[schemes[q]] = tmin;
+      best[q] = tmin;
     }
     t_forced = -1;
+    t_forced_cols = user_cols;
     if (rc == 0) {
-      int choice = (nsch == 2 && best[TGX_SCHEME_PCR] < best[TGX_SCHEME_THOMAS]) ? TGX_SCHEME_PCR
-                                                                               : TGX_SCHEME_THOMAS;
+      int q = 0;
+      for (int c = 1; c < nsch; ++c)
+        if (best[c] < best[q]) q = c;
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tune[tune_key(g, r)] = choice;
+      g_tune[tune_key(g, r)] = schemes[q];
+      if (q > 0) g_tune_cols[tune_key(g, r)] = cols[q];
     }
   }
   t_forced = -1;
+  t_forced_cols = user_cols;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+extern "C" void tgx_ng_force_wave_columns(int cols) { t_forced_cols = cols == 1 || cols == 2 || cols == 4 ? cols : 0; }
+
+extern "C" int tgx_ng_wave_columns_choice(const tgx_ng_grid *g, int routine) {
+  if (!g || routine < 0 || routine >= R_N) return tgx_set_error(1, "tgx_ng_wave_columns_choice: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cols.find(tune_key(g, routine));
+  return it == g_tune_cols.end() ? 4 : it->second;
 }
 
 extern "C" int tgx_ng_scheme_choice(const tgx_ng_grid *g, int routine) {

@@ -73,6 +73,8 @@ _SIGS = {
     "tgx_ng_profile_begin": ([C.c_int], C.c_int),
     "tgx_ng_autotune": ([_P] * 5, C.c_int),
     "tgx_ng_scheme_choice": ([_P, C.c_int], C.c_int),
+    "tgx_ng_wave_columns_choice": ([_P, C.c_int], C.c_int),
+    "tgx_ng_force_wave_columns": ([C.c_int], None),
     "tgx_ng_profile_end": ([_P, _P], C.c_int),
     "tgx_glb_workspace_bytes": ([_P], C.c_ulonglong),
     "tgx_glb_oplus": ([_P] * 5, C.c_int),
@@ -289,6 +291,13 @@ class NestState:
         names = {1: "thomas", 2: "pcr"}
         return {r: names.get(_lib().tgx_ng_scheme_choice(C.byref(self.g), n), "?")
                 for n, r in enumerate(ROUTINES)}
+
+    def wave_columns(self):
+        """Columns (wavefronts) per block of each routine that runs the
+        wavefront-per-column scheme."""
+        sch = self.schemes()
+        return {r: int(_lib().tgx_ng_wave_columns_choice(C.byref(self.g), n))
+                for n, r in enumerate(ROUTINES) if sch[r] == "pcr"}
 
 
 class NestedStep:
