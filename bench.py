@@ -433,7 +433,7 @@ def main():
     kname = names[1] if schemes[dom] == "pcr" else names[0]
     alg = pkg.step.column_kernel_bytes_per_column_level(dom, schemes[dom]) * cols_all * nk
     achieved = alg / (kt[dom] * 1e-3) / 1e9
-    traffic = traffic_from_profiles(kname, args.config, schemes[dom])
+    traffic = traffic_from_profiles(kname, args.config + ("-global" if glb else ""), schemes[dom])
     step_bytes = sum(pkg.step.alg_bytes_per_column_level(r) * cols_all * nk * (nsub if r == "oplus" else 1)
                      for r in routines)
     out = {
