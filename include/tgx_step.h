@@ -120,6 +120,11 @@ int tgx_ng_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out 
  * for capture mode (g->capture must be NULL). */
 int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
                       void *stream, int nsweep);
+/* tgx_ng_step with comp in that precision form (nsweep 0..2; no counterpart
+ * in the reference): the other routines as in tgx_ng_step.  Nest domains
+ * only, g->capture NULL; may be captured into a hipGraph. */
+int tgx_ng_step_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                      void *stream, int nsweep);
 /* The whole column step (dynamics_ng.F90:99-233).  The independent routines
  * (oplus, minors, dt, comp) run concurrently on library-owned streams forked
  * from and joined back into `stream`; duv follows dt and oplus.  Results are

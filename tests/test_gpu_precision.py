@@ -131,3 +131,30 @@ def test_comp_precision_form_refines_to_the_fp64_step(pkg, oracle, name):
     torch.cuda.synchronize()
     for n in COMP_FIELDS:
         assert np.array_equal(ns.fortran(n), ref[n]), n
+
+
+@pytest.mark.parametrize("scheme", ["thomas", "pcr"])
+def test_step_with_comp_precision_form(pkg, scheme):
+    """tgx_ng_step_mixed: the forked step with comp in its precision form (two
+    sweeps) gives the FP64 step's outputs bit for bit for every routine but
+    comp, and O2, O, He within 1e-10 pointwise of it; the entry point
+    rejects a sweep count outside 0..2."""
+    from helpers import field_errs
+    g = pkg.synth.make_grid("small", nstep_sub=2)
+    F = pkg.synth.make_state(g)
+    a = pkg.step.NestState(g, F, scheme=scheme)
+    a.step()
+    b = pkg.step.NestState(g, F, scheme=scheme)
+    b.comp_nsweep = 2
+    b.step()
+    torch.cuda.synchronize()
+    for n in a.out:
+        if n in COMP_FIELDS:
+            e = max(field_errs(n, b.fortran(n), a.fortran(n)))
+            print("%s %s max pointwise error %.3e" % (scheme, n, e))
+            assert e <= 1e-10, (n, e)
+        else:
+            assert torch.equal(a.out[n], b.out[n]), n
+    b.comp_nsweep = 3
+    with pytest.raises(pkg.tgx.TgxError):
+        b.step()

@@ -467,12 +467,20 @@ int run_oplus(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *o
 // pivots, then nsweep (0..2) refinement sweeps with FP64 residuals; the last
 // kernel applies the post pass.  Same inputs, outputs and workspace as
 // tgx_ng_comp.
-extern "C" int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
-                                 void *work, void *stream, int nsweep) {
-  if (int rc = check(g, work)) return rc;
-  if (nsweep < 0 || nsweep > 2) return tgx_set_error(TGX_ERR_ARG, "tgx_ng_comp_mixed: nsweep %d outside 0..2", nsweep);
-  if (g->capture) return tgx_set_error(TGX_ERR_ARG, "tgx_ng_comp_mixed: no capture mode");
-  hipStream_t s = (hipStream_t)stream;
+namespace {
+// Refinement sweeps of comp's precision form inside tgx_ng_step_mixed on
+// this host thread (-1: the FP64 routine).
+thread_local int t_comp_nsweep = -1;
+
+int check_mixed(const tgx_ng_grid *g, int nsweep, const char *fn) {
+  if (nsweep < 0 || nsweep > 2) return tgx_set_error(TGX_ERR_ARG, "%s: nsweep %d outside 0..2", fn, nsweep);
+  if (g->capture) return tgx_set_error(TGX_ERR_ARG, "%s: no capture mode", fn);
+  if (t_glb) return tgx_set_error(TGX_ERR_ARG, "%s: nest domains only", fn);
+  return 0;
+}
+
+int run_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out, void *work,
+                   hipStream_t s, bool chem_done, int nsweep) {
   ProfScope prof(R_COMP, s);
   void *b[5];
   b[0] = tiles(work, g, T_CE);  // FP32 gamma and forward solution (12 floats per level)
@@ -480,13 +488,22 @@ extern "C" int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, 
   b[2] = plane(work, g, W_CO + 0);
   b[3] = plane(work, g, W_CO + 1);
   b[4] = tiles(work, g, T_CU);
-  if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;
+  if (!chem_done)
+    if (int rc = tgx_ng_launch(K_O2O, g, in, out, b, 0, s)) return rc;
   if (int rc = tgx_ng_launch(K_CO_PRE, g, in, out, b, 0, s)) return rc;
   ProfScope kern(R_N + R_COMP, s);
   if (int rc = tgx_ng_launch(K_COMP_M0, g, in, out, b, 0, s)) return rc;
   for (int sw = 0; sw + 1 < nsweep; ++sw)
     if (int rc = tgx_ng_launch(K_COMP_M_REFINE, g, in, out, b, 0, s)) return rc;
   return tgx_ng_launch(nsweep > 0 ? K_COMP_M_LAST : K_COMP_M_POST, g, in, out, b, 0, s);
+}
+}  // namespace
+
+extern "C" int tgx_ng_comp_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                                 void *work, void *stream, int nsweep) {
+  if (int rc = check(g, work)) return rc;
+  if (int rc = check_mixed(g, nsweep, "tgx_ng_comp_mixed")) return rc;
+  return run_comp_mixed(g, in, out, work, (hipStream_t)stream, false, nsweep);
 }
 
 extern "C" int tgx_ng_duv(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
@@ -516,6 +533,7 @@ int run_comp(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *ou
              void *stream, bool chem_done) {
   if (int rc = check(g, work)) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (t_comp_nsweep >= 0) return run_comp_mixed(g, in, out, work, s, chem_done, t_comp_nsweep);
   ProfScope prof(R_COMP, s);
   void *b[4];
   b[0] = tiles(work, g, T_CE);  // Thomas gamma and forward solution
@@ -720,6 +738,19 @@ extern "C" int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in,
   if (int rc = tgx_ng_dt(g, in, out, work, stream)) return rc;
   if (int rc = tgx_ng_duv(g, in, out, work, stream)) return rc;
   return run_comp(g, in, out, work, stream, true);
+}
+
+// tgx_ng_step with comp in its precision form.
+extern "C" int tgx_ng_step_mixed(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
+                                 void *work, void *stream, int nsweep) {
+  if (int rc = check(g, work)) return rc;
+  if (int rc = check_mixed(g, nsweep, "tgx_ng_step_mixed")) return rc;
+  struct Scope {
+    int prev = t_comp_nsweep;
+    ~Scope() { t_comp_nsweep = prev; }
+  } scope;
+  t_comp_nsweep = nsweep;
+  return tgx_ng_step(g, in, out, work, stream);
 }
 
 // ---------------------------------------------------------------------------

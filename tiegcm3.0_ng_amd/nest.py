@@ -59,12 +59,16 @@ def mapout_box(gout, gin):
 
 class CoupledNests:
     """domains: [(grid, fields), ...] outermost first; nstep_ng[l] sub-steps
-    of level l per step of level l-1 (nstep_ng[0] steps of level 0 per run)."""
+    of level l per step of level l-1 (nstep_ng[0] steps of level 0 per run).
+    comp_nsweep: None for the FP64 step, or 0..2 for the step with comp in
+    its FP32 + FP64-refinement form (forked steps)."""
 
-    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda", forked=True, init_from_outer=True):
+    def __init__(self, domains, nstep_ng, scheme="thomas", device="cuda", forked=True, init_from_outer=True,
+                 comp_nsweep=None):
         self.states = [_step.NestState(g, F, device=device, scheme=scheme) for g, F in domains]
         for st in self.states:
             st.enable_feedback()
+            st.comp_nsweep = comp_nsweep  # comp in its precision form (tgx_ng_step_mixed), forked steps
         self.grids = [g for g, _ in domains]
         self.nstep_ng = list(nstep_ng)
         self.forked = forked

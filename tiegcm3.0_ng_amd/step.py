@@ -68,6 +68,7 @@ _SIGS = {
     "tgx_ng_duv": ([_P] * 5, C.c_int),
     "tgx_ng_comp": ([_P] * 5, C.c_int),
     "tgx_ng_comp_mixed": ([_P] * 5 + [C.c_int], C.c_int),
+    "tgx_ng_step_mixed": ([_P] * 5 + [C.c_int], C.c_int),
     "tgx_ng_step": ([_P] * 5, C.c_int),
     "tgx_ng_step_serial": ([_P] * 5, C.c_int),
     "tgx_ng_profile_begin": ([C.c_int], C.c_int),
@@ -240,17 +241,19 @@ class NestState:
         return t.contiguous().cpu().numpy()
 
     # ---- entry points -----------------------------------------------------------
-    def _call(self, name, stream=None):
+    _STEPS = ("tgx_ng_step", "tgx_ng_step_serial", "tgx_ng_step_mixed")
+
+    def _call(self, name, stream=None, extra=()):
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
         f, o = (self.f, self.o)
         phases = getattr(self, "_phases", None)
-        if phases and name in ("tgx_ng_step", "tgx_ng_step_serial"):
+        if phases and name in self._STEPS:
             f, o = phases[self.phase]
         rc = getattr(_lib(), name)(C.byref(self.g), C.byref(f), C.byref(o),
-                                   C.c_void_p(self.work.data_ptr()), C.c_void_p(s))
+                                   C.c_void_p(self.work.data_ptr()), C.c_void_p(s), *extra)
         if rc != 0:
             raise _tgx.TgxError(_tgx.last_error())
-        if phases and name in ("tgx_ng_step", "tgx_ng_step_serial"):
+        if phases and name in self._STEPS:
             self.phase ^= 1
 
     def oplus(self, stream=None):
@@ -277,8 +280,15 @@ class NestState:
         if rc != 0:
             raise _tgx.TgxError(_tgx.last_error())
 
+    # comp's refinement sweeps in step() (None: the FP64 routine; 0..2: its
+    # precision form, tgx_ng_step_mixed)
+    comp_nsweep = None
+
     def step(self, stream=None):
-        self._call("tgx_ng_step", stream)
+        if self.comp_nsweep is None:
+            self._call("tgx_ng_step", stream)
+        else:
+            self._call("tgx_ng_step_mixed", stream, (C.c_int(int(self.comp_nsweep)),))
 
     def step_serial(self, stream=None):
         self._call("tgx_ng_step_serial", stream)
