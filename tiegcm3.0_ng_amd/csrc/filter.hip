@@ -1,15 +1,19 @@
 // The high-latitude longitude ring filter of the global model
 // (ringfilter.F:1-111), applied after the column solves of dt, duv, minor,
 // comp and oplus (filter_tn dt.F:431-511 and its siblings) to the latitudes
-// next to each pole.  A block takes one latitude row and a run of levels:
-// its threads first tabulate cos(m ang) and sin(m ang) of the kept harmonics
-// in LDS (the only transcendental work, shared by every level of the run),
-// then each wavefront filters one level at a time with the row's longitudes
-// in registers, the Fourier coefficients as wavefront reductions and the
-// residual and chunk averages in its own LDS region.  The global routines
-// launch it once per routine for all the fields they filter, with their post
-// pass (time smoothing, top level, floors) applied to each filtered value as
-// it is written.
+// next to each pole.  A wavefront filters one level of one latitude row at a
+// time: the row's longitudes in registers, the Fourier coefficients of the
+// kept harmonics as wavefront reductions over a table of cos(m ang) and
+// sin(m ang) (the only transcendental work), the residual and the chunk
+// averages in its own LDS region.  Two forms of the launch:
+//  - tgx_ringfilter_dev and the host drop-in: a block of four wavefronts
+//    takes one row and a run of levels and first builds the table in LDS;
+//  - the global routines (tgx_ringfilter_post_dev): the table is built once
+//    per call in the workspace, a block is one wavefront and one level (small
+//    blocks that find room beside the other routines' column kernels), all
+//    the routine's fields go in one launch, and the routine's post pass (time
+//    smoothing, top level, floors) is applied to each filtered value as it is
+//    written.
 #include <hip/hip_runtime.h>
 
 #include "../../include/tgx.h"
