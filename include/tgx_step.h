@@ -197,6 +197,22 @@ int tgx_ng_interp2d_dev(const double *src, int nxp, int nyp, int ld, const doubl
  * f1 (t = t1), linear in t, to dst[(t - ta)*n + p]. */
 int tgx_ng_time_interp_dev(const double *f0, const double *f1, double *dst, long n, int t1, int ta, int tb,
                            void *stream);
+/* The same operations on up to 32 fields of one geometry in one launch (the
+ * coupling's set_bndry, map_in, map_out and init move 4 to 22 fields at a
+ * time): job q reads src[q] (interp3d: log-linear in z when zlog[q]) and
+ * writes dst[q], every other argument as above.  In the time batch each job
+ * has its own size and times, and t1[q] = 0 makes job q a plain copy
+ * dst[q] = f1[q] (f0[q], ta[q], tb[q] unused).  No counterparts in the
+ * reference (which loops over fields); host arrays of device pointers. */
+int tgx_ng_interp3d_batch_dev(int njobs, const double *const *src, double *const *dst, const int *zlog, int nzp,
+                              int nxp, int nyp, int ld, const double *zp, const double *xp, const double *yp,
+                              int nz, int nx, int ny, long dk, long di, long dj, const double *z, const double *x,
+                              const double *y, void *stream);
+int tgx_ng_interp2d_batch_dev(int njobs, const double *const *src, double *const *dst, int nxp, int nyp, int ld,
+                              const double *xp, const double *yp, int nx, int ny, long di, long dj, const double *x,
+                              const double *y, void *stream);
+int tgx_ng_time_interp_batch_dev(int njobs, const double *const *f0, const double *const *f1, double *const *dst,
+                                 const long *n, const int *t1, const int *ta, const int *tb, void *stream);
 
 /* A host-layout context for one nest domain: device copies of every field of
  * tgx_ng_fields and tgx_ng_out in level planes, a workspace and a stream,

@@ -74,6 +74,49 @@ def test_interp2d_and_time_kernels_match_oracle(pkg, oracle):
     assert np.array_equal(tout.cpu().numpy(), tw)
 
 
+def test_batched_coupling_kernels_equal_single_launches(pkg):
+    """The batched interp3d / interp2d / time-interpolation launches (one
+    launch for up to 32 fields, as set_bndry, map_in, map_out and the
+    sub-step inputs use them) give bit for bit what one launch per field
+    gives, the log-linear and linear forms mixed in one batch, time jobs of
+    different sizes and plain copies beside interpolations."""
+    import torch
+    rng = np.random.default_rng(11)
+    nzp, nxp, nyp, ld = 9, 13, 11, 16
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    zp, xp, yp = dev(np.linspace(-7, 7, nzp)), dev(_axes(rng, nxp, 10., 2.)), dev(_axes(rng, nyp, -20., 3.))
+    z, x, y = dev(np.sort(rng.uniform(-6, 6, 7))), dev(np.sort(rng.uniform(11, 33, 6))), dev(
+        np.sort(rng.uniform(-18, 8, 5)))
+    srcs = [dev(rng.uniform(0.5, 2., (nzp, nyp, ld))) for _ in range(5)]
+    zlogs = [1, 0, 1, 1, 0]
+    dk, di, dj = 6 * 5, 1, 6
+    one = [torch.zeros(7 * 6 * 5 + 3, dtype=torch.float64, device="cuda") for _ in srcs]
+    bat = [torch.zeros_like(t) for t in one]
+    for sr, o, zl in zip(srcs, one, zlogs):
+        pkg.tgx.interp3d_dev(sr, zp, xp, yp, o, z, x, y, dk, di, dj, zl, offset=3)
+    pkg.tgx.interp3d_batch_dev(srcs, [(t, 3) for t in bat], zlogs, zp, xp, yp, z, x, y, dk, di, dj)
+    planes = [sr[0] for sr in srcs]
+    one2 = [torch.zeros(6 * 5, dtype=torch.float64, device="cuda") for _ in planes]
+    bat2 = [torch.zeros_like(t) for t in one2]
+    for sr, o in zip(planes, one2):
+        pkg.tgx.interp2d_dev(sr, xp, yp, o, x, y, 1, 6)
+    pkg.tgx.interp2d_batch_dev(planes, bat2, xp, yp, x, y, 1, 6)
+    f0 = [dev(rng.normal(size=n)) for n in (40, 7, 123)]
+    f1 = [dev(rng.normal(size=n)) for n in (40, 7, 123)]
+    one3 = [torch.zeros(3 * 40, dtype=torch.float64, device="cuda"), torch.zeros(7, dtype=torch.float64,
+                                                                                 device="cuda"),
+            torch.zeros(123, dtype=torch.float64, device="cuda")]
+    bat3 = [torch.zeros_like(t) for t in one3]
+    pkg.tgx.time_interp_dev(f0[0], f1[0], one3[0], 10, 4, 6)
+    pkg.tgx.time_interp_dev(f0[1], f1[1], one3[1], 5, 5, 5)
+    one3[2].copy_(f1[2])
+    pkg.tgx.time_interp_batch_dev([(f0[0], f1[0], bat3[0], 10, 4, 6), (f0[1], f1[1], bat3[1], 5, 5, 5),
+                                   (None, f1[2], bat3[2], 0, 0, 0)])
+    torch.cuda.synchronize()
+    for a, b in zip(one + one2 + one3, bat + bat2 + bat3):
+        assert torch.equal(a, b)
+
+
 def test_coupled_nests_match_oracle_recursion(pkg, oracle):
     """small (16x12x29) with nest_s1 (20x12x29, 2 sub-steps) inside and
     nest_s2 (10x8x57, 3 sub-steps, another level set) inside that: two outer

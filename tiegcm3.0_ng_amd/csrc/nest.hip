@@ -115,6 +115,79 @@ __global__ void __launch_bounds__(256) time_interp_kernel(const double *__restri
   for (int s = 0; s < nt; ++s) f[p + n * (long)s] = lin((double)(ta + s), 0., x1, a, b);
 }
 
+// Batches: up to kMaxJobs fields with the same geometry in one launch (the
+// coupling of one nest level takes 22 fields per map_out, 11 per set_bndry).
+constexpr int kMaxJobs = 32;
+struct Jobs3 {
+  const double *src[kMaxJobs];
+  double *dst[kMaxJobs];
+  unsigned zlog;  // bit q: job q log-linear in z
+  int n;
+};
+
+__global__ void __launch_bounds__(256) interp3d_batch_kernel(Src S, Dst D, Jobs3 J) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long np = (long)D.nz * D.nx * D.ny;
+  const int q = (int)blockIdx.y;
+  if (t >= np || q >= J.n) return;
+  const int k = (int)(t % D.nz);
+  const long r = t / D.nz;
+  const int i = (int)(r % D.nx), j = (int)(r / D.nx);
+  const double x = D.x[i], y = D.y[j], z = D.z[k];
+  int ix0, ix1, jy0, jy1, kz0, kz1;
+  bracket(S.xp, S.nxp, x, ix0, ix1);
+  bracket(S.yp, S.nyp, y, jy0, jy1);
+  bracket(S.zp, S.nzp, z, kz0, kz1);
+  Src Sq = S;
+  Sq.f = J.src[q];
+  const double f0 = at_xy(Sq, kz0, ix0, ix1, jy0, jy1, x, y);
+  const double f1 = at_xy(Sq, kz1, ix0, ix1, jy0, jy1, x, y);
+  const double v = (J.zlog >> q) & 1u ? exp(lin(z, S.zp[kz0], S.zp[kz1], log(f0), log(f1)))
+                                      : lin(z, S.zp[kz0], S.zp[kz1], f0, f1);
+  J.dst[q][(size_t)k * D.dk + (size_t)i * D.di + (size_t)j * D.dj] = v;
+}
+
+__global__ void __launch_bounds__(256) interp2d_batch_kernel(Src S, Dst D, Jobs3 J) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = (int)blockIdx.y;
+  if (t >= (long)D.nx * D.ny || q >= J.n) return;
+  const int i = (int)(t % D.nx), j = (int)(t / D.nx);
+  const double x = D.x[i], y = D.y[j];
+  int ix0, ix1, jy0, jy1;
+  bracket(S.xp, S.nxp, x, ix0, ix1);
+  bracket(S.yp, S.nyp, y, jy0, jy1);
+  Src Sq = S;
+  Sq.f = J.src[q];
+  J.dst[q][(size_t)i * D.di + (size_t)j * D.dj] = at_xy(Sq, 0, ix0, ix1, jy0, jy1, x, y);
+}
+
+// Time interpolations (time_interp_kernel) and plain copies (copy: dst = f1)
+// of fields of different sizes in one launch: blockIdx.y is the job.
+struct TJob {
+  const double *f0, *f1;
+  double *dst;
+  long n;
+  int t1, ta, nt, copy;
+};
+struct TJobs {
+  TJob j[kMaxJobs];
+  int n;
+};
+
+__global__ void __launch_bounds__(256) time_batch_kernel(TJobs J) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = (int)blockIdx.y;
+  if (q >= J.n) return;
+  const TJob &b = J.j[q];
+  if (p >= b.n) return;
+  if (b.copy) {
+    b.dst[p] = b.f1[p];
+    return;
+  }
+  const double f0 = b.f0[p], f1 = b.f1[p], x1 = (double)b.t1;
+  for (int s = 0; s < b.nt; ++s) b.dst[p + b.n * (long)s] = lin((double)(b.ta + s), 0., x1, f0, f1);
+}
+
 }  // namespace nest
 }  // namespace tgx
 
@@ -167,6 +240,85 @@ extern "C" int tgx_ng_time_interp_dev(const double *f0, const double *f1, double
   if (n == 0) return 0;
   hipLaunchKernelGGL(tgx::nest::time_interp_kernel, dim3(blocks(n)), dim3(256), 0, (hipStream_t)stream, f0, f1,
                      dst, n, t1, ta, tb - ta + 1);
+  TGX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int tgx_ng_interp3d_batch_dev(int njobs, const double *const *src, double *const *dst, const int *zlog,
+                                         int nzp, int nxp, int nyp, int ld, const double *zp, const double *xp,
+                                         const double *yp, int nz, int nx, int ny, long dk, long di, long dj,
+                                         const double *z, const double *x, const double *y, void *stream) {
+  using namespace tgx::nest;
+  if (njobs < 0 || njobs > kMaxJobs || (njobs > 0 && (!src || !dst || !zlog)))
+    return tgx_set_error(TGX_ERR_ARG, "tgx_ng_interp3d_batch_dev: %d jobs (0..%d)", njobs, kMaxJobs);
+  if (njobs == 0) return 0;
+  if (int rc = check_axis("tgx_ng_interp3d_batch_dev zp", nzp, zp, 2)) return rc;
+  if (int rc = check_axis("tgx_ng_interp3d_batch_dev xp", nxp, xp, 2)) return rc;
+  if (int rc = check_axis("tgx_ng_interp3d_batch_dev yp", nyp, yp, 2)) return rc;
+  if (!z || !x || !y || ld < nxp || nz < 1 || nx < 1 || ny < 1)
+    return tgx_set_error(TGX_ERR_ARG, "tgx_ng_interp3d_batch_dev: bad arguments");
+  Jobs3 J = {};
+  for (int q = 0; q < njobs; ++q) {
+    if (!src[q] || !dst[q]) return tgx_set_error(TGX_ERR_ARG, "tgx_ng_interp3d_batch_dev: job %d has a NULL array", q);
+    J.src[q] = src[q];
+    J.dst[q] = dst[q];
+    if (zlog[q]) J.zlog |= 1u << q;
+  }
+  J.n = njobs;
+  const Src S{nullptr, nzp, nxp, nyp, ld, (long)ld * nyp, zp, xp, yp};
+  const Dst D{nullptr, nz, nx, ny, dk, di, dj, z, x, y};
+  const long n = (long)nz * nx * ny;
+  hipLaunchKernelGGL(interp3d_batch_kernel, dim3(blocks(n), (unsigned)njobs), dim3(256), 0, (hipStream_t)stream, S,
+                     D, J);
+  TGX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int tgx_ng_interp2d_batch_dev(int njobs, const double *const *src, double *const *dst, int nxp, int nyp,
+                                         int ld, const double *xp, const double *yp, int nx, int ny, long di,
+                                         long dj, const double *x, const double *y, void *stream) {
+  using namespace tgx::nest;
+  if (njobs < 0 || njobs > kMaxJobs || (njobs > 0 && (!src || !dst)))
+    return tgx_set_error(TGX_ERR_ARG, "tgx_ng_interp2d_batch_dev: %d jobs (0..%d)", njobs, kMaxJobs);
+  if (njobs == 0) return 0;
+  if (int rc = check_axis("tgx_ng_interp2d_batch_dev xp", nxp, xp, 2)) return rc;
+  if (int rc = check_axis("tgx_ng_interp2d_batch_dev yp", nyp, yp, 2)) return rc;
+  if (!x || !y || ld < nxp || nx < 1 || ny < 1)
+    return tgx_set_error(TGX_ERR_ARG, "tgx_ng_interp2d_batch_dev: bad arguments");
+  Jobs3 J = {};
+  for (int q = 0; q < njobs; ++q) {
+    if (!src[q] || !dst[q]) return tgx_set_error(TGX_ERR_ARG, "tgx_ng_interp2d_batch_dev: job %d has a NULL array", q);
+    J.src[q] = src[q];
+    J.dst[q] = dst[q];
+  }
+  J.n = njobs;
+  const Src S{nullptr, 1, nxp, nyp, ld, (long)ld * nyp, nullptr, xp, yp};
+  const Dst D{nullptr, 1, nx, ny, 0, di, dj, nullptr, x, y};
+  hipLaunchKernelGGL(interp2d_batch_kernel, dim3(blocks((long)nx * ny), (unsigned)njobs), dim3(256), 0,
+                     (hipStream_t)stream, S, D, J);
+  TGX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int tgx_ng_time_interp_batch_dev(int njobs, const double *const *f0, const double *const *f1,
+                                            double *const *dst, const long *n, const int *t1, const int *ta,
+                                            const int *tb, void *stream) {
+  using namespace tgx::nest;
+  if (njobs < 0 || njobs > kMaxJobs || (njobs > 0 && (!f0 || !f1 || !dst || !n || !t1 || !ta || !tb)))
+    return tgx_set_error(TGX_ERR_ARG, "tgx_ng_time_interp_batch_dev: %d jobs (0..%d)", njobs, kMaxJobs);
+  TJobs J = {};
+  long nmax = 0;
+  for (int q = 0; q < njobs; ++q) {
+    const bool copy = t1[q] == 0;
+    if (!f1[q] || !dst[q] || n[q] < 0 || (!copy && (!f0[q] || t1[q] < 1 || ta[q] < 0 || tb[q] < ta[q] || tb[q] > t1[q])))
+      return tgx_set_error(TGX_ERR_ARG, "tgx_ng_time_interp_batch_dev: job %d: n %ld t1 %d ta %d tb %d", q, n[q],
+                           t1[q], ta[q], tb[q]);
+    J.j[q] = TJob{f0[q], f1[q], dst[q], n[q], t1[q], ta[q], copy ? 1 : tb[q] - ta[q] + 1, copy ? 1 : 0};
+    if (n[q] > nmax) nmax = n[q];
+  }
+  J.n = njobs;
+  if (njobs == 0 || nmax == 0) return 0;
+  hipLaunchKernelGGL(time_batch_kernel, dim3(blocks(nmax), (unsigned)njobs), dim3(256), 0, (hipStream_t)stream, J);
   TGX_HIP(hipGetLastError());
   return 0;
 }

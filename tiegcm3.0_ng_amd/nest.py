@@ -75,6 +75,9 @@ class CoupledNests:
         kw = dict(dtype=torch.float64, device=device)
         self.axes = [dict(z=torch.tensor(g["zpint"], **kw), x=torch.tensor(g["glon"], **kw),
                           y=torch.tensor(g["glat"], **kw)) for g in self.grids]
+        for ax in self.axes:  # the two lateral boundary lines of each direction
+            ax["x2"] = ax["x"][[0, -1]].contiguous()
+            ax["y2"] = ax["y"][[0, -1]].contiguous()
         self.anchors = [None]
         self.boxes = [None]
         for l in range(1, len(self.states)):
@@ -109,27 +112,29 @@ class CoupledNests:
     def init_fields(self, l):
         """init_fields_ng.F90:128-140: the nest's prognostic fields (the state
         its next step reads) interpolated from the outermost domain's state
-        over the nest's whole grid, ghost points included."""
+        over the nest's whole grid, ghost points included (one launch)."""
         s0, a0, ai, st = self.states[0], self.axes[0], self.axes[l], self.states[l]
-        for n, zlog in MAPOUT:
-            _tgx.interp3d_dev(s0.current(n), a0["z"], a0["x"], a0["y"], st.current(n), ai["z"], ai["x"], ai["y"],
-                              st.nlatd * st.ld, 1, st.ld, zlog)
+        _tgx.interp3d_batch_dev([s0.current(n) for n, _ in MAPOUT], [st.current(n) for n, _ in MAPOUT],
+                                [z for _, z in MAPOUT], a0["z"], a0["x"], a0["y"], ai["z"], ai["x"], ai["y"],
+                                st.nlatd * st.ld, 1, st.ld)
 
     def set_bndry(self, l, slot, src=None):
         """levels_inner_ng.F90:299-410: the outer state at the nest's lateral
         boundary lines into the nest's anchors[slot]; src is the source
-        level (the parent, or the outermost domain at initialisation)."""
+        level (the parent, or the outermost domain at initialisation).  One
+        launch for the longitude sides of every field (the two lines are the
+        target x points, side-major in each anchor), one for the latitude
+        sides."""
         src = l - 1 if src is None else src
         so, ao, ai = self.states[src], self.axes[src], self.axes[l]
         nlatd, ld = self.states[l].nlatd, self.states[l].ld
         a = self.anchors[l]
-        for n, zlog in BNDRY:
-            src = so.current(n)
-            for side, sl in ((0, slice(0, 1)), (1, slice(-1, None))):
-                _tgx.interp3d_dev(src, ao["z"], ao["x"], ao["y"], a["lonb_" + n][slot], ai["z"], ai["x"][sl],
-                                  ai["y"], 2 * nlatd, 0, 1, zlog, offset=side * nlatd)
-                _tgx.interp3d_dev(src, ao["z"], ao["x"], ao["y"], a["latb_" + n][slot], ai["z"], ai["x"],
-                                  ai["y"][sl], 2 * ld, 1, 0, zlog, offset=side * ld)
+        srcs = [so.current(n) for n, _ in BNDRY]
+        zl = [z for _, z in BNDRY]
+        _tgx.interp3d_batch_dev(srcs, [a["lonb_" + n][slot] for n, _ in BNDRY], zl, ao["z"], ao["x"], ao["y"],
+                                ai["z"], ai["x2"], ai["y"], 2 * nlatd, nlatd, 1)
+        _tgx.interp3d_batch_dev(srcs, [a["latb_" + n][slot] for n, _ in BNDRY], zl, ao["z"], ao["x"], ao["y"],
+                                ai["z"], ai["x"], ai["y2"], 2 * ld, 1, ld)
 
     def map_in(self, l, slot, src=None):
         """levels_inner_ng.F90:131-214 (the 2-D lower-boundary fields): the
@@ -137,41 +142,43 @@ class CoupledNests:
         src = l - 1 if src is None else src
         so, ao, ai = self.states[src], self.axes[src], self.axes[l]
         ld = self.states[l].ld
-        for n in LBC2D:
-            src = self._input(so, n) if n == "flx_he" else so.inp[n]
-            _tgx.interp2d_dev(src, ao["x"], ao["y"], self.anchors[l][n][slot], ai["x"], ai["y"], 1, ld)
+        srcs = [self._input(so, n) if n == "flx_he" else so.inp[n] for n in LBC2D]
+        _tgx.interp2d_batch_dev(srcs, [self.anchors[l][n][slot] for n in LBC2D], ao["x"], ao["y"], ai["x"], ai["y"],
+                                1, ld)
 
     def substep_inputs(self, l, s):
         """interp_fields_ng.F90:17-63: the nest's lateral data and lower-boundary
         fields of sub-step s (1..nstep_ng), and its O+ lateral data of the
-        sub-step's O+ sub-cycles."""
+        sub-step's O+ sub-cycles (one launch)."""
         st, a, t1 = self.states[l], self.anchors[l], self.nstep_ng[l]
         nsub = int(self.grids[l]["nstep_sub"])
+        jobs = []
         for key, pair in a.items():
             if key.endswith("_op"):
                 # op_lon_b / op_lat_b at (s-1)*nsub+1 .. s*nsub of 0..t1*nsub
-                _tgx.time_interp_dev(pair[0], pair[1], st.inp[key], t1 * nsub, (s - 1) * nsub + 1, s * nsub)
+                jobs.append((pair[0], pair[1], st.inp[key], t1 * nsub, (s - 1) * nsub + 1, s * nsub))
                 continue
             dst = self._input(st, key) if key == "flx_he" else st.inp[key]
             if s == t1:
-                dst.copy_(pair[1])
+                jobs.append((None, pair[1], dst, 0, 0, 0))
             else:
-                _tgx.time_interp_dev(pair[0], pair[1], dst, t1, s, s)
+                jobs.append((pair[0], pair[1], dst, t1, s, s))
+        _tgx.time_interp_batch_dev(jobs)
 
     def map_out(self, l):
         """levels_inner_ng.F90:216-297: the nest's state onto the outer grid
-        points inside it."""
+        points inside it (one launch)."""
         box = self.boxes[l]
         if box is None:
             return
         lonbeg, lonend, latbeg, latend = box
         so, si, ao, ai = self.states[l - 1], self.states[l], self.axes[l - 1], self.axes[l]
-        x = ao["x"][lonbeg:lonend + 1]
-        y = ao["y"][latbeg:latend + 1]
-        for n, zlog in MAPOUT:
-            dst = so.current(n)
-            _tgx.interp3d_dev(si.current(n), ai["z"], ai["x"], ai["y"], dst, ao["z"], x, y, so.nlatd * so.ld, 1,
-                              so.ld, zlog, offset=latbeg * so.ld + lonbeg)
+        x = ao["x"][lonbeg:lonend + 1].contiguous()
+        y = ao["y"][latbeg:latend + 1].contiguous()
+        off = latbeg * so.ld + lonbeg
+        _tgx.interp3d_batch_dev([si.current(n) for n, _ in MAPOUT], [(so.current(n), off) for n, _ in MAPOUT],
+                                [z for _, z in MAPOUT], ai["z"], ai["x"], ai["y"], ao["z"], x, y, so.nlatd * so.ld,
+                                1, so.ld)
 
     # ---- the recursion --------------------------------------------------------
     def _step(self, l):
@@ -187,8 +194,7 @@ class CoupledNests:
             self._step(l)
             if l + 1 < len(self.states):
                 self._advance(l + 1)
-        for pair in self.anchors[l].values():
-            pair[0].copy_(pair[1])
+        _tgx.time_interp_batch_dev([(None, pair[1], pair[0], 0, 0, 0) for pair in self.anchors[l].values()])
         self.map_out(l)
 
     def run(self):

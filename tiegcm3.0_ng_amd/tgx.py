@@ -45,6 +45,10 @@ _SIGS = {
     "tgx_ng_interp2d_dev": ([_D] + [_I] * 3 + [_D] * 3 + [_I] * 2 + [C.c_long] * 2 + [_D] * 3, _I),
     "tgx_ng_time_interp_dev": ([_D] * 3 + [C.c_long] + [_I] * 3 + [_D], _I),
     "tgx_ng_unpack_levels": ([_D, _D] + [_I] * 4 + [_D], _I),
+    "tgx_ng_interp3d_batch_dev": ([_I, _D, _D, _D] + [_I] * 4 + [_D] * 3 + [_I] * 3 + [C.c_long] * 3 + [_D] * 4,
+                                  _I),
+    "tgx_ng_interp2d_batch_dev": ([_I, _D, _D] + [_I] * 3 + [_D] * 2 + [_I] * 2 + [C.c_long] * 2 + [_D] * 3, _I),
+    "tgx_ng_time_interp_batch_dev": ([_I] + [_D] * 7 + [_D], _I),
 }
 
 
@@ -238,6 +242,50 @@ def time_interp_dev(f0, f1, dst, t1, ta, tb, stream=None):
     """dst[(t - ta)*n + p] for t = ta..tb between f0 (t = 0) and f1 (t = t1)."""
     _check(lib().tgx_ng_time_interp_dev(_dp(f0), _dp(f1), _dp(dst), f0.numel(), int(t1), int(ta), int(tb),
                                         _stream(stream)))
+
+
+def _ptrs(items):
+    """Host array of device pointers: tensors, or (tensor, element offset) pairs."""
+    vals = []
+    for it in items:
+        t, off = it if isinstance(it, tuple) else (it, 0)
+        vals.append(t.data_ptr() + 8 * int(off))
+    return (C.c_void_p * max(1, len(vals)))(*vals)
+
+
+def interp3d_batch_dev(srcs, dsts, zlogs, zp, xp, yp, z, x, y, dk, di, dj, stream=None):
+    """interp3d_dev of several level-plane tensors of one shape (nzp, nyp, ld)
+    on the same axes to the same target points, one launch; dsts: tensors or
+    (tensor, element offset) pairs."""
+    nzp, nyp, ld = srcs[0].shape
+    zl = (C.c_int * max(1, len(zlogs)))(*[int(bool(v)) for v in zlogs])
+    _check(lib().tgx_ng_interp3d_batch_dev(len(srcs), C.cast(_ptrs(srcs), _D), C.cast(_ptrs(dsts), _D),
+                                           C.cast(zl, _D), nzp, len(xp), nyp, ld, _dp(zp), _dp(xp), _dp(yp), len(z),
+                                           len(x), len(y), int(dk), int(di), int(dj), _dp(z), _dp(x), _dp(y),
+                                           _stream(stream)))
+
+
+def interp2d_batch_dev(srcs, dsts, xp, yp, x, y, di, dj, stream=None):
+    """interp2d_dev of several planes (nyp, ld) on the same axes, one launch."""
+    nyp, ld = srcs[0].shape
+    _check(lib().tgx_ng_interp2d_batch_dev(len(srcs), C.cast(_ptrs(srcs), _D), C.cast(_ptrs(dsts), _D), len(xp),
+                                           nyp, ld, _dp(xp), _dp(yp), len(x), len(y), int(di), int(dj), _dp(x),
+                                           _dp(y), _stream(stream)))
+
+
+def time_interp_batch_dev(jobs, stream=None):
+    """time_interp_dev of several fields in one launch: jobs of (f0, f1, dst,
+    t1, ta, tb), or (None, src, dst, 0, 0, 0) for a plain copy dst = src."""
+    n = len(jobs)
+    f0 = _ptrs([j[0] if j[0] is not None else j[1] for j in jobs])
+    f1 = _ptrs([j[1] for j in jobs])
+    dst = _ptrs([j[2] for j in jobs])
+    cnt = (C.c_long * max(1, n))(*[j[1].numel() for j in jobs])
+    t1 = (C.c_int * max(1, n))(*[int(j[3]) for j in jobs])
+    ta = (C.c_int * max(1, n))(*[int(j[4]) for j in jobs])
+    tb = (C.c_int * max(1, n))(*[int(j[5]) for j in jobs])
+    _check(lib().tgx_ng_time_interp_batch_dev(n, C.cast(f0, _D), C.cast(f1, _D), C.cast(dst, _D), C.cast(cnt, _D),
+                                              C.cast(t1, _D), C.cast(ta, _D), C.cast(tb, _D), _stream(stream)))
 
 
 def ringfilter_dev(f, nlon, nlat, order, oplus, dlamda, chunk, stream=None):
