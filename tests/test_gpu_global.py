@@ -216,3 +216,30 @@ def test_global_full_size_step_matches_oracle(pkg, oracle):
             assert (o[n + sfx] >= 1e-12).all(), n + sfx
         assert (o["op" + sfx][..., :-1] >= 0).all()
     assert (o["tn"][..., :-1] >= 100).all()
+
+
+@pytest.mark.parametrize("nsub", [1, 2])
+def test_global_step_without_filtered_latitudes(pkg, oracle, nsub):
+    """A global grid whose chunk table filters no latitude (nlat_filter = 0):
+    the column kernels then apply every routine's post pass on every row
+    and no ring filter runs; the step matches the oracle's global forms with
+    the same empty table, and the forked step equals the serial one."""
+    import torch
+    g = pkg.synth.make_grid("g5", nstep_sub=nsub, glb=True)
+    g["chunk"] = []
+    F = pkg.synth.make_global_state(g)
+    ref = pkg.step.run_oracle_step(oracle, g, pkg.synth.to_fortran_layout(F))
+    ns = pkg.step.GlobalState(g, F)
+    ns.step()
+    ser = pkg.step.GlobalState(g, F)
+    ser.step_serial()
+    torch.cuda.synchronize()
+    compare(g, ns, ref, OUT, "g5-nofilter")
+    rows = slice(2, g["nlat"] + 2)
+    for n in ns.out:
+        a, b = ns.out[n], ser.out[n]
+        if a.dim() == 3:
+            a, b = a[:, rows], b[:, rows]
+        else:
+            a, b = a[rows], b[rows]
+        assert torch.equal(a, b), n
