@@ -501,24 +501,40 @@ def run_oracle_step(oracle, grid, Fo):
     return res
 
 
+# Top levels the global forms set to spval (TN, UN, VN) or zero (O+)
+# (dt.F:407-408, duv.F:385-386, :512-513, oplus.F:922-923).
+_FIXED_TOP = ("tn", "tn_nm", "un", "un_nm", "vn", "vn_nm", "op", "op_nm")
+
+
 def smoke_check(oracle, name="tiny", tol=1e-12):
-    """One small fused column step on cuda:0 against the oracle."""
+    """One small fused column step on cuda:0 against the oracle, as a nest
+    domain (tgx_ng_step) and as a global grid (tgx_glb_step, with its ring
+    filters), every output field within tol of the oracle relative to the
+    field's largest magnitude (on the global grid: the interior latitudes,
+    below the spval / zero top, which must match exactly)."""
     import numpy as np
     from . import synth
-    g = synth.make_grid(name, nstep_sub=2)
-    F = synth.make_state(g)
-    Fo = synth.to_fortran_layout(F)
-    ref = run_oracle_step(oracle, g, Fo)
-    ns = NestState(g, F)
-    ns.step()
-    torch.cuda.synchronize()
     worst = 0.
-    for n in OUT_3D:
-        got, exp = ns.fortran(n), ref[n]
-        err = float(np.abs(got - exp).max() / max(np.abs(exp).max(), 1e-300))
-        worst = max(worst, err)
-        assert err <= tol, (n, err)
-    print("smoke ok: fused column step on %s, worst field-relative error %.3e" % (name, worst))
+    for glb in (False, True):
+        g = synth.make_grid(name, nstep_sub=2, glb=glb)
+        F = synth.make_global_state(g) if glb else synth.make_state(g)
+        Fo = synth.to_fortran_layout(F)
+        ref = run_oracle_step(oracle, g, Fo)
+        ns = GlobalState(g, F) if glb else NestState(g, F)
+        ns.step()
+        torch.cuda.synchronize()
+        for n in OUT_3D:
+            got, exp = ns.fortran(n), ref[n]
+            if glb:
+                got, exp = got[2:g["nlat"] + 2], exp[2:g["nlat"] + 2]
+                if n in _FIXED_TOP:
+                    assert np.array_equal(got[..., -1], exp[..., -1]), n
+                    got, exp = got[..., :-1], exp[..., :-1]
+            err = float(np.abs(got - exp).max() / max(np.abs(exp).max(), 1e-300))
+            worst = max(worst, err)
+            assert err <= tol, ("global" if glb else "nest", n, err)
+    print("smoke ok: fused column step on %s as a nest domain and as a global grid, worst field-relative "
+          "error %.3e" % (name, worst))
     return worst
 
 
