@@ -1498,22 +1498,66 @@ __global__ void __launch_bounds__(256) chem_kernel(tgx_ng_grid g, Geo G, tgx_ng_
 // =============================================================================
 // One wavefront per column.  Lane l holds level l ("lo") and level l+64
 // ("hi"), so a wave covers up to 128 levels; neighbouring levels come from
-// other lanes by ds_bpermute, and the column's tridiagonal systems are solved
-// in registers by parallel cyclic reduction (no ratio planes, no second pass
-// over memory).  A workgroup holds consecutive columns of one latitude row, so
-// the waves of a workgroup share the cache lines of every level plane.
+// other lanes, and the column's tridiagonal systems are solved in registers by
+// parallel cyclic reduction (no ratio planes, no second pass over memory).  A
+// workgroup holds consecutive columns of one latitude row, so the waves of a
+// workgroup share the cache lines of every level plane.
+//
+// Lane exchanges: values one lane away (the neighbouring level, and the first
+// reduction step) move by DPP wave rotations, which run on the VALU; longer
+// distances go through ds_bpermute.  Both return, for every lane, the value
+// the source lane holds, so the two forms give the same bits.  Every lane of
+// the wavefront is active wherever they are used.
+
+// x of lane (lane - 1) & 63 (wave_ror:1) and of lane (lane + 1) & 63
+// (wave_rol:1), one 32-bit half at a time.
+template <int CTRL>
+__device__ __forceinline__ double dpp_rot(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+constexpr int DPP_WAVE_ROL1 = 0x134, DPP_WAVE_ROR1 = 0x13C;
+__device__ __forceinline__ double lane_prev(double x) { return dpp_rot<DPP_WAVE_ROR1>(x); }
+__device__ __forceinline__ double lane_next(double x) { return dpp_rot<DPP_WAVE_ROL1>(x); }
+
+// Partners of a reduction step at distance d: m = lane (lane - d) & 63, p =
+// lane (lane + d) & 63.
+struct XchgRot {  // d = 1
+  static constexpr bool same = false;
+  __device__ double m(double v) const { return lane_prev(v); }
+  __device__ double p(double v) const { return lane_next(v); }
+};
+struct XchgPerm {  // 1 < d < 64
+  static constexpr bool same = false;
+  int sm, sp;
+  __device__ XchgPerm(int lane, int d) : sm((lane - d) & 63), sp((lane + d) & 63) {}
+  __device__ double m(double v) const { return __shfl(v, sm, 64); }
+  __device__ double p(double v) const { return __shfl(v, sp, 64); }
+};
+struct XchgXor32 {  // d = 32: both partners are lane ^ 32 (same = true: one exchange serves both)
+  static constexpr bool same = true;
+  int s;
+  __device__ explicit XchgXor32(int lane) : s(lane ^ 32) {}
+  __device__ double m(double v) const { return __shfl(v, s, 64); }
+  __device__ double p(double v) const { return __shfl(v, s, 64); }
+};
+struct XchgOwn {  // d = 64 with two rows per lane: the partner row is in this lane
+  static constexpr bool same = true;
+  __device__ double m(double v) const { return v; }
+  __device__ double p(double v) const { return v; }
+};
 
 // Value at level L-1 (dn) and L+1 (up) for the lo and hi rows of this lane.
 // Lane 0 lo (level -1) and lane 63 hi (level 128) receive unused values.
 __device__ __forceinline__ void wave_dn(double lo, double hi, int lane, double &dlo, double &dhi) {
-  const int src = (lane - 1) & 63;
-  dlo = __shfl(lo, src, 64);
-  dhi = __shfl(lane == 63 ? lo : hi, src, 64);  // lane 0 hi (level 64) takes lane 63 lo
+  dlo = lane_prev(lo);
+  dhi = lane_prev(lane == 63 ? lo : hi);  // lane 0 hi (level 64) takes lane 63 lo
 }
 __device__ __forceinline__ void wave_up(double lo, double hi, int lane, double &ulo, double &uhi) {
-  const int src = (lane + 1) & 63;
-  ulo = __shfl(lane == 0 ? hi : lo, src, 64);  // lane 63 lo (level 63) takes lane 0 hi
-  uhi = __shfl(hi, src, 64);
+  ulo = lane_next(lane == 0 ? hi : lo);  // lane 63 lo (level 63) takes lane 0 hi
+  uhi = lane_next(hi);
 }
 
 // The same for NH rows per lane: NH = 1 when the column has at most 64
@@ -1523,7 +1567,7 @@ __device__ __forceinline__ void wave_dn_h(double lo, double hi, int lane, double
   if constexpr (NH == 2) {
     wave_dn(lo, hi, lane, dlo, dhi);
   } else {
-    dlo = __shfl(lo, (lane - 1) & 63, 64);
+    dlo = lane_prev(lo);
   }
 }
 template <int NH>
@@ -1531,7 +1575,7 @@ __device__ __forceinline__ void wave_up_h(double lo, double hi, int lane, double
   if constexpr (NH == 2) {
     wave_up(lo, hi, lane, ulo, uhi);
   } else {
-    ulo = __shfl(lo, (lane + 1) & 63, 64);
+    ulo = lane_next(lo);
   }
 }
 
@@ -1539,8 +1583,13 @@ struct TriRow { double a, b, c, f; };  // a x(k-1) + b x(k) + c x(k+1) = f
 
 __device__ __forceinline__ TriRow tri_ident() { return TriRow{0., 1., 0., 0.}; }
 
-__device__ __forceinline__ TriRow tri_shfl(const TriRow &r, int src) {
-  return TriRow{__shfl(r.a, src, 64), __shfl(r.b, src, 64), __shfl(r.c, src, 64), __shfl(r.f, src, 64)};
+template <class X>
+__device__ __forceinline__ TriRow tri_m(const X &x, const TriRow &r) {
+  return TriRow{x.m(r.a), x.m(r.b), x.m(r.c), x.m(r.f)};
+}
+template <class X>
+__device__ __forceinline__ TriRow tri_p(const X &x, const TriRow &r) {
+  return TriRow{x.p(r.a), x.p(r.b), x.p(r.c), x.p(r.f)};
 }
 
 // Eliminates the couplings of row r to rows m (r-d) and p (r+d).
@@ -1557,26 +1606,29 @@ __device__ __forceinline__ TriRow tri_reduce(const TriRow &r, const TriRow &m, c
 // Parallel cyclic reduction of NS independent 128-row systems held as
 // (lo, hi) rows per lane.  Rows beyond the system are identity rows; the
 // first row has a = 0 and the last c = 0.  On return x = f / b.
+template <int NS, class X>
+__device__ __forceinline__ void pcr_step(TriRow (&lo)[NS], TriRow (&hi)[NS], int lane, int d, const X &x) {
+  const bool has_m = lane >= d, low_p = lane + d < 64;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const TriRow Lm = tri_m(x, lo[q]), Hm = tri_m(x, hi[q]);
+    const TriRow Lp = X::same ? Lm : tri_p(x, lo[q]), Hp = X::same ? Hm : tri_p(x, hi[q]);
+    TriRow lm = has_m ? Lm : tri_ident();
+    TriRow lp = low_p ? Lp : Hp;
+    TriRow hm = has_m ? Hm : Lm;
+    TriRow hp = low_p ? Hp : tri_ident();
+    TriRow nlo = tri_reduce(lo[q], lm, lp);
+    TriRow nhi = tri_reduce(hi[q], hm, hp);
+    lo[q] = nlo;
+    hi[q] = nhi;
+  }
+}
 template <int NS>
 __device__ __forceinline__ void wave_pcr(TriRow (&lo)[NS], TriRow (&hi)[NS], int lane) {
+  pcr_step<NS>(lo, hi, lane, 1, XchgRot{});
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
-    const bool has_m = lane >= d, low_p = lane + d < 64;
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      TriRow Lm = tri_shfl(lo[q], sm), Hm = tri_shfl(hi[q], sm);
-      TriRow Lp = tri_shfl(lo[q], sp), Hp = tri_shfl(hi[q], sp);
-      TriRow lm = has_m ? Lm : tri_ident();
-      TriRow lp = low_p ? Lp : Hp;
-      TriRow hm = has_m ? Hm : Lm;
-      TriRow hp = low_p ? Hp : tri_ident();
-      TriRow nlo = tri_reduce(lo[q], lm, lp);
-      TriRow nhi = tri_reduce(hi[q], hm, hp);
-      lo[q] = nlo;
-      hi[q] = nhi;
-    }
-  }
+  for (int d = 2; d < 32; d <<= 1) pcr_step<NS>(lo, hi, lane, d, XchgPerm(lane, d));
+  pcr_step<NS>(lo, hi, lane, 32, XchgXor32(lane));
 #pragma unroll
   for (int q = 0; q < NS; ++q) {  // d = 64: lo and hi of the same lane
     TriRow nlo = tri_reduce(lo[q], tri_ident(), hi[q]);
@@ -1588,18 +1640,21 @@ __device__ __forceinline__ void wave_pcr(TriRow (&lo)[NS], TriRow (&hi)[NS], int
 
 // The same for one row per lane (systems of at most 64 rows): partners past
 // either end of the wave are absent.
+template <int NS, class X>
+__device__ __forceinline__ void pcr1_step(TriRow (&lo)[NS], int lane, int d, const X &x) {
+  const bool has_m = lane >= d, has_p = lane + d < 64;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const TriRow Lm = tri_m(x, lo[q]), Lp = X::same ? Lm : tri_p(x, lo[q]);
+    lo[q] = tri_reduce(lo[q], has_m ? Lm : tri_ident(), has_p ? Lp : tri_ident());
+  }
+}
 template <int NS>
 __device__ __forceinline__ void wave_pcr1(TriRow (&lo)[NS], int lane) {
+  pcr1_step<NS>(lo, lane, 1, XchgRot{});
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
-    const bool has_m = lane >= d, has_p = lane + d < 64;
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      TriRow Lm = tri_shfl(lo[q], sm), Lp = tri_shfl(lo[q], sp);
-      lo[q] = tri_reduce(lo[q], has_m ? Lm : tri_ident(), has_p ? Lp : tri_ident());
-    }
-  }
+  for (int d = 2; d < 32; d <<= 1) pcr1_step<NS>(lo, lane, d, XchgPerm(lane, d));
+  pcr1_step<NS>(lo, lane, 32, XchgXor32(lane));
 }
 
 // Block rows A x(k-1) + B x(k) + C x(k+1) = F with N x N blocks, row-major
@@ -1662,13 +1717,16 @@ __device__ __forceinline__ BlkNorm<N> blk_normalize(const BlkRow<N> &r) {
   blk_mv<N>(Bi, r.F, o.F);
   return o;
 }
-template <int N>
-__device__ __forceinline__ BlkNorm<N> blk_shfl(const BlkNorm<N> &r, int src) {
+template <int N, bool M, class X>
+__device__ __forceinline__ BlkNorm<N> blk_xchg(const X &x, const BlkNorm<N> &r) {  // M: from m, else p
   BlkNorm<N> o;
 #pragma unroll
-  for (int e = 0; e < N * N; ++e) { o.A[e] = __shfl(r.A[e], src, 64); o.C[e] = __shfl(r.C[e], src, 64); }
+  for (int e = 0; e < N * N; ++e) {
+    o.A[e] = M ? x.m(r.A[e]) : x.p(r.A[e]);
+    o.C[e] = M ? x.m(r.C[e]) : x.p(r.C[e]);
+  }
 #pragma unroll
-  for (int e = 0; e < N; ++e) o.F[e] = __shfl(r.F[e], src, 64);
+  for (int e = 0; e < N; ++e) o.F[e] = M ? x.m(r.F[e]) : x.p(r.F[e]);
   return o;
 }
 template <int N>
@@ -1713,20 +1771,23 @@ __device__ __forceinline__ void blk_reduce(BlkRow<N> &r, const BlkNorm<N> &m, co
 
 // Block parallel cyclic reduction over 128 rows held as (lo, hi) per lane;
 // rows beyond the system are identity rows.  On return x = B^-1 F per row.
+template <int N, class X>
+__device__ __forceinline__ void blk_pcr_step(BlkRow<N> &lo, BlkRow<N> &hi, int lane, int d, const X &x) {
+  const bool has_m = lane >= d, low_p = lane + d < 64;
+  BlkNorm<N> nlo = blk_normalize<N>(lo), nhi = blk_normalize<N>(hi);
+  BlkNorm<N> Lm = blk_xchg<N, true>(x, nlo), Hm = blk_xchg<N, true>(x, nhi);
+  BlkNorm<N> Lp = X::same ? Lm : blk_xchg<N, false>(x, nlo), Hp = X::same ? Hm : blk_xchg<N, false>(x, nhi);
+  const BlkNorm<N> Z = blk_zero<N>();
+  blk_reduce<N>(lo, blk_sel<N>(has_m, Lm, Z), blk_sel<N>(low_p, Lp, Hp));
+  blk_reduce<N>(hi, blk_sel<N>(has_m, Hm, Lm), blk_sel<N>(low_p, Hp, Z));
+}
 template <int N>
 __device__ __forceinline__ void wave_block_pcr(BlkRow<N> &lo, BlkRow<N> &hi, int lane, double *xlo,
                                                double *xhi) {
+  blk_pcr_step<N>(lo, hi, lane, 1, XchgRot{});
 #pragma unroll 1
-  for (int d = 1; d < 64; d <<= 1) {
-    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
-    const bool has_m = lane >= d, low_p = lane + d < 64;
-    BlkNorm<N> nlo = blk_normalize<N>(lo), nhi = blk_normalize<N>(hi);
-    BlkNorm<N> Lm = blk_shfl<N>(nlo, sm), Hm = blk_shfl<N>(nhi, sm);
-    BlkNorm<N> Lp = blk_shfl<N>(nlo, sp), Hp = blk_shfl<N>(nhi, sp);
-    const BlkNorm<N> Z = blk_zero<N>();
-    blk_reduce<N>(lo, blk_sel<N>(has_m, Lm, Z), blk_sel<N>(low_p, Lp, Hp));
-    blk_reduce<N>(hi, blk_sel<N>(has_m, Hm, Lm), blk_sel<N>(low_p, Hp, Z));
-  }
+  for (int d = 2; d < 32; d <<= 1) blk_pcr_step<N>(lo, hi, lane, d, XchgPerm(lane, d));
+  blk_pcr_step<N>(lo, hi, lane, 32, XchgXor32(lane));
   {  // d = 64
     BlkNorm<N> nlo = blk_normalize<N>(lo), nhi = blk_normalize<N>(hi);
     blk_reduce<N>(lo, blk_zero<N>(), nhi);
@@ -1741,17 +1802,20 @@ __device__ __forceinline__ void wave_block_pcr(BlkRow<N> &lo, BlkRow<N> &hi, int
 
 // The same for one row per lane (at most 64 rows): partners past either end
 // of the wave are absent.
+template <int N, class X>
+__device__ __forceinline__ void blk_pcr1_step(BlkRow<N> &lo, int lane, int d, const X &x) {
+  const bool has_m = lane >= d, has_p = lane + d < 64;
+  BlkNorm<N> nlo = blk_normalize<N>(lo);
+  BlkNorm<N> Lm = blk_xchg<N, true>(x, nlo), Lp = X::same ? Lm : blk_xchg<N, false>(x, nlo);
+  const BlkNorm<N> Z = blk_zero<N>();
+  blk_reduce<N>(lo, blk_sel<N>(has_m, Lm, Z), blk_sel<N>(has_p, Lp, Z));
+}
 template <int N>
 __device__ __forceinline__ void wave_block_pcr1(BlkRow<N> &lo, int lane, double *xlo) {
+  blk_pcr1_step<N>(lo, lane, 1, XchgRot{});
 #pragma unroll 1
-  for (int d = 1; d < 64; d <<= 1) {
-    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
-    const bool has_m = lane >= d, has_p = lane + d < 64;
-    BlkNorm<N> nlo = blk_normalize<N>(lo);
-    BlkNorm<N> Lm = blk_shfl<N>(nlo, sm), Lp = blk_shfl<N>(nlo, sp);
-    const BlkNorm<N> Z = blk_zero<N>();
-    blk_reduce<N>(lo, blk_sel<N>(has_m, Lm, Z), blk_sel<N>(has_p, Lp, Z));
-  }
+  for (int d = 2; d < 32; d <<= 1) blk_pcr1_step<N>(lo, lane, d, XchgPerm(lane, d));
+  blk_pcr1_step<N>(lo, lane, 32, XchgXor32(lane));
   double Bi[N * N];
   blk_inv<N>(lo.B, Bi);
   blk_mv<N>(Bi, lo.F, xlo);
@@ -1760,44 +1824,48 @@ __device__ __forceinline__ void wave_block_pcr1(BlkRow<N> &lo, int lane, double 
 struct Nrm3 { double A[9], C[9], F[3]; };
 
 // dst = (ok ? the source lane's (pick_a ? a : b) : 0), element by element
-template <int K>
-__device__ __forceinline__ void fetchk(const double *a, const double *b, bool pick_a, int lane_src,
-                                       bool ok, double *dst) {
+// (from the m partner if M, else from the p partner)
+template <int K, bool M, class X>
+__device__ __forceinline__ void fetchk(const X &x, const double *a, const double *b, bool pick_a, bool ok,
+                                       double *dst) {
 #pragma unroll
   for (int e = 0; e < K; ++e) {
-    double t = __shfl(pick_a ? a[e] : b[e], lane_src, 64);
+    const double v = pick_a ? a[e] : b[e];
+    const double t = M ? x.m(v) : x.p(v);
     dst[e] = ok ? t : 0.;
   }
 }
-// The reduced row of r from its neighbours m (lane sm, row ma or mb) and p
-// (lane sp, row pa or pb), normalised again by its new diagonal block.
-__device__ __forceinline__ void nrm3_update(const Nrm3 &r, int sm, const Nrm3 &ma, const Nrm3 &mb,
-                                            bool m_a, bool okm, int sp, const Nrm3 &pa,
-                                            const Nrm3 &pb, bool p_a, bool okp, Nrm3 &o) {
+// The reduced row of r from its neighbours m (partner lane m of x, row ma or
+// mb) and p (partner lane p, row pa or pb), normalised again by its new
+// diagonal block.
+template <class X>
+__device__ __forceinline__ void nrm3_update(const Nrm3 &r, const X &x, const Nrm3 &ma, const Nrm3 &mb,
+                                            bool m_a, bool okm, const Nrm3 &pa, const Nrm3 &pb, bool p_a,
+                                            bool okp, Nrm3 &o) {
   double Bp[9] = {1., 0., 0., 0., 1., 0., 0., 0., 1.}, Ap[9], Cp[9], Fp[3], T[9], t[3], nb[9], nf[3];
   // m side: B -= A Cm, A' = -A Am, F -= A Fm
-  fetchk<9>(ma.C, mb.C, m_a, sm, okm, nb);
+  fetchk<9, true>(x, ma.C, mb.C, m_a, okm, nb);
   blk_mm<3>(r.A, nb, T);
 #pragma unroll
   for (int e = 0; e < 9; ++e) Bp[e] -= T[e];
-  fetchk<9>(ma.A, mb.A, m_a, sm, okm, nb);
+  fetchk<9, true>(x, ma.A, mb.A, m_a, okm, nb);
   blk_mm<3>(r.A, nb, T);
 #pragma unroll
   for (int e = 0; e < 9; ++e) Ap[e] = -T[e];
-  fetchk<3>(ma.F, mb.F, m_a, sm, okm, nf);
+  fetchk<3, true>(x, ma.F, mb.F, m_a, okm, nf);
   blk_mv<3>(r.A, nf, t);
 #pragma unroll
   for (int e = 0; e < 3; ++e) Fp[e] = r.F[e] - t[e];
   // p side: B -= C Ap, C' = -C Cp, F -= C Fp
-  fetchk<9>(pa.A, pb.A, p_a, sp, okp, nb);
+  fetchk<9, false>(x, pa.A, pb.A, p_a, okp, nb);
   blk_mm<3>(r.C, nb, T);
 #pragma unroll
   for (int e = 0; e < 9; ++e) Bp[e] -= T[e];
-  fetchk<9>(pa.C, pb.C, p_a, sp, okp, nb);
+  fetchk<9, false>(x, pa.C, pb.C, p_a, okp, nb);
   blk_mm<3>(r.C, nb, T);
 #pragma unroll
   for (int e = 0; e < 9; ++e) Cp[e] = -T[e];
-  fetchk<3>(pa.F, pb.F, p_a, sp, okp, nf);
+  fetchk<3, false>(x, pa.F, pb.F, p_a, okp, nf);
   blk_mv<3>(r.C, nf, t);
 #pragma unroll
   for (int e = 0; e < 3; ++e) Fp[e] -= t[e];
@@ -1811,43 +1879,45 @@ __device__ __forceinline__ void nrm3_update(const Nrm3 &r, int sm, const Nrm3 &m
 // block (B = I): each lane holds A, C, F of its lo and hi rows.  A source
 // lane offers the row its requester needs, so one neighbour block at a time
 // is live.  On return lo.F / hi.F hold the solution.
-__device__ __forceinline__ void wave_pcr3(Nrm3 &lo, Nrm3 &hi, int lane) {
-#pragma unroll 1
-  for (int d = 1; d <= 64; d <<= 1) {
-    int sm, sp;
-    bool lo_m_ok, hi_p_ok, offer_p_lo, offer_m_hi;
-    if (d < 64) {
-      sm = (lane - d) & 63;
-      sp = (lane + d) & 63;
-      lo_m_ok = lane >= d;       // lo's m-neighbour: the source's lo
-      hi_p_ok = lane + d < 64;   // hi's p-neighbour: the source's hi
-      offer_p_lo = lane >= d;    // lo's p-neighbour: source lo if source >= d, else its hi
-      offer_m_hi = lane + d < 64;  // hi's m-neighbour: source hi if source + d < 64, else its lo
-    } else {                     // the partner row sits in the same lane
-      sm = sp = lane;
-      lo_m_ok = false;
-      hi_p_ok = false;
-      offer_p_lo = false;        // lo's p-neighbour: own hi
-      offer_m_hi = false;        // hi's m-neighbour: own lo
-    }
-    Nrm3 nlo, nhi;
-    nrm3_update(lo, sm, lo, lo, true, lo_m_ok, sp, lo, hi, offer_p_lo, true, nlo);
-    nrm3_update(hi, sm, hi, lo, offer_m_hi, true, sp, hi, hi, true, hi_p_ok, nhi);
-    lo = nlo;
-    hi = nhi;
+template <class X>
+__device__ __forceinline__ void pcr3_step(Nrm3 &lo, Nrm3 &hi, int lane, int d, const X &x) {
+  bool lo_m_ok, hi_p_ok, offer_p_lo, offer_m_hi;
+  if (d < 64) {
+    lo_m_ok = lane >= d;       // lo's m-neighbour: the source's lo
+    hi_p_ok = lane + d < 64;   // hi's p-neighbour: the source's hi
+    offer_p_lo = lane >= d;    // lo's p-neighbour: source lo if source >= d, else its hi
+    offer_m_hi = lane + d < 64;  // hi's m-neighbour: source hi if source + d < 64, else its lo
+  } else {                     // the partner row sits in the same lane
+    lo_m_ok = false;
+    hi_p_ok = false;
+    offer_p_lo = false;        // lo's p-neighbour: own hi
+    offer_m_hi = false;        // hi's m-neighbour: own lo
   }
+  Nrm3 nlo, nhi;
+  nrm3_update(lo, x, lo, lo, true, lo_m_ok, lo, hi, offer_p_lo, true, nlo);
+  nrm3_update(hi, x, hi, lo, offer_m_hi, true, hi, hi, true, hi_p_ok, nhi);
+  lo = nlo;
+  hi = nhi;
+}
+__device__ __forceinline__ void wave_pcr3(Nrm3 &lo, Nrm3 &hi, int lane) {
+  pcr3_step(lo, hi, lane, 1, XchgRot{});
+#pragma unroll 1
+  for (int d = 2; d < 64; d <<= 1) pcr3_step(lo, hi, lane, d, XchgPerm(lane, d));
+  pcr3_step(lo, hi, lane, 64, XchgOwn{});
 }
 
 // The same reduction for one row per lane (at most 64 rows): partners past
 // either end of the wave are absent.
+template <class X>
+__device__ __forceinline__ void pcr3_1_step(Nrm3 &lo, int lane, int d, const X &x) {
+  Nrm3 nlo;
+  nrm3_update(lo, x, lo, lo, true, lane >= d, lo, lo, true, lane + d < 64, nlo);
+  lo = nlo;
+}
 __device__ __forceinline__ void wave_pcr3_1(Nrm3 &lo, int lane) {
+  pcr3_1_step(lo, lane, 1, XchgRot{});
 #pragma unroll 1
-  for (int d = 1; d < 64; d <<= 1) {
-    const int sm = (lane - d) & 63, sp = (lane + d) & 63;
-    Nrm3 nlo;
-    nrm3_update(lo, sm, lo, lo, true, lane >= d, sp, lo, lo, true, lane + d < 64, nlo);
-    lo = nlo;
-  }
+  for (int d = 2; d < 64; d <<= 1) pcr3_1_step(lo, lane, d, XchgPerm(lane, d));
 }
 
 // Column index of this wave (consecutive waves take consecutive columns).
