@@ -148,6 +148,11 @@ int tgx_ng_wave_columns_choice(const tgx_ng_grid *g, int routine);
  * from the calling host thread, in place of the autotuner's choice; 0 returns
  * to the choice. */
 void tgx_ng_force_wave_columns(int cols);
+/* Whether the TN column kernel may load its LDS tiles straight from HBM
+ * (on, the default, where TN, UN, VN and TN_NM are 16-byte aligned and g.ld
+ * is even) or stages them through registers (0), for launches from the
+ * calling host thread.  Both give the same results.  No reference counterpart. */
+void tgx_ng_set_tile_dma(int on);
 /* The same step strictly in the reference's order on `stream` alone. */
 int tgx_ng_step_serial(const tgx_ng_grid *g, const tgx_ng_fields *in, const tgx_ng_out *out,
                        void *work, void *stream);

@@ -36,10 +36,11 @@ def config1_system(rng, ncol, nk):
 
 def test_trsolv_dropin_config1_per_latitude(pkg, oracle):
     # dt.F:330 calls trsolv per latitude: lev0..lev1 = 1..29, rows 1..28,
-    # lon0..lon1 = 1..76 (nlonp4 = 76, periodic columns skipped).
+    # lon0..lon1 = 1..76 (nlonp4 = 76, periodic columns skipped); every one
+    # of the 5 degree grid's 36 latitudes, as one step of the model calls it.
     rng = np.random.default_rng(0x7E6C0000)
     nlev, nlonp4 = 29, 76
-    for lat in range(1, 37, 7):
+    for lat in range(1, 37):
         a, b, c, f = config1_system(rng, nlonp4, nlev)
         x = np.full((nlonp4, nlev), 1e30)
         y = x.copy()

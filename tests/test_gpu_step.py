@@ -192,6 +192,33 @@ def test_wave_kernels_with_fewer_columns_per_block_equal_four(pkg, cols):
         assert torch.equal(a.out[n], b.out[n]), n
 
 
+@pytest.mark.parametrize("name,glb", [("small", False), ("nest1", False), ("g2p5", True)])
+def test_dt_tiles_loaded_straight_into_lds_equal_register_staging(pkg, name, glb):
+    """The dt tile kernel with its tiles loaded from HBM straight into LDS
+    (the default where TN, UN, VN, TN_NM are 16-byte aligned and the row pitch
+    is even) gives bitwise the step of the form that stages them through
+    registers."""
+    import torch
+    g = pkg.synth.make_grid(name, nstep_sub=2, glb=glb)
+    if glb:
+        st = [pkg.step.GlobalState(g, pkg.synth.make_global_state(g)) for _ in range(2)]
+    else:
+        F = pkg.synth.make_state(g)
+        st = [pkg.step.NestState(g, F, scheme="thomas") for _ in range(2)]
+    a, b = st
+    assert a.ld % 2 == 0 and all(a.inp[n].data_ptr() % 16 == 0 for n in ("tn", "un", "vn", "tn_nm"))
+    a.step()
+    torch.cuda.synchronize()
+    pkg.step._lib().tgx_ng_set_tile_dma(0)
+    try:
+        b.step()
+    finally:
+        pkg.step._lib().tgx_ng_set_tile_dma(1)
+    torch.cuda.synchronize()
+    for n in a.out:
+        assert torch.equal(a.out[n], b.out[n]), n
+
+
 def test_profile_hooks_count_calls(pkg):
     import torch
     g = pkg.synth.make_grid("tiny", nstep_sub=1)

@@ -98,6 +98,10 @@ thread_local int t_glb_mode = 0, t_glb_nfilt = 0;
 // step_api.cpp before each routine's launches from the autotuner's choice.
 thread_local int t_wave_cols = 4;
 extern "C" void tgx_ng_set_wave_columns(int cols) { t_wave_cols = cols == 1 || cols == 2 ? cols : 4; }
+// Whether the dt tile kernel may stage its tiles by direct-to-LDS loads
+// (launches from the calling host thread; on by default, off to compare).
+thread_local int t_tile_dma = 1;
+extern "C" void tgx_ng_set_tile_dma(int on) { t_tile_dma = on != 0; }
 extern "C" void tgx_ng_set_global_mode(int on, int nfilt) {
   t_glb_mode = on;
   t_glb_nfilt = on ? nfilt : 0;
@@ -552,11 +556,29 @@ using KindsDuv = StencilKinds<ST_DIVX, ST_DIVY, ST_SMOOTH, ST_SMOOTH, ST_ADVEC, 
 // evaluated from LDS, and the column is assembled and eliminated by Thomas
 // (trsolv_ng).  Every thread of the block takes part in the staging and the
 // barriers; lateral-boundary and out-of-domain columns do not solve.
-template <bool CAP>
-__global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in,
-                                                      tgx_ng_out out, double *__restrict__ w2s) {
+//
+// DMA: the tiles go from HBM straight to LDS (global_load_lds_dwordx4: lane l
+// of wavefront w writes tile points 128 w + 2 l and 2 l + 1, one row of the
+// tile, at LDS base + 16 l), so no registers hold them in flight; the kernel
+// then fits three wavefronts per SIMD and every tile of a 0.625 deg grid is
+// resident in one round.  Needs 16-byte aligned TN, UN, VN, TN_NM and an even
+// row pitch (the host checks).
+template <bool DMA>
+__device__ __forceinline__ void tile_dma16(const double *src, double *lds_wave_base) {
+  __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)src,
+                                   (__attribute__((address_space(3))) void *)lds_wave_base, 16, 0, 0);
+}
+// Waits for this wavefront's outstanding vector memory operations, the tile
+// loads into LDS among them (before the barrier that publishes them).
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+template <bool CAP, bool DMA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMA ? 3 : 1)))
+dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_ng_fields in, tgx_ng_out out, double *__restrict__ w2s) {
   constexpr int TS = TILE_R * TILE_W, NF = 4;
-  __shared__ double tb[2][NF][TS];
+  constexpr int TSB = DMA ? 4 * 128 : TS;  // DMA: each of the 4 wavefronts fills 128 tile points
+  static_assert(TS <= 4 * 128 && TILE_W % 2 == 0, "tile rows of whole point pairs, 128 points per wavefront");
+  __shared__ double tb[2][NF][TSB];
   __shared__ double tls[TILE_Y * TILE_W];
   const int ntx = (G.nlond + TILE_X - 1) / TILE_X;
   const int bb = tgx_xcd_block(gridDim.x);
@@ -594,6 +616,19 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
 #pragma unroll
       for (int sl = 0; sl < 2; ++sl)
         if (tok[sl]) tb[buf][f][te[sl]] = stage[f][sl];
+  };
+  // DMA staging: this lane's pair of tile points (a pair is inside the grid
+  // or outside it as a whole on the left, top and bottom; on the right a pair
+  // may end one point past the grid, in the row pitch, which is even)
+  const int e2 = 2 * (int)threadIdx.x, wv = (int)threadIdx.x >> 6;
+  const int di = i0 - TILE_H + e2 % TILE_W, dj = j0 - TILE_H + e2 / TILE_W;
+  const bool dok = DMA && e2 < TS && di >= 0 && di < G.nlond && dj >= 0 && dj < G.nlatd;
+  const unsigned doff = dok ? (unsigned)((size_t)dj * G.ld + di) : 0u;  // within one plane
+  auto tile_dma = [&](int k, int buf) {
+    if (dok) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) tile_dma16<DMA>(tsrc[f] + (size_t)k * G.P + doff, &tb[buf][f][128 * wv]);
+    }
   };
   const int ic = active ? i : 0, jc = active ? j : 0;
   const size_t c2 = G.a2(ic, jc);
@@ -650,12 +685,17 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
   }
   double un_m = 0., vn_m = 0.;
   double w1 = 0., w3 = 0., rprev = 0., dudz_prev = 0., dvdz_prev = 0.;
-  tile_load(0);
-  tile_store(0);
+  if (DMA) {
+    tile_dma(0, 0);
+    wait_vmem();
+  } else {
+    tile_load(0);
+    tile_store(0);
+  }
   for (int k = 0; k <= nk - 2; ++k) {
     DS ds = {};
     double tn_pp = 0.;
-    if (k + 1 <= nk - 2) tile_load(k + 1);
+    if (!DMA && k + 1 <= nk - 2) tile_load(k + 1);
     if (solve) {
       load_dl(k + 1, dn);
       load_ds(k, ds);
@@ -664,6 +704,8 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
     const double expzk = g.expz[k], expzk1 = g.expz[k + 1], dift1 = g.dift[k + 1];
     asm volatile("" ::: "memory");
     __syncthreads();  // the level-k tiles are in buffer k&1
+    // buffer (k+1)&1 was last read in iteration k-1, which every wave has left
+    if (DMA && k + 1 <= nk - 2) tile_dma(k + 1, (k + 1) & 1);
     const int bk = k & 1;
     const LdsPlane Ftn{tb[bk][0], i0 - TILE_H, j0 - TILE_H}, U{tb[bk][1], i0 - TILE_H, j0 - TILE_H},
         V{tb[bk][2], i0 - TILE_H, j0 - TILE_H}, Fnm{tb[bk][3], i0 - TILE_H, j0 - TILE_H};
@@ -771,7 +813,12 @@ __global__ void __launch_bounds__(256) dt_tile_kernel(tgx_ng_grid g, Geo G, tgx_
     tn_m = tn_c; tn_c = tn_p; tn_p = tn_pp;
     un_m = dc.un; vn_m = dc.vn;
     dc = dn;
-    if (k + 1 <= nk - 2) tile_store((k + 1) & 1);
+    if (k + 1 <= nk - 2) {
+      if (DMA)
+        wait_vmem();  // this wave's level-(k+1) tile loads have landed in LDS
+      else
+        tile_store((k + 1) & 1);
+    }
   }
   if (!solve) return;
   // back substitution (:165-167) fused with the output stage (:208-213); the
@@ -4758,10 +4805,16 @@ extern "C" int tgx_ng_launch(int which, const tgx_ng_grid *gp, const tgx_ng_fiel
     }
     case 24: {  // dt with the stencils fused, one block per column tile: bufs = {Thomas tiles (DW_NV)}
       dim3 gridt((unsigned)(((G.nlond + TILE_X - 1) / TILE_X) * ((G.nlatd + TILE_Y - 1) / TILE_Y)));
+      // the tiles by direct-to-LDS loads where their fields are 16-byte aligned
+      // and the row pitch is even
+      const bool dma = g.ld % 2 == 0 && ((uintptr_t)in.tn | (uintptr_t)in.un | (uintptr_t)in.vn |
+                                         (uintptr_t)in.tn_nm) % 16 == 0 && t_tile_dma;
       if (g.capture)
-        hipLaunchKernelGGL(dt_tile_kernel<true>, gridt, block, 0, s, g, G, in, out, D(0));
+        hipLaunchKernelGGL((dt_tile_kernel<true, false>), gridt, block, 0, s, g, G, in, out, D(0));
+      else if (dma)
+        hipLaunchKernelGGL((dt_tile_kernel<false, true>), gridt, block, 0, s, g, G, in, out, D(0));
       else
-        hipLaunchKernelGGL(dt_tile_kernel<false>, gridt, block, 0, s, g, G, in, out, D(0));
+        hipLaunchKernelGGL((dt_tile_kernel<false, false>), gridt, block, 0, s, g, G, in, out, D(0));
       break;
     }
     case 1:

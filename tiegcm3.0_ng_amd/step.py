@@ -76,6 +76,7 @@ _SIGS = {
     "tgx_ng_scheme_choice": ([_P, C.c_int], C.c_int),
     "tgx_ng_wave_columns_choice": ([_P, C.c_int], C.c_int),
     "tgx_ng_force_wave_columns": ([C.c_int], None),
+    "tgx_ng_set_tile_dma": ([C.c_int], None),
     "tgx_ng_profile_end": ([_P, _P], C.c_int),
     "tgx_glb_workspace_bytes": ([_P], C.c_ulonglong),
     "tgx_glb_oplus": ([_P] * 5, C.c_int),
