@@ -192,15 +192,23 @@ def bench_nests(pkg, args):
     # the recursion with its map_in / set_bndry / time interpolation / map_out
     ns = pkg.nest.CoupledNests(doms, [1, 2, 5], scheme=args.scheme, forked=not args.serial)
     ms = timed(ns)
-    # the step and the interpolation have no value-dependent branches, so the
-    # time does not depend on the values; a long run on fixed synthetic
-    # lateral data can still drive a nest's O+ to its zero clamp
+    # The synthetic nested state is no equilibrium: with its lateral data
+    # held fixed, a strip of nest 1's O+ reaches zero in the third outer step
+    # and the state turns non-finite from there (the oracle's recursion does
+    # the same, at the same 154 points).  The kernels' work does not depend on
+    # the values, so the timing stands; the differences below are taken after
+    # one outer step, where every value is finite.
     finite = all(bool(torch.isfinite(st.current(n)).all()) for st in ns.states for n in pkg.step.PROGNOSTIC)
+    one = pkg.nest.CoupledNests(doms, [1, 2, 5], scheme=args.scheme, forked=not args.serial)
+    one.run()
+    torch.cuda.synchronize()
+    finite1 = all(bool(torch.isfinite(st.current(n)).all()) for st in one.states for n in pkg.step.PROGNOSTIC)
     # the same recursion with comp in its FP32 + FP64-refinement form
-    # (tgx_ng_step_mixed), the same number of outer steps from the same
-    # state: time, and the largest pointwise relative difference of every
-    # domain's prognostic fields from the FP64 run's (floors of the reference's
-    # clamps; the winds against their column maximum)
+    # (tgx_ng_step_mixed): time over the same number of outer steps, and the
+    # largest pointwise relative difference of every domain's prognostic
+    # fields from the FP64 recursion's after one outer step from the same
+    # state (floors of the reference's clamps; the winds against their column
+    # maximum)
     floors = {"o2": 1e-9, "o1": 1e-9, "he": 1e-9, "no": 1e-12, "n4s": 1e-12, "n2d": 1e-12, "ar": 1e-12,
               "tn": 100., "op": 1e-6}
     precision = {}
@@ -208,11 +216,15 @@ def bench_nests(pkg, args):
         for nsweep in (1, 2):
             nm = pkg.nest.CoupledNests(doms, [1, 2, 5], scheme=args.scheme, forked=True, comp_nsweep=nsweep)
             ms_m = timed(nm)
+            del nm
+            m1 = pkg.nest.CoupledNests(doms, [1, 2, 5], scheme=args.scheme, forked=True, comp_nsweep=nsweep)
+            m1.run()
+            torch.cuda.synchronize()
             err = {}
             for n in pkg.step.PROGNOSTIC:
                 worst = 0.
                 base = n[:-3] if n.endswith("_nm") else n
-                for a, b in zip(ns.states, nm.states):
+                for a, b in zip(one.states, m1.states):
                     x, y = a.current(n), b.current(n)
                     if base in floors:
                         den = torch.clamp(x.abs(), min=floors[base])
@@ -220,13 +232,15 @@ def bench_nests(pkg, args):
                         den = x.abs().amax(dim=0, keepdim=True).clamp(min=1e-30)
                     else:
                         den = x.abs().max().clamp(min=1e-30)
-                    worst = max(worst, float(((x - y).abs() / den).max()))
+                    d = float(((x - y).abs() / den).max())
+                    worst = d if d != d else max(worst, d)  # a NaN is reported, not dropped
                 err[n] = float("%.3e" % worst)
             precision["comp_fp32_refine%d" % nsweep] = {"ms_per_step": round(ms_m, 4),
                                                         "vs_fp64": round(ms / ms_m, 4),
-                                                        "max_rel_diff_vs_fp64": err}
-            del nm
+                                                        "max_rel_diff_vs_fp64_after_one_outer_step": err}
+            del m1
             torch.cuda.empty_cache()
+    del one
     unknowns = 10 + args.nstep_sub
     colev = sum(d[0]["nlon"] * d[0]["nlat"] * d[0]["nk"] * sub[i] for i, d in enumerate(doms)) * unknowns
     alg = sum(pkg.step.alg_bytes_per_column_level(r) * (d[0]["nlon"] + 4) * (d[0]["nlat"] + 4) * d[0]["nk"]
@@ -243,7 +257,10 @@ def bench_nests(pkg, args):
                    "substeps": {names[i]: n for i, n in sub.items()}, "column_levels_per_step": colev,
                    "coupling": "map_in, set_bndry, time interpolation and map_out on the device",
                    "precision_form": precision,
-                   "ms_without_coupling": round(ms_plain, 4), "state_finite_after_run": finite,
+                   "ms_without_coupling": round(ms_plain, 4),
+                   "state_finite_after_one_outer_step": finite1,
+                   "state_finite_after_run": finite,
+                   "outer_steps_run": 1 + args.warmup + args.steps,
                    "scheme": args.scheme, "schemes": ns.states[0].schemes(),
                    "wave_columns_per_block": {names[i]: st.wave_columns() for i, st in enumerate(ns.states)},
                    "schedule": "serial" if args.serial else "concurrent"},
